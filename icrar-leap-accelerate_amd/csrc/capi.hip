@@ -1,0 +1,585 @@
+// C ABI of the gfx950 LEAP calibrator: contexts, metadata, the two halves of
+// PhaseRotate, the full Calibrate loop, synthetic sets and small helpers.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+#include "leap_internal.h"
+#include "synth.h"
+
+using namespace leap;
+
+namespace {
+
+thread_local std::string g_error;  // failures without a context
+
+void set_error(const leap_ctx* ctx, const std::string& msg) {
+    if (ctx)
+        const_cast<leap_ctx*>(ctx)->error = msg;
+    g_error = msg;
+}
+
+template <typename F>
+int guarded(const leap_ctx* ctx, F&& f) {
+    try {
+        f();
+        return LEAP_OK;
+    } catch (const Error& e) {
+        set_error(ctx, e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "host allocation failed");
+        return LEAP_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_error(ctx, e.what());
+        return LEAP_ERR_INTERNAL;
+    }
+}
+
+// Makes the context's device current for the scope of a call.
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev) {
+        LEAP_HIP(hipGetDevice(&prev));
+        if (prev != dev) LEAP_HIP(hipSetDevice(dev));
+    }
+    ~DeviceScope() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+leap_options resolve(const leap_options* o) {
+    leap_options r{};
+    if (o) r = *o;
+    require(r.precision == LEAP_PRECISION_FP64 || r.precision == LEAP_PRECISION_MIXED, LEAP_ERR_INVALID_ARGUMENT,
+            "unknown precision");
+    require(r.strategy == LEAP_STRATEGY_PER_DIRECTION || r.strategy == LEAP_STRATEGY_BATCHED,
+            LEAP_ERR_INVALID_ARGUMENT, "unknown strategy");
+    require(!(r.precision == LEAP_PRECISION_MIXED && r.strategy != LEAP_STRATEGY_BATCHED),
+            LEAP_ERR_INVALID_ARGUMENT, "mixed precision needs the batched strategy");
+    require(r.strategy == LEAP_STRATEGY_PER_DIRECTION, LEAP_ERR_NOT_IMPLEMENTED,
+            "the batched strategy is not built into this library");
+    return r;
+}
+
+size_t vis_elem_bytes(const leap_ms_view& ms) {
+    return (ms.vis_dtype == LEAP_VIS_C64 ? 8 : 16) * size_t(ms.num_pols);
+}
+
+void record_rotate_begin(leap_ctx* ctx) {
+    if (ctx->rotate_events_used == ctx->rotate_events.size()) {
+        hipEvent_t a, b;
+        LEAP_HIP(hipEventCreate(&a));
+        LEAP_HIP(hipEventCreate(&b));
+        ctx->rotate_events.emplace_back(a, b);
+    }
+    LEAP_HIP(hipEventRecord(ctx->rotate_events[ctx->rotate_events_used].first, ctx->stream));
+}
+
+void record_rotate_end(leap_ctx* ctx) {
+    LEAP_HIP(hipEventRecord(ctx->rotate_events[ctx->rotate_events_used].second, ctx->stream));
+    ++ctx->rotate_events_used;
+}
+
+void begin_timing(leap_ctx* ctx) {
+    ctx->rotate_events_used = 0;
+    LEAP_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+}
+
+// Call after the stream has been synchronised.
+void end_timing(leap_ctx* ctx) {
+    LEAP_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    LEAP_HIP(hipEventSynchronize(ctx->ev_stop));
+    float total = 0.f;
+    LEAP_HIP(hipEventElapsedTime(&total, ctx->ev_start, ctx->ev_stop));
+    double sum = 0.0;
+    for (size_t i = 0; i < ctx->rotate_events_used; ++i) {
+        float ms = 0.f;
+        LEAP_HIP(hipEventElapsedTime(&ms, ctx->rotate_events[i].first, ctx->rotate_events[i].second));
+        sum += ms;
+    }
+    ctx->last_total_ms = total;
+    ctx->last_rotate_launches = int32_t(ctx->rotate_events_used);
+    ctx->last_rotate_ms_avg = ctx->rotate_events_used ? sum / double(ctx->rotate_events_used) : 0.0;
+}
+
+// Device pointers of visibilities and uvw covering rows [row_begin, row_end);
+// host data is staged into the context's buffers.  Returned pointers address
+// row row_begin.
+struct DeviceRows {
+    const void* vis;
+    const double* uvw;
+};
+
+DeviceRows stage_rows(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin, uint32_t row_end) {
+    require(row_end <= ms.num_rows && row_begin <= row_end, LEAP_ERR_RANGE,
+            "row selection [" + std::to_string(row_begin) + "," + std::to_string(row_end) +
+                "] exceeds total range [0," + std::to_string(ms.num_rows) + "]");
+    require(ms.vis != nullptr, LEAP_ERR_INVALID_ARGUMENT, "measurement set view has no visibilities");
+    const size_t n = size_t(row_end - row_begin);
+    const size_t vb = vis_elem_bytes(ms) * ms.num_channels;
+    DeviceRows r{};
+    if (ms.vis_on_device) {
+        r.vis = static_cast<const char*>(ms.vis) + size_t(row_begin) * vb;
+    } else {
+        ctx->d_vis_stage.reserve(n * vb);
+        LEAP_HIP(hipMemcpyAsync(ctx->d_vis_stage.ptr, static_cast<const char*>(ms.vis) + size_t(row_begin) * vb,
+                                n * vb, hipMemcpyHostToDevice, ctx->stream));
+        r.vis = ctx->d_vis_stage.ptr;
+    }
+    if (ms.uvw_device) {
+        r.uvw = ms.uvw_device + size_t(row_begin) * 3;
+    } else {
+        ctx->d_uvw.reserve(n * 3 * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(ctx->d_uvw.ptr, ms.uvw + size_t(row_begin) * 3, n * 3 * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->stream));
+        r.uvw = ctx->d_uvw.as<double>();
+    }
+    return r;
+}
+
+void upload_directions(leap_ctx* ctx, const leap_metadata& md, const double* dirs, uint32_t D) {
+    ctx->d_dirs.reserve(size_t(D) * 2 * sizeof(double));
+    ctx->d_dd.reserve(size_t(D) * 9 * sizeof(double));
+    LEAP_HIP(hipMemcpyAsync(ctx->d_dirs.ptr, dirs, size_t(D) * 2 * sizeof(double), hipMemcpyHostToDevice,
+                            ctx->stream));
+    launch_dd(ctx->stream, md.phase_centre_ra, md.phase_centre_dec, ctx->d_dirs.as<double>(), D,
+              ctx->d_dd.as<double>());
+}
+
+// avgData of D directions for T timesteps starting at the staged rows, into ctx->d_avg.
+void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
+                       uint32_t T, uint32_t D) {
+    const size_t per_dir = size_t(md.nbl) * md.pols * 2;
+    ctx->d_avg.reserve(per_dir * D * sizeof(double));
+    for (uint32_t d = 0; d < D; ++d) {
+        record_rotate_begin(ctx);
+        launch_rotate_f64(ctx->stream, rows.vis, ms.vis_dtype, rows.uvw, ctx->d_dd.as<double>() + 9 * d, 0, T,
+                          md.nbl, md.channels, md.pols, md.d_lambda.as<double>(),
+                          ctx->d_avg.as<double>() + per_dir * d);
+        record_rotate_end(ctx);
+    }
+}
+
+void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D) {
+    const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
+    ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
+    ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
+    ctx->d_delta.reserve(size_t(K) * D * sizeof(double));
+    ctx->d_cal.reserve(size_t(N) * D * sizeof(double));
+    ctx->d_partial.reserve(inversion_partial_doubles(K, N, D) * sizeof(double));
+    launch_inversion_f64(ctx->stream, md, ctx->d_avg.as<double>(), D, ctx->d_phase.as<double>(),
+                         ctx->d_cal1.as<double>(), ctx->d_delta.as<double>(), ctx->d_partial.as<double>(),
+                         ctx->d_cal.as<double>());
+}
+
+void check_metadata_matches(const leap_metadata& md, const leap_ms_view& ms) {
+    require(md.nbl == ms.num_baselines && md.channels == ms.num_channels && md.pols == ms.num_pols,
+            LEAP_ERR_INVALID_ARGUMENT, "metadata was built for a different measurement set");
+}
+
+leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t ref_ant, double thr, int use_cache) {
+    validate_view(ms);
+    auto md = std::make_unique<leap_metadata>();
+    md->device = ctx->device;
+    md->nbl = ms.num_baselines;                                                       // MetaData.cc:50
+    md->reference_antenna = ref_ant >= 0 ? uint32_t(ref_ant) : ms.total_antennas - 1;  // :51
+    md->channels = ms.num_channels;                                                   // :58
+    md->freq_start_hz = ms.freq_start_hz;                                             // :59
+    md->freq_inc_hz = ms.freq_inc_hz;                                                 // :60
+    md->rows = ms.num_rows;
+    md->pols = ms.num_pols;
+    md->phase_centre_ra = ms.phase_centre_ra_rad;
+    md->phase_centre_dec = ms.phase_centre_dec_rad;
+    const std::vector<uint8_t> fg = filtered_baselines(ms, thr);  // :84
+    const int32_t n = int32_t(ms.num_baselines);
+    md->A1 = phase_matrix(ms.antenna1, ms.antenna2, fg.data(), n, int32_t(md->reference_antenna));  // :92
+    md->A = phase_matrix(ms.antenna1, ms.antenna2, fg.data(), n, -1);                                // :97
+    md->G1p = gram_pinv(md->A1);                                                                      // :111
+    if (use_cache) {                                                                                  // :112-120
+        const uint64_t key = hash_phase_matrix(md->A);
+        auto it = ctx->gram_cache.find(key);
+        if (it == ctx->gram_cache.end()) it = ctx->gram_cache.emplace(key, gram_pinv(md->A)).first;
+        md->Gp = it->second;
+    } else {
+        md->Gp = gram_pinv(md->A);                                                                    // :123
+    }
+    const int32_t K = md->A.rows, N = md->A.cols, K1 = md->A1.rows;
+    // Device copies.
+    std::vector<int2> pairs(size_t(K - 1)), pairs1(size_t(K1 - 1));
+    for (int32_t k = 0; k + 1 < K; ++k) pairs[k] = make_int2(md->A.ant1[k], md->A.ant2[k]);
+    for (int32_t k = 0; k + 1 < K1; ++k) pairs1[k] = make_int2(md->A1.ant1[k], md->A1.ant2[k]);
+    std::vector<double> lambda(md->channels);
+    for (uint32_t c = 0; c < md->channels; ++c)
+        lambda[c] = kSpeedOfLight / (md->freq_start_hz + int32_t(c) * md->freq_inc_hz);  // MetaData.h:95-98
+    md->d_pairs.reserve(std::max<size_t>(1, pairs.size()) * sizeof(int2));
+    md->d_I1.reserve(std::max<size_t>(1, md->A1.I.size()) * sizeof(int32_t));
+    md->d_lambda.reserve(std::max<size_t>(1, lambda.size()) * sizeof(double));
+    md->d_AdT.reserve(size_t(K) * N * sizeof(double));
+    md->d_Ad1T.reserve(size_t(K1) * N * sizeof(double));
+    md->d_Gp.reserve(size_t(N) * N * sizeof(double) * 2);
+    DeviceBuffer d_pairs1;
+    d_pairs1.reserve(std::max<size_t>(1, pairs1.size()) * sizeof(int2));
+    hipStream_t s = ctx->stream;
+    if (!pairs.empty())
+        LEAP_HIP(hipMemcpyAsync(md->d_pairs.ptr, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    if (!pairs1.empty())
+        LEAP_HIP(hipMemcpyAsync(d_pairs1.ptr, pairs1.data(), pairs1.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    if (!md->A1.I.empty())
+        LEAP_HIP(hipMemcpyAsync(md->d_I1.ptr, md->A1.I.data(), md->A1.I.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+    if (!lambda.empty())
+        LEAP_HIP(hipMemcpyAsync(md->d_lambda.ptr, lambda.data(), lambda.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+    double* gp = md->d_Gp.as<double>();
+    double* g1p = gp + size_t(N) * N;
+    LEAP_HIP(hipMemcpyAsync(gp, md->Gp.data(), size_t(N) * N * sizeof(double), hipMemcpyHostToDevice, s));
+    LEAP_HIP(hipMemcpyAsync(g1p, md->G1p.data(), size_t(N) * N * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_assemble_ad(s, gp, N, md->d_pairs.as<int2>(), K - 1, md->A.constraint, md->d_AdT.as<double>());
+    launch_assemble_ad(s, g1p, N, d_pairs1.as<int2>(), K1 - 1, md->A1.constraint, md->d_Ad1T.as<double>());
+    LEAP_HIP(hipStreamSynchronize(s));
+    return md.release();
+}
+
+// The Calibrate loop (CpuLeapCalibrator.cc:61-145) over prepared metadata.
+void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const double* dirs, uint32_t D,
+                    int32_t s_start, int32_t s_interval, int32_t s_end, const leap_options& opts,
+                    leap_solution_cb cb, void* user) {
+    (void)opts;
+    validate_view(ms);
+    check_metadata_matches(md, ms);
+    require(dirs != nullptr || D == 0, LEAP_ERR_INVALID_ARGUMENT, "null directions");
+    const int32_t T = int32_t(ms.num_rows / ms.num_baselines);  // MeasurementSet.cc:115-118
+    int32_t range[3];
+    evaluate_slice(s_start, s_interval, s_end, T, range);      // CpuLeapCalibrator.cc:90
+    const int32_t solutions = (range[2] - range[0]) / range[1];  // Range::GetSize
+    const uint32_t interval = uint32_t(range[1]);
+    const uint32_t nbl = ms.num_baselines;
+    const int32_t N = md.A.cols;
+    begin_timing(ctx);
+    if (D > 0) upload_directions(ctx, md, dirs, D);
+    ctx->h_cal.resize(size_t(N) * D);
+    for (int32_t s = 0; s < solutions; ++s) {  // :104
+        // :113-119 — rows [s * interval * nbl, +interval * nbl); the slice start is not added.
+        const uint32_t row_begin = uint32_t(size_t(s) * interval * nbl);
+        const uint32_t row_end = uint32_t(size_t(s + 1) * interval * nbl);
+        leap_solution sol{};
+        sol.index = uint32_t(s);
+        sol.start_epoch = ms.time[size_t(s) * interval * nbl];               // :107-109
+        sol.end_epoch = ms.time[(size_t(s + 1) * interval - 1) * nbl];
+        sol.n_directions = D;
+        sol.directions = dirs;
+        sol.n_antennas = uint32_t(N);
+        sol.calibrations = ctx->h_cal.data();
+        if (D > 0) {
+            const DeviceRows rows = stage_rows(ctx, ms, row_begin, row_end);
+            rotate_directions(ctx, md, ms, rows, interval, D);
+            invert_directions(ctx, md, D);
+            LEAP_HIP(hipMemcpyAsync(ctx->h_cal.data(), ctx->d_cal.ptr, size_t(N) * D * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+            LEAP_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (cb && cb(user, &sol) != 0) throw Error(LEAP_ERR_CALLBACK, "output callback requested a stop");
+    }
+    LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    end_timing(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* leap_hip_version(void) { return "leap-hip 0.1.0 (gfx950)"; }
+
+const char* leap_hip_last_error(const leap_ctx* ctx) {
+    if (ctx) return ctx->error.c_str();
+    return g_error.c_str();
+}
+
+int leap_hip_create(leap_ctx** out, int32_t device) {
+    return guarded(nullptr, [&] {
+        require(out != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null output pointer");
+        int count = 0;
+        LEAP_HIP(hipGetDeviceCount(&count));
+        require(device >= 0 && device < count, LEAP_ERR_INVALID_ARGUMENT,
+                "device " + std::to_string(device) + " not available (" + std::to_string(count) + " devices)");
+        auto ctx = std::make_unique<leap_ctx>();
+        ctx->device = device;
+        DeviceScope scope(device);
+        LEAP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        LEAP_HIP(hipEventCreate(&ctx->ev_start));
+        LEAP_HIP(hipEventCreate(&ctx->ev_stop));
+        *out = ctx.release();
+    });
+}
+
+void leap_hip_destroy(leap_ctx* ctx) {
+    if (!ctx) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->rotate_events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    hipStream_t s = ctx->stream;
+    delete ctx;  // frees the workspace buffers on this device
+    if (s) (void)hipStreamDestroy(s);
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+int leap_hip_metadata_create(leap_ctx* ctx, const leap_ms_view* ms, int32_t ref_ant, double thr, int32_t use_cache,
+                             leap_metadata** out) {
+    return guarded(ctx, [&] {
+        require(ctx && ms && out, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceScope scope(ctx->device);
+        *out = create_metadata(ctx, *ms, ref_ant, thr, use_cache);
+    });
+}
+
+void leap_hip_metadata_destroy(leap_metadata* md) {
+    if (!md) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(md->device);
+    delete md;
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+void leap_hip_metadata_dims(const leap_metadata* md, int32_t* dims) {
+    dims[0] = md->A.rows;
+    dims[1] = md->A.cols;
+    dims[2] = md->A1.rows;
+    dims[3] = int32_t(md->nbl);
+    dims[4] = int32_t(md->reference_antenna);
+}
+
+int leap_hip_metadata_copy(const leap_metadata* md, double* A, int32_t* I, double* Ad, double* A1, int32_t* I1,
+                           double* Ad1) {
+    return guarded(nullptr, [&] {
+        require(md != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null metadata");
+        DeviceScope scope(md->device);
+        if (A) {
+            const auto a = md->A.dense();
+            std::copy(a.begin(), a.end(), A);
+        }
+        if (I) std::copy(md->A.I.begin(), md->A.I.end(), I);
+        if (A1) {
+            const auto a = md->A1.dense();
+            std::copy(a.begin(), a.end(), A1);
+        }
+        if (I1) std::copy(md->A1.I.begin(), md->A1.I.end(), I1);
+        // AdT [K][N] row-major is Ad (N x K) column-major.
+        if (Ad)
+            LEAP_HIP(hipMemcpy(Ad, md->d_AdT.ptr, size_t(md->A.rows) * md->A.cols * sizeof(double),
+                               hipMemcpyDeviceToHost));
+        if (Ad1)
+            LEAP_HIP(hipMemcpy(Ad1, md->d_Ad1T.ptr, size_t(md->A1.rows) * md->A1.cols * sizeof(double),
+                               hipMemcpyDeviceToHost));
+    });
+}
+
+int leap_hip_rotate_visibilities(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms, uint32_t row_start,
+                                 uint32_t n_rows, const double* dirs, uint32_t D, const leap_options* opts,
+                                 double* avg_out) {
+    return guarded(ctx, [&] {
+        require(ctx && md && ms && avg_out && (dirs || D == 0), LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        resolve(opts);
+        validate_view(*ms);
+        check_metadata_matches(*md, *ms);
+        require(n_rows % ms->num_baselines == 0, LEAP_ERR_RANGE,
+                "integration rows must be a whole number of timesteps");
+        DeviceScope scope(ctx->device);
+        begin_timing(ctx);
+        const DeviceRows rows = stage_rows(ctx, *ms, row_start, row_start + n_rows);
+        upload_directions(ctx, *md, dirs, D);
+        rotate_directions(ctx, *md, *ms, rows, n_rows / ms->num_baselines, D);
+        const size_t n = size_t(md->nbl) * md->pols * 2 * D;
+        if (n) LEAP_HIP(hipMemcpyAsync(avg_out, ctx->d_avg.ptr, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+        end_timing(ctx);
+    });
+}
+
+int leap_hip_apply_inversion(leap_ctx* ctx, const leap_metadata* md, const double* avg, uint32_t D,
+                             const leap_options* opts, double* cal_out) {
+    return guarded(ctx, [&] {
+        require(ctx && md && avg && cal_out, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        resolve(opts);
+        DeviceScope scope(ctx->device);
+        const size_t n = size_t(md->nbl) * md->pols * 2 * D;
+        ctx->d_avg.reserve(n * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(ctx->d_avg.ptr, avg, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        invert_directions(ctx, *md, D);
+        LEAP_HIP(hipMemcpyAsync(cal_out, ctx->d_cal.ptr, size_t(md->A.cols) * D * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int leap_hip_dd_matrix(leap_ctx* ctx, double ra0, double dec0, const double* dirs, uint32_t D, double* dd_out) {
+    return guarded(ctx, [&] {
+        require(ctx && dirs && dd_out, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceScope scope(ctx->device);
+        ctx->d_dirs.reserve(size_t(D) * 2 * sizeof(double));
+        ctx->d_dd.reserve(size_t(D) * 9 * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(ctx->d_dirs.ptr, dirs, size_t(D) * 2 * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream));
+        launch_dd(ctx->stream, ra0, dec0, ctx->d_dirs.as<double>(), D, ctx->d_dd.as<double>());
+        LEAP_HIP(hipMemcpyAsync(dd_out, ctx->d_dd.ptr, size_t(D) * 9 * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int leap_hip_calibrate_md(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms, const double* dirs,
+                          uint32_t D, int32_t s_start, int32_t s_interval, int32_t s_end, const leap_options* opts,
+                          leap_solution_cb cb, void* user) {
+    return guarded(ctx, [&] {
+        require(ctx && md && ms, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        const leap_options o = resolve(opts);
+        DeviceScope scope(ctx->device);
+        calibrate_loop(ctx, *md, *ms, dirs, D, s_start, s_interval, s_end, o, cb, user);
+    });
+}
+
+int leap_hip_calibrate(leap_ctx* ctx, const leap_ms_view* ms, const double* dirs, uint32_t D, int32_t s_start,
+                       int32_t s_interval, int32_t s_end, double thr, int32_t ref_ant, int32_t use_cache,
+                       const leap_options* opts, leap_solution_cb cb, void* user) {
+    return guarded(ctx, [&] {
+        require(ctx && ms, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        const leap_options o = resolve(opts);
+        DeviceScope scope(ctx->device);
+        validate_view(*ms);
+        // Slice errors surface before the metadata work, as Evaluate precedes MetaData (CpuLeapCalibrator.cc:90-99).
+        int32_t range[3];
+        evaluate_slice(s_start, s_interval, s_end, int32_t(ms->num_rows / ms->num_baselines), range);
+        std::unique_ptr<leap_metadata> md(create_metadata(ctx, *ms, ref_ant, thr, use_cache));
+        calibrate_loop(ctx, *md, *ms, dirs, D, s_start, s_interval, s_end, o, cb, user);
+    });
+}
+
+int leap_hip_synth_fill_vis_device(leap_ctx* ctx, const leap_synth_ms* ms, int32_t vis_dtype, void* out_device,
+                                   double* uvw_device_out) {
+    return guarded(ctx, [&] {
+        require(ctx && ms && out_device, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceScope scope(ctx->device);
+        DeviceBuffer d_uvw_tmp, d_blphase;
+        double* d_uvw = uvw_device_out;
+        const size_t uvw_bytes = ms->uvw.size() * sizeof(double);
+        if (!d_uvw) {
+            d_uvw_tmp.reserve(uvw_bytes);
+            d_uvw = d_uvw_tmp.as<double>();
+        }
+        LEAP_HIP(hipMemcpyAsync(d_uvw, ms->uvw.data(), uvw_bytes, hipMemcpyHostToDevice, ctx->stream));
+        d_blphase.reserve(ms->bl_phase.size() * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(d_blphase.ptr, ms->bl_phase.data(), ms->bl_phase.size() * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->stream));
+        leap_synth_ms dev_view;  // parameters with the device phase table
+        dev_view.cfg = ms->cfg;
+        dev_view.n_rows = ms->n_rows;
+        dev_view.params = ms->params;
+        dev_view.params.bl_phase = d_blphase.as<double>();
+        launch_synth_vis(ctx->stream, &dev_view, vis_dtype, out_device, d_uvw);
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int leap_hip_malloc(leap_ctx* ctx, uint64_t bytes, void** out) {
+    return guarded(ctx, [&] {
+        require(ctx && out, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceScope scope(ctx->device);
+        LEAP_HIP(hipMalloc(out, bytes));
+    });
+}
+
+int leap_hip_free(leap_ctx* ctx, void* ptr) {
+    return guarded(ctx, [&] {
+        require(ctx != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null context");
+        DeviceScope scope(ctx->device);
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+        if (ptr) LEAP_HIP(hipFree(ptr));
+    });
+}
+
+int leap_hip_memcpy_d2h(leap_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded(ctx, [&] {
+        require(ctx && dst && src, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceScope scope(ctx->device);
+        LEAP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int leap_hip_memcpy_h2d(leap_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded(ctx, [&] {
+        require(ctx && dst && src, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceScope scope(ctx->device);
+        LEAP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int leap_hip_last_kernel_times(const leap_ctx* ctx, double* rotate_ms_avg, int32_t* rotate_launches,
+                               double* total_ms) {
+    if (!ctx) return LEAP_ERR_INVALID_ARGUMENT;
+    if (rotate_ms_avg) *rotate_ms_avg = ctx->last_rotate_ms_avg;
+    if (rotate_launches) *rotate_launches = ctx->last_rotate_launches;
+    if (total_ms) *total_ms = ctx->last_total_ms;
+    return LEAP_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int leap_phase_matrix(const int32_t* a1, const int32_t* a2, const uint8_t* fg, int32_t n, int32_t ref_ant,
+                      double* A_out, int32_t* I_out, int32_t* K, int32_t* N) {
+    return guarded(nullptr, [&] {
+        require(K && N && (n == 0 || (a1 && a2 && fg)), LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        const PhaseMatrix pm = phase_matrix(a1, a2, fg, n, ref_ant);
+        *K = pm.rows;
+        *N = pm.cols;
+        if (A_out) {
+            const size_t ld = size_t(n) + 1;
+            for (int32_t j = 0; j < pm.cols; ++j)
+                for (int32_t i = 0; i < pm.rows; ++i) A_out[size_t(j) * ld + i] = 0.0;
+            for (int32_t k = 0; k + 1 < pm.rows; ++k) {
+                A_out[size_t(pm.ant1[k]) * ld + k] = 1.0;
+                A_out[size_t(pm.ant2[k]) * ld + k] = -1.0;
+            }
+            A_out[size_t(pm.constraint) * ld + (pm.rows - 1)] = 1.0;
+        }
+        if (I_out) std::copy(pm.I.begin(), pm.I.end(), I_out);
+    });
+}
+
+int leap_phase_matrix_pinv(const int32_t* a1, const int32_t* a2, const uint8_t* fg, int32_t n, int32_t ref_ant,
+                           double* Ad_out) {
+    return guarded(nullptr, [&] {
+        require(Ad_out && (n == 0 || (a1 && a2 && fg)), LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        const PhaseMatrix pm = phase_matrix(a1, a2, fg, n, ref_ant);
+        const std::vector<double> Gp = gram_pinv(pm);
+        const int32_t K = pm.rows, Nn = pm.cols;
+        // Ad = G+ A^T, column k of Ad = G+ (e_a1 - e_a2), last column G+ e_c.
+        for (int32_t k = 0; k < K; ++k) {
+            double* col = Ad_out + size_t(k) * Nn;
+            if (k + 1 < K) {
+                const double* g1 = Gp.data() + size_t(pm.ant1[k]) * Nn;
+                const double* g2 = Gp.data() + size_t(pm.ant2[k]) * Nn;
+                for (int32_t i = 0; i < Nn; ++i) col[i] = g1[i] - g2[i];
+            } else {
+                const double* gc = Gp.data() + size_t(pm.constraint) * Nn;
+                for (int32_t i = 0; i < Nn; ++i) col[i] = gc[i];
+            }
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Internal declarations of the gfx950 LEAP calibrator library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "leap_hip.h"
+
+namespace leap {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kTwoPi = 6.283185307179586476925286766559005768;
+constexpr double kSpeedOfLight = 299792458.0;  // common/constants.h:35 of the reference
+
+// Error carried to the C ABI as a status code plus message.
+struct Error : std::runtime_error {
+    leap_status status;
+    Error(leap_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define LEAP_HIP(call) ::leap::hip_check((call), #call, __FILE__, __LINE__)
+
+inline void require(bool ok, leap_status s, const std::string& msg) {
+    if (!ok) throw Error(s, msg);
+}
+
+// Device buffer owned by a context (RAII, frees on the context's device).
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer();
+    // Grows (never shrinks) to at least n bytes; contents are not kept.
+    void reserve(size_t n);
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Host-side phase matrices, in the reference's layout conventions.
+struct PhaseMatrix {
+    int32_t rows = 0;  // K: baselines used + the constraint row
+    int32_t cols = 0;  // N: max(antenna1, antenna2) + 1
+    std::vector<int32_t> I;       // K - 1 baseline indices
+    std::vector<int32_t> ant1;    // K - 1: antenna1 of baseline I(k)
+    std::vector<int32_t> ant2;    // K - 1: antenna2 of baseline I(k)
+    int32_t constraint = 0;       // column of the constraint row
+    std::vector<double> dense() const;  // K x N column-major
+};
+
+// Builds A (ref_ant < 0) or A1; throws Error with the reference's messages.
+PhaseMatrix phase_matrix(const int32_t* a1, const int32_t* a2, const uint8_t* fg, int32_t n,
+                         int32_t ref_ant);
+
+// Moore-Penrose inverse of the Gram matrix G = A^T A (N x N, column-major)
+// for an incidence-plus-constraint matrix A.
+std::vector<double> gram_pinv(const PhaseMatrix& pm);
+
+// Filtered baselines (flags of the first epoch OR short baselines).
+std::vector<uint8_t> filtered_baselines(const leap_ms_view& ms, double threshold);
+
+// icrar::Slice(...).Evaluate(T) -> {start, interval, end}; throws on bad slices.
+void evaluate_slice(int32_t start, int32_t interval, int32_t end, int32_t T, int32_t out[3]);
+
+// Validates the ms view (MeasurementSet::Validate, ms/MeasurementSet.cc:64-103).
+void validate_view(const leap_ms_view& ms);
+
+uint64_t hash_phase_matrix(const PhaseMatrix& pm);
+
+}  // namespace leap
+
+// Metadata resident on the device (cpu::MetaData + cuda::ConstantBuffer of the reference).
+struct leap_metadata {
+    int device = 0;
+    uint32_t nbl = 0, channels = 0, pols = 0, rows = 0;
+    uint32_t reference_antenna = 0;
+    double freq_start_hz = 0, freq_inc_hz = 0;
+    double phase_centre_ra = 0, phase_centre_dec = 0;
+    leap::PhaseMatrix A, A1;
+    std::vector<double> Gp, G1p;  // N x N pseudo-inverses of A^T A and A1^T A1
+    // device
+    leap::DeviceBuffer d_AdT;      // K x N row-major == Ad column-major (N x K) transposed
+    leap::DeviceBuffer d_Ad1T;     // K1 x N
+    leap::DeviceBuffer d_I1;       // K1 - 1 ints (baseline index of phase angle k)
+    leap::DeviceBuffer d_pairs;    // K - 1 int2: (antenna1, antenna2) of baseline I(k)
+    leap::DeviceBuffer d_lambda;   // channels doubles: wavelength of each channel
+    leap::DeviceBuffer d_Gp;       // scratch for Ad assembly
+};
+
+struct leap_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> rotate_events;
+    size_t rotate_events_used = 0;
+    double last_rotate_ms_avg = 0, last_total_ms = 0;
+    int32_t last_rotate_launches = 0;
+    // workspace
+    leap::DeviceBuffer d_uvw, d_vis_stage, d_dirs, d_dd, d_avg, d_phase, d_cal1, d_delta, d_cal,
+        d_partial;
+    std::vector<double> h_avg, h_cal;
+    // pseudo-inverse cache keyed by a hash of A (use_cache)
+    std::map<uint64_t, std::vector<double>> gram_cache;
+};
+
+namespace leap {
+// Kernel launchers (kernels_f64.hip)
+void launch_dd(hipStream_t s, double ra0, double dec0, const double* d_dirs, uint32_t D,
+               double* d_dd);
+void launch_rotate_f64(hipStream_t s, const void* vis, int32_t vis_dtype, const double* uvw,
+                       const double* dd, uint32_t row_start, uint32_t T, uint32_t nbl, uint32_t C,
+                       uint32_t P, const double* lambda, double* avg);
+void launch_assemble_ad(hipStream_t s, const double* Gp, int32_t N, const int2* pairs,
+                        int32_t n_pairs, int32_t constraint, double* AdT);
+void launch_inversion_f64(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D,
+                          double* phase, double* cal1, double* delta, double* partial, double* cal);
+size_t inversion_partial_doubles(int32_t K, int32_t N, uint32_t D);
+void launch_synth_vis(hipStream_t s, const leap_synth_ms* ms, int32_t vis_dtype, void* out,
+                      const double* d_uvw);
+}  // namespace leap

@@ -1,0 +1,460 @@
+"""Python binding of the gfx950 LEAP calibrator's C ABI (include/leap_hip.h).
+
+A thin ctypes layer used by the test-suite, the smoke check and the
+benchmark.  The shared library is built in-tree (build/libleap_hip.so); if
+it is missing this module raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "build", "libleap_hip.so")
+
+LEAP_OK = 0
+LEAP_VIS_C128 = 0
+LEAP_VIS_C64 = 1
+LEAP_PRECISION_FP64 = 0
+LEAP_PRECISION_MIXED = 1
+LEAP_STRATEGY_PER_DIRECTION = 0
+LEAP_STRATEGY_BATCHED = 1
+
+_STATUS = {
+    -1: "invalid argument",
+    -2: "range",
+    -3: "hip",
+    -4: "out of memory",
+    -5: "not implemented",
+    -6: "callback",
+    -7: "internal",
+}
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_dp = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class LeapError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class MsView(ctypes.Structure):
+    _fields_ = [
+        ("num_stations", ctypes.c_uint32),
+        ("total_antennas", ctypes.c_uint32),
+        ("num_baselines", ctypes.c_uint32),
+        ("num_rows", ctypes.c_uint32),
+        ("num_channels", ctypes.c_uint32),
+        ("num_pols", ctypes.c_uint32),
+        ("freq_start_hz", ctypes.c_double),
+        ("freq_inc_hz", ctypes.c_double),
+        ("phase_centre_ra_rad", ctypes.c_double),
+        ("phase_centre_dec_rad", ctypes.c_double),
+        ("antenna1", _i32p),
+        ("antenna2", _i32p),
+        ("time", _dp),
+        ("flag", _u8p),
+        ("uvw", _dp),
+        ("vis", ctypes.c_void_p),
+        ("vis_dtype", ctypes.c_int32),
+        ("vis_on_device", ctypes.c_int32),
+        ("uvw_device", _dp),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("strategy", ctypes.c_int32),
+        ("direction_block", ctypes.c_int32),
+        ("all_pols", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION, direction_block=0, all_pols=0):
+    o = Options()
+    o.precision = precision
+    o.strategy = strategy
+    o.direction_block = direction_block
+    o.all_pols = all_pols
+    return o
+
+
+class Solution(ctypes.Structure):
+    _fields_ = [
+        ("index", ctypes.c_uint32),
+        ("start_epoch", ctypes.c_double),
+        ("end_epoch", ctypes.c_double),
+        ("n_directions", ctypes.c_uint32),
+        ("directions", _dp),
+        ("n_antennas", ctypes.c_uint32),
+        ("calibrations", _dp),
+    ]
+
+
+SOLUTION_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Solution))
+
+
+class SynthConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_antennas", ctypes.c_uint32),
+        ("layout", ctypes.c_int32),
+        ("array_radius_m", ctypes.c_double),
+        ("autocorrelations", ctypes.c_int32),
+        ("n_timesteps", ctypes.c_uint32),
+        ("integration_s", ctypes.c_double),
+        ("n_channels", ctypes.c_uint32),
+        ("freq_start_hz", ctypes.c_double),
+        ("freq_inc_hz", ctypes.c_double),
+        ("n_pols", ctypes.c_uint32),
+        ("phase_centre_ra_rad", ctypes.c_double),
+        ("phase_centre_dec_rad", ctypes.c_double),
+        ("latitude_rad", ctypes.c_double),
+        ("n_sources", ctypes.c_uint32),
+        ("source_flux_min", ctypes.c_double),
+        ("source_flux_max", ctypes.c_double),
+        ("source_offset_max_rad", ctypes.c_double),
+        ("noise_sigma", ctypes.c_double),
+        ("flag_fraction", ctypes.c_double),
+        ("nan_fraction", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+# Exported symbols of include/leap_hip.h (checked by the CPU test-suite).
+EXPORTS = [
+    "leap_hip_create", "leap_hip_destroy", "leap_hip_last_error", "leap_hip_version",
+    "leap_hip_calibrate", "leap_hip_calibrate_md",
+    "leap_hip_metadata_create", "leap_hip_metadata_destroy", "leap_hip_metadata_dims", "leap_hip_metadata_copy",
+    "leap_hip_rotate_visibilities", "leap_hip_apply_inversion", "leap_hip_dd_matrix",
+    "leap_synth_create", "leap_synth_destroy", "leap_synth_view", "leap_synth_sky", "leap_synth_fill_vis_host",
+    "leap_hip_synth_fill_vis_device",
+    "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
+    "leap_hip_last_kernel_times",
+]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", PKG_DIR, "-j8"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    """Load build/libleap_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C {PKG_DIR}")
+    L = ctypes.CDLL(LIB_PATH)
+    L.leap_hip_last_error.restype = ctypes.c_char_p
+    L.leap_hip_last_error.argtypes = [ctypes.c_void_p]
+    L.leap_hip_version.restype = ctypes.c_char_p
+    L.leap_hip_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32]
+    L.leap_hip_destroy.argtypes = [ctypes.c_void_p]
+    L.leap_hip_destroy.restype = None
+    L.leap_hip_calibrate.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), _dp, ctypes.c_uint32,
+                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                     ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Options), SOLUTION_CB,
+                                     ctypes.c_void_p]
+    L.leap_hip_calibrate_md.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp,
+                                        ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.POINTER(Options), SOLUTION_CB, ctypes.c_void_p]
+    L.leap_hip_metadata_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), ctypes.c_int32,
+                                           ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    L.leap_hip_metadata_destroy.argtypes = [ctypes.c_void_p]
+    L.leap_hip_metadata_destroy.restype = None
+    L.leap_hip_metadata_dims.argtypes = [ctypes.c_void_p, _i32p]
+    L.leap_hip_metadata_dims.restype = None
+    L.leap_hip_metadata_copy.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp, _dp, _i32p, _dp]
+    L.leap_hip_rotate_visibilities.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView),
+                                               ctypes.c_uint32, ctypes.c_uint32, _dp, ctypes.c_uint32,
+                                               ctypes.POINTER(Options), _dp]
+    L.leap_hip_apply_inversion.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_uint32,
+                                           ctypes.POINTER(Options), _dp]
+    L.leap_hip_dd_matrix.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_uint32, _dp]
+    L.leap_synth_create.argtypes = [ctypes.POINTER(SynthConfig), ctypes.POINTER(ctypes.c_void_p)]
+    L.leap_synth_destroy.argtypes = [ctypes.c_void_p]
+    L.leap_synth_destroy.restype = None
+    L.leap_synth_view.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView)]
+    L.leap_synth_view.restype = None
+    L.leap_synth_sky.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.leap_synth_sky.restype = None
+    L.leap_synth_fill_vis_host.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+                                           ctypes.c_void_p]
+    L.leap_hip_synth_fill_vis_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                 ctypes.c_void_p, _dp]
+    L.leap_hip_malloc.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+    L.leap_hip_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.leap_hip_memcpy_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    L.leap_hip_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    L.leap_hip_last_kernel_times.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp]
+    _lib = L
+    return L
+
+
+def _check(rc, ctx=None):
+    if rc != LEAP_OK:
+        msg = lib().leap_hip_last_error(ctx).decode()
+        raise LeapError(rc, msg)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic measurement sets
+# ---------------------------------------------------------------------------
+def synth_config(**kw):
+    """Defaults follow the 128-antenna MWA-like set (1.5 km disk, 14 x 2 s, 48 channels)."""
+    c = SynthConfig()
+    d = dict(n_antennas=128, layout=0, array_radius_m=750.0, autocorrelations=1, n_timesteps=14, integration_s=2.0,
+             n_channels=48, freq_start_hz=169.6e6, freq_inc_hz=40e3, n_pols=4, phase_centre_ra_rad=0.5,
+             phase_centre_dec_rad=np.deg2rad(-27.0), latitude_rad=np.deg2rad(-26.7033), n_sources=5,
+             source_flux_min=1.0, source_flux_max=10.0, source_offset_max_rad=0.05, noise_sigma=0.1,
+             flag_fraction=0.01, nan_fraction=0.0, seed=20201)
+    d.update(kw)
+    for k, v in d.items():
+        setattr(c, k, v)
+    return c
+
+
+class SynthMS:
+    """A synthetic measurement set: host columns, plus host or device visibilities."""
+
+    def __init__(self, **kw):
+        self.cfg = synth_config(**kw)
+        h = ctypes.c_void_p()
+        rc = lib().leap_synth_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        _check(rc)
+        self.h = h
+        v = MsView()
+        lib().leap_synth_view(h, ctypes.byref(v))
+        self.view_template = v
+        self.num_rows = v.num_rows
+        self.num_baselines = v.num_baselines
+        self.num_channels = v.num_channels
+        self.num_pols = v.num_pols
+        n = self.cfg.n_antennas
+        nb = v.num_baselines
+        self.antenna1 = np.ctypeslib.as_array(v.antenna1, shape=(nb,)).copy()
+        self.antenna2 = np.ctypeslib.as_array(v.antenna2, shape=(nb,)).copy()
+        self.flag = np.ctypeslib.as_array(v.flag, shape=(nb,)).copy()
+        self.uvw = np.ctypeslib.as_array(v.uvw, shape=(v.num_rows, 3)).copy()
+        self.time = np.ctypeslib.as_array(v.time, shape=(v.num_rows,)).copy()
+        self.sources = np.zeros((self.cfg.n_sources, 3))
+        self.antenna_phase = np.zeros(n)
+        lib().leap_synth_sky(h, _dptr(self.sources) if self.cfg.n_sources else None, _dptr(self.antenna_phase))
+        self.vis = None
+
+    def fill_host(self, dtype=np.complex128, row_start=0, n_rows=None):
+        if n_rows is None:
+            n_rows = self.num_rows - row_start
+        out = np.zeros((n_rows, self.num_channels, self.num_pols), dtype=dtype)
+        vd = LEAP_VIS_C64 if dtype == np.complex64 else LEAP_VIS_C128
+        _check(lib().leap_synth_fill_vis_host(self.h, row_start, n_rows, vd, out.ctypes.data))
+        if row_start == 0 and n_rows == self.num_rows:
+            self.vis = out
+        return out
+
+    def host_dict(self, vis=None):
+        """Dictionary form used by the oracle bindings in tests/."""
+        vis = self.vis if vis is None else vis
+        v = self.view_template
+        return {
+            "num_stations": v.num_stations, "total_antennas": v.total_antennas, "num_baselines": v.num_baselines,
+            "num_rows": v.num_rows, "num_channels": v.num_channels, "num_pols": v.num_pols,
+            "freq_start_hz": v.freq_start_hz, "freq_inc_hz": v.freq_inc_hz,
+            "phase_centre": (v.phase_centre_ra_rad, v.phase_centre_dec_rad),
+            "antenna1": self.antenna1, "antenna2": self.antenna2, "time": self.time, "flag": self.flag,
+            "uvw": self.uvw, "vis": vis,
+        }
+
+    def __del__(self):
+        try:
+            lib().leap_synth_destroy(self.h)
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def view_from_arrays(ms, vis=None, vis_device=None, vis_dtype=None, uvw_device=None):
+    """Build an MsView from a host MS dict; vis may be host numpy or a device pointer."""
+    v = MsView()
+    v.num_stations = ms["num_stations"]
+    v.total_antennas = ms["total_antennas"]
+    v.num_baselines = ms["num_baselines"]
+    v.num_rows = ms["num_rows"]
+    v.num_channels = ms["num_channels"]
+    v.num_pols = ms["num_pols"]
+    v.freq_start_hz = ms["freq_start_hz"]
+    v.freq_inc_hz = ms["freq_inc_hz"]
+    v.phase_centre_ra_rad, v.phase_centre_dec_rad = ms["phase_centre"]
+    keep = {}
+    for name, dt, ct in (("antenna1", np.int32, ctypes.c_int32), ("antenna2", np.int32, ctypes.c_int32),
+                         ("time", np.float64, ctypes.c_double), ("flag", np.uint8, ctypes.c_uint8),
+                         ("uvw", np.float64, ctypes.c_double)):
+        a = np.ascontiguousarray(ms[name], dtype=dt)
+        keep[name] = a
+        setattr(v, name, a.ctypes.data_as(ctypes.POINTER(ct)))
+    if vis_device is not None:
+        v.vis = vis_device
+        v.vis_on_device = 1
+        v.vis_dtype = vis_dtype
+    else:
+        vis = ms["vis"] if vis is None else vis
+        vis = np.ascontiguousarray(vis)
+        keep["vis"] = vis
+        v.vis = vis.ctypes.data
+        v.vis_on_device = 0
+        v.vis_dtype = LEAP_VIS_C64 if vis.dtype == np.complex64 else LEAP_VIS_C128
+    if uvw_device is not None:
+        v.uvw_device = ctypes.cast(uvw_device, _dp)
+    v._keep = keep  # noqa: SLF001
+    return v
+
+
+# ---------------------------------------------------------------------------
+# Calibrator context
+# ---------------------------------------------------------------------------
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().leap_hip_create(ctypes.byref(h), device))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().leap_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def _chk(self, rc):
+        _check(rc, self.h)
+
+    # device memory --------------------------------------------------------
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        self._chk(lib().leap_hip_malloc(self.h, nbytes, ctypes.byref(p)))
+        return p.value
+
+    def free(self, ptr):
+        self._chk(lib().leap_hip_free(self.h, ptr))
+
+    def d2h(self, dst_array, src_ptr, nbytes=None):
+        nbytes = dst_array.nbytes if nbytes is None else nbytes
+        self._chk(lib().leap_hip_memcpy_d2h(self.h, dst_array.ctypes.data, src_ptr, nbytes))
+
+    def h2d(self, dst_ptr, src_array):
+        self._chk(lib().leap_hip_memcpy_h2d(self.h, dst_ptr, src_array.ctypes.data, src_array.nbytes))
+
+    def synth_fill_device(self, synth, vis_dtype, vis_ptr, uvw_ptr):
+        self._chk(lib().leap_hip_synth_fill_vis_device(self.h, synth.h, vis_dtype, vis_ptr,
+                                                       ctypes.cast(uvw_ptr, _dp) if uvw_ptr else None))
+
+    # metadata / kernels ---------------------------------------------------
+    def metadata(self, view, ref_ant=None, threshold=0.0, use_cache=False):
+        return Metadata(self, view, ref_ant, threshold, use_cache)
+
+    def dd(self, centre, directions):
+        dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((len(dirs), 3, 3))
+        self._chk(lib().leap_hip_dd_matrix(self.h, centre[0], centre[1], _dptr(dirs), len(dirs), _dptr(out)))
+        return out
+
+    def calibrate(self, view, directions, slice_=(0, -1, -1), threshold=0.0, ref_ant=None, use_cache=False,
+                  opts=None, md=None):
+        """Returns (cal [solutions][directions][N], epochs [solutions][2])."""
+        dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)
+        cals, epochs = [], []
+
+        def cb(_user, sol_p):
+            s = sol_p.contents
+            n = s.n_directions * s.n_antennas
+            cals.append(np.ctypeslib.as_array(s.calibrations, shape=(n,)).copy().reshape(s.n_directions,
+                                                                                          s.n_antennas)
+                        if n else np.zeros((s.n_directions, s.n_antennas)))
+            epochs.append((s.start_epoch, s.end_epoch))
+            return 0
+
+        cbf = SOLUTION_CB(cb)
+        o = opts if opts is not None else options()
+        if md is None:
+            rc = lib().leap_hip_calibrate(self.h, ctypes.byref(view), _dptr(dirs), len(dirs), *slice_, threshold,
+                                          -1 if ref_ant is None else ref_ant, 1 if use_cache else 0,
+                                          ctypes.byref(o), cbf, None)
+        else:
+            rc = lib().leap_hip_calibrate_md(self.h, md.h, ctypes.byref(view), _dptr(dirs), len(dirs), *slice_,
+                                             ctypes.byref(o), cbf, None)
+        self._chk(rc)
+        return np.array(cals), np.array(epochs)
+
+    def last_kernel_times(self):
+        a = ctypes.c_double()
+        n = ctypes.c_int32()
+        t = ctypes.c_double()
+        self._chk(lib().leap_hip_last_kernel_times(self.h, ctypes.byref(a), ctypes.byref(n), ctypes.byref(t)))
+        return a.value, n.value, t.value
+
+
+class Metadata:
+    def __init__(self, ctx, view, ref_ant, threshold, use_cache):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._chk(lib().leap_hip_metadata_create(ctx.h, ctypes.byref(view), -1 if ref_ant is None else ref_ant,
+                                                threshold, 1 if use_cache else 0, ctypes.byref(h)))
+        self.h = h
+        dims = np.zeros(5, dtype=np.int32)
+        lib().leap_hip_metadata_dims(h, dims.ctypes.data_as(_i32p))
+        self.K, self.N, self.K1, self.nbl, self.reference_antenna = (int(x) for x in dims)
+        self.num_pols = view.num_pols
+
+    def copy(self):
+        A = np.zeros((self.N, self.K))
+        Ad = np.zeros((self.K, self.N))
+        A1 = np.zeros((self.N, self.K1))
+        Ad1 = np.zeros((self.K1, self.N))
+        I = np.zeros(max(self.K - 1, 1), dtype=np.int32)
+        I1 = np.zeros(max(self.K1 - 1, 1), dtype=np.int32)
+        _check(lib().leap_hip_metadata_copy(self.h, _dptr(A), I.ctypes.data_as(_i32p), _dptr(Ad), _dptr(A1),
+                                            I1.ctypes.data_as(_i32p), _dptr(Ad1)))
+        return {"A": A.T.copy(), "I": I[: self.K - 1].copy(), "Ad": Ad.T.copy(), "A1": A1.T.copy(),
+                "I1": I1[: self.K1 - 1].copy(), "Ad1": Ad1.T.copy()}
+
+    def rotate(self, view, directions, row_start=0, n_rows=None, opts=None):
+        dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)
+        if n_rows is None:
+            n_rows = view.num_rows - row_start
+        out = np.zeros((len(dirs), self.nbl, self.num_pols), dtype=np.complex128)
+        o = opts if opts is not None else options()
+        self.ctx._chk(lib().leap_hip_rotate_visibilities(self.ctx.h, self.h, ctypes.byref(view), row_start, n_rows,
+                                                         _dptr(dirs), len(dirs), ctypes.byref(o),
+                                                         out.ctypes.data_as(_dp)))
+        return out
+
+    def apply_inversion(self, avg, opts=None):
+        avg = np.ascontiguousarray(avg, dtype=np.complex128).reshape(-1, self.nbl, self.num_pols)
+        cal = np.zeros((avg.shape[0], self.N))
+        o = opts if opts is not None else options()
+        self.ctx._chk(lib().leap_hip_apply_inversion(self.ctx.h, self.h, avg.ctypes.data_as(_dp), avg.shape[0],
+                                                     ctypes.byref(o), _dptr(cal)))
+        return cal
+
+    def __del__(self):
+        try:
+            lib().leap_hip_metadata_destroy(self.h)
+        except Exception:  # noqa: BLE001
+            pass

@@ -1,0 +1,267 @@
+/*
+ * leap_hip.h — C ABI of the MI355X (gfx950) LEAP calibrator.
+ *
+ * The drop-in boundary: plain pointers, sizes and status codes, no C++ or
+ * framework types.  The C++ class icrar::hip::HipLeapCalibrator (see
+ * icrar-leap-accelerate_amd/cpp/) implements the reference's ILeapCalibrator
+ * over these entry points, and the Python test harness binds them with ctypes.
+ *
+ * Reference interfaces replaced (paths relative to src/icrar/leap-accelerate/
+ * of ska-telescope/icrar-leap-accelerate v0.8.0):
+ *   leap_hip_calibrate              ILeapCalibrator::Calibrate
+ *                                   (algorithm/ILeapCalibrator.h:62-68), as done by
+ *                                   cuda::CudaLeapCalibrator::Calibrate
+ *                                   (algorithm/cuda/CudaLeapCalibrator.cu:94-233)
+ *   leap_hip_create/destroy         CudaLeapCalibrator ctor/dtor (CudaLeapCalibrator.cu:73-92)
+ *   leap_hip_metadata_create        cpu::MetaData::MetaData (model/cpu/MetaData.cc:42-137)
+ *                                   + cuda::DeviceMetaData upload (model/cuda/DeviceMetaData.cu)
+ *   leap_hip_rotate_visibilities    CudaLeapCalibrator::RotateVisibilities
+ *                                   (CudaLeapCalibrator.cu:355-400; CPU form
+ *                                   CpuLeapCalibrator.cc:181-217)
+ *   leap_hip_apply_inversion        the inversion half of CudaLeapCalibrator::PhaseRotate
+ *                                   (CudaLeapCalibrator.cu:248-261; CPU form
+ *                                   CpuLeapCalibrator.cc:159-178)
+ *   leap_hip_dd_matrix              cpu::MetaData::GenerateDDMatrix (MetaData.cc:166-203)
+ *   leap_ms_view                    the accessors of icrar::MeasurementSet used on the
+ *                                   path (ms/MeasurementSet.h:54-224, MetaData.cc:46-89,
+ *                                   Integration.cc:54-55)
+ *
+ * Errors: every entry point returns LEAP_OK (0) or a negative leap_status; the
+ * message is available from leap_hip_last_error().  No entry point exits the
+ * process (the reference's CUDA path calls exit() on CUDA errors,
+ * cuda/helper_cuda.cuh:613-624).  Argument errors the reference raises as
+ * icrar::invalid_argument_exception / icrar::exception carry the reference's
+ * message text.
+ *
+ * Threading: a leap_ctx owns one HIP stream on one device and its device
+ * workspace; use one context per host thread.  Calls are synchronous: on
+ * return all device work of the call has completed.  Device pointers passed in
+ * a leap_ms_view must be fully written (the caller synchronised) before the call.
+ */
+#ifndef LEAP_HIP_H
+#define LEAP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum leap_status {
+    LEAP_OK = 0,
+    LEAP_ERR_INVALID_ARGUMENT = -1, /* icrar::invalid_argument_exception */
+    LEAP_ERR_RANGE = -2,            /* icrar::exception (ranges, slices, ms layout) */
+    LEAP_ERR_HIP = -3,              /* a HIP runtime call failed */
+    LEAP_ERR_OUT_OF_MEMORY = -4,
+    LEAP_ERR_NOT_IMPLEMENTED = -5,
+    LEAP_ERR_CALLBACK = -6,         /* the output callback returned non-zero */
+    LEAP_ERR_INTERNAL = -7
+} leap_status;
+
+typedef enum leap_vis_dtype {
+    LEAP_VIS_C128 = 0, /* complex<double> */
+    LEAP_VIS_C64 = 1   /* complex<float>, the DATA column's own type */
+} leap_vis_dtype;
+
+/* In-memory measurement set: the accessor contract of icrar::MeasurementSet.
+ * antenna1/antenna2/time/flag/uvw are host memory.  vis is host memory, or
+ * device memory of the context's device when vis_on_device is 1.  uvw_device
+ * optionally gives a device copy of uvw (saves the upload).  The field order up
+ * to vis_on_device is shared with the CPU oracle's view. */
+typedef struct leap_ms_view {
+    uint32_t num_stations;          /* GetNumStations (MeasurementSet.cc:132-135) */
+    uint32_t total_antennas;        /* GetTotalAntennas: ANTENNA table rows (:110-113) */
+    uint32_t num_baselines;         /* GetNumBaselines (:154-167) */
+    uint32_t num_rows;              /* GetNumRows (:105-108) */
+    uint32_t num_channels;          /* SPECTRAL_WINDOW NUM_CHAN[0] (:169-179) */
+    uint32_t num_pols;              /* POLARIZATION NUM_CORR[0] (:137-147) */
+    double freq_start_hz;           /* SPECTRAL_WINDOW REF_FREQUENCY[0] (MetaData.cc:59) */
+    double freq_inc_hz;             /* SPECTRAL_WINDOW CHAN_WIDTH[0][0] (MetaData.cc:60) */
+    double phase_centre_ra_rad;     /* FIELD PHASE_DIR[0] (MetaData.cc:67-79) */
+    double phase_centre_dec_rad;
+    const int32_t* antenna1;        /* ANTENNA1 of rows [0, num_baselines) */
+    const int32_t* antenna2;        /* ANTENNA2 of rows [0, num_baselines) */
+    const double* time;             /* TIME of all rows (epochs, MeasurementSet.cc:120-130) */
+    const uint8_t* flag;            /* FLAG[row][0][0] for rows [0, num_baselines) (:181-197) */
+    const double* uvw;              /* UVW, num_rows x 3 */
+    const void* vis;                /* DATA, [num_rows][num_channels][num_pols] complex */
+    int32_t vis_dtype;              /* leap_vis_dtype */
+    int32_t vis_on_device;          /* 1: vis is a device pointer */
+    const double* uvw_device;       /* optional device copy of uvw, or NULL */
+} leap_ms_view;
+
+typedef enum leap_precision {
+    /* FP64 phase, FP64 sincos, FP64 sums: the reference's arithmetic. */
+    LEAP_PRECISION_FP64 = 0,
+    /* FP64 phase argument and range reduction; FP32 sincos and phasor
+     * recurrence; FP32 sums with compensated (Kahan) accumulation across
+     * channel blocks and timesteps; Ad*dphi as split-3 x BF16 MFMA with FP32
+     * accumulation.  Needs the batched strategy. */
+    LEAP_PRECISION_MIXED = 1
+} leap_precision;
+
+typedef enum leap_strategy {
+    /* One pass over the visibilities per direction (the reference's schedule). */
+    LEAP_STRATEGY_PER_DIRECTION = 0,
+    /* One pass over the visibilities for a block of directions; the
+     * pseudo-inverse products become one GEMM per block. */
+    LEAP_STRATEGY_BATCHED = 1
+} leap_strategy;
+
+typedef struct leap_options {
+    int32_t precision;        /* leap_precision */
+    int32_t strategy;         /* leap_strategy */
+    int32_t direction_block;  /* directions per pass for the batched strategy; 0 = tuned */
+    int32_t all_pols;         /* batched: also accumulate polarisations 1..P-1 of avgData */
+    int32_t reserved[4];
+} leap_options;
+
+/* One solution interval's result: cpu::Calibration (model/cpu/calibration/Calibration.h:42-111)
+ * holding one cpu::BeamCalibration per direction (BeamCalibration.h:45-124). */
+typedef struct leap_solution {
+    uint32_t index;               /* solution number */
+    double start_epoch;           /* epochs[s * interval] (CpuLeapCalibrator.cc:107-109) */
+    double end_epoch;             /* epochs[(s + 1) * interval - 1] */
+    uint32_t n_directions;
+    const double* directions;     /* [n_directions][2] (ra, dec) */
+    uint32_t n_antennas;          /* A.cols() = max(antenna1, antenna2) + 1 */
+    const double* calibrations;   /* [n_directions][n_antennas], valid during the callback */
+} leap_solution;
+
+/* Called once per solution interval, in order.  Return 0 to continue. */
+typedef int (*leap_solution_cb)(void* user, const leap_solution* solution);
+
+typedef struct leap_ctx leap_ctx;
+typedef struct leap_metadata leap_metadata;
+
+/* ---- context -------------------------------------------------------------- */
+int leap_hip_create(leap_ctx** out, int32_t device);
+void leap_hip_destroy(leap_ctx* ctx);
+/* Message of the last failure of ctx on this thread (ctx may be NULL). */
+const char* leap_hip_last_error(const leap_ctx* ctx);
+/* Library version string. */
+const char* leap_hip_version(void);
+
+/* ---- calibration (drop-in for ILeapCalibrator::Calibrate) ----------------- */
+/* slice_* follow icrar::Slice(start, interval, end) with -1 wildcards
+ * (common/Slice.cc:36-63); reference_antenna < 0 means none; use_cache keeps
+ * the pseudo-inverse of A in the context keyed by a hash of A (the reference
+ * caches it in files A.hash / Ad.cache, MetaData.cc:112-120).  opts may be NULL
+ * (FP64, per direction). */
+int leap_hip_calibrate(leap_ctx* ctx, const leap_ms_view* ms, const double* directions,
+                       uint32_t n_directions, int32_t slice_start, int32_t slice_interval,
+                       int32_t slice_end, double min_baseline_threshold,
+                       int32_t reference_antenna, int32_t use_cache, const leap_options* opts,
+                       leap_solution_cb callback, void* user);
+
+/* Same, with metadata built beforehand by leap_hip_metadata_create for this ms. */
+int leap_hip_calibrate_md(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
+                          const double* directions, uint32_t n_directions, int32_t slice_start,
+                          int32_t slice_interval, int32_t slice_end, const leap_options* opts,
+                          leap_solution_cb callback, void* user);
+
+/* ---- metadata (A, I, A1, I1 and their pseudo-inverses, on the device) ------ */
+int leap_hip_metadata_create(leap_ctx* ctx, const leap_ms_view* ms, int32_t reference_antenna,
+                             double min_baseline_threshold, int32_t use_cache,
+                             leap_metadata** out);
+void leap_hip_metadata_destroy(leap_metadata* md);
+/* dims = {K = A.rows, N = A.cols, K1 = A1.rows, nbaselines, referenceAntenna} */
+void leap_hip_metadata_dims(const leap_metadata* md, int32_t* dims);
+/* Host copies, column-major like Eigen; any pointer may be NULL.
+ * A: K x N, I: K-1, Ad: N x K, A1: K1 x N, I1: K1-1, Ad1: N x K1. */
+int leap_hip_metadata_copy(const leap_metadata* md, double* A, int32_t* I, double* Ad,
+                           double* A1, int32_t* I1, double* Ad1);
+
+/* ---- the two halves of PhaseRotate ---------------------------------------- */
+/* avgData for rows [row_start, row_start + n_rows) and n_directions
+ * directions; avg_out is host [n_directions][nbaselines][num_pols] complex128
+ * (interleaved re, im).  With opts->all_pols == 0 under the batched strategy
+ * only polarisation 0 is written (the others are left 0). */
+int leap_hip_rotate_visibilities(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
+                                 uint32_t row_start, uint32_t n_rows, const double* directions,
+                                 uint32_t n_directions, const leap_options* opts, double* avg_out);
+
+/* cal = Ad * dphi + Ad1 * phi1 from host avgData [n_directions][nbaselines][num_pols]
+ * complex128; cal_out is host [n_directions][N]. */
+int leap_hip_apply_inversion(leap_ctx* ctx, const leap_metadata* md, const double* avg,
+                             uint32_t n_directions, const leap_options* opts, double* cal_out);
+
+/* The direction-dependent rotation matrices, computed by the device kernel;
+ * dd_out is host [n_directions][3][3] row-major. */
+int leap_hip_dd_matrix(leap_ctx* ctx, double phase_centre_ra, double phase_centre_dec,
+                       const double* directions, uint32_t n_directions, double* dd_out);
+
+/* ---- host model (no GPU needed) ------------------------------------------- */
+/* cpu::PhaseMatrixFunction (algorithm/cpu/PhaseMatrixFunction.cc:35-93).
+ * ref_ant < 0 means none.  A_out holds (n + 1) x (max(a1, a2) + 1) doubles,
+ * column-major with leading dimension n + 1, of which the first *K rows are
+ * set; I_out holds n ints, of which *K - 1 are set. */
+int leap_phase_matrix(const int32_t* a1, const int32_t* a2, const uint8_t* fg, int32_t n,
+                      int32_t ref_ant, double* A_out, int32_t* I_out, int32_t* K, int32_t* N);
+/* cpu::PseudoInverse (math/cpu/matrix_invert.h:70-74) of the phase matrix that
+ * leap_phase_matrix builds from the same arguments, computed on the host the
+ * way the device metadata computes it; Ad_out is N x K column-major. */
+int leap_phase_matrix_pinv(const int32_t* a1, const int32_t* a2, const uint8_t* fg, int32_t n,
+                           int32_t ref_ant, double* Ad_out);
+
+/* ---- synthetic measurement sets ------------------------------------------- */
+typedef struct leap_synth_config {
+    uint32_t n_antennas;
+    int32_t layout;             /* 0: uniform disk, 1: three-arm log spiral */
+    double array_radius_m;      /* disk radius, or outer radius of the spiral */
+    int32_t autocorrelations;   /* include (a, a) baselines */
+    uint32_t n_timesteps;
+    double integration_s;
+    uint32_t n_channels;
+    double freq_start_hz;
+    double freq_inc_hz;
+    uint32_t n_pols;            /* 1..4 */
+    double phase_centre_ra_rad;
+    double phase_centre_dec_rad;
+    double latitude_rad;
+    uint32_t n_sources;
+    double source_flux_min, source_flux_max;   /* Jy, uniform */
+    double source_offset_max_rad;              /* |d ra|, |d dec| uniform in [-x, x] */
+    double noise_sigma;                        /* per real / imaginary part */
+    double flag_fraction;                      /* fraction of baselines flagged */
+    double nan_fraction;                       /* fraction of (row, chan, pol) set to NaN */
+    uint64_t seed;
+} leap_synth_config;
+
+typedef struct leap_synth_ms leap_synth_ms;
+
+/* Builds the host-side columns (antennas, uvw, time, flags) and the sky model;
+ * no visibilities yet.  Works without a GPU. */
+int leap_synth_create(const leap_synth_config* cfg, leap_synth_ms** out);
+void leap_synth_destroy(leap_synth_ms* ms);
+/* Fills view with the host columns; vis is NULL until a fill call. */
+void leap_synth_view(const leap_synth_ms* ms, leap_ms_view* view);
+/* Sky model for tests: n_sources x {ra, dec, flux}, and per-antenna phases. */
+void leap_synth_sky(const leap_synth_ms* ms, double* sources, double* antenna_phase);
+/* Host visibilities for rows [row_start, row_start + n_rows) into out
+ * ([n_rows][C][P] of vis_dtype).  Works without a GPU. */
+int leap_synth_fill_vis_host(const leap_synth_ms* ms, uint32_t row_start, uint32_t n_rows,
+                             int32_t vis_dtype, void* out);
+/* Device visibilities for all rows into the device buffer out (vis_dtype),
+ * generated by a kernel; also uploads uvw.  The view's vis / uvw_device then
+ * point at device memory.  out must hold num_rows * C * P complex values. */
+int leap_hip_synth_fill_vis_device(leap_ctx* ctx, const leap_synth_ms* ms, int32_t vis_dtype,
+                                   void* out_device, double* uvw_device_out);
+
+/* ---- device memory helpers (for harnesses without their own allocator) ---- */
+int leap_hip_malloc(leap_ctx* ctx, uint64_t bytes, void** out);
+int leap_hip_free(leap_ctx* ctx, void* ptr);
+int leap_hip_memcpy_d2h(leap_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int leap_hip_memcpy_h2d(leap_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- timing of the last calibrate / rotate call --------------------------- */
+/* Average device time per launch (ms) of the rotate/average kernel and number of
+ * launches in the last call, measured with HIP events on the context's stream. */
+int leap_hip_last_kernel_times(const leap_ctx* ctx, double* rotate_ms_avg, int32_t* rotate_launches,
+                               double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LEAP_HIP_H */
