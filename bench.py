@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 LEAP calibrator on the 256-station workload.
+
+Workload (synthetic, generated on the device): 256 stations in a 40 km
+three-arm log spiral, 32,896 baselines including autocorrelations, 128
+timesteps of 1 s, 384 channels from 50 MHz at 781.25 kHz, 4 polarisations,
+2% of baselines flagged, five point sources under per-antenna phase screens
+plus noise.  One solution interval covers all 128 timesteps.
+
+A step is one calibration of one direction (per-direction FP64 path over the
+complex128 cube, 103.5 GB resident in HBM before timing starts).  value is
+directions calibrated per second.  The JSON line also carries the roofline
+of the dominant kernel (rotate/average, HBM-bound) from HIP events, and a CPU
+baseline: the oracle on a bounded sample of the same workload (host cores).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "icrar-leap-accelerate_amd"))
+
+import leap_amd as la  # noqa: E402
+
+METRIC = "directions calibrated/s (256 st,128 t,384 ch) batched-mixed vs plain FP64; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def c1_config():
+    return dict(n_antennas=256, layout=1, array_radius_m=20000.0, autocorrelations=1, n_timesteps=128,
+                integration_s=1.0, n_channels=384, freq_start_hz=50e6, freq_inc_hz=781.25e3, n_pols=4,
+                flag_fraction=0.02, seed=1)
+
+
+def directions(centre, n, seed=7, spread=0.02):
+    rng = np.random.default_rng(seed)
+    return np.column_stack([centre[0] + rng.uniform(-spread, spread, n), centre[1] + rng.uniform(-spread, spread, n)])
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the rotate kernel measured with rocprofv3 PMC counters, if committed."""
+    p = os.path.join(REPO, "profiles", "rotate_average_f64_c1_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(ctx, synth, vis_ptr, timesteps):
+    """The oracle (single thread) on the first `timesteps` timesteps of the same cube, scaled in T."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_bindings import Oracle  # the checker, used only for this reported baseline
+
+    nbl = synth.num_baselines
+    rows = nbl * timesteps
+    host = np.zeros((rows, synth.num_channels, synth.num_pols), dtype=np.complex128)
+    ctx.d2h(host, vis_ptr)
+    ms = dict(synth.host_dict(host))
+    ms["num_rows"] = rows
+    ms["uvw"] = synth.uvw[:rows]
+    ms["time"] = synth.time[:rows]
+    o = Oracle()
+    t = time.perf_counter()
+    md = o.metadata(ms)
+    t_md = time.perf_counter() - t
+    d = directions(ms["phase_centre"], 1, seed=99)[0]
+    t = time.perf_counter()
+    avg = md.rotate(d)
+    t_rot = time.perf_counter() - t
+    t = time.perf_counter()
+    md.apply_inversion(avg)
+    t_inv = time.perf_counter() - t
+    T = synth.cfg.n_timesteps
+    per_dir = t_rot * (T / timesteps) + t_inv
+    return {
+        "value": 1.0 / per_dir,
+        "unit": "directions/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (FP64 C++ restatement, 1 thread, {platform.processor() or platform.machine()}) on "
+                   f"{timesteps} of {T} timesteps of the same cube, 1 direction; RotateVisibilities time "
+                   f"{t_rot:.2f} s scaled x{T // timesteps} in T plus inversion {t_inv * 1e3:.1f} ms; "
+                   f"one-off metadata (SVD pseudo-inverse) {t_md:.1f} s not counted"),
+    }
+
+
+def copy_probe(gb):
+    """hipMemcpy device-to-device of a known size, 3 times; prints the byte count."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(gb * 1e9) // 4096 * 4096
+    ctx = la.Context(0)
+    a = ctx.malloc(n)
+    b = ctx.malloc(n)
+    for _ in range(3):
+        rc = hip.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), ctypes.c_size_t(n), 3)  # device to device
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+    hip.hipDeviceSynchronize()
+    ctx.free(a)
+    ctx.free(b)
+    print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
+    return 0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-timesteps", type=int, default=2, help="timesteps in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--copy-probe-gb", type=float, default=0.0,
+                    help="only run device-to-device copies of this size (calibrates rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE against a copy of known bytes) and exit")
+    args = ap.parse_args()
+    if args.copy_probe_gb > 0:
+        return copy_probe(args.copy_probe_gb)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the per-GPU work is the same for every GPU count: directions or rows shard without "
+              "changing a GPU's kernel).", file=sys.stderr)
+        return 2
+
+    cfg = c1_config()
+    log("building synthetic measurement set (host columns)")
+    synth = la.SynthMS(**cfg)
+    R, C, P, nbl = synth.num_rows, synth.num_channels, synth.num_pols, synth.num_baselines
+    ctx = la.Context(0)
+    vis_bytes = R * C * P * 16
+    log(f"allocating {vis_bytes / 1e9:.1f} GB complex128 cube on the device and generating it")
+    vis_ptr = ctx.malloc(vis_bytes)
+    uvw_ptr = ctx.malloc(R * 3 * 8)
+    t = time.perf_counter()
+    ctx.synth_fill_device(synth, la.LEAP_VIS_C128, vis_ptr, uvw_ptr)
+    log(f"cube generated in {time.perf_counter() - t:.2f} s")
+    view = la.view_from_arrays(synth.host_dict(np.zeros((0, C, P), np.complex128)), vis_device=vis_ptr,
+                               vis_dtype=la.LEAP_VIS_C128, uvw_device=uvw_ptr)
+    view.num_rows = R
+    t = time.perf_counter()
+    md = ctx.metadata(view)
+    t_md = time.perf_counter() - t
+    log(f"metadata (A {md.K}x{md.N}, A1 {md.K1}x{md.N}, pseudo-inverses) in {t_md:.2f} s")
+
+    dirs = directions((view.phase_centre_ra_rad, view.phase_centre_dec_rad), args.warmup + args.steps)
+    for i in range(args.warmup):
+        ctx.calibrate(view, dirs[i:i + 1], md=md)
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    rot_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.calibrate(view, dirs[args.warmup + i:args.warmup + i + 1], md=md)
+        rot_ms.append(ctx.last_kernel_times()[0])
+    elapsed = time.perf_counter() - t0
+    log(f"{args.steps} timed steps in {elapsed:.3f} s")
+    value = args.steps / elapsed
+    k_ms = float(np.mean(rot_ms))
+    alg_bytes = R * C * P * 16 + R * 24 + nbl * P * 16 + C * 8
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile()
+
+    cpu = None
+    if args.cpu_timesteps > 0:
+        log(f"CPU baseline: oracle on {args.cpu_timesteps} timesteps")
+        cpu = cpu_baseline(ctx, synth, vis_ptr, args.cpu_timesteps)
+        log("CPU baseline done")
+    ctx.free(vis_ptr)
+    ctx.free(uvw_ptr)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "directions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded point sources x per-antenna phase screens + noise, generated on the device)",
+        "config": {
+            "workload": "plain FP64 per-direction calibration, 256 stations / 32896 baselines (autos) / 128 t / "
+                        "384 ch / 4 pol, complex128 cube 103.5 GB resident in HBM, 1 direction per step, "
+                        "1 solution interval",
+            "stations": 256, "baselines": nbl, "timesteps": 128, "channels": C, "pols": P,
+            "rows": R, "vis_dtype": "complex128", "directions_per_step": 1, "strategy": "per-direction",
+            "precision": "fp64", "metadata_s": round(t_md, 3),
+        },
+        "roofline": {
+            "kernel": "rotate_average_f64",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel_ms": round(k_ms, 4),
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
