@@ -57,14 +57,15 @@ def traffic_from_profile():
         return None
 
 
-def cpu_baseline(ctx, synth, vis_ptr, timesteps):
+def cpu_baseline(ctx, synth, vis_ptr, vis_dtype, timesteps):
     """The oracle (single thread) on the first `timesteps` timesteps of the same cube, scaled in T."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from oracle_bindings import Oracle  # the checker, used only for this reported baseline
 
     nbl = synth.num_baselines
     rows = nbl * timesteps
-    host = np.zeros((rows, synth.num_channels, synth.num_pols), dtype=np.complex128)
+    host = np.zeros((rows, synth.num_channels, synth.num_pols),
+                    dtype=np.complex64 if vis_dtype == la.LEAP_VIS_C64 else np.complex128)
     ctx.d2h(host, vis_ptr)
     ms = dict(synth.host_dict(host))
     ms["num_rows"] = rows
@@ -120,6 +121,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="plain", choices=["plain", "batched-fp64", "batched-mixed"],
+                    help="plain: FP64 per direction on the complex128 cube; batched-fp64: FP64, one pass per "
+                         "block of directions on the complex128 cube; batched-mixed: mixed precision on the "
+                         "complex64 cube")
+    ap.add_argument("--directions", type=int, default=0, help="directions per step (default 1 plain, 64 "
+                    "batched-fp64, 256 batched-mixed)")
+    ap.add_argument("--direction-block", type=int, default=0, help="0 = autotuned")
+    ap.add_argument("--lane-directions", type=int, default=0, help="directions per lane (0 = autotuned)")
+    ap.add_argument("--fp32-blocks", action="store_true",
+                    help="batched-mixed only: FP32 Horner blocks re-anchored every 32 channels instead of the "
+                         "full-row FP64 Horner")
     ap.add_argument("--cpu-timesteps", type=int, default=2, help="timesteps in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--copy-probe-gb", type=float, default=0.0,
                     help="only run device-to-device copies of this size (calibrates rocprofv3 FETCH_SIZE / "
@@ -133,51 +145,68 @@ def main():
               "changing a GPU's kernel).", file=sys.stderr)
         return 2
 
+    mode = args.mode
+    D = args.directions or {"plain": 1, "batched-fp64": 64, "batched-mixed": 256}[mode]
+    vis_dtype = la.LEAP_VIS_C64 if mode == "batched-mixed" else la.LEAP_VIS_C128
+    elem = 8 if vis_dtype == la.LEAP_VIS_C64 else 16
+    opts = la.options(precision=la.LEAP_PRECISION_MIXED if mode == "batched-mixed" else la.LEAP_PRECISION_FP64,
+                      strategy=la.LEAP_STRATEGY_PER_DIRECTION if mode == "plain" else la.LEAP_STRATEGY_BATCHED,
+                      direction_block=args.direction_block)
+    opts.fp32_blocks = 1 if args.fp32_blocks else 0
+    opts.lane_directions = args.lane_directions
+
     cfg = c1_config()
     log("building synthetic measurement set (host columns)")
     synth = la.SynthMS(**cfg)
     R, C, P, nbl = synth.num_rows, synth.num_channels, synth.num_pols, synth.num_baselines
     ctx = la.Context(0)
-    vis_bytes = R * C * P * 16
-    log(f"allocating {vis_bytes / 1e9:.1f} GB complex128 cube on the device and generating it")
+    vis_bytes = R * C * P * elem
+    log(f"allocating {vis_bytes / 1e9:.1f} GB {'complex64' if elem == 8 else 'complex128'} cube on the device "
+        "and generating it")
     vis_ptr = ctx.malloc(vis_bytes)
     uvw_ptr = ctx.malloc(R * 3 * 8)
     t = time.perf_counter()
-    ctx.synth_fill_device(synth, la.LEAP_VIS_C128, vis_ptr, uvw_ptr)
+    ctx.synth_fill_device(synth, vis_dtype, vis_ptr, uvw_ptr)
     log(f"cube generated in {time.perf_counter() - t:.2f} s")
-    view = la.view_from_arrays(synth.host_dict(np.zeros((0, C, P), np.complex128)), vis_device=vis_ptr,
-                               vis_dtype=la.LEAP_VIS_C128, uvw_device=uvw_ptr)
-    view.num_rows = R
+    view = la.view_from_arrays(synth.host_dict(), vis_device=vis_ptr, vis_dtype=vis_dtype, uvw_device=uvw_ptr)
     t = time.perf_counter()
     md = ctx.metadata(view)
     t_md = time.perf_counter() - t
     log(f"metadata (A {md.K}x{md.N}, A1 {md.K1}x{md.N}, pseudo-inverses) in {t_md:.2f} s")
 
-    dirs = directions((view.phase_centre_ra_rad, view.phase_centre_dec_rad), args.warmup + args.steps)
+    dirs = directions((view.phase_centre_ra_rad, view.phase_centre_dec_rad), (args.warmup + args.steps) * D)
     for i in range(args.warmup):
-        ctx.calibrate(view, dirs[i:i + 1], md=md)
+        ctx.calibrate(view, dirs[i * D:(i + 1) * D], md=md, opts=opts)
         log(f"warmup step {i + 1}/{args.warmup} done")
     rot_ms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        ctx.calibrate(view, dirs[args.warmup + i:args.warmup + i + 1], md=md)
+        j = args.warmup + i
+        ctx.calibrate(view, dirs[j * D:(j + 1) * D], md=md, opts=opts)
         rot_ms.append(ctx.last_kernel_times()[0])
     elapsed = time.perf_counter() - t0
-    log(f"{args.steps} timed steps in {elapsed:.3f} s")
-    value = args.steps / elapsed
+    log(f"{args.steps} timed steps of {D} directions in {elapsed:.3f} s")
+    value = args.steps * D / elapsed
     k_ms = float(np.mean(rot_ms))
-    alg_bytes = R * C * P * 16 + R * 24 + nbl * P * 16 + C * 8
+    alg_bytes = R * C * P * elem + R * 24 + D * nbl * P * 16 + C * 8
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile()
+    traffic = traffic_from_profile() if mode == "plain" else None
 
     cpu = None
     if args.cpu_timesteps > 0:
         log(f"CPU baseline: oracle on {args.cpu_timesteps} timesteps")
-        cpu = cpu_baseline(ctx, synth, vis_ptr, args.cpu_timesteps)
+        cpu = cpu_baseline(ctx, synth, vis_ptr, vis_dtype, args.cpu_timesteps)
         log("CPU baseline done")
     ctx.free(vis_ptr)
     ctx.free(uvw_ptr)
 
+    workload = {
+        "plain": "plain FP64 per-direction calibration, complex128 cube 103.5 GB resident in HBM, 1 direction per step",
+        "batched-fp64": f"batched FP64 calibration, complex128 cube 103.5 GB resident in HBM, {D} directions per "
+                        "step in one pass over the cube",
+        "batched-mixed": f"batched mixed-precision calibration, complex64 cube 51.7 GB resident in HBM, {D} "
+                         "directions per step in one pass over the cube",
+    }[mode]
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -189,18 +218,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if mode != "batched-mixed" else "f32+f64",
         "data": "synthetic (seeded point sources x per-antenna phase screens + noise, generated on the device)",
         "config": {
-            "workload": "plain FP64 per-direction calibration, 256 stations / 32896 baselines (autos) / 128 t / "
-                        "384 ch / 4 pol, complex128 cube 103.5 GB resident in HBM, 1 direction per step, "
+            "workload": workload + "; 256 stations / 32896 baselines (autos) / 128 t / 384 ch / 4 pol, "
                         "1 solution interval",
-            "stations": 256, "baselines": nbl, "timesteps": 128, "channels": C, "pols": P,
-            "rows": R, "vis_dtype": "complex128", "directions_per_step": 1, "strategy": "per-direction",
-            "precision": "fp64", "metadata_s": round(t_md, 3),
+            "stations": 256, "baselines": nbl, "timesteps": 128, "channels": C, "pols": P, "rows": R,
+            "vis_dtype": "complex64" if elem == 8 else "complex128", "directions_per_step": D, "mode": mode,
+            "tile": ctx.last_direction_block(), "metadata_s": round(t_md, 3),
         },
         "roofline": {
-            "kernel": "rotate_average_f64",
+            "kernel": "rotate_average_f64" if mode == "plain" else "rotate_average_batched",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

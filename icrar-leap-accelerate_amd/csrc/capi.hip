@@ -60,8 +60,15 @@ leap_options resolve(const leap_options* o) {
             LEAP_ERR_INVALID_ARGUMENT, "unknown strategy");
     require(!(r.precision == LEAP_PRECISION_MIXED && r.strategy != LEAP_STRATEGY_BATCHED),
             LEAP_ERR_INVALID_ARGUMENT, "mixed precision needs the batched strategy");
-    require(r.strategy == LEAP_STRATEGY_PER_DIRECTION, LEAP_ERR_NOT_IMPLEMENTED,
-            "the batched strategy is not built into this library");
+    require(r.direction_block == 0 || r.direction_block == 32 || r.direction_block == 64 ||
+                r.direction_block == 128 || r.direction_block == 256,
+            LEAP_ERR_INVALID_ARGUMENT, "direction block must be 0 (tuned), 32, 64, 128 or 256");
+    require(r.lane_directions == 0 || r.lane_directions == 1 || r.lane_directions == 2 || r.lane_directions == 4,
+            LEAP_ERR_INVALID_ARGUMENT, "lane directions must be 0 (tuned), 1, 2 or 4");
+    require(r.lane_directions <= 1 || !(r.precision == LEAP_PRECISION_MIXED && r.fp32_blocks), LEAP_ERR_INVALID_ARGUMENT,
+            "the FP32 blocked kernel runs one direction per lane");
+    require(r.lane_directions == 0 || r.direction_block == 0 || r.direction_block >= 32 * r.lane_directions,
+            LEAP_ERR_INVALID_ARGUMENT, "direction block too small for the lane directions");
     return r;
 }
 
@@ -150,11 +157,74 @@ void upload_directions(leap_ctx* ctx, const leap_metadata& md, const double* dir
               ctx->d_dd.as<double>());
 }
 
+// Direction block for the batched kernel: the fastest of the candidates on a
+// slice of the baselines, timed once per shape and kept in the context.
+int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
+                              uint32_t T, uint32_t D, const leap_options& opts) {
+    const bool fp32 = opts.precision == LEAP_PRECISION_MIXED && opts.fp32_blocks;
+    if (opts.direction_block != 0 && (opts.lane_directions != 0 || fp32))
+        return opts.direction_block * 10 + (fp32 ? 1 : opts.lane_directions);
+    if (opts.direction_block != 0 && opts.lane_directions == 0) return opts.direction_block * 10 + 1;
+    const TuneKey key{opts.precision * 4 + (opts.fp32_blocks ? 2 : 0) + (opts.lane_directions ? 1 : 0) +
+                          16 * opts.lane_directions,
+                      ms.vis_dtype, opts.all_pols, D, T, md.channels, md.pols};
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) return it->second;
+    // Candidate tiles (directions per workgroup * 10 + directions per lane).
+    std::vector<int32_t> candidates;
+    const int32_t all64[] = {321, 641, 642, 1281, 1282, 1284, 2561, 2562, 2564};
+    const int32_t all32[] = {321, 641, 1281, 2561};
+    if (fp32) {
+        candidates.assign(std::begin(all32), std::end(all32));
+    } else {
+        for (int32_t t : all64)
+            if (opts.lane_directions == 0 || t % 10 == opts.lane_directions) candidates.push_back(t);
+    }
+    const uint32_t probe_bl = std::min<uint32_t>(md.nbl, 2048);
+    int32_t best = candidates.front();
+    float best_ms = 0.f;
+    hipEvent_t a, b;
+    LEAP_HIP(hipEventCreate(&a));
+    LEAP_HIP(hipEventCreate(&b));
+    for (int32_t tile : candidates) {
+        const uint32_t db = uint32_t(tile / 10);
+        if (db > 32 && db >= 2 * D) continue;  // mostly idle lanes
+        float best_of = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // the first repetition warms the code object
+            LEAP_HIP(hipEventRecord(a, ctx->stream));
+            launch_rotate_batched(ctx->stream, opts.precision, tile, opts.all_pols != 0, rows.vis, ms.vis_dtype,
+                                  rows.uvw, ctx->d_dd.as<double>(), D, T, md.nbl, probe_bl, md.channels, md.pols,
+                                  md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), opts.fp32_blocks);
+            LEAP_HIP(hipEventRecord(b, ctx->stream));
+            LEAP_HIP(hipEventSynchronize(b));
+            LEAP_HIP(hipEventElapsedTime(&best_of, a, b));
+        }
+        if (best_ms == 0.f || best_of < best_ms) {
+            best_ms = best_of;
+            best = tile;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    ctx->tuned.emplace(key, best);
+    return best;
+}
+
 // avgData of D directions for T timesteps starting at the staged rows, into ctx->d_avg.
 void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
-                       uint32_t T, uint32_t D) {
+                       uint32_t T, uint32_t D, const leap_options& opts) {
     const size_t per_dir = size_t(md.nbl) * md.pols * 2;
     ctx->d_avg.reserve(per_dir * D * sizeof(double));
+    if (opts.strategy == LEAP_STRATEGY_BATCHED) {
+        const int32_t db = tuned_direction_block(ctx, md, ms, rows, T, D, opts);
+        ctx->last_direction_block = db;
+        record_rotate_begin(ctx);
+        launch_rotate_batched(ctx->stream, opts.precision, db, opts.all_pols != 0, rows.vis, ms.vis_dtype, rows.uvw,
+                              ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
+                              md.freq_inc_hz, ctx->d_avg.as<double>(), opts.fp32_blocks);
+        record_rotate_end(ctx);
+        return;
+    }
     for (uint32_t d = 0; d < D; ++d) {
         record_rotate_begin(ctx);
         launch_rotate_f64(ctx->stream, rows.vis, ms.vis_dtype, rows.uvw, ctx->d_dd.as<double>() + 9 * d, 0, T,
@@ -248,7 +318,6 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
 void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const double* dirs, uint32_t D,
                     int32_t s_start, int32_t s_interval, int32_t s_end, const leap_options& opts,
                     leap_solution_cb cb, void* user) {
-    (void)opts;
     validate_view(ms);
     check_metadata_matches(md, ms);
     require(dirs != nullptr || D == 0, LEAP_ERR_INVALID_ARGUMENT, "null directions");
@@ -276,7 +345,7 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
         sol.calibrations = ctx->h_cal.data();
         if (D > 0) {
             const DeviceRows rows = stage_rows(ctx, ms, row_begin, row_end);
-            rotate_directions(ctx, md, ms, rows, interval, D);
+            rotate_directions(ctx, md, ms, rows, interval, D, opts);
             invert_directions(ctx, md, D);
             LEAP_HIP(hipMemcpyAsync(ctx->h_cal.data(), ctx->d_cal.ptr, size_t(N) * D * sizeof(double),
                                     hipMemcpyDeviceToHost, ctx->stream));
@@ -390,7 +459,7 @@ int leap_hip_rotate_visibilities(leap_ctx* ctx, const leap_metadata* md, const l
                                  double* avg_out) {
     return guarded(ctx, [&] {
         require(ctx && md && ms && avg_out && (dirs || D == 0), LEAP_ERR_INVALID_ARGUMENT, "null argument");
-        resolve(opts);
+        const leap_options o = resolve(opts);
         validate_view(*ms);
         check_metadata_matches(*md, *ms);
         require(n_rows % ms->num_baselines == 0, LEAP_ERR_RANGE,
@@ -399,7 +468,7 @@ int leap_hip_rotate_visibilities(leap_ctx* ctx, const leap_metadata* md, const l
         begin_timing(ctx);
         const DeviceRows rows = stage_rows(ctx, *ms, row_start, row_start + n_rows);
         upload_directions(ctx, *md, dirs, D);
-        rotate_directions(ctx, *md, *ms, rows, n_rows / ms->num_baselines, D);
+        rotate_directions(ctx, *md, *ms, rows, n_rows / ms->num_baselines, D, o);
         const size_t n = size_t(md->nbl) * md->pols * 2 * D;
         if (n) LEAP_HIP(hipMemcpyAsync(avg_out, ctx->d_avg.ptr, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         LEAP_HIP(hipStreamSynchronize(ctx->stream));
@@ -583,3 +652,5 @@ int leap_phase_matrix_pinv(const int32_t* a1, const int32_t* a2, const uint8_t* 
 }
 
 }  // extern "C"
+
+extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ? ctx->last_direction_block : 0; }

@@ -1,0 +1,510 @@
+// Batched rotate/average for gfx950: one pass over the visibilities serves a
+// block of directions.
+//
+// For one row (t, baseline) and direction d the reference sums, per pol p,
+//   sum_c vis[c][p] * exp(i * sf_d * kappa_c),   kappa_c = (f0 + c df) / c_light,
+// skipping channels whose rotated pols hold a NaN (CpuLeapCalibrator.cc:181-217).
+// kappa_c is linear in c, so over a block of B channels starting at b
+//   sum_k v[b + k] exp(i sf (kappa_b + k dkappa)) = exp(i sf kappa_b) * P(z),
+//   P(z) = sum_k v[b + k] z^k,  z = exp(i sf dkappa),
+// and P(z) is evaluated by Horner's rule: four fused multiply-adds per channel
+// and direction, with the exact phasor recomputed at every block start.
+//
+// Precision variants:
+//   FP64:  phases, sincos, Horner and sums in FP64.
+//   MIXED: the phase argument sf * kappa and its reduction modulo 2 pi in FP64;
+//          sincos, Horner and the sums in FP32, the block sums accumulated over
+//          blocks and timesteps with Kahan compensation.
+//
+// Work decomposition: a workgroup owns one baseline and a block of DB
+// directions (lane = direction); 256 / DB timesteps run side by side.  Each
+// channel chunk of those rows is staged once into LDS (all pols read, NaN/Inf
+// mask applied, the accumulated pols kept) and broadcast to every direction
+// lane.  No atomics: the per-timestep partials are combined in LDS in a fixed
+// order, so results are reproducible.
+//
+// Mask rule: a (row, channel) whose polarisations hold any non-finite value is
+// skipped for every direction.  The reference tests the rotated values for NaN;
+// for finite and NaN inputs the two rules agree, and for an infinite input the
+// reference's result depends on the direction (inf * 0 = NaN), where here the
+// channel is always skipped.
+#include "leap_internal.h"
+
+namespace leap {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kBlockCh = 32;  // Horner block length (channels)
+
+template <typename T>
+struct C2 {
+    T x, y;
+};
+
+template <typename T>
+__device__ inline C2<T> cmul(C2<T> a, C2<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// a * z + v with fused multiply-adds.
+template <typename T>
+__device__ inline C2<T> horner_step(C2<T> a, C2<T> z, C2<T> v) {
+    C2<T> r;
+    r.x = fma(a.x, z.x, fma(-a.y, z.y, v.x));
+    r.y = fma(a.x, z.y, fma(a.y, z.x, v.y));
+    return r;
+}
+
+constexpr double kTwoPiHi = 6.283185307179586232;           // double(2 pi)
+constexpr double kTwoPiLo = 2.4492935982947064e-16;         // 2 pi - double(2 pi)
+constexpr double kInvTwoPi = 0.15915494309189533577;
+
+// exp(i theta) with the argument reduced modulo 2 pi in FP64 and the sincos in FP32.
+__device__ inline C2<float> phasor_mixed(double theta) {
+    const double n = rint(theta * kInvTwoPi);
+    double r = fma(-n, kTwoPiHi, theta);
+    r = fma(-n, kTwoPiLo, r);
+    float s, c;
+    sincosf(float(r), &s, &c);
+    return {c, s};
+}
+
+__device__ inline C2<double> phasor_f64(double theta) {
+    double s, c;
+    sincos(theta, &s, &c);
+    return {c, s};
+}
+
+template <typename A>
+__device__ inline C2<A> phasor(double theta);
+template <>
+__device__ inline C2<float> phasor<float>(double theta) { return phasor_mixed(theta); }
+template <>
+__device__ inline C2<double> phasor<double>(double theta) { return phasor_f64(theta); }
+
+// Running sum: plain for FP64, Kahan-compensated for FP32.
+template <typename A>
+struct Accum {
+    C2<A> s{0, 0};
+    C2<A> comp{0, 0};
+    __device__ inline void add(C2<A> v) {
+        if constexpr (sizeof(A) == 8) {
+            s.x += v.x;
+            s.y += v.y;
+        } else {
+            const A yx = v.x - comp.x;
+            const A tx = s.x + yx;
+            comp.x = (tx - s.x) - yx;
+            s.x = tx;
+            const A yy = v.y - comp.y;
+            const A ty = s.y + yy;
+            comp.y = (ty - s.y) - yy;
+            s.y = ty;
+        }
+    }
+    __device__ inline double re() const { return double(s.x) - double(comp.x); }
+    __device__ inline double im() const { return double(s.y) - double(comp.y); }
+};
+
+// VT: stored component type (double: complex128, float: complex64).
+// A: arithmetic type.  DB: directions per workgroup.  NP: pols accumulated
+// (1 = pol 0 only, else all P <= 4).  CC: channels per LDS chunk.
+template <typename VT, typename A, int DB, int NP, int CC>
+__global__ __launch_bounds__(kThreads) void rotate_average_batched(
+    const VT* __restrict__ vis, const double* __restrict__ uvw, const double* __restrict__ dd,
+    uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa,
+    double* __restrict__ avg) {
+    constexpr int RP = kThreads / DB;  // timesteps in flight
+    __shared__ C2<A> s_v[RP][NP][CC];
+    __shared__ double s_red[(RP > 1) ? RP : 1][DB][NP][2];
+
+    const uint32_t bl = blockIdx.x;
+    const uint32_t lane_d = threadIdx.x % DB;
+    const uint32_t rsub = threadIdx.x / DB;
+    const uint32_t d = blockIdx.y * DB + lane_d;
+    const bool dir_ok = d < D;
+    const uint32_t dc = dir_ok ? d : 0;
+    // (dd * uvw).z - w = dd20 u + dd21 v + dd22 w - w (CpuLeapCalibrator.cc:190)
+    const double dz0 = dd[9 * dc + 6], dz1 = dd[9 * dc + 7], dz2 = dd[9 * dc + 8];
+
+    Accum<A> acc[NP];
+
+    for (uint32_t tb = 0; tb < T; tb += RP) {
+        const uint32_t t = tb + rsub;
+        const bool row_ok = t < T;
+        double sf = 0.0;
+        if (row_ok) {
+            const size_t row = size_t(t) * nbl + bl;
+            const double u = uvw[3 * row], v = uvw[3 * row + 1], w = uvw[3 * row + 2];
+            sf = -kTwoPi * ((dz0 * u + dz1 * v + dz2 * w) - w);
+        }
+        const C2<A> z = phasor<A>(sf * dkappa);
+        for (uint32_t c0 = 0; c0 < C; c0 += CC) {
+            const uint32_t cn = min(uint32_t(CC), C - c0);
+            __syncthreads();
+            // Stage rows [tb, tb + RP) x channels [c0, c0 + cn) into LDS.
+            for (uint32_t i = threadIdx.x; i < RP * uint32_t(CC); i += kThreads) {
+                const uint32_t r = i / CC, c = i - r * CC;
+                const uint32_t tt = tb + r;
+                C2<A> val[NP];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) val[q] = {0, 0};
+                if (tt < T && c < cn) {
+                    const size_t row = size_t(tt) * nbl + bl;
+                    const VT* p = vis + ((row * C + c0 + c) * P) * 2;
+                    bool finite = true;
+                    VT x[8];
+                    for (uint32_t q = 0; q < P; ++q) {
+                        x[2 * q] = p[2 * q];
+                        x[2 * q + 1] = p[2 * q + 1];
+                        finite = finite && __builtin_isfinite(x[2 * q]) && __builtin_isfinite(x[2 * q + 1]);
+                    }
+                    if (finite) {
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) val[q] = {A(x[2 * q]), A(x[2 * q + 1])};
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NP; ++q) s_v[r][q][c] = val[q];
+            }
+            __syncthreads();
+            if (!row_ok || !dir_ok) continue;
+            for (uint32_t b = 0; b < cn; b += kBlockCh) {
+                const uint32_t len = min(uint32_t(kBlockCh), cn - b);
+                const C2<A> base = phasor<A>(sf * (kappa0 + double(c0 + b) * dkappa));
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    C2<A> h = s_v[rsub][q][b + len - 1];
+                    if (len == uint32_t(kBlockCh)) {
+#pragma unroll
+                        for (int k = kBlockCh - 2; k >= 0; --k) h = horner_step(h, z, s_v[rsub][q][b + k]);
+                    } else {
+                        for (int k = int(len) - 2; k >= 0; --k) h = horner_step(h, z, s_v[rsub][q][b + k]);
+                    }
+                    acc[q].add(cmul(base, h));
+                }
+            }
+        }
+    }
+    // Combine the RP timestep lanes of each direction in a fixed order.
+    if constexpr (RP > 1) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            s_red[rsub][lane_d][q][0] = acc[q].re();
+            s_red[rsub][lane_d][q][1] = acc[q].im();
+        }
+        __syncthreads();
+        if (rsub == 0 && dir_ok) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                double re = 0.0, im = 0.0;
+                for (int r = 0; r < RP; ++r) {
+                    re += s_red[r][lane_d][q][0];
+                    im += s_red[r][lane_d][q][1];
+                }
+                double* o = avg + (size_t(d) * nbl + bl) * P * 2 + 2 * q;
+                o[0] = re;
+                o[1] = im;
+            }
+            if constexpr (NP == 1) {
+                double* o = avg + (size_t(d) * nbl + bl) * P * 2;
+                for (uint32_t q = 2; q < 2 * P; ++q) o[q] = 0.0;  // pols not accumulated
+            }
+        }
+    } else {
+        if (dir_ok) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                double* o = avg + (size_t(d) * nbl + bl) * P * 2 + 2 * q;
+                o[0] = acc[q].re();
+                o[1] = acc[q].im();
+            }
+            if constexpr (NP == 1) {
+                double* o = avg + (size_t(d) * nbl + bl) * P * 2;
+                for (uint32_t q = 2; q < 2 * P; ++q) o[q] = 0.0;  // pols not accumulated
+            }
+        }
+    }
+}
+
+// Full-row FP64 Horner: per (row, direction) one exact phasor exp(i sf kappa0)
+// and the step z = exp(i sf dkappa) are computed with FP64 sincos, and the
+// whole channel axis is evaluated as P(z) = P_even(z^2) + z P_odd(z^2) by two
+// interleaved Horner chains (independent, so the FP64 FMA pipeline stays
+// busy).  Horner's error grows as n * eps (Higham, Accuracy and Stability of
+// Numerical Algorithms, 5.1), about 1e-13 relative for n = 384 in FP64, so no
+// re-anchoring of the phasor is needed.  On gfx950 FP64 FMA issues at the
+// same rate as non-packed FP32 FMA, so this costs no more than an FP32 chain.
+// Channel chunks are staged into LDS from the last to the first, and the
+// chains carry across chunks.
+// Each lane carries ND directions of its row, so one LDS read of a channel
+// feeds 2 * ND independent chains (LDS return bandwidth, not the FMA pipe,
+// limits a one-direction-per-lane layout).  A workgroup covers DB directions
+// with L = DB / ND lanes per row and RP = 256 / L rows in flight.
+template <typename VT, int DB, int ND, int NP, int CC>
+__global__ __launch_bounds__(kThreads) void rotate_average_horner64(
+    const VT* __restrict__ vis, const double* __restrict__ uvw, const double* __restrict__ dd,
+    uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa,
+    double* __restrict__ avg) {
+    static_assert(CC % 2 == 0, "chunks hold whole channel pairs");
+    static_assert(DB % ND == 0 && kThreads % (DB / ND) == 0, "lane layout");
+    constexpr int L = DB / ND;
+    constexpr int RP = kThreads / L;
+    __shared__ C2<double> s_v[RP][NP][CC];
+    __shared__ double s_red[(RP > 1) ? RP : 1][L][ND][NP][2];
+
+    const uint32_t bl = blockIdx.x;
+    const uint32_t lane_d = threadIdx.x % L;
+    const uint32_t rsub = threadIdx.x / L;
+    uint32_t dir[ND];
+    bool dir_ok[ND];
+    double dz[ND][3];
+    bool any_dir = false;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        dir[j] = blockIdx.y * DB + lane_d + j * L;
+        dir_ok[j] = dir[j] < D;
+        any_dir |= dir_ok[j];
+        const uint32_t dc = dir_ok[j] ? dir[j] : 0;
+        dz[j][0] = dd[9 * dc + 6];
+        dz[j][1] = dd[9 * dc + 7];
+        dz[j][2] = dd[9 * dc + 8];
+    }
+    const uint32_t n_chunks = (C + CC - 1) / CC;
+
+    C2<double> acc[ND][NP];
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) acc[j][q] = {0.0, 0.0};
+
+    for (uint32_t tb = 0; tb < T; tb += RP) {
+        const uint32_t t = tb + rsub;
+        const bool row_ok = t < T;
+        double u = 0.0, v = 0.0, w = 0.0;
+        if (row_ok) {
+            const size_t row = size_t(t) * nbl + bl;
+            u = uvw[3 * row];
+            v = uvw[3 * row + 1];
+            w = uvw[3 * row + 2];
+        }
+        double sf[ND];
+        C2<double> z[ND], z2[ND];
+        C2<double> he[ND][NP], ho[ND][NP];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            sf[j] = -kTwoPi * ((dz[j][0] * u + dz[j][1] * v + dz[j][2] * w) - w);  // CpuLeapCalibrator.cc:190
+            z[j] = phasor_f64(sf[j] * dkappa);
+            z2[j] = cmul(z[j], z[j]);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                he[j][q] = {0.0, 0.0};
+                ho[j][q] = {0.0, 0.0};
+            }
+        }
+        for (int ch = int(n_chunks) - 1; ch >= 0; --ch) {
+            const uint32_t c0 = uint32_t(ch) * CC;
+            const uint32_t cn = min(uint32_t(CC), C - c0);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < RP * uint32_t(CC); i += kThreads) {
+                const uint32_t r = i / CC, c = i - r * CC;
+                const uint32_t tt = tb + r;
+                C2<double> val[NP];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) val[q] = {0.0, 0.0};
+                if (tt < T && c < cn) {
+                    const size_t row = size_t(tt) * nbl + bl;
+                    const VT* p = vis + ((row * C + c0 + c) * P) * 2;
+                    bool finite = true;
+                    VT x[8];
+                    for (uint32_t q = 0; q < P; ++q) {
+                        x[2 * q] = p[2 * q];
+                        x[2 * q + 1] = p[2 * q + 1];
+                        finite = finite && __builtin_isfinite(x[2 * q]) && __builtin_isfinite(x[2 * q + 1]);
+                    }
+                    if (finite) {
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) val[q] = {double(x[2 * q]), double(x[2 * q + 1])};
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NP; ++q) s_v[r][q][c] = val[q];
+            }
+            __syncthreads();
+            if (!row_ok || !any_dir) continue;
+            // Entries past cn are zero in LDS, so every chunk runs whole pairs.
+            const int npairs = int((cn + 1) / 2);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                if (npairs == CC / 2) {
+#pragma unroll 4
+                    for (int k = CC / 2 - 1; k >= 0; --k) {
+                        const C2<double> ve = s_v[rsub][q][2 * k], vo = s_v[rsub][q][2 * k + 1];
+#pragma unroll
+                        for (int j = 0; j < ND; ++j) {
+                            he[j][q] = horner_step(he[j][q], z2[j], ve);
+                            ho[j][q] = horner_step(ho[j][q], z2[j], vo);
+                        }
+                    }
+                } else {
+                    for (int k = npairs - 1; k >= 0; --k) {
+                        const C2<double> ve = s_v[rsub][q][2 * k], vo = s_v[rsub][q][2 * k + 1];
+#pragma unroll
+                        for (int j = 0; j < ND; ++j) {
+                            he[j][q] = horner_step(he[j][q], z2[j], ve);
+                            ho[j][q] = horner_step(ho[j][q], z2[j], vo);
+                        }
+                    }
+                }
+            }
+        }
+        if (row_ok) {
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                const C2<double> e0 = phasor_f64(sf[j] * kappa0);
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    const C2<double> zo = cmul(z[j], ho[j][q]);
+                    const C2<double> h = {he[j][q].x + zo.x, he[j][q].y + zo.y};
+                    const C2<double> r = cmul(e0, h);
+                    acc[j][q].x += r.x;
+                    acc[j][q].y += r.y;
+                }
+            }
+        }
+    }
+    if constexpr (RP > 1) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                s_red[rsub][lane_d][j][q][0] = acc[j][q].x;
+                s_red[rsub][lane_d][j][q][1] = acc[j][q].y;
+            }
+        __syncthreads();
+        if (rsub == 0) {
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                if (!dir_ok[j]) continue;
+                double* o = avg + (size_t(dir[j]) * nbl + bl) * P * 2;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    double re = 0.0, im = 0.0;
+                    for (int r = 0; r < RP; ++r) {
+                        re += s_red[r][lane_d][j][q][0];
+                        im += s_red[r][lane_d][j][q][1];
+                    }
+                    o[2 * q] = re;
+                    o[2 * q + 1] = im;
+                }
+                for (uint32_t q = 2 * NP; q < 2 * P; ++q) o[q] = 0.0;  // pols not accumulated
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            if (!dir_ok[j]) continue;
+            double* o = avg + (size_t(dir[j]) * nbl + bl) * P * 2;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                o[2 * q] = acc[j][q].x;
+                o[2 * q + 1] = acc[j][q].y;
+            }
+            for (uint32_t q = 2 * NP; q < 2 * P; ++q) o[q] = 0.0;
+        }
+    }
+}
+
+// A == double selects the full-row FP64 Horner kernel (ND directions per
+// lane); A == float the FP32 blocked kernel (one direction per lane,
+// 32-channel Horner blocks re-anchored by FP64-reduced FP32 sincos, Kahan sums).
+template <typename VT, typename A, int DB, int ND, int NP>
+void launch_one(hipStream_t s, const void* vis, const double* uvw, const double* dd, uint32_t D, uint32_t T,
+                uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double kappa0, double dkappa, double* avg) {
+    constexpr int CC = (NP == 1) ? 256 : 64;
+    const dim3 grid(nbl_grid, (D + DB - 1) / DB);
+    if constexpr (sizeof(A) == 8) {
+        rotate_average_horner64<VT, DB, ND, NP, CC><<<grid, kThreads, 0, s>>>(
+            static_cast<const VT*>(vis), uvw, dd, D, T, nbl, C, P, kappa0, dkappa, avg);
+    } else {
+        rotate_average_batched<VT, A, DB, NP, CC><<<grid, kThreads, 0, s>>>(
+            static_cast<const VT*>(vis), uvw, dd, D, T, nbl, C, P, kappa0, dkappa, avg);
+    }
+    LEAP_HIP(hipGetLastError());
+}
+
+// Tile code: DB * 10 + ND (directions per workgroup, directions per lane).
+template <typename VT, typename A, int NP>
+void launch_db(hipStream_t s, int tile, const void* vis, const double* uvw, const double* dd, uint32_t D,
+               uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double kappa0, double dkappa,
+               double* avg) {
+#define LEAP_TILE(DB_, ND_)                                                                                   \
+    case DB_ * 10 + ND_:                                                                                      \
+        launch_one<VT, A, DB_, ND_, NP>(s, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg); \
+        return;
+    if constexpr (sizeof(A) == 8) {
+        switch (tile) {
+            LEAP_TILE(32, 1)
+            LEAP_TILE(64, 1)
+            LEAP_TILE(64, 2)
+            LEAP_TILE(128, 1)
+            LEAP_TILE(128, 2)
+            LEAP_TILE(128, 4)
+            LEAP_TILE(256, 1)
+            LEAP_TILE(256, 2)
+            LEAP_TILE(256, 4)
+            default: break;
+        }
+    } else {
+        switch (tile) {
+            LEAP_TILE(32, 1)
+            LEAP_TILE(64, 1)
+            LEAP_TILE(128, 1)
+            LEAP_TILE(256, 1)
+            default: break;
+        }
+    }
+#undef LEAP_TILE
+    throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported batched tile " + std::to_string(tile));
+}
+
+template <typename VT, typename A>
+void launch_np(hipStream_t s, int db, bool all_pols, const void* vis, const double* uvw, const double* dd,
+               uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double kappa0, double dkappa,
+               double* avg) {
+    if (!all_pols || P == 1) {
+        launch_db<VT, A, 1>(s, db, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
+        return;
+    }
+    switch (P) {
+        case 2: launch_db<VT, A, 2>(s, db, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg); break;
+        case 3: launch_db<VT, A, 3>(s, db, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg); break;
+        case 4: launch_db<VT, A, 4>(s, db, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg); break;
+        default: throw Error(LEAP_ERR_NOT_IMPLEMENTED, "unsupported polarisation count");
+    }
+}
+
+}  // namespace
+
+void launch_rotate_batched(hipStream_t s, int32_t precision, int32_t direction_block, bool all_pols,
+                           const void* vis, int32_t vis_dtype, const double* uvw, const double* dd, uint32_t D,
+                           uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,
+                           double df, double* avg, int32_t fp32_blocks) {
+    if (nbl_grid == 0 || D == 0) return;
+    const double kappa0 = f0 / kSpeedOfLight;
+    const double dkappa = df / kSpeedOfLight;
+    const bool c64 = vis_dtype == LEAP_VIS_C64;
+    if (precision == LEAP_PRECISION_MIXED && fp32_blocks) {
+        if (c64)
+            launch_np<float, float>(s, direction_block, all_pols, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
+        else
+            launch_np<double, float>(s, direction_block, all_pols, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
+    } else {
+        if (c64)
+            launch_np<float, double>(s, direction_block, all_pols, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
+        else
+            launch_np<double, double>(s, direction_block, all_pols, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
+    }
+}
+
+}  // namespace leap

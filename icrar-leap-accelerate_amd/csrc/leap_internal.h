@@ -95,6 +95,18 @@ struct leap_metadata {
     leap::DeviceBuffer d_Gp;       // scratch for Ad assembly
 };
 
+struct TuneKey {
+    int32_t precision, vis_dtype, all_pols;
+    uint32_t D, T, C, P;
+    bool operator<(const TuneKey& o) const {
+        const int64_t a[7] = {precision, vis_dtype, all_pols, D, T, C, P};
+        const int64_t b[7] = {o.precision, o.vis_dtype, o.all_pols, o.D, o.T, o.C, o.P};
+        for (int i = 0; i < 7; ++i)
+            if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+};
+
 struct leap_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -110,6 +122,9 @@ struct leap_ctx {
     std::vector<double> h_avg, h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;
+    // autotuned direction blocks of the batched kernel
+    std::map<TuneKey, int32_t> tuned;
+    int32_t last_direction_block = 0;
 };
 
 namespace leap {
@@ -123,6 +138,10 @@ void launch_assemble_ad(hipStream_t s, const double* Gp, int32_t N, const int2* 
                         int32_t n_pairs, int32_t constraint, double* AdT);
 void launch_inversion_f64(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D,
                           double* phase, double* cal1, double* delta, double* partial, double* cal);
+void launch_rotate_batched(hipStream_t s, int32_t precision, int32_t direction_block, bool all_pols,
+                           const void* vis, int32_t vis_dtype, const double* uvw, const double* dd, uint32_t D,
+                           uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,
+                           double df, double* avg, int32_t fp32_blocks);
 size_t inversion_partial_doubles(int32_t K, int32_t N, uint32_t D);
 void launch_synth_vis(hipStream_t s, const leap_synth_ms* ms, int32_t vis_dtype, void* out,
                       const double* d_uvw);

@@ -73,16 +73,21 @@ class Options(ctypes.Structure):
         ("strategy", ctypes.c_int32),
         ("direction_block", ctypes.c_int32),
         ("all_pols", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("fp32_blocks", ctypes.c_int32),
+        ("lane_directions", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
-def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION, direction_block=0, all_pols=0):
+def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION, direction_block=0, all_pols=0,
+            fp32_blocks=0, lane_directions=0):
     o = Options()
     o.precision = precision
     o.strategy = strategy
     o.direction_block = direction_block
     o.all_pols = all_pols
+    o.fp32_blocks = fp32_blocks
+    o.lane_directions = lane_directions
     return o
 
 
@@ -136,7 +141,7 @@ EXPORTS = [
     "leap_synth_create", "leap_synth_destroy", "leap_synth_view", "leap_synth_sky", "leap_synth_fill_vis_host",
     "leap_hip_synth_fill_vis_device",
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
-    "leap_hip_last_kernel_times",
+    "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_phase_matrix", "leap_phase_matrix_pinv",
 ]
 
 
@@ -197,6 +202,7 @@ def lib():
     L.leap_hip_memcpy_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.leap_hip_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.leap_hip_last_kernel_times.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp]
+    L.leap_hip_last_direction_block.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -401,6 +407,9 @@ class Context:
                                              ctypes.byref(o), cbf, None)
         self._chk(rc)
         return np.array(cals), np.array(epochs)
+
+    def last_direction_block(self):
+        return int(lib().leap_hip_last_direction_block(self.h))
 
     def last_kernel_times(self):
         a = ctypes.c_double()

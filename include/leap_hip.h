@@ -93,10 +93,12 @@ typedef struct leap_ms_view {
 typedef enum leap_precision {
     /* FP64 phase, FP64 sincos, FP64 sums: the reference's arithmetic. */
     LEAP_PRECISION_FP64 = 0,
-    /* FP64 phase argument and range reduction; FP32 sincos and phasor
-     * recurrence; FP32 sums with compensated (Kahan) accumulation across
-     * channel blocks and timesteps; Ad*dphi as split-3 x BF16 MFMA with FP32
-     * accumulation.  Needs the batched strategy. */
+    /* Visibilities stored as complex64 (or converted on load), the phase
+     * argument and its reduction in FP64, the channel sums as a full-row FP64
+     * Horner evaluation (FP64 FMA issues at the rate of non-packed FP32 on
+     * gfx950), Ad * dphi as split-3 x BF16 MFMA with FP32 accumulation.  The
+     * fp32_blocks option selects FP32 sincos / Horner / Kahan sums instead.
+     * Needs the batched strategy. */
     LEAP_PRECISION_MIXED = 1
 } leap_precision;
 
@@ -111,9 +113,13 @@ typedef enum leap_strategy {
 typedef struct leap_options {
     int32_t precision;        /* leap_precision */
     int32_t strategy;         /* leap_strategy */
-    int32_t direction_block;  /* directions per pass for the batched strategy; 0 = tuned */
+    int32_t direction_block;  /* batched: directions per workgroup (32, 64, 128, 256); 0 = tuned */
     int32_t all_pols;         /* batched: also accumulate polarisations 1..P-1 of avgData */
-    int32_t reserved[4];
+    int32_t fp32_blocks;      /* mixed: FP32 Horner blocks of 32 channels re-anchored by FP32
+                                 sincos, Kahan sums (faster; outside the 1e-5 rad budget on
+                                 decorrelated baselines); 0 = full-row FP64 Horner */
+    int32_t lane_directions;  /* batched FP64 Horner: directions per lane (1, 2, 4); 0 = tuned */
+    int32_t reserved[2];
 } leap_options;
 
 /* One solution interval's result: cpu::Calibration (model/cpu/calibration/Calibration.h:42-111)
@@ -259,6 +265,8 @@ int leap_hip_memcpy_h2d(leap_ctx* ctx, void* dst, const void* src, uint64_t byte
  * launches in the last call, measured with HIP events on the context's stream. */
 int leap_hip_last_kernel_times(const leap_ctx* ctx, double* rotate_ms_avg, int32_t* rotate_launches,
                                double* total_ms);
+/* Direction block the batched kernel used in the last call (0 if none). */
+int leap_hip_last_direction_block(const leap_ctx* ctx);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,147 @@
+"""The batched strategy (one pass over the visibilities per block of
+directions) against the oracle.
+
+Bars: FP64 batched — avgData pol 0 within 1e-12 of its largest magnitude and
+calibrations within 1e-9 rad wrapped; mixed precision — calibrations within
+1e-5 rad wrapped (max and RMS are printed per case).
+"""
+import numpy as np
+import pytest
+
+import np_restatement as npr
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9
+MIXED_TOL = 1e-5
+
+
+def wrapped(a, b):
+    return np.abs(npr.wrap(np.asarray(a) - np.asarray(b)))
+
+
+def dirs_near(centre, n, seed, spread=0.02):
+    rng = np.random.default_rng(seed)
+    return np.column_stack([centre[0] + rng.uniform(-spread, spread, n), centre[1] + rng.uniform(-spread, spread, n)])
+
+
+@pytest.fixture(scope="module")
+def small(leap):
+    s = leap.SynthMS(n_antennas=40, n_timesteps=5, n_channels=77, flag_fraction=0.02, nan_fraction=0.002, seed=31)
+    s.fill_host()
+    return s
+
+
+@pytest.mark.parametrize("db", [0, 32, 64, 128, 256])
+@pytest.mark.parametrize("n_dirs", [1, 37, 70])
+def test_batched_fp64_rotate_matches_oracle(leap, ctx, oracle, small, db, n_dirs):
+    ms = small.host_dict()
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    omd = oracle.metadata(ms)
+    dirs = dirs_near(ms["phase_centre"], n_dirs, n_dirs)
+    opts = leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db)
+    got = md.rotate(view, dirs, opts=opts)
+    for d in range(0, n_dirs, max(1, n_dirs // 5)):
+        want = omd.rotate(dirs[d])
+        assert np.abs(got[d, :, 0] - want[:, 0]).max() <= 1e-12 * np.abs(want[:, 0]).max()
+        assert not got[d, :, 1:].any()  # only pol 0 is accumulated by default
+
+
+@pytest.mark.parametrize("db,nd", [(64, 2), (128, 4), (256, 2), (256, 4)])
+def test_batched_lane_directions(leap, ctx, oracle, small, db, nd):
+    ms = small.host_dict()
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    omd = oracle.metadata(ms)
+    dirs = dirs_near(ms["phase_centre"], 41, 11)
+    got = md.rotate(view, dirs, opts=leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
+                                                  lane_directions=nd))
+    assert ctx.last_direction_block() == db * 10 + nd
+    for d in (0, 1, 17, 40):
+        want = omd.rotate(dirs[d])
+        assert np.abs(got[d, :, 0] - want[:, 0]).max() <= 1e-12 * np.abs(want[:, 0]).max()
+
+
+@pytest.mark.parametrize("nd", [1, 2])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_batched_all_pols_matches_oracle(leap, ctx, oracle, dtype, nd):
+    s = leap.SynthMS(n_antennas=16, n_timesteps=7, n_channels=100, nan_fraction=0.003, seed=32)
+    s.fill_host(dtype)
+    ms = s.host_dict()
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    omd = oracle.metadata(ms)
+    dirs = dirs_near(ms["phase_centre"], 5, 2)
+    got = md.rotate(view, dirs, opts=leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, all_pols=1,
+                                                  direction_block=64, lane_directions=nd))
+    for d in range(5):
+        want = omd.rotate(dirs[d])
+        assert np.abs(got[d] - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("slice_", [(0, -1, -1), (0, 2, -1)])
+def test_batched_fp64_calibrate_matches_oracle(leap, ctx, oracle, small, slice_):
+    ms = small.host_dict()
+    view = leap.view_from_arrays(ms)
+    dirs = dirs_near(ms["phase_centre"], 20, 5)
+    got, ge = ctx.calibrate(view, dirs, slice_, ref_ant=0, opts=leap.options(strategy=leap.LEAP_STRATEGY_BATCHED))
+    want, we = oracle.calibrate(ms, dirs, slice_, ref_ant=0)
+    assert np.array_equal(ge, we)
+    assert wrapped(got, want).max() < FP64_TOL
+
+
+@pytest.mark.parametrize("fp32_blocks", [0, 1], ids=["fp64-horner", "fp32-blocks"])
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_mixed_precision_calibrate_error_budget(leap, ctx, oracle, dtype, fp32_blocks):
+    s = leap.SynthMS(n_antennas=48, n_timesteps=8, n_channels=384, freq_start_hz=50e6, freq_inc_hz=781.25e3,
+                     layout=1, array_radius_m=20000.0, flag_fraction=0.02, seed=33)
+    s.fill_host(np.complex128)
+    ms64 = s.host_dict()
+    vis = s.vis.astype(np.complex64) if dtype == np.complex64 else s.vis
+    ms = s.host_dict(vis)
+    view = leap.view_from_arrays(ms)
+    dirs = dirs_near(ms["phase_centre"], 24, 6)
+    got, _ = ctx.calibrate(view, dirs, opts=leap.options(precision=leap.LEAP_PRECISION_MIXED,
+                                                         strategy=leap.LEAP_STRATEGY_BATCHED,
+                                                         fp32_blocks=fp32_blocks))
+    # against the FP64 oracle on the same stored visibilities
+    want, _ = oracle.calibrate(ms, dirs)
+    err = wrapped(got, want)
+    per_dir_max = err.max(axis=(0, 2))
+    per_dir_rms = np.sqrt((err ** 2).mean(axis=(0, 2)))
+    print(f"mixed {np.dtype(dtype).name} fp32_blocks={fp32_blocks}: wrapped error max {per_dir_max.max():.2e} rad, "
+          f"rms {per_dir_rms.max():.2e} rad (worst direction)")
+    # The FP32 blocked variant is kept for comparison; it does not meet the 1e-5 rad budget.
+    assert per_dir_max.max() < (MIXED_TOL if not fp32_blocks else 1e-2)
+    if dtype == np.complex64:
+        # the cube itself was rounded to complex64: report the FP64-cube distance too
+        ref64, _ = oracle.calibrate(ms64, dirs)
+        print(f"  vs FP64 oracle on the complex128 cube: max {wrapped(got, ref64).max():.2e} rad")
+
+
+def test_mixed_rotate_relative_error(leap, ctx, oracle, small):
+    ms = small.host_dict(small.vis.astype(np.complex64))
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    omd = oracle.metadata(ms)
+    dirs = dirs_near(ms["phase_centre"], 8, 9)
+    got = md.rotate(view, dirs, opts=leap.options(precision=leap.LEAP_PRECISION_MIXED,
+                                                  strategy=leap.LEAP_STRATEGY_BATCHED))
+    blk = md.rotate(view, dirs, opts=leap.options(precision=leap.LEAP_PRECISION_MIXED,
+                                                  strategy=leap.LEAP_STRATEGY_BATCHED, fp32_blocks=1))
+    for d in range(8):
+        want = omd.rotate(dirs[d])[:, 0]
+        rel = np.abs(got[d, :, 0] - want).max() / np.abs(want).max()
+        assert rel < 1e-12, rel
+        rel_blk = np.abs(blk[d, :, 0] - want).max() / np.abs(want).max()
+        assert rel_blk < 2e-5, rel_blk
+
+
+def test_mixed_requires_batched(leap, ctx, small):
+    view = leap.view_from_arrays(small.host_dict())
+    with pytest.raises(leap.LeapError, match="batched"):
+        ctx.calibrate(view, [(0.5, -0.47)], opts=leap.options(precision=leap.LEAP_PRECISION_MIXED))
+    with pytest.raises(leap.LeapError, match="direction block"):
+        ctx.calibrate(view, [(0.5, -0.47)], opts=leap.options(strategy=leap.LEAP_STRATEGY_BATCHED,
+                                                               direction_block=48))
