@@ -7,11 +7,13 @@ timesteps of 1 s, 384 channels from 50 MHz at 781.25 kHz, 4 polarisations,
 2% of baselines flagged, five point sources under per-antenna phase screens
 plus noise.  One solution interval covers all 128 timesteps.
 
-A step is one calibration of one direction (per-direction FP64 path over the
-complex128 cube, 103.5 GB resident in HBM before timing starts).  value is
-directions calibrated per second.  The JSON line also carries the roofline
-of the dominant kernel (rotate/average, HBM-bound) from HIP events, and a CPU
-baseline: the oracle on a bounded sample of the same workload (host cores).
+Default (headline): value is the batched mixed-precision rate — a step
+calibrates 256 directions in one pass over the complex64 cube (51.7 GB
+resident in HBM) — and the plain FP64 per-direction rate (a step = one
+direction over the complex128 cube, 103.5 GB) is measured beside it in the
+same run.  The JSON line carries the roofline of the dominant kernel from
+HIP events on the library's stream, and a CPU baseline: the oracle on a
+bounded sample of the same workload (host cores).
 """
 import argparse
 import json
@@ -29,6 +31,7 @@ import leap_amd as la  # noqa: E402
 
 METRIC = "directions calibrated/s (256 st,128 t,384 ch) batched-mixed vs plain FP64; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+VALU_FP64_PEAK_TFLOPS = 78.6
 
 
 def log(msg):
@@ -46,9 +49,9 @@ def directions(centre, n, seed=7, spread=0.02):
     return np.column_stack([centre[0] + rng.uniform(-spread, spread, n), centre[1] + rng.uniform(-spread, spread, n)])
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the rotate kernel measured with rocprofv3 PMC counters, if committed."""
-    p = os.path.join(REPO, "profiles", "rotate_average_f64_c1_traffic.json")
+def traffic_from_profile(name):
+    """HBM bytes per launch of a rotate kernel measured with rocprofv3 PMC counters, if committed."""
+    p = os.path.join(REPO, "profiles", name)
     if not os.path.exists(p):
         return None
     try:
@@ -119,12 +122,12 @@ def copy_probe(gb):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps (the headline's batched leg runs steps/4)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batched-fp64", "batched-mixed"],
-                    help="plain: FP64 per direction on the complex128 cube; batched-fp64: FP64, one pass per "
-                         "block of directions on the complex128 cube; batched-mixed: mixed precision on the "
-                         "complex64 cube")
+    ap.add_argument("--mode", default="headline", choices=["headline", "plain", "batched-fp64", "batched-mixed"],
+                    help="headline: batched-mixed (value) measured beside plain; plain: FP64 per direction on "
+                         "the complex128 cube; batched-fp64: FP64, one pass per block of directions on the "
+                         "complex128 cube; batched-mixed: mixed precision on the complex64 cube")
     ap.add_argument("--directions", type=int, default=0, help="directions per step (default 1 plain, 64 "
                     "batched-fp64, 256 batched-mixed)")
     ap.add_argument("--direction-block", type=int, default=0, help="0 = autotuned")
@@ -145,8 +148,30 @@ def main():
               "changing a GPU's kernel).", file=sys.stderr)
         return 2
 
-    mode = args.mode
-    D = args.directions or {"plain": 1, "batched-fp64": 64, "batched-mixed": 256}[mode]
+    if args.mode == "headline":
+        plain = run_mode(args, "plain", 1, with_cpu=False)
+        mixed = run_mode(args, "batched-mixed", args.directions or 256, with_cpu=args.cpu_timesteps > 0,
+                         steps=max(1, args.steps // 4))
+        out = mixed
+        out["config"]["plain_fp64"] = {
+            "directions_per_s": plain["value"], "ms_per_step": plain["ms_per_step"],
+            "rotate_kernel_ms": plain["roofline"]["kernel_ms"], "roofline": plain["roofline"],
+        }
+        out["config"]["speedup_vs_plain_fp64"] = round(mixed["value"] / plain["value"], 2)
+        out["config"]["workload"] = ("batched mixed-precision calibration (complex64 cube 51.7 GB resident in "
+                                     f"HBM, {out['config']['directions_per_step']} directions per step, one pass "
+                                     "over the cube) vs plain FP64 per-direction calibration (complex128 cube "
+                                     "103.5 GB, 1 direction per step); 256 stations / 32896 baselines (autos) / "
+                                     "128 t / 384 ch / 4 pol, 1 solution interval")
+    else:
+        D = args.directions or {"plain": 1, "batched-fp64": 64, "batched-mixed": 256}[args.mode]
+        out = run_mode(args, args.mode, D, with_cpu=args.cpu_timesteps > 0)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_mode(args, mode, D, with_cpu, steps=None):
+    steps = steps or args.steps
     vis_dtype = la.LEAP_VIS_C64 if mode == "batched-mixed" else la.LEAP_VIS_C128
     elem = 8 if vis_dtype == la.LEAP_VIS_C64 else 16
     opts = la.options(precision=la.LEAP_PRECISION_MIXED if mode == "batched-mixed" else la.LEAP_PRECISION_FP64,
@@ -156,49 +181,66 @@ def main():
     opts.lane_directions = args.lane_directions
 
     cfg = c1_config()
-    log("building synthetic measurement set (host columns)")
+    log(f"[{mode}] building synthetic measurement set (host columns)")
     synth = la.SynthMS(**cfg)
     R, C, P, nbl = synth.num_rows, synth.num_channels, synth.num_pols, synth.num_baselines
     ctx = la.Context(0)
     vis_bytes = R * C * P * elem
-    log(f"allocating {vis_bytes / 1e9:.1f} GB {'complex64' if elem == 8 else 'complex128'} cube on the device "
-        "and generating it")
+    log(f"[{mode}] allocating {vis_bytes / 1e9:.1f} GB {'complex64' if elem == 8 else 'complex128'} cube on the "
+        "device and generating it")
     vis_ptr = ctx.malloc(vis_bytes)
     uvw_ptr = ctx.malloc(R * 3 * 8)
     t = time.perf_counter()
     ctx.synth_fill_device(synth, vis_dtype, vis_ptr, uvw_ptr)
-    log(f"cube generated in {time.perf_counter() - t:.2f} s")
+    log(f"[{mode}] cube generated in {time.perf_counter() - t:.2f} s")
     view = la.view_from_arrays(synth.host_dict(), vis_device=vis_ptr, vis_dtype=vis_dtype, uvw_device=uvw_ptr)
     t = time.perf_counter()
     md = ctx.metadata(view)
     t_md = time.perf_counter() - t
-    log(f"metadata (A {md.K}x{md.N}, A1 {md.K1}x{md.N}, pseudo-inverses) in {t_md:.2f} s")
+    log(f"[{mode}] metadata (A {md.K}x{md.N}, A1 {md.K1}x{md.N}, pseudo-inverses) in {t_md:.2f} s")
 
-    dirs = directions((view.phase_centre_ra_rad, view.phase_centre_dec_rad), (args.warmup + args.steps) * D)
+    dirs = directions((view.phase_centre_ra_rad, view.phase_centre_dec_rad), (args.warmup + steps) * D)
     for i in range(args.warmup):
         ctx.calibrate(view, dirs[i * D:(i + 1) * D], md=md, opts=opts)
-        log(f"warmup step {i + 1}/{args.warmup} done")
+        log(f"[{mode}] warmup step {i + 1}/{args.warmup} done")
     rot_ms = []
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         j = args.warmup + i
         ctx.calibrate(view, dirs[j * D:(j + 1) * D], md=md, opts=opts)
         rot_ms.append(ctx.last_kernel_times()[0])
     elapsed = time.perf_counter() - t0
-    log(f"{args.steps} timed steps of {D} directions in {elapsed:.3f} s")
-    value = args.steps * D / elapsed
+    log(f"[{mode}] {steps} timed steps of {D} directions in {elapsed:.3f} s")
+    value = steps * D / elapsed
     k_ms = float(np.mean(rot_ms))
-    alg_bytes = R * C * P * elem + R * 24 + D * nbl * P * 16 + C * 8
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile() if mode == "plain" else None
+    if mode == "plain":
+        alg_bytes = R * C * P * elem + R * 24 + D * nbl * P * 16 + C * 8
+        achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+        roofline = {"kernel": "rotate_average_f64", "bound": "hbm", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profile("rotate_average_f64_c1_traffic.json"),
+                    "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": alg_bytes}
+    else:
+        # One complex multiply-accumulate (8 flops) per visibility, direction and pass.
+        alg_flops = 8 * R * C * D
+        achieved = alg_flops / (k_ms * 1e-3) / 1e12
+        alg_bytes = R * C * P * elem + R * 24 + D * nbl * P * 16
+        roofline = {"kernel": "rotate_average_horner64" if not args.fp32_blocks else "rotate_average_batched",
+                    "bound": "valu-fp64", "achieved": round(achieved, 2), "peak": VALU_FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / VALU_FP64_PEAK_TFLOPS, 4),
+                    "traffic": traffic_from_profile("rotate_average_horner64_c1_traffic.json"),
+                    "kernel_ms": round(k_ms, 4), "algorithmic_flops_per_launch": alg_flops,
+                    "hbm_gbs": round(alg_bytes / (k_ms * 1e-3) / 1e9, 1)}
 
     cpu = None
-    if args.cpu_timesteps > 0:
-        log(f"CPU baseline: oracle on {args.cpu_timesteps} timesteps")
+    if with_cpu:
+        log(f"[{mode}] CPU baseline: oracle on {args.cpu_timesteps} timesteps")
         cpu = cpu_baseline(ctx, synth, vis_ptr, vis_dtype, args.cpu_timesteps)
-        log("CPU baseline done")
+        log(f"[{mode}] CPU baseline done")
+    tile = ctx.last_direction_block()
     ctx.free(vis_ptr)
     ctx.free(uvw_ptr)
+    ctx.close()
 
     workload = {
         "plain": "plain FP64 per-direction calibration, complex128 cube 103.5 GB resident in HBM, 1 direction per step",
@@ -207,41 +249,29 @@ def main():
         "batched-mixed": f"batched mixed-precision calibration, complex64 cube 51.7 GB resident in HBM, {D} "
                          "directions per step in one pass over the cube",
     }[mode]
-    out = {
+    return {
         "metric": METRIC,
         "value": round(value, 3),
         "unit": "directions/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if mode != "batched-mixed" else "f32+f64",
+        "dtype": "f64" if mode != "batched-mixed" else "c64 storage, f64 phase and Horner",
         "data": "synthetic (seeded point sources x per-antenna phase screens + noise, generated on the device)",
         "config": {
             "workload": workload + "; 256 stations / 32896 baselines (autos) / 128 t / 384 ch / 4 pol, "
                         "1 solution interval",
             "stations": 256, "baselines": nbl, "timesteps": 128, "channels": C, "pols": P, "rows": R,
             "vis_dtype": "complex64" if elem == 8 else "complex128", "directions_per_step": D, "mode": mode,
-            "tile": ctx.last_direction_block(), "metadata_s": round(t_md, 3),
+            "tile": tile, "metadata_s": round(t_md, 3),
         },
-        "roofline": {
-            "kernel": "rotate_average_f64" if mode == "plain" else "rotate_average_batched",
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel_ms": round(k_ms, 4),
-            "algorithmic_bytes_per_launch": alg_bytes,
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
-    return 0
 
 
 if __name__ == "__main__":

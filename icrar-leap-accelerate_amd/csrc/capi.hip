@@ -234,16 +234,32 @@ void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_vie
     }
 }
 
-void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D) {
+void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const leap_options& opts) {
     const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
     ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
     ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
     ctx->d_delta.reserve(size_t(K) * D * sizeof(double));
     ctx->d_cal.reserve(size_t(N) * D * sizeof(double));
-    ctx->d_partial.reserve(inversion_partial_doubles(K, N, D) * sizeof(double));
-    launch_inversion_f64(ctx->stream, md, ctx->d_avg.as<double>(), D, ctx->d_phase.as<double>(),
-                         ctx->d_cal1.as<double>(), ctx->d_delta.as<double>(), ctx->d_partial.as<double>(),
-                         ctx->d_cal.as<double>());
+    if (opts.strategy != LEAP_STRATEGY_BATCHED) {
+        // Per-direction GEMVs, the reference's schedule (CudaLeapCalibrator.cu:256-259).
+        ctx->d_partial.reserve(inversion_partial_doubles(K, N, D) * sizeof(double));
+        launch_inversion_f64(ctx->stream, md, ctx->d_avg.as<double>(), D, ctx->d_phase.as<double>(),
+                             ctx->d_cal1.as<double>(), ctx->d_delta.as<double>(), ctx->d_partial.as<double>(),
+                             ctx->d_cal.as<double>());
+        return;
+    }
+    launch_inversion_head(ctx->stream, md, ctx->d_avg.as<double>(), D, ctx->d_phase.as<double>(),
+                          ctx->d_cal1.as<double>(), ctx->d_delta.as<double>());
+    if (opts.precision == LEAP_PRECISION_MIXED) {
+        ctx->d_a_planes.reserve(bf16_planes_bytes((D + 63) / 64 * 64, K));
+        ctx->d_partial.reserve(mfma_partial_floats(K, D, N) * sizeof(float));
+        launch_gemm_bf16x3(ctx->stream, md.d_AdT_planes.ptr, K, N, ctx->d_delta.as<double>(), D, ctx->d_a_planes.ptr,
+                           ctx->d_cal1.as<double>(), static_cast<float*>(ctx->d_partial.ptr), ctx->d_cal.as<double>());
+    } else {
+        ctx->d_partial.reserve(gemm_f64_partial_doubles(K, N, D) * sizeof(double));
+        launch_gemm_f64(ctx->stream, md.d_AdT.as<double>(), K, N, ctx->d_delta.as<double>(), D,
+                        ctx->d_cal1.as<double>(), ctx->d_partial.as<double>(), ctx->d_cal.as<double>());
+    }
 }
 
 void check_metadata_matches(const leap_metadata& md, const leap_ms_view& ms) {
@@ -310,6 +326,8 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
     LEAP_HIP(hipMemcpyAsync(g1p, md->G1p.data(), size_t(N) * N * sizeof(double), hipMemcpyHostToDevice, s));
     launch_assemble_ad(s, gp, N, md->d_pairs.as<int2>(), K - 1, md->A.constraint, md->d_AdT.as<double>());
     launch_assemble_ad(s, g1p, N, d_pairs1.as<int2>(), K1 - 1, md->A1.constraint, md->d_Ad1T.as<double>());
+    md->d_AdT_planes.reserve(bf16_planes_bytes(uint32_t(N), K));
+    launch_split_b(s, md->d_AdT.as<double>(), K, N, md->d_AdT_planes.ptr);
     LEAP_HIP(hipStreamSynchronize(s));
     return md.release();
 }
@@ -346,7 +364,7 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
         if (D > 0) {
             const DeviceRows rows = stage_rows(ctx, ms, row_begin, row_end);
             rotate_directions(ctx, md, ms, rows, interval, D, opts);
-            invert_directions(ctx, md, D);
+            invert_directions(ctx, md, D, opts);
             LEAP_HIP(hipMemcpyAsync(ctx->h_cal.data(), ctx->d_cal.ptr, size_t(N) * D * sizeof(double),
                                     hipMemcpyDeviceToHost, ctx->stream));
             LEAP_HIP(hipStreamSynchronize(ctx->stream));
@@ -480,12 +498,12 @@ int leap_hip_apply_inversion(leap_ctx* ctx, const leap_metadata* md, const doubl
                              const leap_options* opts, double* cal_out) {
     return guarded(ctx, [&] {
         require(ctx && md && avg && cal_out, LEAP_ERR_INVALID_ARGUMENT, "null argument");
-        resolve(opts);
+        const leap_options o = resolve(opts);
         DeviceScope scope(ctx->device);
         const size_t n = size_t(md->nbl) * md->pols * 2 * D;
         ctx->d_avg.reserve(n * sizeof(double));
         LEAP_HIP(hipMemcpyAsync(ctx->d_avg.ptr, avg, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        invert_directions(ctx, *md, D);
+        invert_directions(ctx, *md, D, o);
         LEAP_HIP(hipMemcpyAsync(cal_out, ctx->d_cal.ptr, size_t(md->A.cols) * D * sizeof(double),
                                 hipMemcpyDeviceToHost, ctx->stream));
         LEAP_HIP(hipStreamSynchronize(ctx->stream));

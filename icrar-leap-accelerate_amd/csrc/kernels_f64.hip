@@ -316,6 +316,19 @@ void launch_inversion_f64(hipStream_t s, const leap_metadata& md, const double* 
     LEAP_HIP(hipGetLastError());
 }
 
+void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D, double* phase,
+                           double* cal1, double* delta) {
+    if (D == 0) return;
+    const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
+    const uint32_t nbl = md.nbl, P = md.pols;
+    phase_i1_kernel<<<dim3((K1 + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_I1.as<int32_t>(), K1, phase);
+    LEAP_HIP(hipGetLastError());
+    cal1_kernel<<<dim3((N + 255) / 256, D), 256, 0, s>>>(md.d_Ad1T.as<double>(), K1, N, phase, cal1);
+    LEAP_HIP(hipGetLastError());
+    delta_kernel<<<dim3((K + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_pairs.as<int2>(), K, N, cal1, delta);
+    LEAP_HIP(hipGetLastError());
+}
+
 size_t inversion_partial_doubles(int32_t K, int32_t N, uint32_t D) {
     return size_t((K + kGemvChunk - 1) / kGemvChunk) * size_t(N) * D;
 }

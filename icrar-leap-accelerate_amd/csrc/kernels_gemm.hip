@@ -1,0 +1,235 @@
+// Batched pseudo-inverse products cal[d] = Ad * dphi[d] + cal1[d] for D
+// directions: the reference's per-direction GEMV (CpuLeapCalibrator.cc:178,
+// cuBLAS call at CudaLeapCalibrator.cu:259) becomes one GEMM
+//   cal^T [D x N] = dphi [D x K] * AdT [K x N]   (+ cal1).
+//
+// FP64: split-K over baselines, each workgroup holds a block of 16 directions'
+// dphi in LDS and streams its K-chunk of AdT rows once (coalesced over the
+// antenna index), so AdT is read D / 16 times instead of D times.
+//
+// Mixed: split-3 x BF16 on MFMA with FP32 accumulation.  Every operand x is
+// split as x = hi + mid + lo with hi = bf16(x), mid = bf16(x - hi),
+// lo = bf16(x - hi - mid) (about 24 significant bits), and the product keeps
+// the six terms hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid, the ones above
+// 2^-24 relative.  The split planes are stored in the operand layout of
+// v_mfma_f32_16x16x32_bf16 (8 consecutive k of one row or column per lane), so
+// each lane loads 16 contiguous bytes per plane and K-step.
+#include "leap_internal.h"
+
+namespace leap {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kDirTile = 16;   // FP64 GEMM: directions per workgroup
+constexpr int kKChunk = 256;   // FP64 GEMM: baselines per workgroup
+
+__global__ __launch_bounds__(256) void gemm_f64_partial(const double* __restrict__ AdT, int32_t K, int32_t N,
+                                                        const double* __restrict__ delta, uint32_t D,
+                                                        double* __restrict__ partial) {
+    __shared__ double s_x[kDirTile][kKChunk];
+    const int32_t chunk = blockIdx.x;
+    const uint32_t d0 = blockIdx.y * kDirTile;
+    const int32_t k0 = chunk * kKChunk;
+    const int32_t kn = min(kKChunk, K - k0);
+    for (int32_t i = threadIdx.x; i < kDirTile * kKChunk; i += 256) {
+        const int32_t dd = i / kKChunk, k = i - dd * kKChunk;
+        s_x[dd][k] = (d0 + dd < D && k < kn) ? delta[size_t(d0 + dd) * K + k0 + k] : 0.0;
+    }
+    __syncthreads();
+    for (int32_t n = threadIdx.x; n < N; n += 256) {
+        double acc[kDirTile];
+#pragma unroll
+        for (int j = 0; j < kDirTile; ++j) acc[j] = 0.0;
+        for (int32_t k = 0; k < kn; ++k) {
+            const double b = AdT[size_t(k0 + k) * N + n];
+#pragma unroll
+            for (int j = 0; j < kDirTile; ++j) acc[j] = fma(s_x[j][k], b, acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kDirTile; ++j)
+            if (d0 + j < D) partial[(size_t(chunk) * D + d0 + j) * N + n] = acc[j];
+    }
+}
+
+__global__ void gemm_reduce(const double* __restrict__ partial, int32_t n_chunks, uint32_t D, int32_t N,
+                            const double* __restrict__ cal1, double* __restrict__ cal) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(D) * N) return;
+    double s = 0.0;
+    for (int32_t c = 0; c < n_chunks; ++c) s += partial[size_t(c) * D * N + i];
+    cal[i] = s + cal1[i];
+}
+
+// ---- split-3 x BF16 -------------------------------------------------------
+__device__ inline void split3(double x, __bf16& hi, __bf16& mid, __bf16& lo) {
+    hi = __bf16(float(x));
+    const double r1 = x - double(float(hi));
+    mid = __bf16(float(r1));
+    const double r2 = r1 - double(float(mid));
+    lo = __bf16(float(r2));
+}
+
+// Planes of a row-major M x K matrix X (the A operand, rows = output rows):
+// out[p][mt][ks][lane][j] = split_p(X[mt*16 + lane%16][ks*32 + (lane/16)*8 + j]).
+__global__ void split_a_planes(const double* __restrict__ X, uint32_t M, int32_t K, uint32_t Mt, int32_t Ks,
+                               __bf16* __restrict__ out) {
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;  // one (mt, ks, lane)
+    const size_t total = size_t(Mt) * Ks * 64;
+    if (idx >= total) return;
+    const uint32_t lane = idx % 64;
+    const size_t t = idx / 64;
+    const int32_t ks = int32_t(t % Ks);
+    const uint32_t mt = uint32_t(t / Ks);
+    const uint32_t m = mt * 16 + lane % 16;
+    const int32_t kb = ks * 32 + int32_t(lane / 16) * 8;
+    for (int j = 0; j < 8; ++j) {
+        const int32_t k = kb + j;
+        const double x = (m < M && k < K) ? X[size_t(m) * K + k] : 0.0;
+        __bf16 h, md, l;
+        split3(x, h, md, l);
+        out[(0 * total + idx) * 8 + j] = h;
+        out[(1 * total + idx) * 8 + j] = md;
+        out[(2 * total + idx) * 8 + j] = l;
+    }
+}
+
+// Planes of a row-major K x N matrix B (the B operand, columns = output columns):
+// out[p][nt][ks][lane][j] = split_p(B[ks*32 + (lane/16)*8 + j][nt*16 + lane%16]).
+__global__ void split_b_planes(const double* __restrict__ B, int32_t K, int32_t N, int32_t Nt, int32_t Ks,
+                               __bf16* __restrict__ out) {
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t total = size_t(Nt) * Ks * 64;
+    if (idx >= total) return;
+    const uint32_t lane = idx % 64;
+    const size_t t = idx / 64;
+    const int32_t ks = int32_t(t % Ks);
+    const int32_t nt = int32_t(t / Ks);
+    const int32_t n = nt * 16 + int32_t(lane % 16);
+    const int32_t kb = ks * 32 + int32_t(lane / 16) * 8;
+    for (int j = 0; j < 8; ++j) {
+        const int32_t k = kb + j;
+        const double x = (n < N && k < K) ? B[size_t(k) * N + n] : 0.0;
+        __bf16 h, md, l;
+        split3(x, h, md, l);
+        out[(0 * total + idx) * 8 + j] = h;
+        out[(1 * total + idx) * 8 + j] = md;
+        out[(2 * total + idx) * 8 + j] = l;
+    }
+}
+
+// One wave per (16-row m-tile, 16-column n-tile, K-split); 4 waves per
+// workgroup take 4 consecutive m-tiles of one n-tile and K-split.
+// partial[split][m][n] (FP32) for m < Mt*16, n < Nt*16.
+__global__ __launch_bounds__(256) void gemm_bf16x3_mfma(const bf16x8* __restrict__ Ap, const bf16x8* __restrict__ Bp,
+                                                       uint32_t Mt, int32_t Nt, int32_t Ks, int32_t ks_per_split,
+                                                       float* __restrict__ partial) {
+    const uint32_t lane = threadIdx.x % 64;
+    const uint32_t wave = threadIdx.x / 64;
+    const int32_t nt = blockIdx.x;
+    const uint32_t mt = blockIdx.y * 4 + wave;
+    const int32_t split = blockIdx.z;
+    if (mt >= Mt) return;  // whole wave exits together
+    const size_t a_total = size_t(Mt) * Ks * 64;
+    const size_t b_total = size_t(Nt) * Ks * 64;
+    const int32_t ks0 = split * ks_per_split;
+    const int32_t ks1 = min(Ks, ks0 + ks_per_split);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int32_t ks = ks0; ks < ks1; ++ks) {
+        const size_t ai = (size_t(mt) * Ks + ks) * 64 + lane;
+        const size_t bi = (size_t(nt) * Ks + ks) * 64 + lane;
+        const bf16x8 ah = Ap[ai], am = Ap[a_total + ai], al = Ap[2 * a_total + ai];
+        const bf16x8 bh = Bp[bi], bm = Bp[b_total + bi], bl = Bp[2 * b_total + bi];
+        // smallest terms first
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+    }
+    // D layout: lane holds column lane%16, rows (lane/16)*4 + i.
+    const uint32_t M16 = Mt * 16;
+    const uint32_t N16 = uint32_t(Nt) * 16;
+    const uint32_t col = uint32_t(nt) * 16 + lane % 16;
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t row = mt * 16 + (lane / 16) * 4 + i;
+        partial[(size_t(split) * M16 + row) * N16 + col] = acc[i];
+    }
+}
+
+__global__ void mfma_reduce(const float* __restrict__ partial, int32_t n_split, uint32_t D, int32_t N, uint32_t M16,
+                            uint32_t N16, const double* __restrict__ cal1, double* __restrict__ cal) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(D) * N) return;
+    const uint32_t d = uint32_t(i / N), n = uint32_t(i % N);
+    double s = 0.0;
+    for (int32_t c = 0; c < n_split; ++c) s += double(partial[(size_t(c) * M16 + d) * N16 + n]);
+    cal[i] = s + cal1[i];
+}
+
+}  // namespace
+
+size_t gemm_f64_partial_doubles(int32_t K, int32_t N, uint32_t D) {
+    return size_t((K + kKChunk - 1) / kKChunk) * D * N;
+}
+
+void launch_gemm_f64(hipStream_t s, const double* AdT, int32_t K, int32_t N, const double* delta, uint32_t D,
+                     const double* cal1, double* partial, double* cal) {
+    const int32_t chunks = (K + kKChunk - 1) / kKChunk;
+    gemm_f64_partial<<<dim3(chunks, (D + kDirTile - 1) / kDirTile), 256, 0, s>>>(AdT, K, N, delta, D, partial);
+    LEAP_HIP(hipGetLastError());
+    const size_t n = size_t(D) * N;
+    gemm_reduce<<<dim3(uint32_t((n + 255) / 256)), 256, 0, s>>>(partial, chunks, D, N, cal1, cal);
+    LEAP_HIP(hipGetLastError());
+}
+
+size_t bf16_planes_bytes(uint32_t rows, int32_t K) {
+    const size_t tiles = (rows + 15) / 16;
+    const size_t ks = size_t((K + 31) / 32);
+    return 3 * tiles * ks * 64 * 8 * sizeof(__bf16);
+}
+
+void launch_split_b(hipStream_t s, const double* B, int32_t K, int32_t N, void* planes) {
+    const int32_t Nt = (N + 15) / 16, Ks = (K + 31) / 32;
+    const size_t total = size_t(Nt) * Ks * 64;
+    split_b_planes<<<dim3(uint32_t((total + 255) / 256)), 256, 0, s>>>(B, K, N, Nt, Ks,
+                                                                       static_cast<__bf16*>(planes));
+    LEAP_HIP(hipGetLastError());
+}
+
+int32_t mfma_n_split(int32_t K, uint32_t D, int32_t N) {
+    const int32_t Ks = (K + 31) / 32;
+    const int64_t tiles = int64_t((D + 63) / 64) * ((N + 15) / 16);
+    int32_t split = int32_t(std::max<int64_t>(1, 2048 / std::max<int64_t>(1, tiles)));
+    return std::min(split, Ks);
+}
+
+size_t mfma_partial_floats(int32_t K, uint32_t D, int32_t N) {
+    const uint32_t Mt = (D + 63) / 64 * 4;
+    const int32_t Nt = (N + 15) / 16;
+    return size_t(mfma_n_split(K, D, N)) * Mt * 16 * size_t(Nt) * 16;
+}
+
+void launch_gemm_bf16x3(hipStream_t s, const void* b_planes, int32_t K, int32_t N, const double* delta, uint32_t D,
+                        void* a_planes, const double* cal1, float* partial, double* cal) {
+    const uint32_t Mt = (D + 63) / 64 * 4;  // whole 4-wave groups
+    const int32_t Nt = (N + 15) / 16, Ks = (K + 31) / 32;
+    const size_t a_total = size_t(Mt) * Ks * 64;
+    split_a_planes<<<dim3(uint32_t((a_total + 255) / 256)), 256, 0, s>>>(delta, D, K, Mt, Ks,
+                                                                         static_cast<__bf16*>(a_planes));
+    LEAP_HIP(hipGetLastError());
+    const int32_t n_split = mfma_n_split(K, D, N);
+    const int32_t ks_per = (Ks + n_split - 1) / n_split;
+    gemm_bf16x3_mfma<<<dim3(Nt, Mt / 4, n_split), 256, 0, s>>>(static_cast<const bf16x8*>(a_planes),
+                                                               static_cast<const bf16x8*>(b_planes), Mt, Nt, Ks,
+                                                               ks_per, partial);
+    LEAP_HIP(hipGetLastError());
+    const size_t n = size_t(D) * N;
+    mfma_reduce<<<dim3(uint32_t((n + 255) / 256)), 256, 0, s>>>(partial, n_split, D, N, Mt * 16, uint32_t(Nt) * 16,
+                                                                cal1, cal);
+    LEAP_HIP(hipGetLastError());
+}
+
+}  // namespace leap

@@ -93,6 +93,7 @@ struct leap_metadata {
     leap::DeviceBuffer d_pairs;    // K - 1 int2: (antenna1, antenna2) of baseline I(k)
     leap::DeviceBuffer d_lambda;   // channels doubles: wavelength of each channel
     leap::DeviceBuffer d_Gp;       // scratch for Ad assembly
+    leap::DeviceBuffer d_AdT_planes;  // split-3 x BF16 planes of AdT in MFMA operand layout
 };
 
 struct TuneKey {
@@ -118,7 +119,7 @@ struct leap_ctx {
     int32_t last_rotate_launches = 0;
     // workspace
     leap::DeviceBuffer d_uvw, d_vis_stage, d_dirs, d_dd, d_avg, d_phase, d_cal1, d_delta, d_cal,
-        d_partial;
+        d_partial, d_a_planes;
     std::vector<double> h_avg, h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;
@@ -143,6 +144,17 @@ void launch_rotate_batched(hipStream_t s, int32_t precision, int32_t direction_b
                            uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,
                            double df, double* avg, int32_t fp32_blocks);
 size_t inversion_partial_doubles(int32_t K, int32_t N, uint32_t D);
+// Batched inversion pieces: cal1 / delta for D directions, then one GEMM.
+void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D, double* phase,
+                           double* cal1, double* delta);
+size_t gemm_f64_partial_doubles(int32_t K, int32_t N, uint32_t D);
+void launch_gemm_f64(hipStream_t s, const double* AdT, int32_t K, int32_t N, const double* delta, uint32_t D,
+                     const double* cal1, double* partial, double* cal);
+size_t bf16_planes_bytes(uint32_t rows, int32_t K);
+void launch_split_b(hipStream_t s, const double* B, int32_t K, int32_t N, void* planes);
+size_t mfma_partial_floats(int32_t K, uint32_t D, int32_t N);
+void launch_gemm_bf16x3(hipStream_t s, const void* b_planes, int32_t K, int32_t N, const double* delta, uint32_t D,
+                        void* a_planes, const double* cal1, float* partial, double* cal);
 void launch_synth_vis(hipStream_t s, const leap_synth_ms* ms, int32_t vis_dtype, void* out,
                       const double* d_uvw);
 }  // namespace leap

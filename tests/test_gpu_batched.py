@@ -145,3 +145,21 @@ def test_mixed_requires_batched(leap, ctx, small):
     with pytest.raises(leap.LeapError, match="direction block"):
         ctx.calibrate(view, [(0.5, -0.47)], opts=leap.options(strategy=leap.LEAP_STRATEGY_BATCHED,
                                                                direction_block=48))
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 2e-6)], ids=["fp64-gemm", "bf16x3-mfma"])
+@pytest.mark.parametrize("n_dirs", [1, 16, 70])
+def test_batched_inversion_gemm(leap, ctx, oracle, precision, tol, n_dirs):
+    """cal = Ad * dphi + cal1 as one GEMM (FP64) or split-3 x BF16 MFMA (mixed) against the oracle."""
+    s = leap.SynthMS(n_antennas=61, autocorrelations=0, n_timesteps=2, n_channels=4, flag_fraction=0.03, seed=40)
+    s.fill_host()
+    ms = s.host_dict()
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view, ref_ant=3)
+    omd = oracle.metadata(ms, ref_ant=3)
+    rng = np.random.default_rng(n_dirs)
+    avg = rng.standard_normal((n_dirs, ms["num_baselines"], 4)) + 1j * rng.standard_normal((n_dirs, ms["num_baselines"], 4))
+    got = md.apply_inversion(avg, opts=leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED))
+    err = max(wrapped(got[d], omd.apply_inversion(avg[d])).max() for d in range(n_dirs))
+    print(f"inversion precision={precision} D={n_dirs}: max wrapped error {err:.2e} rad")
+    assert err < tol
