@@ -1,0 +1,53 @@
+// icrar::Slice / icrar::Range: solution intervals over timesteps
+// (src/icrar/leap-accelerate/common/Slice.h:36-55, Slice.cc:28-63, Range.h:34-64).
+// Same constructors, wildcard (-1) semantics and validation messages.
+#pragma once
+
+#include <cstdint>
+
+#include "icrar/leap-accelerate/exception/exception.h"
+
+namespace icrar {
+
+struct Range {
+    std::int32_t start, interval, end;
+    Range(int s, int i, int e) {
+        if (s < 0) throw icrar::exception("expected a positive integer", __FILE__, __LINE__);
+        if (i < 1) throw icrar::exception("expected a positive integer", __FILE__, __LINE__);
+        if (e < 0) throw icrar::exception("expected a positive integer", __FILE__, __LINE__);
+        start = s;
+        interval = i;
+        end = e;
+    }
+    int GetSize() const { return (end - start) / interval; }
+};
+
+struct Slice {
+    std::int32_t start = 0, interval = -1, end = -1;
+
+    Slice() = default;
+    explicit Slice(int interval_) : Slice(0, interval_, -1) {}
+    Slice(int start_, int end_) : Slice(start_, end_ == -1 ? -1 : end_ - start_, end_) {}
+    Slice(int start_, int interval_, int end_) {
+        if (start_ < 0) throw icrar::exception("expected a positive integer", __FILE__, __LINE__);
+        if (interval_ < -1) throw icrar::exception("expected a positive integer or -1", __FILE__, __LINE__);
+        if (interval_ == 0) throw icrar::exception("expected a non zero integer", __FILE__, __LINE__);
+        if (end_ < -1) throw icrar::exception("expected a positive integer or -1", __FILE__, __LINE__);
+        if (end_ != -1 && start_ >= end_)
+            throw icrar::exception("range start must be greater than end", __FILE__, __LINE__);
+        if (end_ != -1) {
+            if (interval_ == -1) interval_ = end_ - start_;
+            if (interval_ > end_ - start_) throw icrar::exception("range increment out of bounds", __FILE__, __LINE__);
+        }
+        start = start_;
+        interval = interval_;
+        end = end_;
+    }
+
+    Range Evaluate(int collectionSize) const {
+        return Range(start == -1 ? collectionSize : start, interval == -1 ? collectionSize : interval,
+                     end == -1 ? collectionSize : end);
+    }
+};
+
+}  // namespace icrar

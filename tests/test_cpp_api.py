@@ -1,0 +1,60 @@
+"""The C++ API (icrar::ILeapCalibrator / LeapCalibratorFactory / hip::HipLeapCalibrator
+and the CalibrationCollection JSON), driven by icrar-leap-accelerate_amd/build/hip_calibrator_main."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import np_restatement as npr
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MAIN = os.path.join(REPO, "icrar-leap-accelerate_amd", "build", "hip_calibrator_main")
+
+
+def test_cpp_selftest():
+    """JSON number format, Slice rules and factory errors; host only."""
+    assert os.path.exists(MAIN), "build the library first"
+    out = subprocess.run([MAIN, "selftest"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr + out.stdout
+    assert "selftest ok" in out.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slice_,ref,dtype,mode", [
+    ((0, -1, -1), 0, "c128", "plain"),
+    ((0, 1, -1), -1, "c64", "plain"),
+    ((0, 2, 4), 5, "c64", "batched"),
+    ((0, -1, -1), -1, "c64", "mixed"),
+])
+def test_cpp_calibrate_matches_oracle(leap, oracle, tmp_path, slice_, ref, dtype, mode):
+    ant, T, C, seed, ndir = 24, 4, 16, 77, 3
+    vis_file = tmp_path / "vis.bin"
+    cmd = [MAIN, "calibrate", str(ant), str(T), str(C), str(seed), str(ndir), *map(str, slice_), str(ref), dtype, mode,
+           str(vis_file)]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    result = json.loads(out.stdout)
+    s = leap.SynthMS(n_antennas=ant, n_timesteps=T, n_channels=C, seed=seed)
+    vis = np.fromfile(vis_file, dtype=np.complex64 if dtype == "c64" else np.complex128).reshape(s.num_rows, C, 4)
+    ms = s.host_dict(vis)
+    dirs = np.array([b["direction"] for b in result[0]["calibration"]])
+    want, epochs = oracle.calibrate(ms, dirs, slice_, ref_ant=None if ref < 0 else ref)
+    assert len(result) == len(want)
+    tol = 1e-5 if mode == "mixed" else 1e-9
+    for s_i, sol in enumerate(result):
+        assert sol["epoch"]["start"] == epochs[s_i, 0] and sol["epoch"]["end"] == epochs[s_i, 1]
+        for d, beam in enumerate(sol["calibration"]):
+            got = np.array(beam["beamCalibration"])
+            assert got.shape == want[s_i, d].shape
+            assert np.abs(npr.wrap(got - want[s_i, d])).max() < tol
+
+
+@pytest.mark.gpu
+def test_cpp_errors_are_exceptions(tmp_path):
+    # refAnt beyond the baseline count -> icrar::invalid_argument_exception, exit code 1, message kept
+    out = subprocess.run([MAIN, "calibrate", "6", "2", "4", "1", "1", "0", "-1", "-1", "99", "c64", "plain"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 1
+    assert "refAnt 99 is out of bounds" in out.stderr
