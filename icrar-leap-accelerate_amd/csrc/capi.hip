@@ -672,3 +672,212 @@ int leap_phase_matrix_pinv(const int32_t* a1, const int32_t* a2, const uint8_t* 
 }  // extern "C"
 
 extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ? ctx->last_direction_block : 0; }
+
+// ---------------------------------------------------------------------------
+// Calibration-residual loss and its gradient.
+// ---------------------------------------------------------------------------
+namespace {
+
+using cplx = std::complex<double>;
+
+double wrap_pi(double x) {
+    double r = std::fmod(x + kPi, 2.0 * kPi);
+    if (r < 0) r += 2.0 * kPi;
+    return r - kPi;
+}
+
+// Row 2 of the DD matrix and its derivatives in ra and dec (MetaData.cc:166-203 differentiated).
+void dz_derivs(double ra0, double dec0, double ra, double dec, double dra[3], double ddec[3]) {
+    const double s1 = std::sin(kPi / 2.0 - dec0), c1 = std::cos(kPi / 2.0 - dec0);
+    const double s2 = std::sin(ra - ra0), c2 = std::cos(ra - ra0);
+    const double s3 = std::sin(-kPi / 2.0 + dec), c3 = std::cos(-kPi / 2.0 + dec);
+    dra[0] = -s3 * c2;
+    dra[1] = -s3 * s2 * c1;
+    dra[2] = s3 * s2 * s1;
+    ddec[0] = -c3 * s2;
+    ddec[1] = c3 * c2 * c1 - s3 * s1;
+    ddec[2] = -c3 * c2 * s1 - s3 * c1;
+}
+
+// Ad(i, k) from the Gram pseudo-inverse: column k of pinv(A) = G+ (A row k)^T.
+struct AdView {
+    const leap::PhaseMatrix& pm;
+    const std::vector<double>& Gp;
+    int32_t N;
+    const double* col_a(int32_t k) const {
+        return Gp.data() + size_t(k + 1 < pm.rows ? pm.ant1[k] : pm.constraint) * N;
+    }
+    const double* col_b(int32_t k) const { return k + 1 < pm.rows ? Gp.data() + size_t(pm.ant2[k]) * N : nullptr; }
+    double at(int32_t i, int32_t k) const {
+        const double* b = col_b(k);
+        return col_a(k)[i] - (b ? b[i] : 0.0);
+    }
+};
+
+// Forward inversion, loss and its adjoint for one direction (pol 0 of avgData).
+// Returns the loss; abar[b] receives dL/dRe avg + i dL/dIm avg.
+double inversion_loss_adjoint(const leap_metadata& md, const cplx* avg, cplx* abar) {
+    const leap::PhaseMatrix& A = md.A;
+    const leap::PhaseMatrix& A1 = md.A1;
+    const int32_t N = A.cols, K = A.rows, K1 = A1.rows, nI = K - 1;
+    const AdView Ad{A, md.Gp, N}, Ad1{A1, md.G1p, N};
+    std::vector<double> phi1(size_t(K1), 0.0), cal1(size_t(N), 0.0);
+    for (int32_t k = 0; k + 1 < K1; ++k) phi1[k] = std::arg(avg[A1.I[k]]);
+    for (int32_t k = 0; k < K1; ++k)
+        for (int32_t i = 0; i < N; ++i) cal1[i] += Ad1.at(i, k) * phi1[k];
+    std::vector<double> dphi(size_t(K), 0.0), cal(cal1);
+    for (int32_t n = 0; n < nI; ++n) {
+        const double acal1 = cal1[A.ant1[n]] - cal1[A.ant2[n]];
+        dphi[n] = std::arg(std::exp(cplx(0.0, -kTwoPi * acal1)) * avg[n]);  // row n of avgData
+    }
+    for (int32_t k = 0; k < K; ++k)
+        for (int32_t i = 0; i < N; ++i) cal[i] += Ad.at(i, k) * dphi[k];
+    double loss = 0.0;
+    std::vector<double> res(static_cast<size_t>(nI));
+    for (int32_t n = 0; n < nI; ++n) {
+        res[n] = wrap_pi(cal[A.ant1[n]] - cal[A.ant2[n]] - std::arg(avg[A.I[n]]));
+        loss += res[n] * res[n];
+    }
+    // adjoint
+    const uint32_t nbl = md.nbl;
+    std::vector<double> g_arg(nbl, 0.0), g_cal(size_t(N), 0.0);
+    for (int32_t n = 0; n < nI; ++n) {
+        g_arg[A.I[n]] -= 2.0 * res[n];
+        g_cal[A.ant1[n]] += 2.0 * res[n];
+        g_cal[A.ant2[n]] -= 2.0 * res[n];
+    }
+    std::vector<double> g_cal1(g_cal);
+    for (int32_t n = 0; n < nI; ++n) {
+        double g = 0.0;  // (Ad^T g_cal)[n]
+        for (int32_t i = 0; i < N; ++i) g += Ad.at(i, n) * g_cal[i];
+        g_arg[n] += g;
+        const double g_acal1 = -kTwoPi * g;
+        g_cal1[A.ant1[n]] += g_acal1;
+        g_cal1[A.ant2[n]] -= g_acal1;
+    }
+    for (int32_t k = 0; k + 1 < K1; ++k) {
+        double g = 0.0;
+        for (int32_t i = 0; i < N; ++i) g += Ad1.at(i, k) * g_cal1[i];
+        g_arg[A1.I[k]] += g;
+    }
+    for (uint32_t b = 0; b < nbl; ++b)
+        abar[b] = (avg[b] == cplx(0, 0)) ? cplx(0, 0) : cplx(0.0, g_arg[b]) / std::conj(avg[b]);
+    return loss;
+}
+
+}  // namespace
+
+extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
+                                           const double* directions, const double* offsets, uint32_t D,
+                                           const double* antenna_delta, const double* enu_to_uvw, double* loss_out,
+                                           double* grad_offsets, double* grad_antenna) {
+    return guarded(ctx, [&] {
+        require(ctx && md && ms && directions && enu_to_uvw && loss_out, LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        require(D >= 1 && D <= 256, LEAP_ERR_NOT_IMPLEMENTED, "the adjoint handles 1 to 256 directions per call");
+        validate_view(*ms);
+        check_metadata_matches(*md, *ms);
+        DeviceScope scope(ctx->device);
+        hipStream_t s = ctx->stream;
+        const uint32_t nbl = ms->num_baselines, R = ms->num_rows, T = R / nbl;
+        const int32_t N = md->A.cols;
+        std::vector<double> dirs(size_t(D) * 2);
+        for (uint32_t d = 0; d < 2 * D; ++d) dirs[d] = directions[d] + (offsets ? offsets[d] : 0.0);
+        upload_directions(ctx, *md, dirs.data(), D);
+        const DeviceRows rows = stage_rows(ctx, *ms, 0, R);
+        // perturbed uvw
+        DeviceBuffer d_M, d_ants, d_dp, d_uvwp;
+        d_M.reserve(size_t(T) * 9 * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(d_M.ptr, enu_to_uvw, size_t(T) * 9 * sizeof(double), hipMemcpyHostToDevice, s));
+        const double* uvw = rows.uvw;
+        if (antenna_delta) {
+            std::vector<int2> ants(nbl);
+            for (uint32_t b = 0; b < nbl; ++b) ants[b] = make_int2(ms->antenna1[b], ms->antenna2[b]);
+            d_ants.reserve(nbl * sizeof(int2));
+            d_dp.reserve(size_t(N) * 3 * sizeof(double));
+            d_uvwp.reserve(size_t(R) * 3 * sizeof(double));
+            LEAP_HIP(hipMemcpyAsync(d_ants.ptr, ants.data(), nbl * sizeof(int2), hipMemcpyHostToDevice, s));
+            LEAP_HIP(hipMemcpyAsync(d_dp.ptr, antenna_delta, size_t(N) * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+            launch_perturb_uvw(s, rows.uvw, d_M.as<double>(), d_ants.as<int2>(), d_dp.as<double>(), nbl, R,
+                               d_uvwp.as<double>());
+            uvw = d_uvwp.as<double>();
+        }
+        // forward avgData (pol 0), FP64
+        const size_t per_dir = size_t(nbl) * md->pols * 2;
+        ctx->d_avg.reserve(per_dir * D * sizeof(double));
+        const int32_t tile = D <= 32 ? 321 : D <= 64 ? 641 : D <= 128 ? 1282 : 2562;
+        launch_rotate_batched(s, LEAP_PRECISION_FP64, tile, false, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(),
+                              D, T, nbl, nbl, md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
+                              ctx->d_avg.as<double>(), 0);
+        std::vector<double> h_avg(per_dir * D);
+        LEAP_HIP(hipMemcpyAsync(h_avg.data(), ctx->d_avg.ptr, h_avg.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        LEAP_HIP(hipStreamSynchronize(s));
+        // inversion, loss and its adjoint on the host (O(D K N))
+        std::vector<double> h_abar(size_t(D) * nbl * 2);
+        std::vector<cplx> avg0(nbl), ab(nbl);
+        double loss = 0.0;
+        for (uint32_t d = 0; d < D; ++d) {
+            for (uint32_t b = 0; b < nbl; ++b)
+                avg0[b] = cplx(h_avg[d * per_dir + b * md->pols * 2], h_avg[d * per_dir + b * md->pols * 2 + 1]);
+            loss += inversion_loss_adjoint(*md, avg0.data(), ab.data());
+            for (uint32_t b = 0; b < nbl; ++b) {
+                h_abar[2 * (size_t(d) * nbl + b)] = ab[b].real();
+                h_abar[2 * (size_t(d) * nbl + b) + 1] = ab[b].imag();
+            }
+        }
+        *loss_out = loss;
+        if (!grad_offsets && !grad_antenna) return;
+        // adjoint of rotate -> average on the device
+        DeviceBuffer d_abar, d_guvw, d_gdzp, d_gdz;
+        d_abar.reserve(h_abar.size() * sizeof(double));
+        d_guvw.reserve(size_t(R) * 3 * sizeof(double));
+        d_gdzp.reserve(size_t(nbl) * D * 3 * sizeof(double));
+        d_gdz.reserve(size_t(D) * 3 * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(d_abar.ptr, h_abar.data(), h_abar.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_rotate_adjoint(s, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(), d_abar.as<double>(), D, T, nbl,
+                              md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz, d_guvw.as<double>(),
+                              d_gdzp.as<double>());
+        launch_reduce_gdz(s, d_gdzp.as<double>(), nbl, D, d_gdz.as<double>());
+        std::vector<double> g_dz(size_t(D) * 3);
+        LEAP_HIP(hipMemcpyAsync(g_dz.data(), d_gdz.ptr, g_dz.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (grad_antenna) {
+            // baselines of each antenna: +1 where it is antenna2, -1 where antenna1
+            std::vector<int32_t> off(size_t(N) + 1, 0), bls, sgn;
+            std::vector<std::vector<std::pair<int32_t, int32_t>>> lists(static_cast<size_t>(N));
+            for (uint32_t b = 0; b < nbl; ++b) {
+                if (ms->antenna1[b] == ms->antenna2[b]) continue;  // uvw of an autocorrelation is 0
+                lists[ms->antenna2[b]].emplace_back(int32_t(b), 1);
+                lists[ms->antenna1[b]].emplace_back(int32_t(b), -1);
+            }
+            for (int32_t a = 0; a < N; ++a) {
+                off[a + 1] = off[a] + int32_t(lists[a].size());
+                for (auto& e : lists[a]) {
+                    bls.push_back(e.first);
+                    sgn.push_back(e.second);
+                }
+            }
+            DeviceBuffer d_off, d_bls, d_sgn, d_gant;
+            d_off.reserve(off.size() * sizeof(int32_t));
+            d_bls.reserve(std::max<size_t>(1, bls.size()) * sizeof(int32_t));
+            d_sgn.reserve(std::max<size_t>(1, sgn.size()) * sizeof(int32_t));
+            d_gant.reserve(size_t(N) * 3 * sizeof(double));
+            LEAP_HIP(hipMemcpyAsync(d_off.ptr, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            if (!bls.empty()) {
+                LEAP_HIP(hipMemcpyAsync(d_bls.ptr, bls.data(), bls.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+                LEAP_HIP(hipMemcpyAsync(d_sgn.ptr, sgn.data(), sgn.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+            }
+            launch_antenna_grad(s, d_guvw.as<double>(), d_M.as<double>(), d_off.as<int32_t>(), d_bls.as<int32_t>(),
+                                d_sgn.as<int32_t>(), N, nbl, T, d_gant.as<double>());
+            LEAP_HIP(hipMemcpyAsync(grad_antenna, d_gant.ptr, size_t(N) * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+            LEAP_HIP(hipStreamSynchronize(s));
+        }
+        LEAP_HIP(hipStreamSynchronize(s));
+        if (grad_offsets) {
+            for (uint32_t d = 0; d < D; ++d) {
+                double dra[3], ddec[3];
+                dz_derivs(md->phase_centre_ra, md->phase_centre_dec, dirs[2 * d], dirs[2 * d + 1], dra, ddec);
+                grad_offsets[2 * d] = g_dz[3 * d] * dra[0] + g_dz[3 * d + 1] * dra[1] + g_dz[3 * d + 2] * dra[2];
+                grad_offsets[2 * d + 1] = g_dz[3 * d] * ddec[0] + g_dz[3 * d + 1] * ddec[1] + g_dz[3 * d + 2] * ddec[2];
+            }
+        }
+    });
+}

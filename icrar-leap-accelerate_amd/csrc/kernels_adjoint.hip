@@ -1,0 +1,244 @@
+// Adjoint of rotate -> average for the calibration-residual gradient.
+//
+// Forward (per row r = (t, baseline), direction d, pol 0):
+//   sf(r, d)  = -2 pi ((dz_d . uvw(r)) - w(r))
+//   avg(b, d) = sum_{t, c} v(r, c) exp(i sf kappa_c),   kappa_c = kappa0 + c dkappa
+// Given abar(b, d) = dL/dRe avg + i dL/dIm avg (so dL = Re(conj(abar) d avg)):
+//   dL/dsf(r, d) = Re(conj(abar) * i * sum_c kappa_c v_c exp(i sf kappa_c))
+//                = Re(conj(abar) * i * e0 * (kappa0 P(z) + dkappa z P'(z)))
+// with e0 = exp(i sf kappa0), z = exp(i sf dkappa), P(z) = sum_c v_c z^c.  P and P'
+// are evaluated together by Horner's rule (p' <- p' z + p; p <- p z + v), so the
+// phasors are recomputed rather than stored.  Then
+//   dL/d dz_d   += -2 pi dL/dsf(r, d) uvw(r)            (per baseline, reduced later)
+//   dL/d uvw(r)  = sum_d -2 pi dL/dsf(r, d) (dz_d - e_z)
+// and the antenna gradient is dL/d dp_a = sum_t M_t^T (sum_{b: a2 = a} g_uvw - sum_{b: a1 = a} g_uvw).
+#include "leap_internal.h"
+
+namespace leap {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kCC = 256;  // channels per LDS chunk
+
+struct Cx {
+    double x, y;
+};
+
+__device__ inline Cx cmulx(Cx a, Cx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline Cx hstep(Cx a, Cx z, Cx v) {
+    return {fma(a.x, z.x, fma(-a.y, z.y, v.x)), fma(a.x, z.y, fma(a.y, z.x, v.y))};
+}
+__device__ inline Cx expi(double t) {
+    double s, c;
+    sincos(t, &s, &c);
+    return {c, s};
+}
+
+// uvw'(row) = uvw0(row) + M_t (dp[a2] - dp[a1])
+__global__ void perturb_uvw_kernel(const double* __restrict__ uvw0, const double* __restrict__ M,
+                                   const int2* __restrict__ bl_ants, const double* __restrict__ dp, uint32_t nbl,
+                                   uint32_t rows, double* __restrict__ uvw) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const uint32_t t = r / nbl, b = r - t * nbl;
+    const int2 a = bl_ants[b];
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = dp[3 * a.y + k] - dp[3 * a.x + k];
+    for (int i = 0; i < 3; ++i) {
+        double s = uvw0[3 * size_t(r) + i];
+        for (int k = 0; k < 3; ++k) s += M[9 * size_t(t) + 3 * i + k] * d[k];
+        uvw[3 * size_t(r) + i] = s;
+    }
+}
+
+// One workgroup per baseline, one lane per direction (D <= 256), 256/DL rows side by side.
+template <int DL>
+__global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
+    const void* __restrict__ vis, int32_t c64, const double* __restrict__ uvw, const double* __restrict__ dd,
+    const double* __restrict__ abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0,
+    double dkappa, double* __restrict__ g_uvw, double* __restrict__ g_dz_partial) {
+    constexpr int RP = kThreads / DL;
+    __shared__ Cx s_v[RP][kCC];
+    __shared__ double s_g[RP][DL][3];
+    const uint32_t bl = blockIdx.x;
+    const uint32_t lane_d = threadIdx.x % DL;
+    const uint32_t rsub = threadIdx.x / DL;
+    const bool dir_ok = lane_d < D;
+    const uint32_t dc = dir_ok ? lane_d : 0;
+    const double dz0 = dd[9 * dc + 6], dz1 = dd[9 * dc + 7], dz2 = dd[9 * dc + 8];
+    Cx ab = {0.0, 0.0};
+    if (dir_ok) ab = {abar[2 * (size_t(dc) * nbl + bl)], abar[2 * (size_t(dc) * nbl + bl) + 1]};
+    double gdz[3] = {0.0, 0.0, 0.0};
+    const uint32_t n_chunks = (C + kCC - 1) / kCC;
+
+    for (uint32_t tb = 0; tb < T; tb += RP) {
+        const uint32_t t = tb + rsub;
+        const bool row_ok = t < T;
+        double u = 0.0, v = 0.0, w = 0.0;
+        if (row_ok) {
+            const size_t row = size_t(t) * nbl + bl;
+            u = uvw[3 * row];
+            v = uvw[3 * row + 1];
+            w = uvw[3 * row + 2];
+        }
+        const double sf = -kTwoPi * ((dz0 * u + dz1 * v + dz2 * w) - w);
+        const Cx z = expi(sf * dkappa);
+        Cx p = {0.0, 0.0}, dp = {0.0, 0.0};
+        for (int ch = int(n_chunks) - 1; ch >= 0; --ch) {
+            const uint32_t c0 = uint32_t(ch) * kCC;
+            const uint32_t cn = min(uint32_t(kCC), C - c0);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < RP * uint32_t(kCC); i += kThreads) {
+                const uint32_t r = i / kCC, c = i - r * kCC;
+                const uint32_t tt = tb + r;
+                Cx val = {0.0, 0.0};
+                if (tt < T && c < cn) {
+                    const size_t e = ((size_t(tt) * nbl + bl) * C + c0 + c) * P;
+                    bool finite = true;
+                    double x0 = 0.0, y0 = 0.0;
+                    for (uint32_t q = 0; q < P; ++q) {
+                        double xr, xi;
+                        if (c64) {
+                            xr = static_cast<const float*>(vis)[2 * (e + q)];
+                            xi = static_cast<const float*>(vis)[2 * (e + q) + 1];
+                        } else {
+                            xr = static_cast<const double*>(vis)[2 * (e + q)];
+                            xi = static_cast<const double*>(vis)[2 * (e + q) + 1];
+                        }
+                        finite = finite && __builtin_isfinite(xr) && __builtin_isfinite(xi);
+                        if (q == 0) {
+                            x0 = xr;
+                            y0 = xi;
+                        }
+                    }
+                    if (finite) val = {x0, y0};
+                }
+                s_v[r][c] = val;
+            }
+            __syncthreads();
+            if (!row_ok || !dir_ok) continue;
+            for (int k = int(cn) - 1; k >= 0; --k) {
+                dp = hstep(dp, z, p);  // P' <- P' z + P (uses P before its update)
+                p = hstep(p, z, s_v[rsub][k]);
+            }
+        }
+        double gsf = 0.0;
+        if (row_ok && dir_ok) {
+            const Cx e0 = expi(sf * kappa0);
+            const Cx zdp = cmulx(z, dp);
+            const Cx s1 = cmulx(e0, Cx{kappa0 * p.x + dkappa * zdp.x, kappa0 * p.y + dkappa * zdp.y});
+            // Re(conj(ab) * i * s1) with conj(ab) = (ab.x, -ab.y) and i * s1 = (-s1.y, s1.x)
+            gsf = -ab.x * s1.y + ab.y * s1.x;
+            gdz[0] += -kTwoPi * gsf * u;
+            gdz[1] += -kTwoPi * gsf * v;
+            gdz[2] += -kTwoPi * gsf * w;
+        }
+        // g_uvw(row) = sum over directions of -2 pi gsf (dz - e_z), reduced in LDS.
+        s_g[rsub][lane_d][0] = -kTwoPi * gsf * dz0;
+        s_g[rsub][lane_d][1] = -kTwoPi * gsf * dz1;
+        s_g[rsub][lane_d][2] = -kTwoPi * gsf * (dz2 - 1.0);
+        __syncthreads();
+        if (lane_d < 3 && row_ok) {
+            double s = 0.0;
+            for (int j = 0; j < DL; ++j) s += s_g[rsub][j][lane_d];
+            g_uvw[3 * (size_t(t) * nbl + bl) + lane_d] = s;
+        }
+    }
+    // Partials of this baseline's rows per rsub lane; the RP lanes are summed in order below.
+    __syncthreads();
+    s_g[rsub][lane_d][0] = dir_ok ? gdz[0] : 0.0;
+    s_g[rsub][lane_d][1] = dir_ok ? gdz[1] : 0.0;
+    s_g[rsub][lane_d][2] = dir_ok ? gdz[2] : 0.0;
+    __syncthreads();
+    if (rsub == 0 && dir_ok) {
+        for (int k = 0; k < 3; ++k) {
+            double s = 0.0;
+            for (int r = 0; r < RP; ++r) s += s_g[r][lane_d][k];
+            g_dz_partial[(size_t(bl) * D + lane_d) * 3 + k] = s;
+        }
+    }
+}
+
+// g_dz[d][k] = sum over baselines of g_dz_partial[b][d][k]
+__global__ void reduce_gdz_kernel(const double* __restrict__ part, uint32_t nbl, uint32_t D, double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D * 3) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nbl; ++b) s += part[size_t(b) * D * 3 + i];
+    out[i] = s;
+}
+
+// One workgroup per antenna: sum over its baselines (sign +1 as antenna2, -1 as
+// antenna1) and timesteps of M_t^T g_uvw(t, b).
+__global__ __launch_bounds__(256) void antenna_grad_kernel(const double* __restrict__ g_uvw,
+                                                           const double* __restrict__ M,
+                                                           const int32_t* __restrict__ csr_off,
+                                                           const int32_t* __restrict__ csr_bl,
+                                                           const int32_t* __restrict__ csr_sign, uint32_t nbl,
+                                                           uint32_t T, double* __restrict__ out) {
+    __shared__ double s_red[256][3];
+    const int32_t a = blockIdx.x;
+    const int32_t j0 = csr_off[a], j1 = csr_off[a + 1];
+    const uint32_t nj = uint32_t(j1 - j0);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (uint32_t i = threadIdx.x; i < nj * T; i += 256) {
+        const uint32_t t = i / nj, j = i - t * nj;
+        const int32_t b = csr_bl[j0 + j];
+        const double sgn = double(csr_sign[j0 + j]);
+        const double* g = g_uvw + 3 * (size_t(t) * nbl + b);
+        const double* m = M + 9 * size_t(t);
+        for (int k = 0; k < 3; ++k) acc[k] += sgn * (m[k] * g[0] + m[3 + k] * g[1] + m[6 + k] * g[2]);
+    }
+    for (int k = 0; k < 3; ++k) s_red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (int(threadIdx.x) < stride)
+            for (int k = 0; k < 3; ++k) s_red[threadIdx.x][k] += s_red[threadIdx.x + stride][k];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[3 * a + threadIdx.x] = s_red[0][threadIdx.x];
+}
+
+}  // namespace
+
+void launch_perturb_uvw(hipStream_t s, const double* uvw0, const double* M, const int2* bl_ants, const double* dp,
+                        uint32_t nbl, uint32_t rows, double* uvw) {
+    perturb_uvw_kernel<<<(rows + 255) / 256, 256, 0, s>>>(uvw0, M, bl_ants, dp, nbl, rows, uvw);
+    LEAP_HIP(hipGetLastError());
+}
+
+void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, const double* uvw, const double* dd,
+                           const double* abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
+                           double df, double* g_uvw, double* g_dz_partial) {
+    const double kappa0 = f0 / kSpeedOfLight, dkappa = df / kSpeedOfLight;
+    const int32_t c64 = vis_dtype == LEAP_VIS_C64 ? 1 : 0;
+    require(D >= 1 && D <= 256, LEAP_ERR_NOT_IMPLEMENTED, "the adjoint handles 1 to 256 directions per call");
+    const dim3 grid(nbl);
+    if (D <= 32)
+        rotate_adjoint_kernel<32><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, dkappa,
+                                                            g_uvw, g_dz_partial);
+    else if (D <= 64)
+        rotate_adjoint_kernel<64><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, dkappa,
+                                                            g_uvw, g_dz_partial);
+    else if (D <= 128)
+        rotate_adjoint_kernel<128><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0,
+                                                             dkappa, g_uvw, g_dz_partial);
+    else
+        rotate_adjoint_kernel<256><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0,
+                                                             dkappa, g_uvw, g_dz_partial);
+    LEAP_HIP(hipGetLastError());
+}
+
+void launch_reduce_gdz(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double* out) {
+    reduce_gdz_kernel<<<(D * 3 + 255) / 256, 256, 0, s>>>(part, nbl, D, out);
+    LEAP_HIP(hipGetLastError());
+}
+
+void launch_antenna_grad(hipStream_t s, const double* g_uvw, const double* M, const int32_t* csr_off,
+                         const int32_t* csr_bl, const int32_t* csr_sign, int32_t n_ant, uint32_t nbl, uint32_t T,
+                         double* out) {
+    antenna_grad_kernel<<<n_ant, 256, 0, s>>>(g_uvw, M, csr_off, csr_bl, csr_sign, nbl, T, out);
+    LEAP_HIP(hipGetLastError());
+}
+
+}  // namespace leap

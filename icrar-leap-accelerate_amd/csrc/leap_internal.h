@@ -155,6 +155,15 @@ void launch_split_b(hipStream_t s, const double* B, int32_t K, int32_t N, void* 
 size_t mfma_partial_floats(int32_t K, uint32_t D, int32_t N);
 void launch_gemm_bf16x3(hipStream_t s, const void* b_planes, int32_t K, int32_t N, const double* delta, uint32_t D,
                         void* a_planes, const double* cal1, float* partial, double* cal);
+void launch_perturb_uvw(hipStream_t s, const double* uvw0, const double* M, const int2* bl_ants, const double* dp,
+                        uint32_t nbl, uint32_t rows, double* uvw);
+void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, const double* uvw, const double* dd,
+                           const double* abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
+                           double df, double* g_uvw, double* g_dz_partial);
+void launch_reduce_gdz(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double* out);
+void launch_antenna_grad(hipStream_t s, const double* g_uvw, const double* M, const int32_t* csr_off,
+                         const int32_t* csr_bl, const int32_t* csr_sign, int32_t n_ant, uint32_t nbl, uint32_t T,
+                         double* out);
 void launch_synth_vis(hipStream_t s, const leap_synth_ms* ms, int32_t vis_dtype, void* out,
                       const double* d_uvw);
 }  // namespace leap

@@ -113,5 +113,6 @@ struct leap_synth_ms {
     std::vector<double> antenna_phase;  // [N]
     std::vector<double> bl_phase;       // [nbl]
     std::vector<double> sources;        // [n_src][3] ra, dec, flux
+    std::vector<double> enu_to_uvw;     // [T][3][3]: uvw = M_t * enu, per timestep
     leap::SynthParams params{};
 };

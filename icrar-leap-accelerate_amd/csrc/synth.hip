@@ -142,10 +142,23 @@ int leap_synth_create(const leap_synth_config* cfg_in, leap_synth_ms** out) {
         ms->uvw.resize(size_t(rows) * 3);
         ms->time.resize(size_t(rows));
         std::vector<double> auvw(size_t(N) * 3);
+        ms->enu_to_uvw.assign(size_t(cfg.n_timesteps) * 9, 0.0);
         const double h0 = -0.5 * kEarthRotation * cfg.integration_s * cfg.n_timesteps;
         for (uint32_t t = 0; t < cfg.n_timesteps; ++t) {
             const double H = h0 + kEarthRotation * cfg.integration_s * (t + 0.5);
             const double sh = std::sin(H), ch = std::cos(H), sd = std::sin(dec), cdd = std::cos(dec);
+            {
+                // M_t = W(H, dec) * E(lat): east-north-up metres to uvw metres.
+                const double W[9] = {sh, ch, 0.0, -sd * ch, sd * sh, cdd, cdd * ch, -cdd * sh, sd};
+                const double E[9] = {0.0, -std::sin(lat), std::cos(lat), 1.0, 0.0, 0.0, 0.0, std::cos(lat),
+                                     std::sin(lat)};
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        double v = 0.0;
+                        for (int k = 0; k < 3; ++k) v += W[3 * i + k] * E[3 * k + j];
+                        ms->enu_to_uvw[size_t(t) * 9 + 3 * i + j] = v;
+                    }
+            }
             for (uint32_t a = 0; a < N; ++a) {
                 const double X = xyz[3 * a], Y = xyz[3 * a + 1], Z = xyz[3 * a + 2];
                 auvw[3 * a] = sh * X + ch * Y;
@@ -221,6 +234,10 @@ void leap_synth_view(const leap_synth_ms* ms, leap_ms_view* v) {
     v->vis_dtype = LEAP_VIS_C128;
     v->vis_on_device = 0;
     v->uvw_device = nullptr;
+}
+
+void leap_synth_enu_to_uvw(const leap_synth_ms* ms, double* out) {
+    std::copy(ms->enu_to_uvw.begin(), ms->enu_to_uvw.end(), out);
 }
 
 void leap_synth_sky(const leap_synth_ms* ms, double* sources, double* antenna_phase) {

@@ -142,6 +142,7 @@ EXPORTS = [
     "leap_hip_synth_fill_vis_device",
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
     "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_phase_matrix", "leap_phase_matrix_pinv",
+    "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw",
 ]
 
 
@@ -203,6 +204,10 @@ def lib():
     L.leap_hip_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.leap_hip_last_kernel_times.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp]
     L.leap_hip_last_direction_block.argtypes = [ctypes.c_void_p]
+    L.leap_hip_residual_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp, _dp,
+                                              ctypes.c_uint32, _dp, _dp, _dp, _dp, _dp]
+    L.leap_synth_enu_to_uvw.argtypes = [ctypes.c_void_p, _dp]
+    L.leap_synth_enu_to_uvw.restype = None
     _lib = L
     return L
 
@@ -260,6 +265,8 @@ class SynthMS:
         self.sources = np.zeros((self.cfg.n_sources, 3))
         self.antenna_phase = np.zeros(n)
         lib().leap_synth_sky(h, _dptr(self.sources) if self.cfg.n_sources else None, _dptr(self.antenna_phase))
+        self.enu_to_uvw = np.zeros((self.cfg.n_timesteps, 3, 3))
+        lib().leap_synth_enu_to_uvw(h, _dptr(self.enu_to_uvw))
         self.vis = None
 
     def fill_host(self, dtype=np.complex128, row_start=0, n_rows=None):
@@ -461,6 +468,22 @@ class Metadata:
         self.ctx._chk(lib().leap_hip_apply_inversion(self.ctx.h, self.h, avg.ctypes.data_as(_dp), avg.shape[0],
                                                      ctypes.byref(o), _dptr(cal)))
         return cal
+
+    def loss_grad(self, view, directions, enu_to_uvw, offsets=None, antenna_delta=None, want_grad=True):
+        """Calibration-residual loss and its gradients (offsets [D, 2], antenna positions [N, 3])."""
+        dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)
+        D = len(dirs)
+        off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.float64).reshape(D, 2)
+        dp = None if antenna_delta is None else np.ascontiguousarray(antenna_delta, dtype=np.float64).reshape(self.N, 3)
+        M = np.ascontiguousarray(enu_to_uvw, dtype=np.float64)
+        loss = ctypes.c_double()
+        g_off = np.zeros((D, 2))
+        g_ant = np.zeros((self.N, 3))
+        self.ctx._chk(lib().leap_hip_residual_loss_grad(
+            self.ctx.h, self.h, ctypes.byref(view), _dptr(dirs), None if off is None else _dptr(off), D,
+            None if dp is None else _dptr(dp), _dptr(M), ctypes.byref(loss),
+            _dptr(g_off) if want_grad else None, _dptr(g_ant) if want_grad else None))
+        return loss.value, g_off, g_ant
 
     def __del__(self):
         try:

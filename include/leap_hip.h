@@ -197,6 +197,25 @@ int leap_hip_apply_inversion(leap_ctx* ctx, const leap_metadata* md, const doubl
 int leap_hip_dd_matrix(leap_ctx* ctx, double phase_centre_ra, double phase_centre_dec,
                        const double* directions, uint32_t n_directions, double* dd_out);
 
+/* ---- differentiation ------------------------------------------------------ */
+/* Calibration-residual loss of one solution interval over all rows and its
+ * gradient with respect to direction offsets and antenna positions.  No
+ * reference counterpart: it differentiates the reference's calibration
+ * (CpuLeapCalibrator.cc:147-217) for direction and array refinement.
+ *   uvw(row)  = uvw(row) + M_t (dp[antenna2] - dp[antenna1]),  M_t = enu_to_uvw[t] (3x3 row-major)
+ *   direction = directions[d] + offsets[d]                      (ra, dec radians)
+ *   loss      = sum_d sum_{n < K-1} wrap((A cal_d)(n) - arg avgData_d(I(n)))^2
+ * with avgData and cal computed in FP64 as leap_hip_calibrate does (pol 0).
+ * The gradient runs a device adjoint of rotate/average that recomputes the
+ * phasors (Horner's rule for the sum and its derivative) and the adjoint of
+ * arg and the pseudo-inverse products on the host.  offsets (D x 2),
+ * antenna_delta (N x 3, metres east/north/up) may be NULL (zero); grad_offsets
+ * (D x 2) and grad_antenna (N x 3) may be NULL.  1 <= D <= 256. */
+int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
+                                const double* directions, const double* offsets, uint32_t n_directions,
+                                const double* antenna_delta, const double* enu_to_uvw, double* loss,
+                                double* grad_offsets, double* grad_antenna);
+
 /* ---- host model (no GPU needed) ------------------------------------------- */
 /* cpu::PhaseMatrixFunction (algorithm/cpu/PhaseMatrixFunction.cc:35-93).
  * ref_ant < 0 means none.  A_out holds (n + 1) x (max(a1, a2) + 1) doubles,
@@ -242,6 +261,8 @@ int leap_synth_create(const leap_synth_config* cfg, leap_synth_ms** out);
 void leap_synth_destroy(leap_synth_ms* ms);
 /* Fills view with the host columns; vis is NULL until a fill call. */
 void leap_synth_view(const leap_synth_ms* ms, leap_ms_view* view);
+/* Per-timestep 3x3 matrices M_t (row-major, [T][3][3]) with uvw = M_t * (enu_a2 - enu_a1). */
+void leap_synth_enu_to_uvw(const leap_synth_ms* ms, double* out);
 /* Sky model for tests: n_sources x {ra, dec, flux}, and per-antenna phases. */
 void leap_synth_sky(const leap_synth_ms* ms, double* sources, double* antenna_phase);
 /* Host visibilities for rows [row_start, row_start + n_rows) into out

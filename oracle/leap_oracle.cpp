@@ -622,3 +622,181 @@ int oracle_calibrate(const oracle_ms_view* ms, const double* directions, int32_t
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Residual loss and its gradient, written out with explicit sums.
+// ---------------------------------------------------------------------------
+namespace {
+
+inline double wrap_pi(double x) {
+    double r = std::fmod(x + kPi, 2.0 * kPi);
+    if (r < 0) r += 2.0 * kPi;
+    return r - kPi;
+}
+
+// dd row 2 and its derivatives in ra and dec (MetaData.cc:166-203 differentiated).
+void dz_and_derivs(double ra0, double dec0, double ra, double dec, double dz[3], double dra[3], double ddec[3]) {
+    const double s1 = std::sin(kPi / 2.0 - dec0), c1 = std::cos(kPi / 2.0 - dec0);
+    const double s2 = std::sin(ra - ra0), c2 = std::cos(ra - ra0);
+    const double s3 = std::sin(-kPi / 2.0 + dec), c3 = std::cos(-kPi / 2.0 + dec);
+    dz[0] = -s3 * s2;
+    dz[1] = s3 * c2 * c1 + c3 * s1;
+    dz[2] = -s3 * c2 * s1 + c3 * c1;
+    dra[0] = -s3 * c2;
+    dra[1] = -s3 * s2 * c1;
+    dra[2] = s3 * s2 * s1;
+    ddec[0] = -c3 * s2;
+    ddec[1] = c3 * c2 * c1 - s3 * s1;
+    ddec[2] = -c3 * c2 * s1 - s3 * c1;
+}
+
+}  // namespace
+
+extern "C" int oracle_residual_loss_grad(const oracle_ms_view* ms, const double* directions, const double* offsets,
+                                         int32_t D, const double* antenna_delta, const double* enu_to_uvw,
+                                         int32_t ref_ant, double* loss_out, double* grad_offsets,
+                                         double* grad_antenna) {
+    return guarded([&] {
+        check_view(ms);
+        std::unique_ptr<oracle_metadata> md(metadata_create(*ms, ref_ant, 0.0));
+        const oracle_metadata& m = *md;
+        const uint32_t nbl = ms->num_baselines, C = ms->num_channels, P = ms->num_pols, R = ms->num_rows;
+        const int32_t N = m.A.cols, K = m.A.rows, K1 = m.A1.rows;
+        const int32_t nI = K - 1;
+        // perturbed uvw
+        std::vector<double> uvw(size_t(R) * 3);
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t t = r / nbl, b = r % nbl;
+            double dp[3] = {0, 0, 0};
+            if (antenna_delta) {
+                for (int k = 0; k < 3; ++k)
+                    dp[k] = antenna_delta[3 * ms->antenna2[b] + k] - antenna_delta[3 * ms->antenna1[b] + k];
+            }
+            for (int i = 0; i < 3; ++i) {
+                double s = ms->uvw[3 * size_t(r) + i];
+                for (int k = 0; k < 3; ++k) s += enu_to_uvw[9 * size_t(t) + 3 * i + k] * dp[k];
+                uvw[3 * size_t(r) + i] = s;
+            }
+        }
+        std::vector<double> kappa(C);
+        for (uint32_t c = 0; c < C; ++c) kappa[c] = (m.freq_start_hz + double(c) * m.freq_inc_hz) / kSpeedOfLight;
+        double loss = 0.0;
+        std::vector<double> g_uvw(size_t(R) * 3, 0.0);
+        if (grad_offsets) std::fill(grad_offsets, grad_offsets + 2 * size_t(D), 0.0);
+        for (int32_t d = 0; d < D; ++d) {
+            const double ra = directions[2 * d] + (offsets ? offsets[2 * d] : 0.0);
+            const double dec = directions[2 * d + 1] + (offsets ? offsets[2 * d + 1] : 0.0);
+            double dz[3], dra[3], ddec[3];
+            dz_and_derivs(m.phase_centre_ra_rad, m.phase_centre_dec_rad, ra, dec, dz, dra, ddec);
+            // forward: avgData pol 0 with the NaN rule over all pols
+            std::vector<cd> avg(nbl, cd(0, 0));
+            std::vector<double> sf(R);
+            for (uint32_t r = 0; r < R; ++r) {
+                const double* u = &uvw[3 * size_t(r)];
+                sf[r] = -kTwoPi * ((dz[0] * u[0] + dz[1] * u[1] + dz[2] * u[2]) - u[2]);
+                for (uint32_t c = 0; c < C; ++c) {
+                    bool finite = true;
+                    for (uint32_t p = 0; p < P; ++p) {
+                        const cd v = ms->vis_dtype == 1 ? load_vis<float>(ms->vis, (size_t(r) * C + c) * P + p)
+                                                        : load_vis<double>(ms->vis, (size_t(r) * C + c) * P + p);
+                        finite = finite && std::isfinite(v.real()) && std::isfinite(v.imag());
+                    }
+                    if (!finite) continue;
+                    const cd v0 = ms->vis_dtype == 1 ? load_vis<float>(ms->vis, (size_t(r) * C + c) * P)
+                                                     : load_vis<double>(ms->vis, (size_t(r) * C + c) * P);
+                    avg[r % nbl] += v0 * std::exp(cd(0.0, sf[r] * kappa[c]));
+                }
+            }
+            // forward inversion
+            std::vector<double> phi1(size_t(K1), 0.0);
+            for (int32_t k = 0; k + 1 < K1; ++k) phi1[k] = std::arg(avg[m.I1[k]]);
+            std::vector<double> cal1(size_t(N), 0.0);
+            for (int32_t k = 0; k < K1; ++k)
+                for (int32_t i = 0; i < N; ++i) cal1[i] += m.Ad1(i, k) * phi1[k];
+            std::vector<double> acal1(size_t(K), 0.0);
+            for (int32_t j = 0; j < N; ++j)
+                for (int32_t n = 0; n < K; ++n) acal1[n] += m.A(n, j) * cal1[j];
+            std::vector<double> dphi(size_t(K), 0.0);
+            for (int32_t n = 0; n < nI; ++n) dphi[n] = std::arg(std::exp(cd(0.0, -kTwoPi * acal1[n])) * avg[n]);
+            std::vector<double> cal(cal1);
+            for (int32_t k = 0; k < K; ++k)
+                for (int32_t i = 0; i < N; ++i) cal[i] += m.Ad(i, k) * dphi[k];
+            // residuals
+            std::vector<double> res(static_cast<size_t>(nI));
+            for (int32_t n = 0; n < nI; ++n) {
+                double ac = 0.0;
+                for (int32_t j = 0; j < N; ++j) ac += m.A(n, j) * cal[j];
+                res[n] = wrap_pi(ac - std::arg(avg[m.I[n]]));
+                loss += res[n] * res[n];
+            }
+            if (!grad_offsets && !grad_antenna) continue;
+            // backward through the inversion: g_arg[b] = dL/d arg(avg[b])
+            std::vector<double> g_arg(nbl, 0.0), g_cal(size_t(N), 0.0);
+            for (int32_t n = 0; n < nI; ++n) {
+                g_arg[m.I[n]] -= 2.0 * res[n];
+                for (int32_t j = 0; j < N; ++j) g_cal[j] += 2.0 * res[n] * m.A(n, j);
+            }
+            std::vector<double> g_dphi(size_t(K), 0.0), g_cal1(g_cal);
+            for (int32_t k = 0; k < K; ++k)
+                for (int32_t i = 0; i < N; ++i) g_dphi[k] += m.Ad(i, k) * g_cal[i];
+            std::vector<double> g_acal1(size_t(K), 0.0);
+            for (int32_t n = 0; n < nI; ++n) {
+                g_arg[n] += g_dphi[n];  // dInt row n uses avgData row n
+                g_acal1[n] = -kTwoPi * g_dphi[n];
+            }
+            for (int32_t n = 0; n < K; ++n)
+                for (int32_t j = 0; j < N; ++j) g_cal1[j] += m.A(n, j) * g_acal1[n];
+            for (int32_t k = 0; k + 1 < K1; ++k) {
+                double g = 0.0;
+                for (int32_t i = 0; i < N; ++i) g += m.Ad1(i, k) * g_cal1[i];
+                g_arg[m.I1[k]] += g;
+            }
+            // d arg(z) = Im(conj(z) dz) / |z|^2  ->  adjoint abar = i g / conj(z)
+            std::vector<cd> abar(nbl);
+            for (uint32_t b = 0; b < nbl; ++b)
+                abar[b] = (avg[b] == cd(0, 0)) ? cd(0, 0) : cd(0.0, g_arg[b]) / std::conj(avg[b]);
+            // backward through the rotation: dL/dsf(row) = Re(conj(abar) * sum_c i kappa_c v_c e^{i sf kappa_c})
+            double g_dz[3] = {0, 0, 0};
+            for (uint32_t r = 0; r < R; ++r) {
+                cd s1(0, 0);
+                for (uint32_t c = 0; c < C; ++c) {
+                    bool finite = true;
+                    for (uint32_t p = 0; p < P; ++p) {
+                        const cd v = ms->vis_dtype == 1 ? load_vis<float>(ms->vis, (size_t(r) * C + c) * P + p)
+                                                        : load_vis<double>(ms->vis, (size_t(r) * C + c) * P + p);
+                        finite = finite && std::isfinite(v.real()) && std::isfinite(v.imag());
+                    }
+                    if (!finite) continue;
+                    const cd v0 = ms->vis_dtype == 1 ? load_vis<float>(ms->vis, (size_t(r) * C + c) * P)
+                                                     : load_vis<double>(ms->vis, (size_t(r) * C + c) * P);
+                    s1 += kappa[c] * v0 * std::exp(cd(0.0, sf[r] * kappa[c]));
+                }
+                const double g_sf = (std::conj(abar[r % nbl]) * cd(0.0, 1.0) * s1).real();
+                const double* u = &uvw[3 * size_t(r)];
+                for (int i = 0; i < 3; ++i) g_dz[i] += -kTwoPi * g_sf * u[i];
+                g_uvw[3 * size_t(r) + 0] += -kTwoPi * g_sf * dz[0];
+                g_uvw[3 * size_t(r) + 1] += -kTwoPi * g_sf * dz[1];
+                g_uvw[3 * size_t(r) + 2] += -kTwoPi * g_sf * (dz[2] - 1.0);
+            }
+            if (grad_offsets) {
+                grad_offsets[2 * d] = g_dz[0] * dra[0] + g_dz[1] * dra[1] + g_dz[2] * dra[2];
+                grad_offsets[2 * d + 1] = g_dz[0] * ddec[0] + g_dz[1] * ddec[1] + g_dz[2] * ddec[2];
+            }
+        }
+        *loss_out = loss;
+        if (grad_antenna) {
+            int32_t nant = 0;
+            for (uint32_t b = 0; b < nbl; ++b) nant = std::max(nant, std::max(ms->antenna1[b], ms->antenna2[b]) + 1);
+            std::fill(grad_antenna, grad_antenna + 3 * size_t(nant), 0.0);
+            for (uint32_t r = 0; r < R; ++r) {
+                const uint32_t t = r / nbl, b = r % nbl;
+                for (int k = 0; k < 3; ++k) {
+                    double g = 0.0;  // (M_t^T g_uvw)_k
+                    for (int i = 0; i < 3; ++i) g += enu_to_uvw[9 * size_t(t) + 3 * i + k] * g_uvw[3 * size_t(r) + i];
+                    grad_antenna[3 * size_t(ms->antenna2[b]) + k] += g;
+                    grad_antenna[3 * size_t(ms->antenna1[b]) + k] -= g;
+                }
+            }
+        }
+    });
+}

@@ -121,3 +121,24 @@ int oracle_slice_solutions(int32_t start, int32_t interval, int32_t end, int32_t
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Calibration-residual loss and its gradient (test infrastructure for the
+ * product's adjoint).  One solution interval over all rows, threshold 0.
+ *   uvw(row)  = uvw0(row) + M_t (dp[a2] - dp[a1]),  M_t = enu_to_uvw[t] (3x3 row-major)
+ *   direction = directions[d] + offsets[d]           (ra, dec radians)
+ *   avgData, cal per direction as CpuLeapCalibrator (pol 0 reaches cal)
+ *   loss      = sum_d sum_{n < K-1} wrap((A cal_d)(n) - arg avgData_d(I(n)))^2
+ * Gradients by the chain rule written out with explicit sums (no recurrences).
+ * antenna_delta may be NULL (zero).  grad_offsets [D][2], grad_antenna [N_ant][3]
+ * (N_ant = max antenna index + 1); either may be NULL. */
+int oracle_residual_loss_grad(const oracle_ms_view* ms, const double* directions, const double* offsets,
+                              int32_t n_directions, const double* antenna_delta, const double* enu_to_uvw,
+                              int32_t ref_ant, double* loss, double* grad_offsets, double* grad_antenna);
+
+#ifdef __cplusplus
+}
+#endif

@@ -109,6 +109,8 @@ class Oracle:
                                          _c_dp, _c_dp, _c_i32p, _c_i32p]
         lib.oracle_slice_solutions.argtypes = [ctypes.c_int32] * 4
         lib.oracle_filtered_baselines.argtypes = [ctypes.POINTER(MsView), ctypes.c_double, _c_u8p]
+        lib.oracle_residual_loss_grad.argtypes = [ctypes.POINTER(MsView), _c_dp, _c_dp, ctypes.c_int32, _c_dp, _c_dp,
+                                                  ctypes.c_int32, _c_dp, _c_dp, _c_dp]
 
     def _check(self, rc):
         if rc != 0:
@@ -163,6 +165,25 @@ class Oracle:
         out = np.zeros(ms["num_baselines"], dtype=np.uint8)
         self._check(self.lib.oracle_filtered_baselines(ctypes.byref(v), threshold, _ptr(out, ctypes.c_uint8)))
         return out.astype(bool)
+
+    def loss_grad(self, ms, directions, enu_to_uvw, offsets=None, antenna_delta=None, ref_ant=None, want_grad=True):
+        v = make_view(ms)
+        dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)
+        D = len(dirs)
+        n_ant = int(max(ms["antenna1"].max(), ms["antenna2"].max())) + 1
+        off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.float64).reshape(D, 2)
+        dp = None if antenna_delta is None else np.ascontiguousarray(antenna_delta, dtype=np.float64).reshape(n_ant, 3)
+        M = np.ascontiguousarray(enu_to_uvw, dtype=np.float64)
+        loss = ctypes.c_double()
+        g_off = np.zeros((D, 2))
+        g_ant = np.zeros((n_ant, 3))
+        null = ctypes.POINTER(ctypes.c_double)()
+        self._check(self.lib.oracle_residual_loss_grad(
+            ctypes.byref(v), dirs.ctypes.data_as(_c_dp), null if off is None else off.ctypes.data_as(_c_dp), D,
+            null if dp is None else dp.ctypes.data_as(_c_dp), M.ctypes.data_as(_c_dp), -1 if ref_ant is None else ref_ant,
+            ctypes.byref(loss), g_off.ctypes.data_as(_c_dp) if want_grad else null,
+            g_ant.ctypes.data_as(_c_dp) if want_grad else null))
+        return loss.value, g_off, g_ant
 
     # -- metadata / calibration -------------------------------------------
     def metadata(self, ms, ref_ant=None, threshold=0.0):

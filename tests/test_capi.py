@@ -139,3 +139,15 @@ def test_synth_one_source_phase_matches_geometry(leap, oracle):
     avg = npr.rotate_visibilities(ms, s.sources[0, :2])
     bl_phase = s.antenna_phase[s.antenna1] - s.antenna_phase[s.antenna2]
     assert np.abs(npr.wrap(np.angle(avg[:, 0]) - bl_phase)).max() < 1e-9
+
+
+def test_synth_enu_to_uvw_is_the_uvw_geometry(leap):
+    """uvw(t, b) = M_t (enu_a2 - enu_a1): recover the baseline vectors at t = 0, then predict every t."""
+    s = leap.SynthMS(n_antennas=10, n_timesteps=5, n_channels=2, seed=8)
+    nbl = s.num_baselines
+    uvw = s.uvw.reshape(5, nbl, 3)
+    M = s.enu_to_uvw
+    enu_d = np.linalg.solve(M[0], uvw[0].T).T  # [nbl, 3]
+    for t in range(5):
+        assert np.abs(uvw[t] - enu_d @ M[t].T).max() < 1e-9 * np.abs(uvw).max()
+    assert np.abs(enu_d[:, 2]).max() < 1e-9  # a flat array

@@ -283,3 +283,36 @@ def test_calibrate_composes_rotate_and_inversion(oracle, small_ms, slice_):
             avg = npr.rotate_visibilities(ms, dirs[d], s * interval * nbl, interval * nbl)
             expect = npr.apply_inversion(avg, md.A, md.I, md.A1, md.I1)
             assert np.abs(npr.wrap(cal[s, d] - expect)).max() < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# Residual loss and gradient (the checker of the product's adjoint)
+# ---------------------------------------------------------------------------
+def test_oracle_gradient_matches_central_differences(oracle, leap):
+    s = leap.SynthMS(n_antennas=8, n_timesteps=3, n_channels=6, seed=5)
+    s.fill_host()
+    ms = s.host_dict()
+    c0 = ms["phase_centre"]
+    dirs = np.array([[c0[0] + 0.01, c0[1] - 0.01], [c0[0] - 0.012, c0[1] + 0.004]])
+    rng = np.random.default_rng(0)
+    off = rng.normal(0, 1e-3, (2, 2))
+    dp = rng.normal(0, 0.01, (8, 3))
+    M = s.enu_to_uvw
+    _, g_off, g_ant = oracle.loss_grad(ms, dirs, M, off, dp)
+    h = 1e-6
+
+    def loss(o, a):
+        return oracle.loss_grad(ms, dirs, M, o, a, want_grad=False)[0]
+
+    for i in range(2):
+        for j in range(2):
+            e = np.zeros_like(off)
+            e[i, j] = h
+            fd = (loss(off + e, dp) - loss(off - e, dp)) / (2 * h)
+            assert abs(fd - g_off[i, j]) <= 1e-6 * abs(g_off).max()
+    for a in (0, 3, 7):
+        for k in range(3):
+            e = np.zeros_like(dp)
+            e[a, k] = h
+            fd = (loss(off, dp + e) - loss(off, dp - e)) / (2 * h)
+            assert abs(fd - g_ant[a, k]) <= 1e-5 * abs(g_ant).max()
