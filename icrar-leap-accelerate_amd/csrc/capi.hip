@@ -69,6 +69,8 @@ leap_options resolve(const leap_options* o) {
             "the FP32 blocked kernel runs one direction per lane");
     require(r.lane_directions == 0 || r.direction_block == 0 || r.direction_block >= 32 * r.lane_directions,
             LEAP_ERR_INVALID_ARGUMENT, "direction block too small for the lane directions");
+    require(r.workgroups_per_cu >= -1 && r.workgroups_per_cu <= 64, LEAP_ERR_INVALID_ARGUMENT,
+            "workgroups per CU must be -1 (one work item each), 0 (tuned) or 1 to 64");
     return r;
 }
 
@@ -121,7 +123,8 @@ struct DeviceRows {
     const double* uvw;
 };
 
-DeviceRows stage_rows(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin, uint32_t row_end) {
+DeviceRows stage_rows_into(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin, uint32_t row_end,
+                           DeviceBuffer& vis_buf, DeviceBuffer& uvw_buf) {
     require(row_end <= ms.num_rows && row_begin <= row_end, LEAP_ERR_RANGE,
             "row selection [" + std::to_string(row_begin) + "," + std::to_string(row_end) +
                 "] exceeds total range [0," + std::to_string(ms.num_rows) + "]");
@@ -132,20 +135,24 @@ DeviceRows stage_rows(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin,
     if (ms.vis_on_device) {
         r.vis = static_cast<const char*>(ms.vis) + size_t(row_begin) * vb;
     } else {
-        ctx->d_vis_stage.reserve(n * vb);
-        LEAP_HIP(hipMemcpyAsync(ctx->d_vis_stage.ptr, static_cast<const char*>(ms.vis) + size_t(row_begin) * vb,
-                                n * vb, hipMemcpyHostToDevice, ctx->stream));
-        r.vis = ctx->d_vis_stage.ptr;
+        vis_buf.reserve(n * vb);
+        LEAP_HIP(hipMemcpyAsync(vis_buf.ptr, static_cast<const char*>(ms.vis) + size_t(row_begin) * vb, n * vb,
+                                hipMemcpyHostToDevice, ctx->stream));
+        r.vis = vis_buf.ptr;
     }
     if (ms.uvw_device) {
         r.uvw = ms.uvw_device + size_t(row_begin) * 3;
     } else {
-        ctx->d_uvw.reserve(n * 3 * sizeof(double));
-        LEAP_HIP(hipMemcpyAsync(ctx->d_uvw.ptr, ms.uvw + size_t(row_begin) * 3, n * 3 * sizeof(double),
+        uvw_buf.reserve(n * 3 * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(uvw_buf.ptr, ms.uvw + size_t(row_begin) * 3, n * 3 * sizeof(double),
                                 hipMemcpyHostToDevice, ctx->stream));
-        r.uvw = ctx->d_uvw.as<double>();
+        r.uvw = uvw_buf.as<double>();
     }
     return r;
+}
+
+DeviceRows stage_rows(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin, uint32_t row_end) {
+    return stage_rows_into(ctx, ms, row_begin, row_end, ctx->d_vis_stage, ctx->d_uvw);
 }
 
 void upload_directions(leap_ctx* ctx, const leap_metadata& md, const double* dirs, uint32_t D) {
@@ -234,31 +241,37 @@ void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_vie
     }
 }
 
-void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const leap_options& opts) {
+// cal[D][N] of D directions from avgData at avg (ctx->d_avg unless given).
+void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const leap_options& opts,
+                       const double* avg = nullptr, double* cal = nullptr) {
     const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
     ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
     ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
     ctx->d_delta.reserve(size_t(K) * D * sizeof(double));
-    ctx->d_cal.reserve(size_t(N) * D * sizeof(double));
+    if (cal == nullptr) {
+        ctx->d_cal.reserve(size_t(N) * D * sizeof(double));
+        cal = ctx->d_cal.as<double>();
+    }
+    if (avg == nullptr) avg = ctx->d_avg.as<double>();
     if (opts.strategy != LEAP_STRATEGY_BATCHED) {
         // Per-direction GEMVs, the reference's schedule (CudaLeapCalibrator.cu:256-259).
         ctx->d_partial.reserve(inversion_partial_doubles(K, N, D) * sizeof(double));
-        launch_inversion_f64(ctx->stream, md, ctx->d_avg.as<double>(), D, ctx->d_phase.as<double>(),
+        launch_inversion_f64(ctx->stream, md, avg, D, ctx->d_phase.as<double>(),
                              ctx->d_cal1.as<double>(), ctx->d_delta.as<double>(), ctx->d_partial.as<double>(),
-                             ctx->d_cal.as<double>());
+                             cal);
         return;
     }
-    launch_inversion_head(ctx->stream, md, ctx->d_avg.as<double>(), D, ctx->d_phase.as<double>(),
+    launch_inversion_head(ctx->stream, md, avg, D, ctx->d_phase.as<double>(),
                           ctx->d_cal1.as<double>(), ctx->d_delta.as<double>());
     if (opts.precision == LEAP_PRECISION_MIXED) {
         ctx->d_a_planes.reserve(bf16_planes_bytes((D + 63) / 64 * 64, K));
         ctx->d_partial.reserve(mfma_partial_floats(K, D, N) * sizeof(float));
         launch_gemm_bf16x3(ctx->stream, md.d_AdT_planes.ptr, K, N, ctx->d_delta.as<double>(), D, ctx->d_a_planes.ptr,
-                           ctx->d_cal1.as<double>(), static_cast<float*>(ctx->d_partial.ptr), ctx->d_cal.as<double>());
+                           ctx->d_cal1.as<double>(), static_cast<float*>(ctx->d_partial.ptr), cal);
     } else {
         ctx->d_partial.reserve(gemm_f64_partial_doubles(K, N, D) * sizeof(double));
         launch_gemm_f64(ctx->stream, md.d_AdT.as<double>(), K, N, ctx->d_delta.as<double>(), D,
-                        ctx->d_cal1.as<double>(), ctx->d_partial.as<double>(), ctx->d_cal.as<double>());
+                        ctx->d_cal1.as<double>(), ctx->d_partial.as<double>(), cal);
     }
 }
 
@@ -370,6 +383,223 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
             LEAP_HIP(hipStreamSynchronize(ctx->stream));
         }
         if (cb && cb(user, &sol) != 0) throw Error(LEAP_ERR_CALLBACK, "output callback requested a stop");
+    }
+    LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    end_timing(ctx);
+}
+
+// ---- several measurement sets in one persistent launch ----------------------
+
+// Workgroups of the multi-set grid: per CU the occupancy limit (per_cu 0), the
+// caller's count, or one work item per workgroup (per_cu < 0: the hardware
+// dispatcher balances, largest items first).
+uint32_t multi_grid(leap_ctx* ctx, int32_t tile, int32_t per_cu, size_t n_items) {
+    if (per_cu < 0) return uint32_t(std::max<size_t>(n_items, 1));
+    if (ctx->cu_count == 0)
+        LEAP_HIP(hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const size_t g = size_t(std::max(1, ctx->cu_count)) * uint32_t(per_cu > 0 ? per_cu : multi_blocks_per_cu(tile));
+    return uint32_t(std::min(g, std::max<size_t>(n_items, 1)));
+}
+
+// Plans, uploads and launches one multi-set rotate/average.
+void run_multi(leap_ctx* ctx, const std::vector<MultiSetDesc>& descs, const std::vector<MultiShape>& shapes,
+               int32_t tile, int32_t per_cu, hipEvent_t after_upload = nullptr, bool timed = false) {
+    size_t n_items = 0;
+    const uint32_t db = uint32_t(tile / 10);
+    for (const MultiShape& m : shapes) n_items += size_t(m.nbl) * ((m.D + db - 1) / db);
+    const uint32_t blocks = multi_grid(ctx, tile, per_cu, n_items);
+    const MultiPlan plan = plan_multi(shapes, tile, blocks);
+    ctx->d_descs.reserve(descs.size() * sizeof(MultiSetDesc));
+    ctx->d_items.reserve(std::max<size_t>(plan.items.size(), 1) * sizeof(MultiItem));
+    ctx->d_offsets.reserve(plan.off.size() * sizeof(uint32_t));
+    LEAP_HIP(hipMemcpyAsync(ctx->d_descs.ptr, descs.data(), descs.size() * sizeof(MultiSetDesc),
+                            hipMemcpyHostToDevice, ctx->stream));
+    if (!plan.items.empty())
+        LEAP_HIP(hipMemcpyAsync(ctx->d_items.ptr, plan.items.data(), plan.items.size() * sizeof(MultiItem),
+                                hipMemcpyHostToDevice, ctx->stream));
+    LEAP_HIP(hipMemcpyAsync(ctx->d_offsets.ptr, plan.off.data(), plan.off.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, ctx->stream));
+    if (after_upload) LEAP_HIP(hipEventRecord(after_upload, ctx->stream));
+    if (n_items == 0) return;
+    if (timed) record_rotate_begin(ctx);
+    launch_rotate_multi(ctx->stream, tile, ctx->d_descs.as<MultiSetDesc>(), ctx->d_items.as<uint2>(),
+                        ctx->d_offsets.as<uint32_t>(), blocks);
+    if (timed) record_rotate_end(ctx);
+}
+
+// Tile and workgroups per CU of the multi-set kernel: the caller's, or the
+// fastest candidates on the launch itself, timed once per combination of shapes.
+std::pair<int32_t, int32_t> multi_tile(leap_ctx* ctx, const std::vector<MultiSetDesc>& descs,
+                                       const std::vector<MultiShape>& shapes, const leap_options& opts) {
+    int32_t fixed_tile = 0;
+    if (opts.direction_block != 0) {
+        fixed_tile = opts.direction_block * 10 + (opts.lane_directions ? opts.lane_directions : 1);
+        require(multi_tile_supported(fixed_tile), LEAP_ERR_INVALID_ARGUMENT,
+                "direction block " + std::to_string(opts.direction_block) + " x " +
+                    std::to_string(opts.lane_directions ? opts.lane_directions : 1) +
+                    " directions per lane is not a multi-set tile");
+        if (opts.workgroups_per_cu != 0) return {fixed_tile, opts.workgroups_per_cu};
+    }
+    std::vector<uint32_t> key{uint32_t(fixed_tile), uint32_t(opts.workgroups_per_cu)};
+    uint32_t max_d = 0;
+    for (size_t i = 0; i < shapes.size(); ++i) {
+        key.insert(key.end(), {shapes[i].nbl, shapes[i].T, shapes[i].C, shapes[i].D, uint32_t(descs[i].c64)});
+        max_d = std::max(max_d, shapes[i].D);
+    }
+    auto it = ctx->tuned_multi.find(key);
+    if (it != ctx->tuned_multi.end()) return it->second;
+    std::vector<int32_t> tiles;
+    if (fixed_tile) {
+        tiles.push_back(fixed_tile);
+    } else {
+        for (int32_t tile : {321, 641, 1281, 1282, 2562}) {
+            const uint32_t db = uint32_t(tile / 10);
+            if (!(db > 32 && db >= 2 * max_d)) tiles.push_back(tile);  // else mostly idle lanes
+        }
+    }
+    hipEvent_t a, b;
+    LEAP_HIP(hipEventCreate(&a));
+    LEAP_HIP(hipEventCreate(&b));
+    auto time_of = [&](int32_t tile, int32_t per_cu) {
+        float best_t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {  // the first repetition warms the code object
+            float t = 0.f;
+            run_multi(ctx, descs, shapes, tile, per_cu, a);
+            LEAP_HIP(hipEventRecord(b, ctx->stream));
+            LEAP_HIP(hipEventSynchronize(b));
+            LEAP_HIP(hipEventElapsedTime(&t, a, b));
+            if (rep == 1 || (rep > 1 && t < best_t)) best_t = t;
+        }
+        return best_t;
+    };
+    // Tile at the occupancy limit, then the grid depth for that tile.
+    std::pair<int32_t, int32_t> best{tiles.front(), opts.workgroups_per_cu};
+    float best_ms = 0.f;
+    for (int32_t tile : tiles) {
+        const float t = time_of(tile, opts.workgroups_per_cu);
+        if (best_ms == 0.f || t < best_ms) {
+            best_ms = t;
+            best = {tile, opts.workgroups_per_cu};
+        }
+    }
+    if (opts.workgroups_per_cu == 0) {
+        const int32_t occ = multi_blocks_per_cu(best.first);
+        best.second = occ;
+        for (int32_t k : {2, 4, 8, -1}) {
+            const int32_t per_cu = k > 0 ? occ * k : -1;
+            const float t = time_of(best.first, per_cu);
+            if (t < best_ms) {
+                best_ms = t;
+                best.second = per_cu;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    ctx->tuned_multi.emplace(key, best);
+    return best;
+}
+
+struct BatchState {
+    const leap_metadata* md;
+    const leap_ms_view* ms;
+    const double* dirs;
+    uint32_t D;
+    int32_t solutions;
+    uint32_t interval;
+    std::vector<double> h_cal;
+};
+
+void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets, const leap_options& opts,
+                     leap_batch_cb cb, void* user) {
+    require(sets != nullptr || n_sets == 0, LEAP_ERR_INVALID_ARGUMENT, "null set list");
+    require(opts.strategy == LEAP_STRATEGY_BATCHED, LEAP_ERR_INVALID_ARGUMENT,
+            "several measurement sets in one launch need the batched strategy");
+    require(!opts.all_pols && !opts.fp32_blocks, LEAP_ERR_NOT_IMPLEMENTED,
+            "the multi-set launch accumulates polarisation 0 with the FP64 Horner kernel only");
+    std::vector<BatchState> st(n_sets);
+    int32_t rounds = 0;
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        const leap_batch_set& in = sets[i];
+        require(in.metadata != nullptr && in.ms != nullptr, LEAP_ERR_INVALID_ARGUMENT,
+                "set " + std::to_string(i) + " has no metadata or measurement set");
+        require(in.metadata->device == ctx->device, LEAP_ERR_INVALID_ARGUMENT,
+                "metadata belongs to another device");
+        validate_view(*in.ms);
+        check_metadata_matches(*in.metadata, *in.ms);
+        require(in.directions != nullptr || in.n_directions == 0, LEAP_ERR_INVALID_ARGUMENT, "null directions");
+        const int32_t T = int32_t(in.ms->num_rows / in.ms->num_baselines);
+        int32_t range[3];
+        evaluate_slice(in.slice_start, in.slice_interval, in.slice_end, T, range);
+        st[i] = BatchState{in.metadata, in.ms, in.directions, in.n_directions,
+                           (range[2] - range[0]) / range[1], uint32_t(range[1]), {}};
+        rounds = std::max(rounds, st[i].solutions);
+    }
+    while (ctx->batch_sets.size() < n_sets) ctx->batch_sets.push_back(std::make_unique<SetBuffers>());
+    begin_timing(ctx);
+    for (uint32_t i = 0; i < n_sets; ++i) {
+        BatchState& b = st[i];
+        SetBuffers& buf = *ctx->batch_sets[i];
+        const uint32_t N = uint32_t(b.md->A.cols);
+        b.h_cal.assign(size_t(N) * b.D, 0.0);
+        if (b.D == 0) continue;
+        buf.d_dirs.reserve(size_t(b.D) * 2 * sizeof(double));
+        buf.d_dd.reserve(size_t(b.D) * 9 * sizeof(double));
+        buf.d_avg.reserve(size_t(b.D) * b.md->nbl * b.md->pols * 2 * sizeof(double));
+        buf.d_cal.reserve(size_t(N) * b.D * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(buf.d_dirs.ptr, b.dirs, size_t(b.D) * 2 * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream));
+        launch_dd(ctx->stream, b.md->phase_centre_ra, b.md->phase_centre_dec, buf.d_dirs.as<double>(), b.D,
+                  buf.d_dd.as<double>());
+    }
+    std::pair<int32_t, int32_t> launch{0, 0};  // tile, workgroups per CU
+    for (int32_t r = 0; r < rounds; ++r) {
+        std::vector<MultiSetDesc> descs;
+        std::vector<MultiShape> shapes;
+        std::vector<uint32_t> active;
+        for (uint32_t i = 0; i < n_sets; ++i) {
+            BatchState& b = st[i];
+            if (r >= b.solutions || b.D == 0) continue;
+            SetBuffers& buf = *ctx->batch_sets[i];
+            const uint32_t nbl = b.md->nbl;
+            // CpuLeapCalibrator.cc:113-119: rows [r * interval * nbl, +interval * nbl).
+            const DeviceRows rows = stage_rows_into(ctx, *b.ms, uint32_t(size_t(r) * b.interval * nbl),
+                                                    uint32_t(size_t(r + 1) * b.interval * nbl), buf.d_vis, buf.d_uvw);
+            descs.push_back(MultiSetDesc{rows.vis, rows.uvw, buf.d_dd.as<double>(), buf.d_avg.as<double>(), nbl,
+                                         b.interval, b.md->channels, b.md->pols, b.D,
+                                         b.ms->vis_dtype == LEAP_VIS_C64 ? 1 : 0,
+                                         b.md->freq_start_hz / kSpeedOfLight, b.md->freq_inc_hz / kSpeedOfLight});
+            shapes.push_back(MultiShape{nbl, b.interval, b.md->channels, b.D});
+            active.push_back(i);
+        }
+        if (!descs.empty()) {
+            if (launch.first == 0) launch = multi_tile(ctx, descs, shapes, opts);
+            ctx->last_direction_block = launch.first;
+            ctx->last_workgroups_per_cu = launch.second;
+            run_multi(ctx, descs, shapes, launch.first, launch.second, nullptr, true);
+            for (uint32_t i : active) {
+                BatchState& b = st[i];
+                SetBuffers& buf = *ctx->batch_sets[i];
+                invert_directions(ctx, *b.md, b.D, opts, buf.d_avg.as<double>(), buf.d_cal.as<double>());
+                LEAP_HIP(hipMemcpyAsync(b.h_cal.data(), buf.d_cal.ptr, b.h_cal.size() * sizeof(double),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            }
+            LEAP_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        for (uint32_t i = 0; i < n_sets; ++i) {
+            const BatchState& b = st[i];
+            if (r >= b.solutions) continue;
+            const uint32_t nbl = b.md->nbl;
+            leap_solution sol{};
+            sol.index = uint32_t(r);
+            sol.start_epoch = b.ms->time[size_t(r) * b.interval * nbl];  // CpuLeapCalibrator.cc:107-109
+            sol.end_epoch = b.ms->time[(size_t(r + 1) * b.interval - 1) * nbl];
+            sol.n_directions = b.D;
+            sol.directions = b.dirs;
+            sol.n_antennas = uint32_t(b.md->A.cols);
+            sol.calibrations = b.h_cal.data();
+            if (cb && cb(user, i, &sol) != 0) throw Error(LEAP_ERR_CALLBACK, "output callback requested a stop");
+        }
     }
     LEAP_HIP(hipStreamSynchronize(ctx->stream));
     end_timing(ctx);
@@ -552,6 +782,15 @@ int leap_hip_calibrate(leap_ctx* ctx, const leap_ms_view* ms, const double* dirs
     });
 }
 
+int leap_hip_calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets, const leap_options* opts,
+                             leap_batch_cb cb, void* user) {
+    return guarded(ctx, [&] {
+        require(ctx != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null context");
+        DeviceScope scope(ctx->device);
+        calibrate_batch(ctx, sets, n_sets, resolve(opts), cb, user);
+    });
+}
+
 int leap_hip_synth_fill_vis_device(leap_ctx* ctx, const leap_synth_ms* ms, int32_t vis_dtype, void* out_device,
                                    double* uvw_device_out) {
     return guarded(ctx, [&] {
@@ -669,9 +908,33 @@ int leap_phase_matrix_pinv(const int32_t* a1, const int32_t* a2, const uint8_t* 
     });
 }
 
+int leap_multi_schedule(const uint32_t* nbl, const uint32_t* T, const uint32_t* C, const uint32_t* D,
+                        uint32_t n_sets, int32_t tile, uint32_t n_blocks, uint32_t* offsets, uint32_t* item_set,
+                        uint32_t* item_local, double* makespan_out) {
+    return guarded(nullptr, [&] {
+        require(n_sets == 0 || (nbl && T && C && D), LEAP_ERR_INVALID_ARGUMENT, "null shape arrays");
+        require(offsets != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null offsets");
+        std::vector<MultiShape> shapes(n_sets);
+        for (uint32_t i = 0; i < n_sets; ++i) shapes[i] = MultiShape{nbl[i], T[i], C[i], D[i]};
+        const MultiPlan plan = plan_multi(shapes, tile, n_blocks);
+        require(plan.items.empty() || (item_set && item_local), LEAP_ERR_INVALID_ARGUMENT, "null item arrays");
+        std::copy(plan.off.begin(), plan.off.end(), offsets);
+        for (size_t i = 0; i < plan.items.size(); ++i) {
+            item_set[i] = plan.items[i].set;
+            item_local[i] = plan.items[i].local;
+        }
+        if (makespan_out) *makespan_out = plan.makespan;
+    });
+}
+
+double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
+    return multi_tile_supported(tile) ? multi_item_cost(T, C, tile) : -1.0;
+}
+
 }  // extern "C"
 
 extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ? ctx->last_direction_block : 0; }
+extern "C" int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx) { return ctx ? ctx->last_workgroups_per_cu : 0; }
 
 // ---------------------------------------------------------------------------
 // Calibration-residual loss and its gradient.

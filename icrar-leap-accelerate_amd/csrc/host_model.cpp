@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <queue>
 #include <sstream>
 
 #include "leap_internal.h"
@@ -256,6 +257,76 @@ uint64_t hash_phase_matrix(const PhaseMatrix& pm) {
         add(uint64_t(uint32_t(pm.ant1[k])) | (uint64_t(uint32_t(pm.ant2[k])) << 32));
     }
     return h;
+}
+
+bool multi_tile_supported(int32_t tile) {
+    return tile == 321 || tile == 641 || tile == 1281 || tile == 1282 || tile == 2562;
+}
+
+// Work of one (baseline, direction block) item of the multi-set kernel: per
+// group of RP = 256 / (DB / ND) rows, ceil(C / 2) channel pairs on the FMA
+// chains plus the staging of each 256-channel chunk and the row set-up.
+double multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
+    require(multi_tile_supported(tile), LEAP_ERR_INVALID_ARGUMENT,
+            "unsupported multi-set tile " + std::to_string(tile));
+    const uint32_t lanes = uint32_t(tile / 10) / uint32_t(tile % 10);
+    const uint32_t rp = 256 / lanes;
+    const double groups = double((T + rp - 1) / rp);
+    const double chunks = double((C + 255) / 256);
+    return groups * (double((C + 1) / 2) + 48.0 * chunks + 16.0) + 32.0;
+}
+
+// Longest processing time first: items in decreasing cost, each to the
+// least-loaded workgroup (ties to the lower index), so the plan is a
+// deterministic function of the shapes.
+MultiPlan plan_multi(const std::vector<MultiShape>& sets, int32_t tile, uint32_t n_blocks) {
+    require(n_blocks >= 1, LEAP_ERR_INVALID_ARGUMENT, "the schedule needs at least one workgroup");
+    const uint32_t db = uint32_t(tile / 10);
+    std::vector<uint32_t> order(sets.size());
+    std::vector<double> cost(sets.size(), 0.0);
+    size_t n_items = 0;
+    for (size_t s = 0; s < sets.size(); ++s) {
+        order[s] = uint32_t(s);
+        cost[s] = multi_item_cost(sets[s].T, sets[s].C, tile);
+        if (sets[s].D > 0) {
+            require(sets[s].nbl > 0, LEAP_ERR_INVALID_ARGUMENT, "a set has no baselines");
+            n_items += size_t(sets[s].nbl) * ((sets[s].D + db - 1) / db);
+        }
+    }
+    require(n_items < (size_t(1) << 32), LEAP_ERR_RANGE, "too many work items for one launch");
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    using Slot = std::pair<double, uint32_t>;  // (load, workgroup)
+    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+    for (uint32_t g = 0; g < n_blocks; ++g) heap.emplace(0.0, g);
+    std::vector<uint32_t> owner(n_items), count(n_blocks, 0);
+    std::vector<uint32_t> iset(n_items), ilocal(n_items);
+    size_t k = 0;
+    for (uint32_t s : order) {
+        if (sets[s].D == 0) continue;
+        const uint32_t n = sets[s].nbl * ((sets[s].D + db - 1) / db);
+        for (uint32_t i = 0; i < n; ++i, ++k) {
+            Slot top = heap.top();
+            heap.pop();
+            owner[k] = top.second;
+            ++count[top.second];
+            iset[k] = s;
+            ilocal[k] = i;
+            top.first += cost[s];
+            heap.push(top);
+        }
+    }
+    MultiPlan plan;
+    plan.makespan = 0.0;
+    while (!heap.empty()) {
+        plan.makespan = std::max(plan.makespan, heap.top().first);
+        heap.pop();
+    }
+    plan.off.assign(size_t(n_blocks) + 1, 0);
+    for (uint32_t g = 0; g < n_blocks; ++g) plan.off[g + 1] = plan.off[g] + count[g];
+    plan.items.resize(n_items);
+    std::vector<uint32_t> fill(plan.off.begin(), plan.off.end() - 1);
+    for (size_t i = 0; i < n_items; ++i) plan.items[fill[owner[i]]++] = {iset[i], ilocal[i]};
+    return plan;
 }
 
 }  // namespace leap

@@ -241,19 +241,31 @@ __global__ __launch_bounds__(kThreads) void rotate_average_batched(
 // feeds 2 * ND independent chains (LDS return bandwidth, not the FMA pipe,
 // limits a one-direction-per-lane layout).  A workgroup covers DB directions
 // with L = DB / ND lanes per row and RP = 256 / L rows in flight.
-template <typename VT, int DB, int ND, int NP, int CC>
-__global__ __launch_bounds__(kThreads) void rotate_average_horner64(
-    const VT* __restrict__ vis, const double* __restrict__ uvw, const double* __restrict__ dd,
-    uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa,
-    double* __restrict__ avg) {
+template <int DB, int ND, int NP, int CC>
+struct Horner64Smem {
     static_assert(CC % 2 == 0, "chunks hold whole channel pairs");
     static_assert(DB % ND == 0 && kThreads % (DB / ND) == 0, "lane layout");
+    static constexpr int L = DB / ND;
+    static constexpr int RP = kThreads / L;
+    C2<double> v[RP][NP][CC];
+    double red[(RP > 1) ? RP : 1][L][ND][NP][2];
+};
+
+// Global-memory pointers: pointers read from a descriptor table are otherwise
+// generic, and flat memory instructions share the LDS wait counter.
+#define LEAP_GLOBAL __attribute__((address_space(1)))
+
+// One (baseline, direction block) tile; every thread of the workgroup calls it.
+template <typename VT, int DB, int ND, int NP, int CC>
+__device__ __attribute__((always_inline)) inline void horner64_tile(
+    Horner64Smem<DB, ND, NP, CC>& sm, const LEAP_GLOBAL VT* __restrict__ vis, const LEAP_GLOBAL double* __restrict__ uvw,
+    const LEAP_GLOBAL double* __restrict__ dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
+    double kappa0, double dkappa, LEAP_GLOBAL double* __restrict__ avg, uint32_t bl, uint32_t dblock) {
     constexpr int L = DB / ND;
     constexpr int RP = kThreads / L;
-    __shared__ C2<double> s_v[RP][NP][CC];
-    __shared__ double s_red[(RP > 1) ? RP : 1][L][ND][NP][2];
+    auto& s_v = sm.v;
+    auto& s_red = sm.red;
 
-    const uint32_t bl = blockIdx.x;
     const uint32_t lane_d = threadIdx.x % L;
     const uint32_t rsub = threadIdx.x / L;
     uint32_t dir[ND];
@@ -262,7 +274,7 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64(
     bool any_dir = false;
 #pragma unroll
     for (int j = 0; j < ND; ++j) {
-        dir[j] = blockIdx.y * DB + lane_d + j * L;
+        dir[j] = dblock * DB + lane_d + j * L;
         dir_ok[j] = dir[j] < D;
         any_dir |= dir_ok[j];
         const uint32_t dc = dir_ok[j] ? dir[j] : 0;
@@ -314,7 +326,7 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64(
                 for (int q = 0; q < NP; ++q) val[q] = {0.0, 0.0};
                 if (tt < T && c < cn) {
                     const size_t row = size_t(tt) * nbl + bl;
-                    const VT* p = vis + ((row * C + c0 + c) * P) * 2;
+                    const LEAP_GLOBAL VT* p = vis + ((row * C + c0 + c) * P) * 2;
                     bool finite = true;
                     VT x[8];
                     for (uint32_t q = 0; q < P; ++q) {
@@ -386,7 +398,7 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64(
 #pragma unroll
             for (int j = 0; j < ND; ++j) {
                 if (!dir_ok[j]) continue;
-                double* o = avg + (size_t(dir[j]) * nbl + bl) * P * 2;
+                LEAP_GLOBAL double* o = avg + (size_t(dir[j]) * nbl + bl) * P * 2;
 #pragma unroll
                 for (int q = 0; q < NP; ++q) {
                     double re = 0.0, im = 0.0;
@@ -404,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64(
 #pragma unroll
         for (int j = 0; j < ND; ++j) {
             if (!dir_ok[j]) continue;
-            double* o = avg + (size_t(dir[j]) * nbl + bl) * P * 2;
+            LEAP_GLOBAL double* o = avg + (size_t(dir[j]) * nbl + bl) * P * 2;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 o[2 * q] = acc[j][q].x;
@@ -412,6 +424,43 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64(
             }
             for (uint32_t q = 2 * NP; q < 2 * P; ++q) o[q] = 0.0;
         }
+    }
+}
+
+template <typename VT, int DB, int ND, int NP, int CC>
+__global__ __launch_bounds__(kThreads) void rotate_average_horner64(
+    const VT* __restrict__ vis, const double* __restrict__ uvw, const double* __restrict__ dd,
+    uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa,
+    double* __restrict__ avg) {
+    __shared__ Horner64Smem<DB, ND, NP, CC> sm;
+    horner64_tile<VT, DB, ND, NP, CC>(sm, (const LEAP_GLOBAL VT*)vis, (const LEAP_GLOBAL double*)uvw,
+                                      (const LEAP_GLOBAL double*)dd, D, T, nbl, C, P, kappa0, dkappa,
+                                      (LEAP_GLOBAL double*)avg, blockIdx.x, blockIdx.y);
+}
+
+// Several measurement sets in one persistent launch: workgroup g runs the
+// (set, baseline, direction block) items items[off[g] .. off[g + 1]) planned on
+// the host, each against its set's descriptor.
+template <int DB, int ND>
+__global__ __launch_bounds__(kThreads) void rotate_average_multi(const MultiSetDesc* __restrict__ descs,
+                                                                 const uint2* __restrict__ items,
+                                                                 const uint32_t* __restrict__ off) {
+    __shared__ Horner64Smem<DB, ND, 1, 256> sm;
+    const uint32_t i1 = off[blockIdx.x + 1];
+    for (uint32_t i = off[blockIdx.x]; i < i1; ++i) {
+        const uint2 it = items[i];
+        const MultiSetDesc& m = descs[it.x];
+        const uint32_t bl = it.y % m.nbl, db = it.y / m.nbl;
+        const LEAP_GLOBAL double* uvw = (const LEAP_GLOBAL double*)m.uvw;
+        const LEAP_GLOBAL double* dd = (const LEAP_GLOBAL double*)m.dd;
+        LEAP_GLOBAL double* avg = (LEAP_GLOBAL double*)m.avg;
+        if (m.c64)
+            horner64_tile<float, DB, ND, 1, 256>(sm, (const LEAP_GLOBAL float*)m.vis, uvw, dd, m.D, m.T, m.nbl, m.C,
+                                                 m.P, m.kappa0, m.dkappa, avg, bl, db);
+        else
+            horner64_tile<double, DB, ND, 1, 256>(sm, (const LEAP_GLOBAL double*)m.vis, uvw, dd, m.D, m.T, m.nbl,
+                                                  m.C, m.P, m.kappa0, m.dkappa, avg, bl, db);
+        __syncthreads();
     }
 }
 
@@ -485,6 +534,32 @@ void launch_np(hipStream_t s, int db, bool all_pols, const void* vis, const doub
 }
 
 }  // namespace
+
+void launch_rotate_multi(hipStream_t s, int32_t tile, const MultiSetDesc* descs, const uint2* items,
+                         const uint32_t* off, uint32_t n_blocks) {
+    if (n_blocks == 0) return;
+    switch (tile) {
+        case 2562: rotate_average_multi<256, 2><<<n_blocks, kThreads, 0, s>>>(descs, items, off); break;
+        case 1282: rotate_average_multi<128, 2><<<n_blocks, kThreads, 0, s>>>(descs, items, off); break;
+        case 1281: rotate_average_multi<128, 1><<<n_blocks, kThreads, 0, s>>>(descs, items, off); break;
+        case 641: rotate_average_multi<64, 1><<<n_blocks, kThreads, 0, s>>>(descs, items, off); break;
+        case 321: rotate_average_multi<32, 1><<<n_blocks, kThreads, 0, s>>>(descs, items, off); break;
+        default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported multi-set tile " + std::to_string(tile));
+    }
+    LEAP_HIP(hipGetLastError());
+}
+
+int32_t multi_blocks_per_cu(int32_t tile) {
+    int n = 0;
+    switch (tile) {
+        case 2562: LEAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rotate_average_multi<256, 2>, kThreads, 0)); break;
+        case 1282: LEAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rotate_average_multi<128, 2>, kThreads, 0)); break;
+        case 1281: LEAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rotate_average_multi<128, 1>, kThreads, 0)); break;
+        case 641: LEAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rotate_average_multi<64, 1>, kThreads, 0)); break;
+        default: LEAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rotate_average_multi<32, 1>, kThreads, 0)); break;
+    }
+    return n > 0 ? n : 1;
+}
 
 void launch_rotate_batched(hipStream_t s, int32_t precision, int32_t direction_block, bool all_pols,
                            const void* vis, int32_t vis_dtype, const double* uvw, const double* dd, uint32_t D,

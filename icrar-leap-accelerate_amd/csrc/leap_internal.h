@@ -75,6 +75,22 @@ void validate_view(const leap_ms_view& ms);
 
 uint64_t hash_phase_matrix(const PhaseMatrix& pm);
 
+// Static schedule of the multi-set persistent kernel.
+struct MultiShape {
+    uint32_t nbl, T, C, D;
+};
+struct MultiItem {  // the layout of the kernel's uint2 items
+    uint32_t set, local;  // local = baseline + nbl * direction block
+};
+struct MultiPlan {
+    std::vector<uint32_t> off;  // n_blocks + 1
+    std::vector<MultiItem> items;
+    double makespan;
+};
+bool multi_tile_supported(int32_t tile);
+double multi_item_cost(uint32_t T, uint32_t C, int32_t tile);
+MultiPlan plan_multi(const std::vector<MultiShape>& sets, int32_t tile, uint32_t n_blocks);
+
 }  // namespace leap
 
 // Metadata resident on the device (cpu::MetaData + cuda::ConstantBuffer of the reference).
@@ -108,6 +124,13 @@ struct TuneKey {
     }
 };
 
+namespace leap {
+// Per-set device buffers of leap_hip_calibrate_batch.
+struct SetBuffers {
+    DeviceBuffer d_dirs, d_dd, d_avg, d_cal, d_vis, d_uvw;
+};
+}  // namespace leap
+
 struct leap_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -126,9 +149,28 @@ struct leap_ctx {
     // autotuned direction blocks of the batched kernel
     std::map<TuneKey, int32_t> tuned;
     int32_t last_direction_block = 0;
+    // multi-set launches
+    std::vector<std::unique_ptr<leap::SetBuffers>> batch_sets;
+    leap::DeviceBuffer d_descs, d_items, d_offsets;
+    std::map<std::vector<uint32_t>, std::pair<int32_t, int32_t>> tuned_multi;  // -> tile, workgroups per CU
+    int32_t last_workgroups_per_cu = 0;
+    int32_t cu_count = 0;
 };
 
 namespace leap {
+// One measurement set of a multi-set launch (device-resident descriptor).
+struct MultiSetDesc {
+    const void* vis;    // [rows][C][P] complex, rows = T * nbl
+    const double* uvw;  // [rows][3]
+    const double* dd;   // [D][3][3]
+    double* avg;        // [D][nbl][P] complex128 (pol 0 written, others zeroed)
+    uint32_t nbl, T, C, P, D;
+    int32_t c64;
+    double kappa0, dkappa;
+};
+void launch_rotate_multi(hipStream_t s, int32_t tile, const MultiSetDesc* descs, const uint2* items,
+                         const uint32_t* off, uint32_t n_blocks);
+int32_t multi_blocks_per_cu(int32_t tile);
 // Kernel launchers (kernels_f64.hip)
 void launch_dd(hipStream_t s, double ra0, double dec0, const double* d_dirs, uint32_t D,
                double* d_dd);

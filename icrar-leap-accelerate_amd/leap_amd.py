@@ -75,12 +75,13 @@ class Options(ctypes.Structure):
         ("all_pols", ctypes.c_int32),
         ("fp32_blocks", ctypes.c_int32),
         ("lane_directions", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("workgroups_per_cu", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
 def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION, direction_block=0, all_pols=0,
-            fp32_blocks=0, lane_directions=0):
+            fp32_blocks=0, lane_directions=0, workgroups_per_cu=0):
     o = Options()
     o.precision = precision
     o.strategy = strategy
@@ -88,6 +89,7 @@ def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION,
     o.all_pols = all_pols
     o.fp32_blocks = fp32_blocks
     o.lane_directions = lane_directions
+    o.workgroups_per_cu = workgroups_per_cu
     return o
 
 
@@ -104,6 +106,21 @@ class Solution(ctypes.Structure):
 
 
 SOLUTION_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Solution))
+
+
+class BatchSet(ctypes.Structure):
+    _fields_ = [
+        ("metadata", ctypes.c_void_p),
+        ("ms", ctypes.POINTER(MsView)),
+        ("directions", _dp),
+        ("n_directions", ctypes.c_uint32),
+        ("slice_start", ctypes.c_int32),
+        ("slice_interval", ctypes.c_int32),
+        ("slice_end", ctypes.c_int32),
+    ]
+
+
+BATCH_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(Solution))
 
 
 class SynthConfig(ctypes.Structure):
@@ -141,8 +158,10 @@ EXPORTS = [
     "leap_synth_create", "leap_synth_destroy", "leap_synth_view", "leap_synth_sky", "leap_synth_fill_vis_host",
     "leap_hip_synth_fill_vis_device",
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
-    "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_phase_matrix", "leap_phase_matrix_pinv",
-    "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw",
+    "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_workgroups_per_cu",
+    "leap_phase_matrix", "leap_phase_matrix_pinv",
+    "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
+    "leap_multi_item_cost",
 ]
 
 
@@ -174,6 +193,13 @@ def lib():
     L.leap_hip_calibrate_md.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp,
                                         ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.POINTER(Options), SOLUTION_CB, ctypes.c_void_p]
+    L.leap_hip_calibrate_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchSet), ctypes.c_uint32,
+                                           ctypes.POINTER(Options), BATCH_CB, ctypes.c_void_p]
+    _u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.leap_multi_schedule.argtypes = [_u32p, _u32p, _u32p, _u32p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
+                                      _u32p, _u32p, _u32p, _dp]
+    L.leap_multi_item_cost.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
+    L.leap_multi_item_cost.restype = ctypes.c_double
     L.leap_hip_metadata_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), ctypes.c_int32,
                                            ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
     L.leap_hip_metadata_destroy.argtypes = [ctypes.c_void_p]
@@ -204,6 +230,7 @@ def lib():
     L.leap_hip_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.leap_hip_last_kernel_times.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp]
     L.leap_hip_last_direction_block.argtypes = [ctypes.c_void_p]
+    L.leap_hip_last_workgroups_per_cu.argtypes = [ctypes.c_void_p]
     L.leap_hip_residual_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp, _dp,
                                               ctypes.c_uint32, _dp, _dp, _dp, _dp, _dp]
     L.leap_synth_enu_to_uvw.argtypes = [ctypes.c_void_p, _dp]
@@ -335,6 +362,28 @@ def view_from_arrays(ms, vis=None, vis_device=None, vis_dtype=None, uvw_device=N
     return v
 
 
+def multi_schedule(shapes, tile, n_blocks):
+    """Host schedule of the multi-set kernel for shapes [(nbl, T, C, D)]:
+    (offsets [n_blocks + 1], item_set, item_local, makespan)."""
+    sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.uint32).reshape(-1, 4))
+    cols = [np.ascontiguousarray(sh[:, i]) for i in range(4)]
+    db = tile // 10
+    n_items = int(sum(int(n) * ((int(d) + db - 1) // db) for n, _, _, d in sh if d > 0)) if db else 0
+    off = np.zeros(n_blocks + 1, dtype=np.uint32)
+    iset = np.zeros(max(1, n_items), dtype=np.uint32)
+    iloc = np.zeros(max(1, n_items), dtype=np.uint32)
+    mk = ctypes.c_double()
+    u = ctypes.POINTER(ctypes.c_uint32)
+    _check(lib().leap_multi_schedule(*[c.ctypes.data_as(u) for c in cols], len(sh), tile, n_blocks,
+                                     off.ctypes.data_as(u), iset.ctypes.data_as(u), iloc.ctypes.data_as(u),
+                                     ctypes.byref(mk)))
+    return off, iset[:n_items], iloc[:n_items], mk.value
+
+
+def multi_item_cost(T, C, tile):
+    return lib().leap_multi_item_cost(T, C, tile)
+
+
 # ---------------------------------------------------------------------------
 # Calibrator context
 # ---------------------------------------------------------------------------
@@ -415,8 +464,42 @@ class Context:
         self._chk(rc)
         return np.array(cals), np.array(epochs)
 
+    def calibrate_batch(self, sets, opts=None):
+        """sets: [(md, view, directions, slice_)]; returns per set (cal [solutions][D][N], epochs [solutions][2])."""
+        keep = []
+        arr = (BatchSet * max(1, len(sets)))()
+        for i, (md, view, directions, slice_) in enumerate(sets):
+            dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)
+            keep.append(dirs)
+            arr[i].metadata = md.h
+            arr[i].ms = ctypes.pointer(view)
+            arr[i].directions = _dptr(dirs)
+            arr[i].n_directions = len(dirs)
+            arr[i].slice_start, arr[i].slice_interval, arr[i].slice_end = slice_
+        cals = [[] for _ in sets]
+        epochs = [[] for _ in sets]
+        order = []
+
+        def cb(_user, set_index, sol_p):
+            s = sol_p.contents
+            n = s.n_directions * s.n_antennas
+            cals[set_index].append(np.ctypeslib.as_array(s.calibrations, shape=(n,)).copy().reshape(
+                s.n_directions, s.n_antennas) if n else np.zeros((s.n_directions, s.n_antennas)))
+            epochs[set_index].append((s.start_epoch, s.end_epoch))
+            order.append((s.index, set_index))
+            return 0
+
+        cbf = BATCH_CB(cb)
+        o = opts if opts is not None else options(strategy=LEAP_STRATEGY_BATCHED)
+        self._chk(lib().leap_hip_calibrate_batch(self.h, arr, len(sets), ctypes.byref(o), cbf, None))
+        self.last_batch_order = order
+        return [(np.array(c), np.array(e)) for c, e in zip(cals, epochs)]
+
     def last_direction_block(self):
         return int(lib().leap_hip_last_direction_block(self.h))
+
+    def last_workgroups_per_cu(self):
+        return int(lib().leap_hip_last_workgroups_per_cu(self.h))
 
     def last_kernel_times(self):
         a = ctypes.c_double()

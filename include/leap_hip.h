@@ -119,7 +119,9 @@ typedef struct leap_options {
                                  sincos, Kahan sums (faster; outside the 1e-5 rad budget on
                                  decorrelated baselines); 0 = full-row FP64 Horner */
     int32_t lane_directions;  /* batched FP64 Horner: directions per lane (1, 2, 4); 0 = tuned */
-    int32_t reserved[2];
+    int32_t workgroups_per_cu; /* multi-set launch: persistent workgroups per CU (1..64);
+                                  -1 = one work item per workgroup; 0 = tuned */
+    int32_t reserved;
 } leap_options;
 
 /* One solution interval's result: cpu::Calibration (model/cpu/calibration/Calibration.h:42-111)
@@ -165,6 +167,49 @@ int leap_hip_calibrate_md(leap_ctx* ctx, const leap_metadata* md, const leap_ms_
                           const double* directions, uint32_t n_directions, int32_t slice_start,
                           int32_t slice_interval, int32_t slice_end, const leap_options* opts,
                           leap_solution_cb callback, void* user);
+
+/* ---- several measurement sets in one launch -------------------------------- */
+/* One measurement set of leap_hip_calibrate_batch: the metadata built for it by
+ * leap_hip_metadata_create, its directions and its Slice (as leap_hip_calibrate). */
+typedef struct leap_batch_set {
+    const leap_metadata* metadata;
+    const leap_ms_view* ms;
+    const double* directions;  /* [n_directions][2] (ra, dec) */
+    uint32_t n_directions;
+    int32_t slice_start, slice_interval, slice_end;
+} leap_batch_set;
+
+/* Called once per (set, solution interval): interval s of every set that has
+ * one, in set order, before interval s + 1.  Return 0 to continue. */
+typedef int (*leap_batch_cb)(void* user, uint32_t set, const leap_solution* solution);
+
+/* Calibrate of several measurement sets with uneven antenna, baseline, channel
+ * and timestep counts: each set gets the solutions leap_hip_calibrate_md would
+ * give it with the same options.  Solution interval s of all sets, all
+ * directions, is rotated and averaged in ONE persistent launch: a device
+ * descriptor per set and (set, baseline, direction block) work items
+ * statically load-balanced over the grid on the host (longest processing time
+ * first), so the result does not depend on scheduling.  Needs the batched
+ * strategy, all_pols = 0 and fp32_blocks = 0.  No reference counterpart: the
+ * reference calibrates one measurement set per ILeapCalibrator::Calibrate call
+ * (algorithm/ILeapCalibrator.h:62-68). */
+int leap_hip_calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
+                             const leap_options* opts, leap_batch_cb callback, void* user);
+
+/* The static schedule leap_hip_calibrate_batch uses, on the host (no GPU):
+ * the work items of sets with the given shapes for the persistent kernel tile
+ * (directions per workgroup * 10 + directions per lane) over n_blocks
+ * workgroups.  Workgroup g runs items [offsets[g], offsets[g + 1]);
+ * item_set / item_local give the set and baseline + nbl * direction block of
+ * each.  offsets holds n_blocks + 1 entries; item arrays hold
+ * sum_s nbl_s * ceil(D_s / (tile / 10)) entries; *makespan_out (may be NULL)
+ * gets the largest per-workgroup cost in the cost model's units. */
+int leap_multi_schedule(const uint32_t* nbl, const uint32_t* timesteps, const uint32_t* channels,
+                        const uint32_t* n_directions, uint32_t n_sets, int32_t tile, uint32_t n_blocks,
+                        uint32_t* offsets, uint32_t* item_set, uint32_t* item_local, double* makespan_out);
+/* Cost-model units of one work item of a set with the given shape (the weights the
+ * schedule balances). */
+double leap_multi_item_cost(uint32_t timesteps, uint32_t channels, int32_t tile);
 
 /* ---- metadata (A, I, A1, I1 and their pseudo-inverses, on the device) ------ */
 int leap_hip_metadata_create(leap_ctx* ctx, const leap_ms_view* ms, int32_t reference_antenna,
@@ -286,8 +331,12 @@ int leap_hip_memcpy_h2d(leap_ctx* ctx, void* dst, const void* src, uint64_t byte
  * launches in the last call, measured with HIP events on the context's stream. */
 int leap_hip_last_kernel_times(const leap_ctx* ctx, double* rotate_ms_avg, int32_t* rotate_launches,
                                double* total_ms);
-/* Direction block the batched kernel used in the last call (0 if none). */
+/* Tile the batched kernel used in the last call: directions per workgroup * 10
+ * + directions per lane (0 if none). */
 int leap_hip_last_direction_block(const leap_ctx* ctx);
+/* Workgroups per CU of the last multi-set launch (-1: one work item per
+ * workgroup; 0 if none). */
+int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx);
 
 #ifdef __cplusplus
 }

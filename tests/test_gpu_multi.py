@@ -1,0 +1,144 @@
+"""Several measurement sets of uneven shape in one persistent launch.
+
+Small sets run against the oracle (FP64 <= 1e-9 rad wrapped, mixed <= 1e-5
+rad); the three sets of the uneven benchmark batch run at full size against
+the single-set batched path with the same tile, which must agree bit for bit
+(the same FP64 Horner tile code runs per work item), and against the oracle
+for a few directions of the two smaller sets.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import np_restatement as npr
+
+pytestmark = pytest.mark.gpu
+
+
+def wrapped(a, b):
+    return np.abs(npr.wrap(np.asarray(a) - np.asarray(b)))
+
+
+def dirs_near(centre, n, seed, spread=0.02):
+    rng = np.random.default_rng(seed)
+    return np.column_stack([centre[0] + rng.uniform(-spread, spread, n), centre[1] + rng.uniform(-spread, spread, n)])
+
+
+SMALL = [  # synth kwargs, directions, slice, reference antenna, visibility dtype
+    (dict(n_antennas=12, autocorrelations=0, n_timesteps=6, n_channels=40, seed=51, nan_fraction=0.003), 5,
+     (0, 2, -1), 0, np.complex128),
+    (dict(n_antennas=20, n_timesteps=4, n_channels=9, seed=52, flag_fraction=0.05), 70, (0, -1, -1), None,
+     np.complex128),
+    (dict(n_antennas=9, n_timesteps=5, n_channels=300, seed=53), 1, (0, 1, -1), 3, np.complex64),
+    (dict(n_antennas=7, n_timesteps=2, n_channels=3, seed=54), 0, (0, -1, -1), None, np.complex128),
+]
+
+
+@pytest.fixture(scope="module")
+def small_sets(leap):
+    out = []
+    for kw, nd, slice_, ref, dt in SMALL:
+        s = leap.SynthMS(**kw)
+        s.fill_host(np.complex128)
+        ms = s.host_dict(s.vis.astype(dt) if dt == np.complex64 else s.vis)
+        out.append((s, ms, dirs_near(ms["phase_centre"], nd, nd + 1), slice_, ref))
+    return out
+
+
+def _run_small(leap, ctx, small_sets, opts):
+    views = [leap.view_from_arrays(ms) for _, ms, _, _, _ in small_sets]
+    mds = [ctx.metadata(v, ref_ant=ref) for v, (_, _, _, _, ref) in zip(views, small_sets)]
+    res = ctx.calibrate_batch([(md, v, d, sl) for md, v, (_, _, d, sl, _) in zip(mds, views, small_sets)], opts)
+    return res
+
+
+@pytest.mark.parametrize("db,nd", [(0, 0), (32, 1), (64, 1), (128, 1), (128, 2), (256, 2)])
+def test_multi_fp64_matches_oracle(leap, ctx, oracle, small_sets, db, nd):
+    opts = leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db, lane_directions=nd)
+    res = _run_small(leap, ctx, small_sets, opts)
+    if db:
+        assert ctx.last_direction_block() == db * 10 + max(nd, 1)
+    for (s, ms, dirs, slice_, ref), (got, ge) in zip(small_sets, res):
+        want, we = oracle.calibrate(ms, dirs, slice_, ref_ant=ref)
+        assert got.shape == want.shape
+        assert np.array_equal(ge, we)
+        if got.size:
+            assert wrapped(got, want).max() < 1e-9
+    # callbacks come interval by interval, sets in order within an interval
+    assert ctx.last_batch_order == sorted(ctx.last_batch_order)
+
+
+def test_multi_mixed_matches_oracle(leap, ctx, oracle, small_sets):
+    opts = leap.options(precision=leap.LEAP_PRECISION_MIXED, strategy=leap.LEAP_STRATEGY_BATCHED)
+    res = _run_small(leap, ctx, small_sets, opts)
+    worst = 0.0
+    for (s, ms, dirs, slice_, ref), (got, _) in zip(small_sets, res):
+        want, _ = oracle.calibrate(ms, dirs, slice_, ref_ant=ref)
+        if got.size:
+            worst = max(worst, wrapped(got, want).max())
+    print(f"multi-set mixed: max wrapped error {worst:.2e} rad")
+    assert worst < 1e-5
+
+
+def test_multi_rejects_unsupported(leap, ctx, small_sets):
+    s, ms, dirs, slice_, ref = small_sets[0]
+    v = leap.view_from_arrays(ms)
+    md = ctx.metadata(v, ref_ant=ref)
+    with pytest.raises(leap.LeapError, match="batched strategy"):
+        ctx.calibrate_batch([(md, v, dirs, slice_)], leap.options())
+    with pytest.raises(leap.LeapError, match="polarisation 0"):
+        ctx.calibrate_batch([(md, v, dirs, slice_)], leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, all_pols=1))
+    with pytest.raises(leap.LeapError, match="not a multi-set tile"):
+        ctx.calibrate_batch([(md, v, dirs, slice_)], leap.options(strategy=leap.LEAP_STRATEGY_BATCHED,
+                                                                  direction_block=256, lane_directions=4))
+    assert ctx.calibrate_batch([], leap.options(strategy=leap.LEAP_STRATEGY_BATCHED)) == []
+
+
+# The uneven benchmark batch: antennas, autocorrelations, timesteps, channels.
+UNEVEN = [(36, 0, 32, 300), (128, 1, 14, 48), (256, 1, 64, 96)]
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_uneven_batch_full_size(leap, ctx, oracle, precision):
+    dt = np.complex64 if precision else np.complex128
+    vd = leap.LEAP_VIS_C64 if precision else leap.LEAP_VIS_C128
+    sets, ptrs = [], []
+    try:
+        for i, (n_ant, autos, T, C) in enumerate(UNEVEN):
+            s = leap.SynthMS(n_antennas=n_ant, autocorrelations=autos, n_timesteps=T, n_channels=C,
+                             freq_start_hz=50e6, freq_inc_hz=781.25e3 * 384 / C, seed=60 + i)
+            nbytes = s.num_rows * s.num_channels * s.num_pols * np.dtype(dt).itemsize
+            vis_p = ctx.malloc(nbytes)
+            uvw_p = ctx.malloc(s.num_rows * 3 * 8)
+            ptrs += [vis_p, uvw_p]
+            ms = s.host_dict(None)
+            host = None
+            if i < 2:  # host cube for the oracle spot check, uploaded as is
+                host = s.fill_host(dt)
+                ctx.h2d(vis_p, host)
+                ctx.h2d(uvw_p, np.ascontiguousarray(s.uvw))
+            else:
+                ctx.synth_fill_device(s, vd, vis_p, uvw_p)
+            view = leap.view_from_arrays(ms, vis_device=vis_p, vis_dtype=vd, uvw_device=uvw_p)
+            dirs = dirs_near(ms["phase_centre"], 64, 70 + i, spread=0.1)
+            sets.append((s, ms, host, view, ctx.metadata(view), dirs))
+        for db, nd in ((128, 2), (64, 1)):
+            opts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
+                                lane_directions=nd)
+            res = ctx.calibrate_batch([(md, v, d, (0, -1, -1)) for (_, _, _, v, md, d) in sets], opts)
+            multi_ms = ctx.last_kernel_times()[0]
+            single_ms = 0.0
+            for (s, ms, host, view, md, dirs), (got, _) in zip(sets, res):
+                one, _ = ctx.calibrate(view, dirs, opts=opts, md=md)
+                single_ms += ctx.last_kernel_times()[0]
+                assert np.array_equal(got, one), "multi-set and single-set results differ"
+            print(f"uneven batch {'mixed' if precision else 'fp64'} tile {db}x{nd}: one launch {multi_ms:.3f} ms, "
+                  f"three single-set launches {single_ms:.3f} ms")
+        for (s, ms, host, view, md, dirs), (got, _) in zip(sets[:2], res[:2]):
+            pick = [0, 33, 63]
+            want, _ = oracle.calibrate(dict(ms, vis=host), dirs[pick])
+            assert wrapped(got[:, pick], want).max() < (1e-5 if precision else 1e-9)
+    finally:
+        for p in ptrs:
+            ctx.free(p)
