@@ -69,6 +69,7 @@ leap_options resolve(const leap_options* o) {
             "the FP32 blocked kernel runs one direction per lane");
     require(r.lane_directions == 0 || r.direction_block == 0 || r.direction_block >= 32 * r.lane_directions,
             LEAP_ERR_INVALID_ARGUMENT, "direction block too small for the lane directions");
+    require(r.graph == 0 || r.graph == 1, LEAP_ERR_INVALID_ARGUMENT, "graph must be 0 or 1");
     require(r.workgroups_per_cu >= -1 && r.workgroups_per_cu <= 64, LEAP_ERR_INVALID_ARGUMENT,
             "workgroups per CU must be -1 (one work item each), 0 (tuned) or 1 to 64");
     return r;
@@ -79,6 +80,7 @@ size_t vis_elem_bytes(const leap_ms_view& ms) {
 }
 
 void record_rotate_begin(leap_ctx* ctx) {
+    if (ctx->capturing) return;  // a captured sequence is timed as a whole
     if (ctx->rotate_events_used == ctx->rotate_events.size()) {
         hipEvent_t a, b;
         LEAP_HIP(hipEventCreate(&a));
@@ -89,6 +91,7 @@ void record_rotate_begin(leap_ctx* ctx) {
 }
 
 void record_rotate_end(leap_ctx* ctx) {
+    if (ctx->capturing) return;
     LEAP_HIP(hipEventRecord(ctx->rotate_events[ctx->rotate_events_used].second, ctx->stream));
     ++ctx->rotate_events_used;
 }
@@ -219,11 +222,11 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
 
 // avgData of D directions for T timesteps starting at the staged rows, into ctx->d_avg.
 void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
-                       uint32_t T, uint32_t D, const leap_options& opts) {
+                       uint32_t T, uint32_t D, const leap_options& opts, int32_t tile = 0) {
     const size_t per_dir = size_t(md.nbl) * md.pols * 2;
     ctx->d_avg.reserve(per_dir * D * sizeof(double));
     if (opts.strategy == LEAP_STRATEGY_BATCHED) {
-        const int32_t db = tuned_direction_block(ctx, md, ms, rows, T, D, opts);
+        const int32_t db = tile ? tile : tuned_direction_block(ctx, md, ms, rows, T, D, opts);
         ctx->last_direction_block = db;
         record_rotate_begin(ctx);
         launch_rotate_batched(ctx->stream, opts.precision, db, opts.all_pols != 0, rows.vis, ms.vis_dtype, rows.uvw,
@@ -345,6 +348,141 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
     return md.release();
 }
 
+// ---- HIP graphs ---------------------------------------------------------------
+
+// Device buffers one solution interval uses, reserved before a capture so the
+// captured sequence allocates nothing (the sizes rotate_directions and
+// invert_directions ask for).
+void reserve_solution(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const leap_options& opts) {
+    const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
+    ctx->d_avg.reserve(size_t(md.nbl) * md.pols * 2 * D * sizeof(double));
+    ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
+    ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
+    ctx->d_delta.reserve(size_t(K) * D * sizeof(double));
+    ctx->d_cal.reserve(size_t(N) * D * sizeof(double));
+    if (opts.strategy != LEAP_STRATEGY_BATCHED) {
+        ctx->d_partial.reserve(inversion_partial_doubles(K, N, D) * sizeof(double));
+    } else if (opts.precision == LEAP_PRECISION_MIXED) {
+        ctx->d_a_planes.reserve(bf16_planes_bytes((D + 63) / 64 * 64, K));
+        ctx->d_partial.reserve(mfma_partial_floats(K, D, N) * sizeof(float));
+    } else {
+        ctx->d_partial.reserve(gemm_f64_partial_doubles(K, N, D) * sizeof(double));
+    }
+}
+
+uint64_t key_bits(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, sizeof(u));
+    return u;
+}
+
+uint64_t key_ptr(const void* p) { return uint64_t(reinterpret_cast<uintptr_t>(p)); }
+
+// What the kernels of a captured sequence take as arguments: options, shapes
+// and device addresses.  A sequence is replayed only for an equal key.
+void key_common(std::vector<uint64_t>& k, const leap_ctx* ctx, const leap_options& opts) {
+    k.insert(k.end(), {uint64_t(opts.precision), uint64_t(opts.strategy), uint64_t(opts.all_pols),
+                       uint64_t(opts.fp32_blocks), key_ptr(ctx->d_dd.ptr), key_ptr(ctx->d_avg.ptr),
+                       key_ptr(ctx->d_phase.ptr), key_ptr(ctx->d_cal1.ptr), key_ptr(ctx->d_delta.ptr),
+                       key_ptr(ctx->d_cal.ptr), key_ptr(ctx->d_partial.ptr), key_ptr(ctx->d_a_planes.ptr)});
+}
+
+void key_set(std::vector<uint64_t>& k, const leap_metadata& md, const leap_ms_view& ms, uint32_t D,
+             int32_t solutions, uint32_t interval) {
+    k.insert(k.end(), {key_ptr(&md), uint64_t(md.A.rows), uint64_t(md.A.cols), uint64_t(md.A1.rows), md.nbl,
+                       md.channels, md.pols, key_bits(md.freq_start_hz), key_bits(md.freq_inc_hz),
+                       key_ptr(md.d_AdT.ptr), key_ptr(md.d_Ad1T.ptr), key_ptr(md.d_I1.ptr), key_ptr(md.d_pairs.ptr),
+                       key_ptr(md.d_lambda.ptr), key_ptr(md.d_AdT_planes.ptr), key_ptr(ms.vis),
+                       key_ptr(ms.uvw_device), uint64_t(ms.vis_dtype), ms.num_rows, D, uint64_t(solutions),
+                       interval});
+}
+
+// Records what enqueue() puts on the context's stream into seq's graph.
+template <typename F>
+void capture(leap_ctx* ctx, CapturedSequence& seq, F&& enqueue) {
+    LEAP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    try {
+        enqueue();
+    } catch (...) {
+        ctx->capturing = false;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    ctx->capturing = false;
+    LEAP_HIP(hipStreamEndCapture(ctx->stream, &seq.graph));
+    LEAP_HIP(hipGraphInstantiate(&seq.exec, seq.graph, nullptr, nullptr, 0));
+}
+
+void require_device_resident(const leap_ms_view& ms) {
+    require(ms.vis_on_device && ms.uvw_device != nullptr, LEAP_ERR_INVALID_ARGUMENT,
+            "graph capture needs device-resident visibilities and uvw (vis_on_device, uvw_device)");
+}
+
+// calibrate_loop with every solution interval's rotate/average and inversion
+// captured once as a HIP graph and replayed while the key holds; results of
+// all intervals are copied back together and handed to the callback in order.
+void calibrate_loop_graph(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const double* dirs,
+                          uint32_t D, int32_t solutions, uint32_t interval, const leap_options& opts,
+                          leap_solution_cb cb, void* user) {
+    require_device_resident(ms);
+    const uint32_t nbl = ms.num_baselines;
+    const int32_t N = md.A.cols;
+    const size_t per_sol = size_t(N) * D;
+    begin_timing(ctx);
+    CapturedSequence* seq = nullptr;
+    if (D > 0 && solutions > 0) {
+        upload_directions(ctx, md, dirs, D);
+        reserve_solution(ctx, md, D, opts);
+        int32_t tile = 0;
+        if (opts.strategy == LEAP_STRATEGY_BATCHED) {
+            const DeviceRows rows0 = stage_rows(ctx, ms, 0, interval * nbl);
+            tile = tuned_direction_block(ctx, md, ms, rows0, interval, D, opts);
+            ctx->last_direction_block = tile;
+        }
+        std::vector<uint64_t> key{uint64_t(tile)};
+        key_common(key, ctx, opts);
+        key_set(key, md, ms, D, solutions, interval);
+        if (!ctx->seq_single || ctx->seq_single->key != key) {
+            ctx->seq_single = std::make_unique<CapturedSequence>();
+            seq = ctx->seq_single.get();
+            seq->key = key;
+            seq->d_cal.reserve(per_sol * size_t(solutions) * sizeof(double));
+            capture(ctx, *seq, [&] {
+                for (int32_t s = 0; s < solutions; ++s) {
+                    const DeviceRows rows = stage_rows(ctx, ms, uint32_t(size_t(s) * interval * nbl),
+                                                       uint32_t(size_t(s + 1) * interval * nbl));
+                    rotate_directions(ctx, md, ms, rows, interval, D, opts, tile);
+                    invert_directions(ctx, md, D, opts, nullptr, seq->d_cal.as<double>() + per_sol * size_t(s));
+                }
+            });
+            ctx->last_graph = 1;
+        } else {
+            seq = ctx->seq_single.get();
+            ctx->last_graph = 2;
+        }
+        LEAP_HIP(hipGraphLaunch(seq->exec, ctx->stream));
+        seq->h_cal.resize(per_sol * size_t(solutions));
+        LEAP_HIP(hipMemcpyAsync(seq->h_cal.data(), seq->d_cal.ptr, seq->h_cal.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    }
+    LEAP_HIP(hipStreamSynchronize(ctx->stream));
+    end_timing(ctx);
+    for (int32_t s = 0; s < solutions; ++s) {
+        leap_solution sol{};
+        sol.index = uint32_t(s);
+        sol.start_epoch = ms.time[size_t(s) * interval * nbl];  // CpuLeapCalibrator.cc:107-109
+        sol.end_epoch = ms.time[(size_t(s + 1) * interval - 1) * nbl];
+        sol.n_directions = D;
+        sol.directions = dirs;
+        sol.n_antennas = uint32_t(N);
+        sol.calibrations = seq ? seq->h_cal.data() + per_sol * size_t(s) : nullptr;
+        if (cb && cb(user, &sol) != 0) throw Error(LEAP_ERR_CALLBACK, "output callback requested a stop");
+    }
+}
+
 // The Calibrate loop (CpuLeapCalibrator.cc:61-145) over prepared metadata.
 void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const double* dirs, uint32_t D,
                     int32_t s_start, int32_t s_interval, int32_t s_end, const leap_options& opts,
@@ -359,6 +497,11 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
     const uint32_t interval = uint32_t(range[1]);
     const uint32_t nbl = ms.num_baselines;
     const int32_t N = md.A.cols;
+    if (opts.graph) {
+        calibrate_loop_graph(ctx, md, ms, dirs, D, solutions, interval, opts, cb, user);
+        return;
+    }
+    ctx->last_graph = 0;
     begin_timing(ctx);
     if (D > 0) upload_directions(ctx, md, dirs, D);
     ctx->h_cal.resize(size_t(N) * D);
@@ -401,29 +544,56 @@ uint32_t multi_grid(leap_ctx* ctx, int32_t tile, int32_t per_cu, size_t n_items)
     return uint32_t(std::min(g, std::max<size_t>(n_items, 1)));
 }
 
-// Plans, uploads and launches one multi-set rotate/average.
+// Device tables of one multi-set launch inside a byte block: descriptors, work
+// items and per-workgroup item offsets, each 256-byte aligned.
+struct MultiTables {
+    size_t descs = 0, items = 0, offsets = 0;  // byte offsets in the block
+    uint32_t blocks = 0;
+    size_t n_items = 0;
+};
+
+// Plans one launch (host LPT schedule) and appends its tables to bytes.
+MultiTables append_tables(leap_ctx* ctx, std::vector<uint8_t>& bytes, const std::vector<MultiSetDesc>& descs,
+                          const std::vector<MultiShape>& shapes, int32_t tile, int32_t per_cu) {
+    MultiTables t;
+    const uint32_t db = uint32_t(tile / 10);
+    for (const MultiShape& m : shapes) t.n_items += size_t(m.nbl) * ((m.D + db - 1) / db);
+    t.blocks = multi_grid(ctx, tile, per_cu, t.n_items);
+    const MultiPlan plan = plan_multi(shapes, tile, t.blocks);
+    auto put = [&](const void* src, size_t n) {
+        const size_t at = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(at + std::max<size_t>(n, 1), 0);
+        if (n) std::memcpy(bytes.data() + at, src, n);
+        return at;
+    };
+    t.descs = put(descs.data(), descs.size() * sizeof(MultiSetDesc));
+    t.items = put(plan.items.data(), plan.items.size() * sizeof(MultiItem));
+    t.offsets = put(plan.off.data(), plan.off.size() * sizeof(uint32_t));
+    return t;
+}
+
+void launch_tables(leap_ctx* ctx, const void* block, const MultiTables& t, int32_t tile) {
+    if (t.n_items == 0) return;
+    const char* b = static_cast<const char*>(block);
+    launch_rotate_multi(ctx->stream, tile, reinterpret_cast<const MultiSetDesc*>(b + t.descs),
+                        reinterpret_cast<const uint2*>(b + t.items), reinterpret_cast<const uint32_t*>(b + t.offsets),
+                        t.blocks);
+}
+
+// Plans, uploads and launches one multi-set rotate/average.  The host copy of
+// the tables stays in the context until the next launch.
 void run_multi(leap_ctx* ctx, const std::vector<MultiSetDesc>& descs, const std::vector<MultiShape>& shapes,
                int32_t tile, int32_t per_cu, hipEvent_t after_upload = nullptr, bool timed = false) {
-    size_t n_items = 0;
-    const uint32_t db = uint32_t(tile / 10);
-    for (const MultiShape& m : shapes) n_items += size_t(m.nbl) * ((m.D + db - 1) / db);
-    const uint32_t blocks = multi_grid(ctx, tile, per_cu, n_items);
-    const MultiPlan plan = plan_multi(shapes, tile, blocks);
-    ctx->d_descs.reserve(descs.size() * sizeof(MultiSetDesc));
-    ctx->d_items.reserve(std::max<size_t>(plan.items.size(), 1) * sizeof(MultiItem));
-    ctx->d_offsets.reserve(plan.off.size() * sizeof(uint32_t));
-    LEAP_HIP(hipMemcpyAsync(ctx->d_descs.ptr, descs.data(), descs.size() * sizeof(MultiSetDesc),
-                            hipMemcpyHostToDevice, ctx->stream));
-    if (!plan.items.empty())
-        LEAP_HIP(hipMemcpyAsync(ctx->d_items.ptr, plan.items.data(), plan.items.size() * sizeof(MultiItem),
-                                hipMemcpyHostToDevice, ctx->stream));
-    LEAP_HIP(hipMemcpyAsync(ctx->d_offsets.ptr, plan.off.data(), plan.off.size() * sizeof(uint32_t),
-                            hipMemcpyHostToDevice, ctx->stream));
+    LEAP_HIP(hipStreamSynchronize(ctx->stream));  // the previous upload has left h_tables
+    ctx->h_tables.clear();
+    const MultiTables t = append_tables(ctx, ctx->h_tables, descs, shapes, tile, per_cu);
+    ctx->d_tables.reserve(ctx->h_tables.size());
+    LEAP_HIP(hipMemcpyAsync(ctx->d_tables.ptr, ctx->h_tables.data(), ctx->h_tables.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
     if (after_upload) LEAP_HIP(hipEventRecord(after_upload, ctx->stream));
-    if (n_items == 0) return;
+    if (t.n_items == 0) return;
     if (timed) record_rotate_begin(ctx);
-    launch_rotate_multi(ctx->stream, tile, ctx->d_descs.as<MultiSetDesc>(), ctx->d_items.as<uint2>(),
-                        ctx->d_offsets.as<uint32_t>(), blocks);
+    launch_tables(ctx, ctx->d_tables.ptr, t, tile);
     if (timed) record_rotate_end(ctx);
 }
 
@@ -552,11 +722,9 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
         launch_dd(ctx->stream, b.md->phase_centre_ra, b.md->phase_centre_dec, buf.d_dirs.as<double>(), b.D,
                   buf.d_dd.as<double>());
     }
-    std::pair<int32_t, int32_t> launch{0, 0};  // tile, workgroups per CU
-    for (int32_t r = 0; r < rounds; ++r) {
-        std::vector<MultiSetDesc> descs;
-        std::vector<MultiShape> shapes;
-        std::vector<uint32_t> active;
+    // Descriptors of round r: solution interval r of every set that has one.
+    auto round_descs = [&](int32_t r, std::vector<MultiSetDesc>& descs, std::vector<MultiShape>& shapes,
+                           std::vector<uint32_t>& active) {
         for (uint32_t i = 0; i < n_sets; ++i) {
             BatchState& b = st[i];
             if (r >= b.solutions || b.D == 0) continue;
@@ -572,6 +740,104 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             shapes.push_back(MultiShape{nbl, b.interval, b.md->channels, b.D});
             active.push_back(i);
         }
+    };
+    auto deliver = [&](int32_t r, auto&& cal_of) {
+        for (uint32_t i = 0; i < n_sets; ++i) {
+            const BatchState& b = st[i];
+            if (r >= b.solutions) continue;
+            const uint32_t nbl = b.md->nbl;
+            leap_solution sol{};
+            sol.index = uint32_t(r);
+            sol.start_epoch = b.ms->time[size_t(r) * b.interval * nbl];  // CpuLeapCalibrator.cc:107-109
+            sol.end_epoch = b.ms->time[(size_t(r + 1) * b.interval - 1) * nbl];
+            sol.n_directions = b.D;
+            sol.directions = b.dirs;
+            sol.n_antennas = uint32_t(b.md->A.cols);
+            sol.calibrations = cal_of(i);
+            if (cb && cb(user, i, &sol) != 0) throw Error(LEAP_ERR_CALLBACK, "output callback requested a stop");
+        }
+    };
+    std::pair<int32_t, int32_t> launch{0, 0};  // tile, workgroups per CU
+    if (opts.graph) {
+        // Every round's multi-set launch and inversions captured as one graph.
+        const size_t n_rounds = static_cast<size_t>(rounds);
+        std::vector<std::vector<MultiSetDesc>> rdescs(n_rounds);
+        std::vector<std::vector<MultiShape>> rshapes(n_rounds);
+        std::vector<std::vector<uint32_t>> ractive(n_rounds);
+        std::vector<size_t> cal_at(n_sets, 0);
+        size_t cal_total = 0;
+        for (uint32_t i = 0; i < n_sets; ++i) {
+            const BatchState& b = st[i];
+            cal_at[i] = cal_total;
+            if (b.D == 0 || b.solutions == 0) continue;
+            require_device_resident(*b.ms);
+            reserve_solution(ctx, *b.md, b.D, opts);
+            cal_total += size_t(b.md->A.cols) * b.D * size_t(b.solutions);
+        }
+        for (int32_t r = 0; r < rounds; ++r) round_descs(r, rdescs[r], rshapes[r], ractive[r]);
+        for (int32_t r = 0; r < rounds && launch.first == 0; ++r)
+            if (!rdescs[r].empty()) launch = multi_tile(ctx, rdescs[r], rshapes[r], opts);
+        ctx->last_direction_block = launch.first;
+        ctx->last_workgroups_per_cu = launch.second;
+        std::vector<uint64_t> key{uint64_t(uint32_t(launch.first)), uint64_t(uint32_t(launch.second)), n_sets};
+        key_common(key, ctx, opts);
+        for (uint32_t i = 0; i < n_sets; ++i) {
+            const BatchState& b = st[i];
+            key_set(key, *b.md, *b.ms, b.D, b.solutions, b.interval);
+            key.insert(key.end(), {key_ptr(ctx->batch_sets[i]->d_dd.ptr), key_ptr(ctx->batch_sets[i]->d_avg.ptr)});
+        }
+        CapturedSequence* seq = ctx->seq_batch.get();
+        if (cal_total > 0 && (!seq || seq->key != key)) {
+            ctx->seq_batch = std::make_unique<CapturedSequence>();
+            seq = ctx->seq_batch.get();
+            seq->key = key;
+            std::vector<uint8_t> bytes;
+            std::vector<MultiTables> tables(n_rounds);
+            for (int32_t r = 0; r < rounds; ++r)
+                if (!rdescs[r].empty())
+                    tables[r] = append_tables(ctx, bytes, rdescs[r], rshapes[r], launch.first, launch.second);
+            seq->d_tables.reserve(bytes.size());
+            LEAP_HIP(hipMemcpyAsync(seq->d_tables.ptr, bytes.data(), bytes.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+            LEAP_HIP(hipStreamSynchronize(ctx->stream));
+            seq->d_cal.reserve(cal_total * sizeof(double));
+            capture(ctx, *seq, [&] {
+                for (int32_t r = 0; r < rounds; ++r) {
+                    launch_tables(ctx, seq->d_tables.ptr, tables[r], launch.first);
+                    for (uint32_t i : ractive[r]) {
+                        const BatchState& b = st[i];
+                        const size_t per_sol = size_t(b.md->A.cols) * b.D;
+                        invert_directions(ctx, *b.md, b.D, opts, ctx->batch_sets[i]->d_avg.as<double>(),
+                                          seq->d_cal.as<double>() + cal_at[i] + per_sol * size_t(r));
+                    }
+                }
+            });
+            ctx->last_graph = 1;
+        } else {
+            ctx->last_graph = cal_total > 0 ? 2 : 0;
+        }
+        if (cal_total > 0) {
+            LEAP_HIP(hipGraphLaunch(seq->exec, ctx->stream));
+            seq->h_cal.resize(cal_total);
+            LEAP_HIP(hipMemcpyAsync(seq->h_cal.data(), seq->d_cal.ptr, cal_total * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        }
+        LEAP_HIP(hipStreamSynchronize(ctx->stream));
+        end_timing(ctx);
+        for (int32_t r = 0; r < rounds; ++r)
+            deliver(r, [&](uint32_t i) -> const double* {
+                const BatchState& b = st[i];
+                if (b.D == 0) return b.h_cal.data();
+                return seq->h_cal.data() + cal_at[i] + size_t(b.md->A.cols) * b.D * size_t(r);
+            });
+        return;
+    }
+    ctx->last_graph = 0;
+    for (int32_t r = 0; r < rounds; ++r) {
+        std::vector<MultiSetDesc> descs;
+        std::vector<MultiShape> shapes;
+        std::vector<uint32_t> active;
+        round_descs(r, descs, shapes, active);
         if (!descs.empty()) {
             if (launch.first == 0) launch = multi_tile(ctx, descs, shapes, opts);
             ctx->last_direction_block = launch.first;
@@ -586,20 +852,7 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             }
             LEAP_HIP(hipStreamSynchronize(ctx->stream));
         }
-        for (uint32_t i = 0; i < n_sets; ++i) {
-            const BatchState& b = st[i];
-            if (r >= b.solutions) continue;
-            const uint32_t nbl = b.md->nbl;
-            leap_solution sol{};
-            sol.index = uint32_t(r);
-            sol.start_epoch = b.ms->time[size_t(r) * b.interval * nbl];  // CpuLeapCalibrator.cc:107-109
-            sol.end_epoch = b.ms->time[(size_t(r + 1) * b.interval - 1) * nbl];
-            sol.n_directions = b.D;
-            sol.directions = b.dirs;
-            sol.n_antennas = uint32_t(b.md->A.cols);
-            sol.calibrations = b.h_cal.data();
-            if (cb && cb(user, i, &sol) != 0) throw Error(LEAP_ERR_CALLBACK, "output callback requested a stop");
-        }
+        deliver(r, [&](uint32_t i) -> const double* { return st[i].h_cal.data(); });
     }
     LEAP_HIP(hipStreamSynchronize(ctx->stream));
     end_timing(ctx);
@@ -935,6 +1188,7 @@ double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
 
 extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ? ctx->last_direction_block : 0; }
 extern "C" int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx) { return ctx ? ctx->last_workgroups_per_cu : 0; }
+extern "C" int leap_hip_last_graph(const leap_ctx* ctx) { return ctx ? ctx->last_graph : 0; }
 
 // ---------------------------------------------------------------------------
 // Calibration-residual loss and its gradient.

@@ -259,6 +259,11 @@ uint64_t hash_phase_matrix(const PhaseMatrix& pm) {
     return h;
 }
 
+CapturedSequence::~CapturedSequence() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
 bool multi_tile_supported(int32_t tile) {
     return tile == 321 || tile == 641 || tile == 1281 || tile == 1282 || tile == 2562;
 }

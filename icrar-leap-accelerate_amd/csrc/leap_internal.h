@@ -129,6 +129,21 @@ namespace leap {
 struct SetBuffers {
     DeviceBuffer d_dirs, d_dd, d_avg, d_cal, d_vis, d_uvw;
 };
+
+// A calibration sequence captured as a HIP graph, with the key of shapes,
+// options and device addresses it was captured for.
+struct CapturedSequence {
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DeviceBuffer d_cal;     // cal of every solution interval (and set)
+    DeviceBuffer d_tables;  // multi-set descriptor tables of every round
+    std::vector<double> h_cal;
+    CapturedSequence() = default;
+    CapturedSequence(const CapturedSequence&) = delete;
+    CapturedSequence& operator=(const CapturedSequence&) = delete;
+    ~CapturedSequence();
+};
 }  // namespace leap
 
 struct leap_ctx {
@@ -151,10 +166,15 @@ struct leap_ctx {
     int32_t last_direction_block = 0;
     // multi-set launches
     std::vector<std::unique_ptr<leap::SetBuffers>> batch_sets;
-    leap::DeviceBuffer d_descs, d_items, d_offsets;
+    leap::DeviceBuffer d_tables;     // descriptor tables of the last multi-set launch
+    std::vector<uint8_t> h_tables;
     std::map<std::vector<uint32_t>, std::pair<int32_t, int32_t>> tuned_multi;  // -> tile, workgroups per CU
     int32_t last_workgroups_per_cu = 0;
     int32_t cu_count = 0;
+    // HIP graphs
+    bool capturing = false;
+    int32_t last_graph = 0;  // 0 none, 1 captured, 2 replayed
+    std::unique_ptr<leap::CapturedSequence> seq_single, seq_batch;
 };
 
 namespace leap {

@@ -76,12 +76,12 @@ class Options(ctypes.Structure):
         ("fp32_blocks", ctypes.c_int32),
         ("lane_directions", ctypes.c_int32),
         ("workgroups_per_cu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
     ]
 
 
 def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION, direction_block=0, all_pols=0,
-            fp32_blocks=0, lane_directions=0, workgroups_per_cu=0):
+            fp32_blocks=0, lane_directions=0, workgroups_per_cu=0, graph=0):
     o = Options()
     o.precision = precision
     o.strategy = strategy
@@ -90,6 +90,7 @@ def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION,
     o.fp32_blocks = fp32_blocks
     o.lane_directions = lane_directions
     o.workgroups_per_cu = workgroups_per_cu
+    o.graph = graph
     return o
 
 
@@ -159,7 +160,7 @@ EXPORTS = [
     "leap_hip_synth_fill_vis_device",
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
     "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_workgroups_per_cu",
-    "leap_phase_matrix", "leap_phase_matrix_pinv",
+    "leap_hip_last_graph", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
     "leap_multi_item_cost",
 ]
@@ -231,6 +232,7 @@ def lib():
     L.leap_hip_last_kernel_times.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp]
     L.leap_hip_last_direction_block.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_workgroups_per_cu.argtypes = [ctypes.c_void_p]
+    L.leap_hip_last_graph.argtypes = [ctypes.c_void_p]
     L.leap_hip_residual_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp, _dp,
                                               ctypes.c_uint32, _dp, _dp, _dp, _dp, _dp]
     L.leap_synth_enu_to_uvw.argtypes = [ctypes.c_void_p, _dp]
@@ -500,6 +502,9 @@ class Context:
 
     def last_workgroups_per_cu(self):
         return int(lib().leap_hip_last_workgroups_per_cu(self.h))
+
+    def last_graph(self):
+        return int(lib().leap_hip_last_graph(self.h))
 
     def last_kernel_times(self):
         a = ctypes.c_double()

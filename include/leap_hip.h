@@ -121,7 +121,11 @@ typedef struct leap_options {
     int32_t lane_directions;  /* batched FP64 Horner: directions per lane (1, 2, 4); 0 = tuned */
     int32_t workgroups_per_cu; /* multi-set launch: persistent workgroups per CU (1..64);
                                   -1 = one work item per workgroup; 0 = tuned */
-    int32_t reserved;
+    int32_t graph;            /* 1: capture the call's device work (every solution interval's
+                                 rotate/average and inversion) as a HIP graph, kept in the
+                                 context and replayed by later calls with the same shapes,
+                                 options and device addresses; results reach the callback
+                                 after the whole graph.  Needs vis_on_device and uvw_device. */
 } leap_options;
 
 /* One solution interval's result: cpu::Calibration (model/cpu/calibration/Calibration.h:42-111)
@@ -337,6 +341,9 @@ int leap_hip_last_direction_block(const leap_ctx* ctx);
 /* Workgroups per CU of the last multi-set launch (-1: one work item per
  * workgroup; 0 if none). */
 int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx);
+/* Graph use of the last calibrate call: 0 none, 1 captured a new graph,
+ * 2 replayed the context's graph. */
+int leap_hip_last_graph(const leap_ctx* ctx);
 
 #ifdef __cplusplus
 }
