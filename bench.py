@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--fp32-blocks", action="store_true",
                     help="batched-mixed only: FP32 Horner blocks re-anchored every 32 channels instead of the "
                          "full-row FP64 Horner")
+    ap.add_argument("--direct-cube", action="store_true",
+                    help="batched: run the Horner kernel straight from the cube (no pol-0 pack)")
+    ap.add_argument("--pack-timesteps", type=int, default=0, help="timesteps per pack chunk (0 = automatic)")
     ap.add_argument("--cpu-timesteps", type=int, default=2, help="timesteps in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--copy-probe-gb", type=float, default=0.0,
                     help="only run device-to-device copies of this size (calibrates rocprofv3 FETCH_SIZE / "
@@ -179,6 +182,8 @@ def run_mode(args, mode, D, with_cpu, steps=None):
                       direction_block=args.direction_block)
     opts.fp32_blocks = 1 if args.fp32_blocks else 0
     opts.lane_directions = args.lane_directions
+    opts.direct_cube = 1 if args.direct_cube else 0
+    opts.pack_timesteps = args.pack_timesteps
 
     cfg = c1_config()
     log(f"[{mode}] building synthetic measurement set (host columns)")
@@ -203,16 +208,22 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     for i in range(args.warmup):
         ctx.calibrate(view, dirs[i * D:(i + 1) * D], md=md, opts=opts)
         log(f"[{mode}] warmup step {i + 1}/{args.warmup} done")
-    rot_ms = []
+    rot_ms, pack_ms = [], []
     t0 = time.perf_counter()
     for i in range(steps):
         j = args.warmup + i
         ctx.calibrate(view, dirs[j * D:(j + 1) * D], md=md, opts=opts)
-        rot_ms.append(ctx.last_kernel_times()[0])
+        avg_ms, launches, _ = ctx.last_kernel_times()
+        # device time of the rotate/average launches of one step (the plain path launches one per direction)
+        rot_ms.append(avg_ms if mode == "plain" else avg_ms * launches)
+        p_avg, p_n = ctx.last_pack_times()
+        pack_ms.append(p_avg * p_n)
     elapsed = time.perf_counter() - t0
     log(f"[{mode}] {steps} timed steps of {D} directions in {elapsed:.3f} s")
     value = steps * D / elapsed
     k_ms = float(np.mean(rot_ms))
+    tile = ctx.last_direction_block()
+    packed = mode != "plain" and not args.fp32_blocks and not args.direct_cube and tile in (641, 1282, 2564)
     if mode == "plain":
         alg_bytes = R * C * P * elem + R * 24 + D * nbl * P * 16 + C * 8
         achieved = alg_bytes / (k_ms * 1e-3) / 1e9
@@ -225,19 +236,34 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         alg_flops = 8 * R * C * D
         achieved = alg_flops / (k_ms * 1e-3) / 1e12
         alg_bytes = R * C * P * elem + R * 24 + D * nbl * P * 16
-        roofline = {"kernel": "rotate_average_horner64" if not args.fp32_blocks else "rotate_average_batched",
+        if packed:
+            kernel, prof = "rotate_average_packed", "rotate_average_packed_c1_traffic.json"
+            # the Horner kernel reads the packed pol-0 array (16 B per row and channel) instead of the cube
+            alg_bytes = R * C * 16 + R * 24 + D * nbl * P * 16
+        elif args.fp32_blocks:
+            kernel, prof = "rotate_average_batched", None
+        else:
+            kernel, prof = "rotate_average_horner64", "rotate_average_horner64_c1_traffic.json"
+        roofline = {"kernel": kernel,
                     "bound": "valu-fp64", "achieved": round(achieved, 2), "peak": VALU_FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / VALU_FP64_PEAK_TFLOPS, 4),
-                    "traffic": traffic_from_profile("rotate_average_horner64_c1_traffic.json"),
+                    "traffic": traffic_from_profile(prof) if prof else None,
                     "kernel_ms": round(k_ms, 4), "algorithmic_flops_per_launch": alg_flops,
                     "hbm_gbs": round(alg_bytes / (k_ms * 1e-3) / 1e9, 1)}
+        if packed:
+            p_ms = float(np.mean(pack_ms))
+            pack_bytes = R * C * (P * elem + 16)  # read every pol once, write pol 0 as FP64 complex
+            roofline["pack"] = {"kernel": "pack_pol0_kernel", "bound": "hbm", "ms": round(p_ms, 4),
+                                "algorithmic_bytes": pack_bytes,
+                                "achieved": round(pack_bytes / (p_ms * 1e-3) / 1e9, 1) if p_ms > 0 else None,
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(pack_bytes / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if p_ms > 0 else None}
 
     cpu = None
     if with_cpu:
         log(f"[{mode}] CPU baseline: oracle on {args.cpu_timesteps} timesteps")
         cpu = cpu_baseline(ctx, synth, vis_ptr, vis_dtype, args.cpu_timesteps)
         log(f"[{mode}] CPU baseline done")
-    tile = ctx.last_direction_block()
     ctx.free(vis_ptr)
     ctx.free(uvw_ptr)
     ctx.close()

@@ -4,9 +4,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <sstream>
 
 #include "leap_internal.h"
+#include "phasor.h"
 #include "synth.h"
 
 using namespace leap;
@@ -70,6 +72,8 @@ leap_options resolve(const leap_options* o) {
     require(r.lane_directions == 0 || r.direction_block == 0 || r.direction_block >= 32 * r.lane_directions,
             LEAP_ERR_INVALID_ARGUMENT, "direction block too small for the lane directions");
     require(r.graph == 0 || r.graph == 1, LEAP_ERR_INVALID_ARGUMENT, "graph must be 0 or 1");
+    require(r.direct_cube == 0 || r.direct_cube == 1, LEAP_ERR_INVALID_ARGUMENT, "direct_cube must be 0 or 1");
+    require(r.pack_timesteps >= 0, LEAP_ERR_INVALID_ARGUMENT, "pack_timesteps must be >= 0");
     require(r.workgroups_per_cu >= -1 && r.workgroups_per_cu <= 64, LEAP_ERR_INVALID_ARGUMENT,
             "workgroups per CU must be -1 (one work item each), 0 (tuned) or 1 to 64");
     return r;
@@ -96,7 +100,25 @@ void record_rotate_end(leap_ctx* ctx) {
     ++ctx->rotate_events_used;
 }
 
+void record_pack_begin(leap_ctx* ctx, hipStream_t st) {
+    if (ctx->capturing) return;
+    if (ctx->pack_events_used == ctx->pack_events.size()) {
+        hipEvent_t a, b;
+        LEAP_HIP(hipEventCreate(&a));
+        LEAP_HIP(hipEventCreate(&b));
+        ctx->pack_events.emplace_back(a, b);
+    }
+    LEAP_HIP(hipEventRecord(ctx->pack_events[ctx->pack_events_used].first, st));
+}
+
+void record_pack_end(leap_ctx* ctx, hipStream_t st) {
+    if (ctx->capturing) return;
+    LEAP_HIP(hipEventRecord(ctx->pack_events[ctx->pack_events_used].second, st));
+    ++ctx->pack_events_used;
+}
+
 void begin_timing(leap_ctx* ctx) {
+    ctx->pack_events_used = 0;
     ctx->rotate_events_used = 0;
     LEAP_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
 }
@@ -113,6 +135,14 @@ void end_timing(leap_ctx* ctx) {
         LEAP_HIP(hipEventElapsedTime(&ms, ctx->rotate_events[i].first, ctx->rotate_events[i].second));
         sum += ms;
     }
+    double pack_sum = 0.0;
+    for (size_t i = 0; i < ctx->pack_events_used; ++i) {
+        float ms = 0.f;
+        LEAP_HIP(hipEventElapsedTime(&ms, ctx->pack_events[i].first, ctx->pack_events[i].second));
+        pack_sum += ms;
+    }
+    ctx->last_pack_launches = int32_t(ctx->pack_events_used);
+    ctx->last_pack_ms_avg = ctx->pack_events_used ? pack_sum / double(ctx->pack_events_used) : 0.0;
     ctx->last_total_ms = total;
     ctx->last_rotate_launches = int32_t(ctx->rotate_events_used);
     ctx->last_rotate_ms_avg = ctx->rotate_events_used ? sum / double(ctx->rotate_events_used) : 0.0;
@@ -167,16 +197,24 @@ void upload_directions(leap_ctx* ctx, const leap_metadata& md, const double* dir
               ctx->d_dd.as<double>());
 }
 
+// The batched pol-0 path through the pol-0 pack and the scalar-load Horner kernel.
+bool use_packed(const leap_options& opts) {
+    return opts.strategy == LEAP_STRATEGY_BATCHED && !opts.all_pols &&
+           !(opts.precision == LEAP_PRECISION_MIXED && opts.fp32_blocks) && !opts.direct_cube;
+}
+
 // Direction block for the batched kernel: the fastest of the candidates on a
 // slice of the baselines, timed once per shape and kept in the context.
 int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
-                              uint32_t T, uint32_t D, const leap_options& opts) {
+                              uint32_t T, uint32_t D, const leap_options& opts,
+                              const std::function<void()>& before_probe = nullptr) {
     const bool fp32 = opts.precision == LEAP_PRECISION_MIXED && opts.fp32_blocks;
     if (opts.direction_block != 0 && (opts.lane_directions != 0 || fp32))
         return opts.direction_block * 10 + (fp32 ? 1 : opts.lane_directions);
     if (opts.direction_block != 0 && opts.lane_directions == 0) return opts.direction_block * 10 + 1;
+    const bool packed = use_packed(opts);
     const TuneKey key{opts.precision * 4 + (opts.fp32_blocks ? 2 : 0) + (opts.lane_directions ? 1 : 0) +
-                          16 * opts.lane_directions,
+                          16 * opts.lane_directions + (packed ? 1024 : 0),
                       ms.vis_dtype, opts.all_pols, D, T, md.channels, md.pols};
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) return it->second;
@@ -184,13 +222,22 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
     std::vector<int32_t> candidates;
     const int32_t all64[] = {321, 641, 642, 1281, 1282, 1284, 2561, 2562, 2564};
     const int32_t all32[] = {321, 641, 1281, 2561};
-    if (fp32) {
+    if (packed) {
+        // 64 lanes per row: 64, 128 or 256 directions per workgroup
+        for (int32_t t : {641, 1282, 2564})
+            if ((t == 641 || uint32_t(t / 10) <= 2 * D - 2) &&
+                (opts.lane_directions == 0 || t % 10 == opts.lane_directions))
+                candidates.push_back(t);
+        if (candidates.empty()) candidates.push_back(opts.lane_directions == 2 ? 1282 : 2564);
+    } else if (fp32) {
         candidates.assign(std::begin(all32), std::end(all32));
     } else {
         for (int32_t t : all64)
             if (opts.lane_directions == 0 || t % 10 == opts.lane_directions) candidates.push_back(t);
     }
-    const uint32_t probe_bl = std::min<uint32_t>(md.nbl, 2048);
+    // Enough baselines for several rounds of workgroups on every CU.
+    const uint32_t probe_bl = std::min<uint32_t>(md.nbl, 8192);
+    if (before_probe) before_probe();
     int32_t best = candidates.front();
     float best_ms = 0.f;
     hipEvent_t a, b;
@@ -198,13 +245,18 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
     LEAP_HIP(hipEventCreate(&b));
     for (int32_t tile : candidates) {
         const uint32_t db = uint32_t(tile / 10);
-        if (db > 32 && db >= 2 * D) continue;  // mostly idle lanes
+        if (!packed && db > 32 && db >= 2 * D) continue;  // mostly idle lanes
         float best_of = 0.f;
         for (int rep = 0; rep < 2; ++rep) {  // the first repetition warms the code object
             LEAP_HIP(hipEventRecord(a, ctx->stream));
-            launch_rotate_batched(ctx->stream, opts.precision, tile, opts.all_pols != 0, rows.vis, ms.vis_dtype,
-                                  rows.uvw, ctx->d_dd.as<double>(), D, T, md.nbl, probe_bl, md.channels, md.pols,
-                                  md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), opts.fp32_blocks);
+            if (packed)
+                launch_rotate_packed(ctx->stream, tile, ctx->d_packed.as<double>(), rows.uvw, ctx->d_dd.as<double>(),
+                                     D, T, md.nbl, probe_bl, md.channels, md.pols, md.freq_start_hz, md.freq_inc_hz,
+                                     ctx->d_avg.as<double>(), 0);
+            else
+                launch_rotate_batched(ctx->stream, opts.precision, tile, opts.all_pols != 0, rows.vis, ms.vis_dtype,
+                                      rows.uvw, ctx->d_dd.as<double>(), D, T, md.nbl, probe_bl, md.channels, md.pols,
+                                      md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), opts.fp32_blocks);
             LEAP_HIP(hipEventRecord(b, ctx->stream));
             LEAP_HIP(hipEventSynchronize(b));
             LEAP_HIP(hipEventElapsedTime(&best_of, a, b));
@@ -220,14 +272,96 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
     return best;
 }
 
+// Timesteps per pack chunk.  Measured on MI355X at 256 stations / 128
+// timesteps / 384 channels / 256 directions, one chunk (pack, then Horner)
+// and overlapped chunks take the same time: the FP64 Horner kernel runs at the
+// power-limited clock, so a concurrent pack slows it by about what it hides.
+// Chunks therefore only bound the pack buffer: one chunk up to 32 GiB, else
+// the fewest chunks whose two buffers fit in it.
+uint32_t pack_chunk(uint32_t T, uint32_t nbl, uint32_t C, const leap_options& opts) {
+    if (opts.pack_timesteps > 0) return std::min<uint32_t>(T, uint32_t(opts.pack_timesteps));
+    const size_t per_t = size_t(nbl) * C * 2 * sizeof(double);
+    const size_t cap = size_t(32) << 30;
+    if (size_t(T) * per_t <= cap) return T;
+    return uint32_t(std::max<size_t>(1, cap / (2 * per_t)));
+}
+
+// The packed pol-0 path over the staged rows: chunks of timesteps are packed
+// on the pack stream into two alternating buffers while the Horner kernel
+// runs the previous chunk on the context stream, accumulating into avgData.
+// Events order each chunk's pack before its Horner launch and each Horner
+// launch before the pack that reuses its buffer.
+void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
+                            uint32_t T, uint32_t D, const leap_options& opts, int32_t tile) {
+    const uint32_t tc = pack_chunk(T, md.nbl, md.channels, opts);
+    const uint32_t n_chunks = (T + tc - 1) / tc;
+    const size_t per_t = size_t(md.nbl) * md.channels;  // (row, channel)s per timestep
+    const size_t buf_elems = size_t(tc) * per_t;
+    ctx->d_packed.reserve(buf_elems * (n_chunks > 1 ? 2 : 1) * 2 * sizeof(double));
+    const size_t vis_row_bytes = vis_elem_bytes(ms) * md.channels;
+    double* buf[2] = {ctx->d_packed.as<double>(), ctx->d_packed.as<double>() + 2 * buf_elems};
+    auto pack = [&](uint32_t k, hipStream_t st) {
+        const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
+        const void* src = static_cast<const char*>(rows.vis) + size_t(t0) * md.nbl * vis_row_bytes;
+        record_pack_begin(ctx, st);
+        launch_pack_pol0(st, src, ms.vis_dtype, size_t(nt) * per_t, md.pols, buf[k % 2]);
+        record_pack_end(ctx, st);
+    };
+    auto horner = [&](uint32_t k, int32_t db, uint32_t nbl_grid) {
+        const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
+        launch_rotate_packed(ctx->stream, db, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3, ctx->d_dd.as<double>(),
+                             D, nt, md.nbl, nbl_grid, md.channels, md.pols, md.freq_start_hz, md.freq_inc_hz,
+                             ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
+    };
+    if (n_chunks == 1) {
+        pack(0, ctx->stream);
+        record_rotate_begin(ctx);
+        horner(0, tile, md.nbl);
+        record_rotate_end(ctx);
+        return;
+    }
+    // Fork the pack stream from the context stream (staged rows and dd are ready there).
+    LEAP_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    LEAP_HIP(hipStreamWaitEvent(ctx->pack_stream, ctx->ev_fork, 0));
+    for (uint32_t k = 0; k < n_chunks; ++k) {
+        const int b = int(k % 2);
+        if (k >= 2) LEAP_HIP(hipStreamWaitEvent(ctx->pack_stream, ctx->ev_consumed[b], 0));
+        pack(k, ctx->pack_stream);
+        LEAP_HIP(hipEventRecord(ctx->ev_packed[b], ctx->pack_stream));
+        LEAP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_packed[b], 0));
+        record_rotate_begin(ctx);
+        horner(k, tile, md.nbl);
+        record_rotate_end(ctx);
+        LEAP_HIP(hipEventRecord(ctx->ev_consumed[b], ctx->stream));
+    }
+}
+
+// Tile of the batched kernel for this shape: the caller's, the cached one, or
+// tuned now.  The packed path is tuned on its first chunk, packed into the
+// first buffer just before the probe launches.
+int32_t batched_tile(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
+                     uint32_t T, uint32_t D, const leap_options& opts) {
+    if (!use_packed(opts)) return tuned_direction_block(ctx, md, ms, rows, T, D, opts);
+    const uint32_t tc = std::min(T, pack_chunk(T, md.nbl, md.channels, opts));
+    const size_t n = size_t(tc) * md.nbl * md.channels;
+    return tuned_direction_block(ctx, md, ms, rows, tc, D, opts, [&] {
+        ctx->d_packed.reserve(n * 2 * sizeof(double));
+        launch_pack_pol0(ctx->stream, rows.vis, ms.vis_dtype, n, md.pols, ctx->d_packed.as<double>());
+    });
+}
+
 // avgData of D directions for T timesteps starting at the staged rows, into ctx->d_avg.
 void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
                        uint32_t T, uint32_t D, const leap_options& opts, int32_t tile = 0) {
     const size_t per_dir = size_t(md.nbl) * md.pols * 2;
     ctx->d_avg.reserve(per_dir * D * sizeof(double));
     if (opts.strategy == LEAP_STRATEGY_BATCHED) {
-        const int32_t db = tile ? tile : tuned_direction_block(ctx, md, ms, rows, T, D, opts);
+        const int32_t db = tile ? tile : batched_tile(ctx, md, ms, rows, T, D, opts);
         ctx->last_direction_block = db;
+        if (use_packed(opts) && packed_tile_supported(db)) {
+            rotate_packed_pipeline(ctx, md, ms, rows, T, D, opts, db);
+            return;
+        }
         record_rotate_begin(ctx);
         launch_rotate_batched(ctx->stream, opts.precision, db, opts.all_pols != 0, rows.vis, ms.vis_dtype, rows.uvw,
                               ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
@@ -353,9 +487,13 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
 // Device buffers one solution interval uses, reserved before a capture so the
 // captured sequence allocates nothing (the sizes rotate_directions and
 // invert_directions ask for).
-void reserve_solution(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const leap_options& opts) {
+void reserve_solution(leap_ctx* ctx, const leap_metadata& md, uint32_t D, uint32_t T, const leap_options& opts) {
     const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
     ctx->d_avg.reserve(size_t(md.nbl) * md.pols * 2 * D * sizeof(double));
+    if (use_packed(opts)) {
+        const uint32_t tc = pack_chunk(T, md.nbl, md.channels, opts);
+        ctx->d_packed.reserve(size_t(tc) * md.nbl * md.channels * (T > tc ? 2 : 1) * 2 * sizeof(double));
+    }
     ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
     ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
     ctx->d_delta.reserve(size_t(K) * D * sizeof(double));
@@ -382,7 +520,9 @@ uint64_t key_ptr(const void* p) { return uint64_t(reinterpret_cast<uintptr_t>(p)
 // and device addresses.  A sequence is replayed only for an equal key.
 void key_common(std::vector<uint64_t>& k, const leap_ctx* ctx, const leap_options& opts) {
     k.insert(k.end(), {uint64_t(opts.precision), uint64_t(opts.strategy), uint64_t(opts.all_pols),
-                       uint64_t(opts.fp32_blocks), key_ptr(ctx->d_dd.ptr), key_ptr(ctx->d_avg.ptr),
+                       uint64_t(opts.fp32_blocks), uint64_t(opts.direct_cube), uint64_t(opts.pack_timesteps),
+                       key_ptr(ctx->d_packed.ptr),
+                       key_ptr(ctx->d_dd.ptr), key_ptr(ctx->d_avg.ptr),
                        key_ptr(ctx->d_phase.ptr), key_ptr(ctx->d_cal1.ptr), key_ptr(ctx->d_delta.ptr),
                        key_ptr(ctx->d_cal.ptr), key_ptr(ctx->d_partial.ptr), key_ptr(ctx->d_a_planes.ptr)});
 }
@@ -435,11 +575,11 @@ void calibrate_loop_graph(leap_ctx* ctx, const leap_metadata& md, const leap_ms_
     CapturedSequence* seq = nullptr;
     if (D > 0 && solutions > 0) {
         upload_directions(ctx, md, dirs, D);
-        reserve_solution(ctx, md, D, opts);
+        reserve_solution(ctx, md, D, interval, opts);
         int32_t tile = 0;
         if (opts.strategy == LEAP_STRATEGY_BATCHED) {
             const DeviceRows rows0 = stage_rows(ctx, ms, 0, interval * nbl);
-            tile = tuned_direction_block(ctx, md, ms, rows0, interval, D, opts);
+            tile = batched_tile(ctx, md, ms, rows0, interval, D, opts);
             ctx->last_direction_block = tile;
         }
         std::vector<uint64_t> key{uint64_t(tile)};
@@ -771,7 +911,9 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             cal_at[i] = cal_total;
             if (b.D == 0 || b.solutions == 0) continue;
             require_device_resident(*b.ms);
-            reserve_solution(ctx, *b.md, b.D, opts);
+            leap_options ro = opts;
+            ro.direct_cube = 1;  // the multi-set kernel reads the cubes directly
+            reserve_solution(ctx, *b.md, b.D, b.interval, ro);
             cal_total += size_t(b.md->A.cols) * b.D * size_t(b.solutions);
         }
         for (int32_t r = 0; r < rounds; ++r) round_descs(r, rdescs[r], rshapes[r], ractive[r]);
@@ -880,8 +1022,14 @@ int leap_hip_create(leap_ctx** out, int32_t device) {
         ctx->device = device;
         DeviceScope scope(device);
         LEAP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        LEAP_HIP(hipStreamCreateWithFlags(&ctx->pack_stream, hipStreamNonBlocking));
         LEAP_HIP(hipEventCreate(&ctx->ev_start));
         LEAP_HIP(hipEventCreate(&ctx->ev_stop));
+        LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        for (int b = 0; b < 2; ++b) {
+            LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_packed[b], hipEventDisableTiming));
+            LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_consumed[b], hipEventDisableTiming));
+        }
         *out = ctx.release();
     });
 }
@@ -896,11 +1044,23 @@ void leap_hip_destroy(leap_ctx* ctx) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    if (ctx->pack_stream) (void)hipStreamSynchronize(ctx->pack_stream);
+    for (auto& e : ctx->pack_events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->ev_packed[b]) (void)hipEventDestroy(ctx->ev_packed[b]);
+        if (ctx->ev_consumed[b]) (void)hipEventDestroy(ctx->ev_consumed[b]);
+    }
     hipStream_t s = ctx->stream;
+    hipStream_t ps = ctx->pack_stream;
     delete ctx;  // frees the workspace buffers on this device
     if (s) (void)hipStreamDestroy(s);
+    if (ps) (void)hipStreamDestroy(ps);
     if (prev >= 0) (void)hipSetDevice(prev);
 }
 
@@ -1180,6 +1340,17 @@ int leap_multi_schedule(const uint32_t* nbl, const uint32_t* T, const uint32_t* 
     });
 }
 
+int leap_phasor64(const double* theta, uint32_t n, double* cos_out, double* sin_out) {
+    return guarded(nullptr, [&] {
+        require(n == 0 || (theta && cos_out && sin_out), LEAP_ERR_INVALID_ARGUMENT, "null argument");
+        for (uint32_t i = 0; i < n; ++i) {
+            const Phasor64 p = phasor64(theta[i]);
+            cos_out[i] = p.c;
+            sin_out[i] = p.s;
+        }
+    });
+}
+
 double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
     return multi_tile_supported(tile) ? multi_item_cost(T, C, tile) : -1.0;
 }
@@ -1189,6 +1360,12 @@ double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
 extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ? ctx->last_direction_block : 0; }
 extern "C" int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx) { return ctx ? ctx->last_workgroups_per_cu : 0; }
 extern "C" int leap_hip_last_graph(const leap_ctx* ctx) { return ctx ? ctx->last_graph : 0; }
+extern "C" int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg, int32_t* pack_launches) {
+    if (!ctx) return LEAP_ERR_INVALID_ARGUMENT;
+    if (pack_ms_avg) *pack_ms_avg = ctx->last_pack_ms_avg;
+    if (pack_launches) *pack_launches = ctx->last_pack_launches;
+    return LEAP_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Calibration-residual loss and its gradient.

@@ -29,6 +29,7 @@
 // reference's result depends on the direction (inf * 0 = NaN), where here the
 // channel is always skipped.
 #include "leap_internal.h"
+#include "phasor.h"
 
 namespace leap {
 namespace {
@@ -41,9 +42,12 @@ struct C2 {
     T x, y;
 };
 
+// Complex product with the fused operations written out (no contraction), so
+// every kernel evaluates it the same way.
 template <typename T>
 __device__ inline C2<T> cmul(C2<T> a, C2<T> b) {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+#pragma clang fp contract(off)
+    return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)};
 }
 
 // a * z + v with fused multiply-adds.
@@ -70,9 +74,17 @@ __device__ inline C2<float> phasor_mixed(double theta) {
 }
 
 __device__ inline C2<double> phasor_f64(double theta) {
-    double s, c;
-    sincos(theta, &s, &c);
-    return {c, s};
+    const Phasor64 p = phasor64(theta);
+    return {p.c, p.s};
+}
+
+// -2 pi ((dd . uvw).z - w) (CpuLeapCalibrator.cc:190) with the fused operations written out.
+__device__ inline double shift_factor(const double dz[3], double u, double v, double w) {
+#pragma clang fp contract(off)
+    double t = dz[0] * u;
+    t = fma(dz[1], v, t);
+    t = fma(dz[2], w, t);
+    return -kTwoPi * (t - w);
 }
 
 template <typename A>
@@ -261,6 +273,7 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
     Horner64Smem<DB, ND, NP, CC>& sm, const LEAP_GLOBAL VT* __restrict__ vis, const LEAP_GLOBAL double* __restrict__ uvw,
     const LEAP_GLOBAL double* __restrict__ dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
     double kappa0, double dkappa, LEAP_GLOBAL double* __restrict__ avg, uint32_t bl, uint32_t dblock) {
+#pragma clang fp contract(off)
     constexpr int L = DB / ND;
     constexpr int RP = kThreads / L;
     auto& s_v = sm.v;
@@ -305,7 +318,7 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
         C2<double> he[ND][NP], ho[ND][NP];
 #pragma unroll
         for (int j = 0; j < ND; ++j) {
-            sf[j] = -kTwoPi * ((dz[j][0] * u + dz[j][1] * v + dz[j][2] * w) - w);  // CpuLeapCalibrator.cc:190
+            sf[j] = shift_factor(dz[j], u, v, w);
             z[j] = phasor_f64(sf[j] * dkappa);
             z2[j] = cmul(z[j], z[j]);
 #pragma unroll
@@ -464,6 +477,188 @@ __global__ __launch_bounds__(kThreads) void rotate_average_multi(const MultiSetD
     }
 }
 
+// ---- packed pol-0 path ------------------------------------------------------
+// s_waitcnt lgkmcnt(0) with the vector-memory and export counters left free
+// (gfx9 encoding: vmcnt 63, expcnt 7, lgkmcnt 0).
+constexpr int kWaitScalarMem = 0xC07F;
+
+// The pol-0 visibility of every (row, channel) as FP64 complex, 0 where any
+// polarisation of that (row, channel) is not finite (the reference skips the
+// whole (row, channel) then, CpuLeapCalibrator.cc:201-214).  HBM-bound: reads
+// the cube once, writes 16 B per (row, channel).
+template <typename VT, int P>
+__global__ __launch_bounds__(kThreads) void pack_pol0_kernel(const VT* __restrict__ vis, size_t n,
+                                                             C2<double>* __restrict__ out) {
+    // Four independent (row, channel)s per thread and step keep eight loads in
+    // flight per thread; consecutive threads take consecutive elements.
+    constexpr int U = 4;
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    for (; i + (U - 1) * stride < n; i += U * stride) {
+        VT x[U][2 * P];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < 2 * P; ++k) x[u][k] = vis[(i + u * stride) * (2 * P) + k];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            bool finite = true;
+#pragma unroll
+            for (int k = 0; k < 2 * P; ++k) finite = finite && __builtin_isfinite(x[u][k]);
+            out[i + u * stride] = finite ? C2<double>{double(x[u][0]), double(x[u][1])} : C2<double>{0.0, 0.0};
+        }
+    }
+    for (; i < n; i += stride) {
+        VT x[2 * P];
+#pragma unroll
+        for (int k = 0; k < 2 * P; ++k) x[k] = vis[i * (2 * P) + k];
+        bool finite = true;
+#pragma unroll
+        for (int k = 0; k < 2 * P; ++k) finite = finite && __builtin_isfinite(x[k]);
+        out[i] = finite ? C2<double>{double(x[0]), double(x[1])} : C2<double>{0.0, 0.0};
+    }
+}
+
+// Full-row FP64 Horner over packed pol-0 rows.  One wave per row: the row's
+// channel values are wave-uniform, so they are read with scalar loads into
+// SGPRs and feed the FMAs directly (no LDS traffic, no VGPRs for them); each
+// lane carries ND directions, a workgroup 64 * ND directions of one baseline
+// over 4 rows in flight.  Same operation order as rotate_average_horner64
+// with 64 lanes per row, hence the same results.
+template <int ND>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(ND == 4 ? 4 : 1, 8))) void rotate_average_packed(
+    const C2<double>* __restrict__ w, const double* __restrict__ uvw, const double* __restrict__ dd, uint32_t D,
+    uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa, double* __restrict__ avg,
+    int32_t accumulate) {
+#pragma clang fp contract(off)
+    constexpr int L = 64;
+    constexpr int RP = kThreads / L;
+    constexpr int DB = L * ND;
+    // Per-lane state touched once per row lives in LDS (read through volatile
+    // pointers so it is not promoted back to registers): the direction's dd
+    // row 2 and the running row sums.  The registers hold only what the
+    // channel loop needs, for more waves per SIMD.
+    __shared__ double s_dz[L][ND][3];
+    __shared__ double s_red[RP][L][ND][2];
+    const uint32_t lane = threadIdx.x % L;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / L);
+    const uint32_t bl = blockIdx.x;
+    if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            const uint32_t d = blockIdx.y * DB + lane + j * L;
+            const uint32_t dc = d < D ? d : 0;
+            s_dz[lane][j][0] = dd[9 * dc + 6];
+            s_dz[lane][j][1] = dd[9 * dc + 7];
+            s_dz[lane][j][2] = dd[9 * dc + 8];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        s_red[wave][lane][j][0] = 0.0;
+        s_red[wave][lane][j][1] = 0.0;
+    }
+    __syncthreads();
+    volatile double* vdz = &s_dz[lane][0][0];
+    volatile double* vacc = &s_red[wave][lane][0][0];
+    const uint32_t npairs = C / 2;
+    for (uint32_t t = wave; t < T; t += RP) {
+        const size_t row = size_t(t) * nbl + bl;
+        const double u = uvw[3 * row], v = uvw[3 * row + 1], ww = uvw[3 * row + 2];
+        const C2<double>* wr = w + row * C;
+        double sf[ND];
+        C2<double> z[ND], z2[ND], he[ND], ho[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            const double dz[3] = {vdz[3 * j], vdz[3 * j + 1], vdz[3 * j + 2]};
+            sf[j] = shift_factor(dz, u, v, ww);
+            z[j] = phasor_f64(sf[j] * dkappa);
+            z2[j] = cmul(z[j], z[j]);
+            he[j] = {0.0, 0.0};
+            ho[j] = {0.0, 0.0};
+        }
+        if (C & 1) {  // the top channel pairs with a zero
+            const C2<double> ve = wr[C - 1];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                he[j] = horner_step(he[j], z2[j], ve);
+                ho[j] = horner_step(ho[j], z2[j], C2<double>{0.0, 0.0});
+            }
+        }
+        // Blocks of 4 channel pairs in two SGPR buffers: wait for the current
+        // block, issue the next block's scalar loads, then run the current
+        // block, so the loads of one block overlap the FMAs of the other (a
+        // scalar-memory wait covers every outstanding scalar load).
+        uint32_t k = npairs;
+        C2<double> ba[8], bb[8];
+        auto load_block = [&](C2<double>* b, uint32_t top) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) b[i] = wr[2 * (top - 4) + i];
+        };
+        auto run_block = [&](const C2<double>* b) {
+#pragma unroll
+            for (int i = 3; i >= 0; --i) {
+#pragma unroll
+                for (int j = 0; j < ND; ++j) {
+                    he[j] = horner_step(he[j], z2[j], b[2 * i]);
+                    ho[j] = horner_step(ho[j], z2[j], b[2 * i + 1]);
+                }
+            }
+        };
+        if (k >= 4) load_block(ba, k);
+        while (k >= 4) {
+            __builtin_amdgcn_s_waitcnt(kWaitScalarMem);
+            if (k >= 8) load_block(bb, k - 4);
+            run_block(ba);
+            k -= 4;
+            if (k < 4) break;
+            __builtin_amdgcn_s_waitcnt(kWaitScalarMem);
+            if (k >= 8) load_block(ba, k - 4);
+            run_block(bb);
+            k -= 4;
+        }
+        for (; k > 0; --k) {
+            const C2<double> ve = wr[2 * (k - 1)], vo = wr[2 * (k - 1) + 1];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                he[j] = horner_step(he[j], z2[j], ve);
+                ho[j] = horner_step(ho[j], z2[j], vo);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            const C2<double> e0 = phasor_f64(sf[j] * kappa0);
+            const C2<double> zo = cmul(z[j], ho[j]);
+            const C2<double> h = {he[j].x + zo.x, he[j].y + zo.y};
+            const C2<double> r = cmul(e0, h);
+            vacc[2 * j] = vacc[2 * j] + r.x;
+            vacc[2 * j + 1] = vacc[2 * j + 1] + r.y;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            const uint32_t dj = blockIdx.y * DB + lane + j * L;
+            if (dj >= D) continue;
+            double re = 0.0, im = 0.0;
+            for (int r = 0; r < RP; ++r) {
+                re += s_red[r][lane][j][0];
+                im += s_red[r][lane][j][1];
+            }
+            double* o = avg + (size_t(dj) * nbl + bl) * P * 2;
+            if (accumulate) {
+                o[0] += re;
+                o[1] += im;
+            } else {
+                o[0] = re;
+                o[1] = im;
+                for (uint32_t q = 2; q < 2 * P; ++q) o[q] = 0.0;  // pols not accumulated
+            }
+        }
+    }
+}
+
 // A == double selects the full-row FP64 Horner kernel (ND directions per
 // lane); A == float the FP32 blocked kernel (one direction per lane,
 // 32-channel Horner blocks re-anchored by FP64-reduced FP32 sincos, Kahan sums).
@@ -534,6 +729,45 @@ void launch_np(hipStream_t s, int db, bool all_pols, const void* vis, const doub
 }
 
 }  // namespace
+
+void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t n, uint32_t P, double* out) {
+    if (n == 0) return;
+    const size_t want = (n + 4 * kThreads - 1) / (4 * kThreads);
+    const uint32_t grid = uint32_t(std::min<size_t>(want, 256 * 16));
+    C2<double>* o = reinterpret_cast<C2<double>*>(out);
+#define LEAP_PACK(VT, NP_)                                                                                    \
+    pack_pol0_kernel<VT, NP_><<<grid, kThreads, 0, s>>>(static_cast<const VT*>(vis), n, o)
+    const bool c64 = vis_dtype == LEAP_VIS_C64;
+    switch (P) {
+        case 1: if (c64) LEAP_PACK(float, 1); else LEAP_PACK(double, 1); break;
+        case 2: if (c64) LEAP_PACK(float, 2); else LEAP_PACK(double, 2); break;
+        case 3: if (c64) LEAP_PACK(float, 3); else LEAP_PACK(double, 3); break;
+        case 4: if (c64) LEAP_PACK(float, 4); else LEAP_PACK(double, 4); break;
+        default: throw Error(LEAP_ERR_NOT_IMPLEMENTED, "polarisation counts above 4 are not supported");
+    }
+#undef LEAP_PACK
+    LEAP_HIP(hipGetLastError());
+}
+
+bool packed_tile_supported(int32_t tile) { return tile == 641 || tile == 1282 || tile == 2564; }
+
+void launch_rotate_packed(hipStream_t s, int32_t tile, const double* w, const double* uvw, const double* dd,
+                          uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,
+                          double df, double* avg, int32_t accumulate) {
+    if (nbl_grid == 0 || D == 0) return;
+    const double kappa0 = f0 / kSpeedOfLight;
+    const double dkappa = df / kSpeedOfLight;
+    const C2<double>* wp = reinterpret_cast<const C2<double>*>(w);
+    const uint32_t db = uint32_t(tile / 10);
+    const dim3 grid(nbl_grid, (D + db - 1) / db);
+    switch (tile) {
+        case 641: rotate_average_packed<1><<<grid, kThreads, 0, s>>>(wp, uvw, dd, D, T, nbl, C, P, kappa0, dkappa, avg, accumulate); break;
+        case 1282: rotate_average_packed<2><<<grid, kThreads, 0, s>>>(wp, uvw, dd, D, T, nbl, C, P, kappa0, dkappa, avg, accumulate); break;
+        case 2564: rotate_average_packed<4><<<grid, kThreads, 0, s>>>(wp, uvw, dd, D, T, nbl, C, P, kappa0, dkappa, avg, accumulate); break;
+        default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported packed tile " + std::to_string(tile));
+    }
+    LEAP_HIP(hipGetLastError());
+}
 
 void launch_rotate_multi(hipStream_t s, int32_t tile, const MultiSetDesc* descs, const uint2* items,
                          const uint32_t* off, uint32_t n_blocks) {

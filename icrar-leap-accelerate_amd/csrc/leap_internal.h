@@ -149,13 +149,20 @@ struct CapturedSequence {
 struct leap_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t pack_stream = nullptr;  // pol-0 packs overlapping the Horner launches
+    hipEvent_t ev_fork = nullptr, ev_packed[2] = {nullptr, nullptr}, ev_consumed[2] = {nullptr, nullptr};
     std::string error;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> rotate_events;
     size_t rotate_events_used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pack_events;
+    size_t pack_events_used = 0;
+    double last_pack_ms_avg = 0;
+    int32_t last_pack_launches = 0;
     double last_rotate_ms_avg = 0, last_total_ms = 0;
     int32_t last_rotate_launches = 0;
     // workspace
+    leap::DeviceBuffer d_packed;  // pol-0 pack of the staged rows (FP64 complex per row and channel)
     leap::DeviceBuffer d_uvw, d_vis_stage, d_dirs, d_dd, d_avg, d_phase, d_cal1, d_delta, d_cal,
         d_partial, d_a_planes;
     std::vector<double> h_avg, h_cal;
@@ -190,6 +197,12 @@ struct MultiSetDesc {
 };
 void launch_rotate_multi(hipStream_t s, int32_t tile, const MultiSetDesc* descs, const uint2* items,
                          const uint32_t* off, uint32_t n_blocks);
+// Packed pol-0 path: pack the cube, then FP64 Horner fed by scalar loads.
+void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t n, uint32_t P, double* out);
+bool packed_tile_supported(int32_t tile);
+void launch_rotate_packed(hipStream_t s, int32_t tile, const double* w, const double* uvw, const double* dd,
+                          uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,
+                          double df, double* avg, int32_t accumulate);
 int32_t multi_blocks_per_cu(int32_t tile);
 // Kernel launchers (kernels_f64.hip)
 void launch_dd(hipStream_t s, double ra0, double dec0, const double* d_dirs, uint32_t D,

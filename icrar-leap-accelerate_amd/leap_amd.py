@@ -77,11 +77,14 @@ class Options(ctypes.Structure):
         ("lane_directions", ctypes.c_int32),
         ("workgroups_per_cu", ctypes.c_int32),
         ("graph", ctypes.c_int32),
+        ("direct_cube", ctypes.c_int32),
+        ("pack_timesteps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
 def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION, direction_block=0, all_pols=0,
-            fp32_blocks=0, lane_directions=0, workgroups_per_cu=0, graph=0):
+            fp32_blocks=0, lane_directions=0, workgroups_per_cu=0, graph=0, direct_cube=0, pack_timesteps=0):
     o = Options()
     o.precision = precision
     o.strategy = strategy
@@ -91,6 +94,8 @@ def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION,
     o.lane_directions = lane_directions
     o.workgroups_per_cu = workgroups_per_cu
     o.graph = graph
+    o.direct_cube = direct_cube
+    o.pack_timesteps = pack_timesteps
     return o
 
 
@@ -160,9 +165,9 @@ EXPORTS = [
     "leap_hip_synth_fill_vis_device",
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
     "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_workgroups_per_cu",
-    "leap_hip_last_graph", "leap_phase_matrix", "leap_phase_matrix_pinv",
+    "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
-    "leap_multi_item_cost",
+    "leap_multi_item_cost", "leap_phasor64",
 ]
 
 
@@ -200,6 +205,7 @@ def lib():
     L.leap_multi_schedule.argtypes = [_u32p, _u32p, _u32p, _u32p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
                                       _u32p, _u32p, _u32p, _dp]
     L.leap_multi_item_cost.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
+    L.leap_phasor64.argtypes = [_dp, ctypes.c_uint32, _dp, _dp]
     L.leap_multi_item_cost.restype = ctypes.c_double
     L.leap_hip_metadata_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), ctypes.c_int32,
                                            ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
@@ -233,6 +239,7 @@ def lib():
     L.leap_hip_last_direction_block.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_workgroups_per_cu.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_graph.argtypes = [ctypes.c_void_p]
+    L.leap_hip_last_pack_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int32)]
     L.leap_hip_residual_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp, _dp,
                                               ctypes.c_uint32, _dp, _dp, _dp, _dp, _dp]
     L.leap_synth_enu_to_uvw.argtypes = [ctypes.c_void_p, _dp]
@@ -382,6 +389,15 @@ def multi_schedule(shapes, tile, n_blocks):
     return off, iset[:n_items], iloc[:n_items], mk.value
 
 
+def phasor64(theta):
+    """(cos, sin) of theta as the batched kernels compute them (host build of the same routine)."""
+    th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
+    c = np.empty_like(th)
+    s = np.empty_like(th)
+    _check(lib().leap_phasor64(_dptr(th), len(th), _dptr(c), _dptr(s)))
+    return c, s
+
+
 def multi_item_cost(T, C, tile):
     return lib().leap_multi_item_cost(T, C, tile)
 
@@ -505,6 +521,12 @@ class Context:
 
     def last_graph(self):
         return int(lib().leap_hip_last_graph(self.h))
+
+    def last_pack_times(self):
+        a = ctypes.c_double()
+        n = ctypes.c_int32()
+        self._chk(lib().leap_hip_last_pack_times(self.h, ctypes.byref(a), ctypes.byref(n)))
+        return a.value, n.value
 
     def last_kernel_times(self):
         a = ctypes.c_double()

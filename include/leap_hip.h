@@ -126,6 +126,14 @@ typedef struct leap_options {
                                  context and replayed by later calls with the same shapes,
                                  options and device addresses; results reach the callback
                                  after the whole graph.  Needs vis_on_device and uvw_device. */
+    int32_t direct_cube;      /* batched pol 0: 1 = run the Horner kernel straight from the cube
+                                 through LDS; 0 = first pack pol 0 (FP64, non-finite (row,
+                                 channel)s zeroed; 16 B per row and channel of device memory) and
+                                 feed the Horner kernel through scalar loads (faster) */
+    int32_t pack_timesteps;   /* packed path: timesteps per pack chunk; chunks are packed on a
+                                 second stream while the Horner kernel runs the previous one.
+                                 0 = automatic; >= the interval's timesteps = one chunk */
+    int32_t reserved[2];
 } leap_options;
 
 /* One solution interval's result: cpu::Calibration (model/cpu/calibration/Calibration.h:42-111)
@@ -278,6 +286,12 @@ int leap_phase_matrix(const int32_t* a1, const int32_t* a2, const uint8_t* fg, i
 int leap_phase_matrix_pinv(const int32_t* a1, const int32_t* a2, const uint8_t* fg, int32_t n,
                            int32_t ref_ant, double* Ad_out);
 
+/* exp(i theta) as the batched Horner kernels compute it (Cody-Waite reduction
+ * by pi/2, Taylor series), on the host: cos_out[i], sin_out[i] of theta[i].
+ * No reference counterpart (the reference calls std::exp on a complex,
+ * CpuLeapCalibrator.cc:198); exported so the routine can be checked without a GPU. */
+int leap_phasor64(const double* theta, uint32_t n, double* cos_out, double* sin_out);
+
 /* ---- synthetic measurement sets ------------------------------------------- */
 typedef struct leap_synth_config {
     uint32_t n_antennas;
@@ -344,6 +358,9 @@ int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx);
 /* Graph use of the last calibrate call: 0 none, 1 captured a new graph,
  * 2 replayed the context's graph. */
 int leap_hip_last_graph(const leap_ctx* ctx);
+/* Average device time (ms) of the pol-0 pack launches of the last call and
+ * their number (0 when the call packed nothing). */
+int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg, int32_t* pack_launches);
 
 #ifdef __cplusplus
 }

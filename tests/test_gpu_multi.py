@@ -2,8 +2,9 @@
 
 Small sets run against the oracle (FP64 <= 1e-9 rad wrapped, mixed <= 1e-5
 rad); the three sets of the uneven benchmark batch run at full size against
-the single-set batched path with the same tile, which must agree bit for bit
-(the same FP64 Horner tile code runs per work item), and against the oracle
+the single-set batched kernel that reads the cube directly with the same tile,
+which must agree bit for bit (the same FP64 Horner tile code runs per work
+item), and against the oracle
 for a few directions of the two smaller sets.
 """
 import ctypes
@@ -129,8 +130,11 @@ def test_uneven_batch_full_size(leap, ctx, oracle, precision):
             res = ctx.calibrate_batch([(md, v, d, (0, -1, -1)) for (_, _, _, v, md, d) in sets], opts)
             multi_ms = ctx.last_kernel_times()[0]
             single_ms = 0.0
+            # the single-set kernel that shares the multi-set tile code: straight from the cube
+            one_opts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
+                                    lane_directions=nd, direct_cube=1)
             for (s, ms, host, view, md, dirs), (got, _) in zip(sets, res):
-                one, _ = ctx.calibrate(view, dirs, opts=opts, md=md)
+                one, _ = ctx.calibrate(view, dirs, opts=one_opts, md=md)
                 single_ms += ctx.last_kernel_times()[0]
                 assert np.array_equal(got, one), "multi-set and single-set results differ"
             print(f"uneven batch {'mixed' if precision else 'fp64'} tile {db}x{nd}: one launch {multi_ms:.3f} ms, "
