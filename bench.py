@@ -100,7 +100,8 @@ def cpu_baseline(ctx, synth, vis_ptr, vis_dtype, timesteps):
 
 
 def copy_probe(gb):
-    """hipMemcpy device-to-device of a known size, 3 times; prints the byte count."""
+    """hipMemcpy device-to-device of a known size, 3 times; prints the byte count and the best
+    read + write rate (host clock around each synchronous copy)."""
     import ctypes
 
     hip = ctypes.CDLL("libamdhip64.so")
@@ -108,14 +109,19 @@ def copy_probe(gb):
     ctx = la.Context(0)
     a = ctx.malloc(n)
     b = ctx.malloc(n)
+    best = None
     for _ in range(3):
+        t = time.perf_counter()
         rc = hip.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), ctypes.c_size_t(n), 3)  # device to device
+        hip.hipDeviceSynchronize()
+        dt = time.perf_counter() - t
         if rc != 0:
             raise RuntimeError(f"hipMemcpy failed: {rc}")
-    hip.hipDeviceSynchronize()
+        best = dt if best is None else min(best, dt)
     ctx.free(a)
     ctx.free(b)
-    print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
+    print(json.dumps({"copy_probe_bytes": n, "copies": 3, "best_read_write_gbs": round(2 * n / best / 1e9, 1)}),
+          flush=True)
     return 0
 
 
