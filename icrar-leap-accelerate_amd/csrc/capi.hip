@@ -118,6 +118,7 @@ void record_pack_end(leap_ctx* ctx, hipStream_t st) {
 }
 
 void begin_timing(leap_ctx* ctx) {
+    ctx->last_stage_bytes = 0;
     ctx->pack_events_used = 0;
     ctx->rotate_events_used = 0;
     LEAP_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
@@ -149,15 +150,18 @@ void end_timing(leap_ctx* ctx) {
 }
 
 // Device pointers of visibilities and uvw covering rows [row_begin, row_end);
-// host data is staged into the context's buffers.  Returned pointers address
-// row row_begin.
+// host data is staged into the context's buffers (visibilities through the
+// pinned double buffer).  Returned pointers address row row_begin.  With
+// with_vis false host visibilities are left where they are: vis is null and
+// host_vis addresses row row_begin (the packed path stages pol 0 only).
 struct DeviceRows {
     const void* vis;
     const double* uvw;
+    const void* host_vis;
 };
 
 DeviceRows stage_rows_into(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin, uint32_t row_end,
-                           DeviceBuffer& vis_buf, DeviceBuffer& uvw_buf) {
+                           DeviceBuffer& vis_buf, DeviceBuffer& uvw_buf, bool with_vis = true) {
     require(row_end <= ms.num_rows && row_begin <= row_end, LEAP_ERR_RANGE,
             "row selection [" + std::to_string(row_begin) + "," + std::to_string(row_end) +
                 "] exceeds total range [0," + std::to_string(ms.num_rows) + "]");
@@ -167,10 +171,12 @@ DeviceRows stage_rows_into(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_b
     DeviceRows r{};
     if (ms.vis_on_device) {
         r.vis = static_cast<const char*>(ms.vis) + size_t(row_begin) * vb;
+    } else if (!with_vis) {
+        r.vis = nullptr;
+        r.host_vis = static_cast<const char*>(ms.vis) + size_t(row_begin) * vb;
     } else {
         vis_buf.reserve(n * vb);
-        LEAP_HIP(hipMemcpyAsync(vis_buf.ptr, static_cast<const char*>(ms.vis) + size_t(row_begin) * vb, n * vb,
-                                hipMemcpyHostToDevice, ctx->stream));
+        stage_copy(ctx, static_cast<const char*>(ms.vis) + size_t(row_begin) * vb, n * vb, vis_buf.ptr);
         r.vis = vis_buf.ptr;
     }
     if (ms.uvw_device) {
@@ -184,8 +190,9 @@ DeviceRows stage_rows_into(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_b
     return r;
 }
 
-DeviceRows stage_rows(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin, uint32_t row_end) {
-    return stage_rows_into(ctx, ms, row_begin, row_end, ctx->d_vis_stage, ctx->d_uvw);
+DeviceRows stage_rows(leap_ctx* ctx, const leap_ms_view& ms, uint32_t row_begin, uint32_t row_end,
+                      bool with_vis = true) {
+    return stage_rows_into(ctx, ms, row_begin, row_end, ctx->d_vis_stage, ctx->d_uvw, with_vis);
 }
 
 void upload_directions(leap_ctx* ctx, const leap_metadata& md, const double* dirs, uint32_t D) {
@@ -201,6 +208,15 @@ void upload_directions(leap_ctx* ctx, const leap_metadata& md, const double* dir
 bool use_packed(const leap_options& opts) {
     return opts.strategy == LEAP_STRATEGY_BATCHED && !opts.all_pols &&
            !(opts.precision == LEAP_PRECISION_MIXED && opts.fp32_blocks) && !opts.direct_cube;
+}
+
+// Whether a batched rotate takes the packed path: the caller's tile (or, when
+// tuning, the requested one) must be a packed tile.
+bool packed_route(const leap_options& opts, int32_t tile) {
+    if (!use_packed(opts)) return false;
+    if (tile) return packed_tile_supported(tile);
+    if (opts.direction_block == 0) return true;
+    return packed_tile_supported(opts.direction_block * 10 + (opts.lane_directions ? opts.lane_directions : 1));
 }
 
 // Direction block for the batched kernel: the fastest of the candidates on a
@@ -286,11 +302,14 @@ uint32_t pack_chunk(uint32_t T, uint32_t nbl, uint32_t C, const leap_options& op
     return uint32_t(std::max<size_t>(1, cap / (2 * per_t)));
 }
 
-// The packed pol-0 path over the staged rows: chunks of timesteps are packed
-// on the pack stream into two alternating buffers while the Horner kernel
-// runs the previous chunk on the context stream, accumulating into avgData.
-// Events order each chunk's pack before its Horner launch and each Horner
-// launch before the pack that reuses its buffer.
+// The packed pol-0 path over the staged rows.  Chunk 0 is packed first (and
+// the tile tuned on it when needed); further chunks of timesteps are packed on
+// the pack stream into two alternating buffers while the Horner kernel runs the
+// previous chunk on the context stream, accumulating into avgData.  Events
+// order each chunk's pack before its Horner launch and each Horner launch
+// before the pack that reuses its buffer.  Host-resident visibilities are
+// packed by the host staging instead (pol 0 only crosses the bus), which
+// orders itself on the context stream.
 void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
                             uint32_t T, uint32_t D, const leap_options& opts, int32_t tile) {
     const uint32_t tc = pack_chunk(T, md.nbl, md.channels, opts);
@@ -299,39 +318,53 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
     const size_t buf_elems = size_t(tc) * per_t;
     ctx->d_packed.reserve(buf_elems * (n_chunks > 1 ? 2 : 1) * 2 * sizeof(double));
     const size_t vis_row_bytes = vis_elem_bytes(ms) * md.channels;
+    const bool host = rows.vis == nullptr;
     double* buf[2] = {ctx->d_packed.as<double>(), ctx->d_packed.as<double>() + 2 * buf_elems};
     auto pack = [&](uint32_t k, hipStream_t st) {
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
-        const void* src = static_cast<const char*>(rows.vis) + size_t(t0) * md.nbl * vis_row_bytes;
+        const size_t off = size_t(t0) * md.nbl * vis_row_bytes;
+        if (host) {
+            stage_pol0(ctx, static_cast<const char*>(rows.host_vis) + off, ms.vis_dtype, md.pols,
+                       size_t(nt) * per_t, buf[k % 2]);
+            return;
+        }
         record_pack_begin(ctx, st);
-        launch_pack_pol0(st, src, ms.vis_dtype, size_t(nt) * per_t, md.pols, buf[k % 2]);
+        launch_pack_pol0(st, static_cast<const char*>(rows.vis) + off, ms.vis_dtype, size_t(nt) * per_t, md.pols,
+                         buf[k % 2]);
         record_pack_end(ctx, st);
     };
-    auto horner = [&](uint32_t k, int32_t db, uint32_t nbl_grid) {
+    auto horner = [&](uint32_t k) {
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
-        launch_rotate_packed(ctx->stream, db, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3, ctx->d_dd.as<double>(),
-                             D, nt, md.nbl, nbl_grid, md.channels, md.pols, md.freq_start_hz, md.freq_inc_hz,
-                             ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
-    };
-    if (n_chunks == 1) {
-        pack(0, ctx->stream);
         record_rotate_begin(ctx);
-        horner(0, tile, md.nbl);
+        launch_rotate_packed(ctx->stream, tile, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
+                             ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
+                             md.freq_inc_hz, ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
         record_rotate_end(ctx);
+    };
+    pack(0, ctx->stream);
+    // the pack stream may start on chunk 1 once the staged rows are ready
+    LEAP_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    if (tile == 0) tile = tuned_direction_block(ctx, md, ms, rows, std::min(tc, T), D, opts);
+    ctx->last_direction_block = tile;
+    horner(0);
+    if (n_chunks == 1) return;
+    if (host) {
+        // staging chunk k waits for the work queued before it, so buffer k % 2 is free
+        for (uint32_t k = 1; k < n_chunks; ++k) {
+            pack(k, ctx->stream);
+            horner(k);
+        }
         return;
     }
-    // Fork the pack stream from the context stream (staged rows and dd are ready there).
-    LEAP_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    LEAP_HIP(hipEventRecord(ctx->ev_consumed[0], ctx->stream));
     LEAP_HIP(hipStreamWaitEvent(ctx->pack_stream, ctx->ev_fork, 0));
-    for (uint32_t k = 0; k < n_chunks; ++k) {
+    for (uint32_t k = 1; k < n_chunks; ++k) {
         const int b = int(k % 2);
         if (k >= 2) LEAP_HIP(hipStreamWaitEvent(ctx->pack_stream, ctx->ev_consumed[b], 0));
         pack(k, ctx->pack_stream);
         LEAP_HIP(hipEventRecord(ctx->ev_packed[b], ctx->pack_stream));
         LEAP_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_packed[b], 0));
-        record_rotate_begin(ctx);
-        horner(k, tile, md.nbl);
-        record_rotate_end(ctx);
+        horner(k);
         LEAP_HIP(hipEventRecord(ctx->ev_consumed[b], ctx->stream));
     }
 }
@@ -355,13 +388,14 @@ void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_vie
                        uint32_t T, uint32_t D, const leap_options& opts, int32_t tile = 0) {
     const size_t per_dir = size_t(md.nbl) * md.pols * 2;
     ctx->d_avg.reserve(per_dir * D * sizeof(double));
+    if (opts.strategy == LEAP_STRATEGY_BATCHED && packed_route(opts, tile)) {
+        rotate_packed_pipeline(ctx, md, ms, rows, T, D, opts, tile);
+        return;
+    }
+    require(rows.vis != nullptr, LEAP_ERR_INTERNAL, "visibilities were not staged");
     if (opts.strategy == LEAP_STRATEGY_BATCHED) {
         const int32_t db = tile ? tile : batched_tile(ctx, md, ms, rows, T, D, opts);
         ctx->last_direction_block = db;
-        if (use_packed(opts) && packed_tile_supported(db)) {
-            rotate_packed_pipeline(ctx, md, ms, rows, T, D, opts, db);
-            return;
-        }
         record_rotate_begin(ctx);
         launch_rotate_batched(ctx->stream, opts.precision, db, opts.all_pols != 0, rows.vis, ms.vis_dtype, rows.uvw,
                               ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
@@ -658,7 +692,7 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
         sol.n_antennas = uint32_t(N);
         sol.calibrations = ctx->h_cal.data();
         if (D > 0) {
-            const DeviceRows rows = stage_rows(ctx, ms, row_begin, row_end);
+            const DeviceRows rows = stage_rows(ctx, ms, row_begin, row_end, !packed_route(opts, 0));
             rotate_directions(ctx, md, ms, rows, interval, D, opts);
             invert_directions(ctx, md, D, opts);
             LEAP_HIP(hipMemcpyAsync(ctx->h_cal.data(), ctx->d_cal.ptr, size_t(N) * D * sizeof(double),
@@ -1023,12 +1057,17 @@ int leap_hip_create(leap_ctx** out, int32_t device) {
         DeviceScope scope(device);
         LEAP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         LEAP_HIP(hipStreamCreateWithFlags(&ctx->pack_stream, hipStreamNonBlocking));
+        LEAP_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_copy_fork, hipEventDisableTiming));
+        LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_copy_join, hipEventDisableTiming));
         LEAP_HIP(hipEventCreate(&ctx->ev_start));
         LEAP_HIP(hipEventCreate(&ctx->ev_stop));
         LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         for (int b = 0; b < 2; ++b) {
             LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_packed[b], hipEventDisableTiming));
             LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_consumed[b], hipEventDisableTiming));
+            LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_h2d[b], hipEventDisableTiming));
+            LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_widened[b], hipEventDisableTiming));
         }
         *out = ctx.release();
     });
@@ -1045,6 +1084,14 @@ void leap_hip_destroy(leap_ctx* ctx) {
         (void)hipEventDestroy(e.second);
     }
     if (ctx->pack_stream) (void)hipStreamSynchronize(ctx->pack_stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->h_pinned[b]) (void)hipHostFree(ctx->h_pinned[b]);
+        if (ctx->ev_h2d[b]) (void)hipEventDestroy(ctx->ev_h2d[b]);
+        if (ctx->ev_widened[b]) (void)hipEventDestroy(ctx->ev_widened[b]);
+    }
+    if (ctx->ev_copy_fork) (void)hipEventDestroy(ctx->ev_copy_fork);
+    if (ctx->ev_copy_join) (void)hipEventDestroy(ctx->ev_copy_join);
     for (auto& e : ctx->pack_events) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -1058,9 +1105,11 @@ void leap_hip_destroy(leap_ctx* ctx) {
     }
     hipStream_t s = ctx->stream;
     hipStream_t ps = ctx->pack_stream;
+    hipStream_t cs = ctx->copy_stream;
     delete ctx;  // frees the workspace buffers on this device
     if (s) (void)hipStreamDestroy(s);
     if (ps) (void)hipStreamDestroy(ps);
+    if (cs) (void)hipStreamDestroy(cs);
     if (prev >= 0) (void)hipSetDevice(prev);
 }
 
@@ -1127,7 +1176,7 @@ int leap_hip_rotate_visibilities(leap_ctx* ctx, const leap_metadata* md, const l
                 "integration rows must be a whole number of timesteps");
         DeviceScope scope(ctx->device);
         begin_timing(ctx);
-        const DeviceRows rows = stage_rows(ctx, *ms, row_start, row_start + n_rows);
+        const DeviceRows rows = stage_rows(ctx, *ms, row_start, row_start + n_rows, !packed_route(o, 0));
         upload_directions(ctx, *md, dirs, D);
         rotate_directions(ctx, *md, *ms, rows, n_rows / ms->num_baselines, D, o);
         const size_t n = size_t(md->nbl) * md->pols * 2 * D;
@@ -1360,6 +1409,7 @@ double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
 extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ? ctx->last_direction_block : 0; }
 extern "C" int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx) { return ctx ? ctx->last_workgroups_per_cu : 0; }
 extern "C" int leap_hip_last_graph(const leap_ctx* ctx) { return ctx ? ctx->last_graph : 0; }
+extern "C" uint64_t leap_hip_last_staged_bytes(const leap_ctx* ctx) { return ctx ? ctx->last_stage_bytes : 0; }
 extern "C" int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg, int32_t* pack_launches) {
     if (!ctx) return LEAP_ERR_INVALID_ARGUMENT;
     if (pack_ms_avg) *pack_ms_avg = ctx->last_pack_ms_avg;

@@ -519,6 +519,14 @@ __global__ __launch_bounds__(kThreads) void pack_pol0_kernel(const VT* __restric
     }
 }
 
+// complex64 -> complex128, for pol-0 chunks packed on the host.
+__global__ __launch_bounds__(kThreads) void widen_c64_kernel(const C2<float>* __restrict__ in, size_t n,
+                                                             C2<double>* __restrict__ out) {
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    for (size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride)
+        out[i] = C2<double>{double(in[i].x), double(in[i].y)};
+}
+
 // Full-row FP64 Horner over packed pol-0 rows.  One wave per row: the row's
 // channel values are wave-uniform, so they are read with scalar loads into
 // SGPRs and feed the FMAs directly (no LDS traffic, no VGPRs for them); each
@@ -748,6 +756,13 @@ void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t 
         default: throw Error(LEAP_ERR_NOT_IMPLEMENTED, "polarisation counts above 4 are not supported");
     }
 #undef LEAP_PACK
+    LEAP_HIP(hipGetLastError());
+}
+
+void launch_widen_c64(hipStream_t s, const void* in, size_t n, double* out) {
+    if (n == 0) return;
+    const uint32_t grid = uint32_t(std::min<size_t>((n + kThreads - 1) / kThreads, 256 * 64));
+    widen_c64_kernel<<<grid, kThreads, 0, s>>>(static_cast<const C2<float>*>(in), n, reinterpret_cast<C2<double>*>(out));
     LEAP_HIP(hipGetLastError());
 }
 

@@ -150,6 +150,14 @@ struct leap_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t pack_stream = nullptr;  // pol-0 packs overlapping the Horner launches
+    // host -> device staging of host-resident visibilities: pinned double buffer, copy stream
+    hipStream_t copy_stream = nullptr;
+    void* h_pinned[2] = {nullptr, nullptr};
+    size_t pinned_bytes = 0;
+    hipEvent_t ev_h2d[2] = {nullptr, nullptr}, ev_widened[2] = {nullptr, nullptr};
+    bool h2d_pending[2] = {false, false}, widen_pending[2] = {false, false};
+    hipEvent_t ev_copy_fork = nullptr, ev_copy_join = nullptr;
+    size_t last_stage_bytes = 0;  // bytes the last call sent host -> device through the pinned buffers
     hipEvent_t ev_fork = nullptr, ev_packed[2] = {nullptr, nullptr}, ev_consumed[2] = {nullptr, nullptr};
     std::string error;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -163,6 +171,7 @@ struct leap_ctx {
     int32_t last_rotate_launches = 0;
     // workspace
     leap::DeviceBuffer d_packed;  // pol-0 pack of the staged rows (FP64 complex per row and channel)
+    leap::DeviceBuffer d_widen[2];  // complex64 pol-0 chunks from the host, widened into d_packed
     leap::DeviceBuffer d_uvw, d_vis_stage, d_dirs, d_dd, d_avg, d_phase, d_cal1, d_delta, d_cal,
         d_partial, d_a_planes;
     std::vector<double> h_avg, h_cal;
@@ -197,6 +206,16 @@ struct MultiSetDesc {
 };
 void launch_rotate_multi(hipStream_t s, int32_t tile, const MultiSetDesc* descs, const uint2* items,
                          const uint32_t* off, uint32_t n_blocks);
+// Host -> device staging (staging.cpp).  Both order their device writes after
+// the work already on ctx->stream and make later work on ctx->stream wait for
+// them; host memory is read before they return.
+// Copies bytes from host memory to device memory through the pinned buffers.
+void stage_copy(leap_ctx* ctx, const void* host_src, size_t bytes, void* device_dst);
+// Pol 0 of n (row, channel)s of a host cube (VT = float or double, P pols),
+// 0 where any pol is not finite, as FP64 complex into device memory; only
+// pol 0 crosses the bus (complex64 sources are widened on the device).
+void stage_pol0(leap_ctx* ctx, const void* host_vis, int32_t vis_dtype, uint32_t P, size_t n, double* device_dst);
+void launch_widen_c64(hipStream_t s, const void* in, size_t n, double* out);
 // Packed pol-0 path: pack the cube, then FP64 Horner fed by scalar loads.
 void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t n, uint32_t P, double* out);
 bool packed_tile_supported(int32_t tile);

@@ -165,7 +165,8 @@ EXPORTS = [
     "leap_hip_synth_fill_vis_device",
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
     "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_workgroups_per_cu",
-    "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_phase_matrix", "leap_phase_matrix_pinv",
+    "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes",
+    "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
     "leap_multi_item_cost", "leap_phasor64",
 ]
@@ -240,6 +241,8 @@ def lib():
     L.leap_hip_last_workgroups_per_cu.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_graph.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_pack_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int32)]
+    L.leap_hip_last_staged_bytes.argtypes = [ctypes.c_void_p]
+    L.leap_hip_last_staged_bytes.restype = ctypes.c_uint64
     L.leap_hip_residual_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp, _dp,
                                               ctypes.c_uint32, _dp, _dp, _dp, _dp, _dp]
     L.leap_synth_enu_to_uvw.argtypes = [ctypes.c_void_p, _dp]
@@ -521,6 +524,9 @@ class Context:
 
     def last_graph(self):
         return int(lib().leap_hip_last_graph(self.h))
+
+    def last_staged_bytes(self):
+        return int(lib().leap_hip_last_staged_bytes(self.h))
 
     def last_pack_times(self):
         a = ctypes.c_double()

@@ -361,6 +361,9 @@ int leap_hip_last_graph(const leap_ctx* ctx);
 /* Average device time (ms) of the pol-0 pack launches of the last call and
  * their number (0 when the call packed nothing). */
 int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg, int32_t* pack_launches);
+/* Bytes the last call sent host -> device through the pinned staging buffers
+ * (host-resident visibilities; pol 0 only on the packed path). */
+uint64_t leap_hip_last_staged_bytes(const leap_ctx* ctx);
 
 #ifdef __cplusplus
 }
