@@ -5,6 +5,9 @@
 
 #include <cstdint>
 
+#include <string>
+
+#include "icrar/leap-accelerate/common/json_reader.h"
 #include "icrar/leap-accelerate/exception/exception.h"
 
 namespace icrar {
@@ -49,5 +52,23 @@ struct Slice {
                      end == -1 ? collectionSize : end);
     }
 };
+
+// ParseSlice (common/Slice.cc:65-103): an integer is Slice(interval); an array
+// of 3 integers Slice(start, interval, end).  A 2-element array is rejected as
+// the reference's control flow does ("expected 3 integers").
+inline Slice ParseSlice(const JsonValue& doc) {
+    if (doc.IsInt()) return Slice(int(doc.num));
+    if (doc.kind == JsonValue::Array) {
+        if (doc.arr.size() == 3) {
+            for (const auto& e : doc.arr)
+                if (!e.IsInt()) throw icrar::exception("expected 3 integers", __FILE__, __LINE__);
+            return Slice(int(doc.arr[0].num), int(doc.arr[1].num), int(doc.arr[2].num));
+        }
+        throw icrar::exception("expected 3 integers", __FILE__, __LINE__);
+    }
+    throw icrar::exception("expected an integer or array of integers", __FILE__, __LINE__);
+}
+
+inline Slice ParseSlice(const std::string& json) { return ParseSlice(JsonReader::Parse(json)); }
 
 }  // namespace icrar

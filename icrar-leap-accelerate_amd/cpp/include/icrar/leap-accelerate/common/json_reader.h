@@ -1,0 +1,169 @@
+// A small JSON reader (RFC 8259 values; numbers as double) for the
+// configuration file, the command-line direction and slice strings, and the
+// raw measurement-set header.  The reference parses the same inputs with
+// RapidJSON (leap-accelerate-cli/Arguments.cc:311-482).
+#pragma once
+
+#include <cctype>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "icrar/leap-accelerate/exception/exception.h"
+
+namespace icrar {
+
+struct JsonValue {
+    enum Kind { Null, Bool, Number, String, Array, Object } kind = Null;
+    bool b = false;
+    double num = 0.0;
+    bool integral = false;  // a number written without fraction or exponent
+    std::string str;
+    std::vector<JsonValue> arr;
+    std::vector<std::pair<std::string, JsonValue>> obj;
+
+    bool IsNumber() const { return kind == Number; }
+    // RapidJSON's IsInt / IsDouble: an integer literal in int range / a literal with fraction or exponent.
+    bool IsInt() const { return kind == Number && integral && num >= -2147483648.0 && num <= 2147483647.0; }
+    bool IsDouble() const { return kind == Number && !integral; }
+    const JsonValue* Find(const std::string& key) const {
+        for (const auto& kv : obj)
+            if (kv.first == key) return &kv.second;
+        return nullptr;
+    }
+};
+
+class JsonReader {
+    const char* m_p;
+    const char* m_begin;
+
+    [[noreturn]] void Fail(const std::string& what) const {
+        throw json_exception(what + " at offset " + std::to_string(m_p - m_begin), __FILE__, __LINE__);
+    }
+    void Ws() {
+        while (*m_p && std::isspace(static_cast<unsigned char>(*m_p))) ++m_p;
+    }
+    void Expect(char c) {
+        Ws();
+        if (*m_p != c) Fail(std::string("expected '") + c + "'");
+        ++m_p;
+    }
+    std::string ParseString() {
+        Expect('"');
+        std::string out;
+        while (*m_p && *m_p != '"') {
+            if (*m_p == '\\') {
+                ++m_p;
+                switch (*m_p) {
+                    case '"': out += '"'; break;
+                    case '\\': out += '\\'; break;
+                    case '/': out += '/'; break;
+                    case 'b': out += '\b'; break;
+                    case 'f': out += '\f'; break;
+                    case 'n': out += '\n'; break;
+                    case 'r': out += '\r'; break;
+                    case 't': out += '\t'; break;
+                    default: Fail("unsupported escape");
+                }
+                ++m_p;
+            } else {
+                out += *m_p++;
+            }
+        }
+        if (*m_p != '"') Fail("unterminated string");
+        ++m_p;
+        return out;
+    }
+    JsonValue ParseValue(int depth) {
+        if (depth > 64) Fail("nesting too deep");
+        Ws();
+        JsonValue v;
+        if (*m_p == '{') {
+            ++m_p;
+            v.kind = JsonValue::Object;
+            Ws();
+            if (*m_p == '}') {
+                ++m_p;
+                return v;
+            }
+            for (;;) {
+                std::string k = ParseString();
+                Expect(':');
+                v.obj.emplace_back(std::move(k), ParseValue(depth + 1));
+                Ws();
+                if (*m_p == ',') {
+                    ++m_p;
+                    continue;
+                }
+                Expect('}');
+                return v;
+            }
+        }
+        if (*m_p == '[') {
+            ++m_p;
+            v.kind = JsonValue::Array;
+            Ws();
+            if (*m_p == ']') {
+                ++m_p;
+                return v;
+            }
+            for (;;) {
+                v.arr.push_back(ParseValue(depth + 1));
+                Ws();
+                if (*m_p == ',') {
+                    ++m_p;
+                    continue;
+                }
+                Expect(']');
+                return v;
+            }
+        }
+        if (*m_p == '"') {
+            v.kind = JsonValue::String;
+            v.str = ParseString();
+            return v;
+        }
+        if (std::strncmp(m_p, "true", 4) == 0) {
+            m_p += 4;
+            v.kind = JsonValue::Bool;
+            v.b = true;
+            return v;
+        }
+        if (std::strncmp(m_p, "false", 5) == 0) {
+            m_p += 5;
+            v.kind = JsonValue::Bool;
+            return v;
+        }
+        if (std::strncmp(m_p, "null", 4) == 0) {
+            m_p += 4;
+            return v;
+        }
+        if (*m_p == '-' || std::isdigit(static_cast<unsigned char>(*m_p))) {
+            char* end = nullptr;
+            v.num = std::strtod(m_p, &end);
+            if (end == m_p) Fail("bad number");
+            v.integral = true;
+            for (const char* q = m_p; q < end; ++q)
+                if (*q == '.' || *q == 'e' || *q == 'E') v.integral = false;
+            m_p = end;
+            v.kind = JsonValue::Number;
+            return v;
+        }
+        Fail("unexpected character");
+    }
+
+public:
+    // Parses a whole document; trailing characters other than whitespace are an error.
+    static JsonValue Parse(const std::string& text) {
+        JsonReader r;
+        r.m_begin = r.m_p = text.c_str();
+        JsonValue v = r.ParseValue(0);
+        r.Ws();
+        if (*r.m_p) r.Fail("trailing characters");
+        return v;
+    }
+};
+
+}  // namespace icrar

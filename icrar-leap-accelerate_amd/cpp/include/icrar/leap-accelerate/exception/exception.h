@@ -27,6 +27,18 @@ public:
         : exception(arg + " invalid argument exception: " + msg, file, line) {}
 };
 
+class file_exception : public icrar::exception {
+public:
+    file_exception(const std::string& msg, const std::string& filename, const std::string& file, int line)
+        : exception(filename + " file exception: " + msg, file, line) {}
+};
+
+class json_exception : public icrar::exception {
+public:
+    json_exception(const std::string& msg, const std::string& file, int line)
+        : exception("json exception: " + msg, file, line) {}
+};
+
 class not_implemented_exception : public icrar::exception {
 public:
     not_implemented_exception(const std::string& file, int line) : exception("not implemented", file, line) {}

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <memory>
 #include <optional>
 #include <set>
 #include <sstream>
@@ -41,6 +42,7 @@ struct MeasurementSetColumns {
     leap_vis_dtype data_dtype = LEAP_VIS_C64;
     bool data_on_device = false;
     const double* uvw_device = nullptr;       // optional device copy of uvw
+    std::shared_ptr<const void> data_owner;   // keeps `data` alive (e.g. a file mapping), may be empty
 };
 
 class MeasurementSet {

@@ -10,6 +10,9 @@
 //       factory and prints the CalibrationCollection JSON.  When vis_out_file is
 //       given, the visibilities are also written there (raw, [row][chan][pol])
 //       so a checker can recompute the same calibration.
+//   hip_calibrator_main msinfo <raw-ms-dir> [stations|-1] [autocorrelations 0|1]
+//       loads a raw measurement-set directory (host only) and prints its
+//       accessors (stations, baselines, epochs, ...) as JSON.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,6 +24,7 @@
 
 #include "icrar/leap-accelerate/algorithm/LeapCalibratorFactory.h"
 #include "icrar/leap-accelerate/ms/MeasurementSet.h"
+#include "icrar/leap-accelerate/ms/RawMeasurementSet.h"
 
 using namespace icrar;
 
@@ -160,10 +164,49 @@ static int calibrate(int argc, char** argv) {
     return 0;
 }
 
+// msinfo <raw-ms-dir> [stations|-1] [autocorrelations 0|1]: the accessors of a
+// loaded raw measurement set as JSON (host only).
+static int msinfo(int argc, char** argv) {
+    if (argc < 3) {
+        std::cerr << "usage: msinfo <raw-ms-dir> [stations|-1] [autocorrelations 0|1]\n";
+        return 2;
+    }
+    std::optional<int> stations;
+    if (argc > 3 && std::atoi(argv[3]) >= 0) stations = std::atoi(argv[3]);
+    const bool autos = argc > 4 ? std::atoi(argv[4]) != 0 : true;
+    const MeasurementSet ms = LoadRawMeasurementSet(argv[2], stations, autos);
+    const leap_ms_view v = ms.View();
+    std::ostringstream os;
+    os << "{\"stations\": " << ms.GetNumStations() << ", \"total_antennas\": " << ms.GetTotalAntennas()
+       << ", \"baselines\": " << ms.GetNumBaselines() << ", \"rows\": " << ms.GetNumRows()
+       << ", \"timesteps\": " << ms.GetNumTimesteps() << ", \"channels\": " << ms.GetNumChannels()
+       << ", \"pols\": " << ms.GetNumPols() << ", \"flagged_baselines\": " << ms.GetNumFlaggedBaselines()
+       << ", \"freq_start_hz\": " << FormatJsonDouble(v.freq_start_hz)
+       << ", \"freq_inc_hz\": " << FormatJsonDouble(v.freq_inc_hz)
+       << ", \"phase_centre\": [" << FormatJsonDouble(v.phase_centre_ra_rad) << ", "
+       << FormatJsonDouble(v.phase_centre_dec_rad) << "], \"vis_dtype\": " << v.vis_dtype << ", \"epochs\": [";
+    const std::vector<double> ep = ms.GetEpochs();
+    for (size_t i = 0; i < ep.size(); ++i) os << (i ? ", " : "") << FormatJsonDouble(ep[i]);
+    // checksum of the mapped visibilities: sum of all values as doubles
+    double sum = 0.0;
+    const size_t n = size_t(v.num_rows) * v.num_channels * v.num_pols * 2;
+    if (v.vis_dtype == LEAP_VIS_C64) {
+        const float* d = static_cast<const float*>(v.vis);
+        for (size_t i = 0; i < n; ++i) sum += std::isfinite(d[i]) ? double(d[i]) : 0.0;
+    } else {
+        const double* d = static_cast<const double*>(v.vis);
+        for (size_t i = 0; i < n; ++i) sum += std::isfinite(d[i]) ? d[i] : 0.0;
+    }
+    os << "], \"vis_sum\": " << FormatJsonDouble(sum) << "}";
+    std::cout << os.str() << std::endl;
+    return 0;
+}
+
 int main(int argc, char** argv) {
     try {
         if (argc > 1 && std::string(argv[1]) == "selftest") return selftest();
         if (argc > 1 && std::string(argv[1]) == "calibrate") return calibrate(argc, argv);
+        if (argc > 1 && std::string(argv[1]) == "msinfo") return msinfo(argc, argv);
         std::cerr << "usage: hip_calibrator_main selftest | calibrate ...\n";
         return 2;
     } catch (const std::exception& e) {

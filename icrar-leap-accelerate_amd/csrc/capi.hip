@@ -230,7 +230,7 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
     if (opts.direction_block != 0 && opts.lane_directions == 0) return opts.direction_block * 10 + 1;
     const bool packed = use_packed(opts);
     const TuneKey key{opts.precision * 4 + (opts.fp32_blocks ? 2 : 0) + (opts.lane_directions ? 1 : 0) +
-                          16 * opts.lane_directions + (packed ? 1024 : 0),
+                          16 * opts.lane_directions + (packed ? kTunePackedBit : 0),
                       ms.vis_dtype, opts.all_pols, D, T, md.channels, md.pols};
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) return it->second;
@@ -1410,6 +1410,24 @@ extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ?
 extern "C" int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx) { return ctx ? ctx->last_workgroups_per_cu : 0; }
 extern "C" int leap_hip_last_graph(const leap_ctx* ctx) { return ctx ? ctx->last_graph : 0; }
 extern "C" uint64_t leap_hip_last_staged_bytes(const leap_ctx* ctx) { return ctx ? ctx->last_stage_bytes : 0; }
+extern "C" int leap_hip_tuning_save(const leap_ctx* ctx, char* buf, uint64_t cap, uint64_t* needed) {
+    return guarded(ctx, [&] {
+        require(ctx != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null context");
+        const std::string js = tuning_to_json(*ctx);
+        if (needed) *needed = js.size() + 1;
+        if (buf && cap) {
+            const size_t n = std::min<size_t>(size_t(cap) - 1, js.size());
+            std::memcpy(buf, js.data(), n);
+            buf[n] = '\0';
+        }
+    });
+}
+extern "C" int leap_hip_tuning_load(leap_ctx* ctx, const char* json) {
+    return guarded(ctx, [&] {
+        require(ctx != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null context");
+        tuning_from_json(*ctx, json);
+    });
+}
 extern "C" int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg, int32_t* pack_launches) {
     if (!ctx) return LEAP_ERR_INVALID_ARGUMENT;
     if (pack_ms_avg) *pack_ms_avg = ctx->last_pack_ms_avg;

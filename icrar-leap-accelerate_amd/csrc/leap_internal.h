@@ -112,6 +112,9 @@ struct leap_metadata {
     leap::DeviceBuffer d_AdT_planes;  // split-3 x BF16 planes of AdT in MFMA operand layout
 };
 
+// TuneKey::precision holds the batched tuner's mode code; this bit marks the packed path.
+constexpr int32_t kTunePackedBit = 1024;
+
 struct TuneKey {
     int32_t precision, vis_dtype, all_pols;
     uint32_t D, T, C, P;
@@ -206,6 +209,9 @@ struct MultiSetDesc {
 };
 void launch_rotate_multi(hipStream_t s, int32_t tile, const MultiSetDesc* descs, const uint2* items,
                          const uint32_t* off, uint32_t n_blocks);
+// The autotuners' choices as JSON (tuning.cpp).
+std::string tuning_to_json(const leap_ctx& ctx);
+void tuning_from_json(leap_ctx& ctx, const char* text);
 // Host -> device staging (staging.cpp).  Both order their device writes after
 // the work already on ctx->stream and make later work on ctx->stream wait for
 // them; host memory is read before they return.

@@ -5,6 +5,7 @@ benchmark.  The shared library is built in-tree (build/libleap_hip.so); if
 it is missing this module raises instead of falling back to anything else.
 """
 import ctypes
+import json
 import os
 import subprocess
 
@@ -166,7 +167,7 @@ EXPORTS = [
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
     "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_workgroups_per_cu",
     "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes",
-    "leap_phase_matrix", "leap_phase_matrix_pinv",
+    "leap_hip_tuning_save", "leap_hip_tuning_load", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
     "leap_multi_item_cost", "leap_phasor64",
 ]
@@ -242,6 +243,9 @@ def lib():
     L.leap_hip_last_graph.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_pack_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int32)]
     L.leap_hip_last_staged_bytes.argtypes = [ctypes.c_void_p]
+    L.leap_hip_tuning_save.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64,
+                                       ctypes.POINTER(ctypes.c_uint64)]
+    L.leap_hip_tuning_load.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
     L.leap_hip_last_staged_bytes.restype = ctypes.c_uint64
     L.leap_hip_residual_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp, _dp,
                                               ctypes.c_uint32, _dp, _dp, _dp, _dp, _dp]
@@ -392,6 +396,48 @@ def multi_schedule(shapes, tile, n_blocks):
     return off, iset[:n_items], iloc[:n_items], mk.value
 
 
+def write_raw_ms(path, *, antenna1, antenna2, time, flag_c0p0, uvw, data, antenna_table_rows, ref_frequency_hz,
+                 chan_width_hz, phase_dir_rad, data_column="DATA"):
+    """Writes a raw measurement-set directory (the format RawMeasurementSet.h reads): header.json and
+    little-endian column files.  data is [rows][channels][pols] complex64 or complex128; every other
+    column has one entry per row (uvw three)."""
+    data = np.asarray(data)
+    if data.dtype not in (np.complex64, np.complex128) or data.ndim != 3:
+        raise ValueError("data must be a [rows][channels][pols] complex64 or complex128 array")
+    R, C, P = data.shape
+    cols = {"ANTENNA1.i32": np.asarray(antenna1, dtype="<i4"), "ANTENNA2.i32": np.asarray(antenna2, dtype="<i4"),
+            "TIME.f64": np.asarray(time, dtype="<f8"), "FLAG_C0P0.u8": np.asarray(flag_c0p0, dtype=np.uint8),
+            "UVW.f64": np.asarray(uvw, dtype="<f8").reshape(-1, 3)}
+    for name, col in cols.items():
+        if len(col) != R:
+            raise ValueError(f"{name} has {len(col)} rows, DATA has {R}")
+    os.makedirs(path, exist_ok=True)
+    header = {"format": "leap-raw-ms", "version": 1, "rows": int(R), "channels": int(C), "pols": int(P),
+              "antenna_table_rows": int(antenna_table_rows), "ref_frequency_hz": float(ref_frequency_hz),
+              "chan_width_hz": float(chan_width_hz),
+              "phase_dir_rad": [float(phase_dir_rad[0]), float(phase_dir_rad[1])],
+              "data_column": data_column, "data_type": "complex64" if data.dtype == np.complex64 else "complex128"}
+    for name, col in cols.items():
+        np.ascontiguousarray(col).tofile(os.path.join(path, name))
+    np.ascontiguousarray(data).tofile(os.path.join(path, "DATA.c64" if data.dtype == np.complex64 else "DATA.c128"))
+    with open(os.path.join(path, "header.json"), "w") as f:
+        json.dump(header, f)
+
+
+def write_raw_ms_from_dict(path, ms, antenna_table_rows=None):
+    """write_raw_ms from the dictionary form of a measurement set (SynthMS.host_dict): the first-epoch
+    flags are repeated over every epoch."""
+    nbl = ms["num_baselines"]
+    rows = ms["num_rows"]
+    a1 = np.tile(np.asarray(ms["antenna1"]), rows // nbl)
+    a2 = np.tile(np.asarray(ms["antenna2"]), rows // nbl)
+    flag = np.tile(np.asarray(ms["flag"], dtype=np.uint8), rows // nbl)
+    write_raw_ms(path, antenna1=a1, antenna2=a2, time=ms["time"], flag_c0p0=flag, uvw=ms["uvw"], data=ms["vis"],
+                 antenna_table_rows=ms["total_antennas"] if antenna_table_rows is None else antenna_table_rows,
+                 ref_frequency_hz=ms["freq_start_hz"], chan_width_hz=ms["freq_inc_hz"],
+                 phase_dir_rad=ms["phase_centre"])
+
+
 def phasor64(theta):
     """(cos, sin) of theta as the batched kernels compute them (host build of the same routine)."""
     th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
@@ -524,6 +570,17 @@ class Context:
 
     def last_graph(self):
         return int(lib().leap_hip_last_graph(self.h))
+
+    def tuning_json(self):
+        """The autotuners' choices held by this context, as a JSON string."""
+        n = ctypes.c_uint64()
+        self._chk(lib().leap_hip_tuning_save(self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._chk(lib().leap_hip_tuning_save(self.h, buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def load_tuning(self, text):
+        self._chk(lib().leap_hip_tuning_load(self.h, text.encode()))
 
     def last_staged_bytes(self):
         return int(lib().leap_hip_last_staged_bytes(self.h))

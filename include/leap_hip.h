@@ -361,6 +361,17 @@ int leap_hip_last_graph(const leap_ctx* ctx);
 /* Average device time (ms) of the pol-0 pack launches of the last call and
  * their number (0 when the call packed nothing). */
 int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg, int32_t* pack_launches);
+/* ---- autotuner cache ------------------------------------------------------ */
+/* The tiles (and multi-set grid depths) the context's autotuners chose, as a
+ * JSON document {"format": "leap-hip-tuning", "version": 1, "batched": [...],
+ * "multi": [...]}.  Writes at most cap bytes, NUL-terminated, into buf (may be
+ * NULL) and the document's length + 1 into *needed. */
+int leap_hip_tuning_save(const leap_ctx* ctx, char* buf, uint64_t cap, uint64_t* needed);
+/* Adds the choices of a document written by leap_hip_tuning_save; later calls
+ * of those shapes use them without timing candidates.  The document is checked
+ * whole before anything is added. */
+int leap_hip_tuning_load(leap_ctx* ctx, const char* json);
+
 /* Bytes the last call sent host -> device through the pinned staging buffers
  * (host-resident visibilities; pol 0 only on the packed path). */
 uint64_t leap_hip_last_staged_bytes(const leap_ctx* ctx);
