@@ -210,6 +210,28 @@ bool use_packed(const leap_options& opts) {
            !(opts.precision == LEAP_PRECISION_MIXED && opts.fp32_blocks) && !opts.direct_cube;
 }
 
+// The batched pol-0 route for a tuned call on a device-resident cube: the pack
+// pass (every pol read, pol 0 written as FP64) pays only when the FP64 work is
+// large enough.  Rates measured on MI355X at the 256-station set (32,896
+// baselines, 128 t, 384 ch): pack 5.2 TB/s; packed Horner 32 / 50 / 55 TFLOP/s
+// with 1 / 2 / 4 directions per lane (up to 64, 128, more directions); the
+// Horner kernel straight from the cube reads at most 6.4 TB/s and computes
+// 30 / 41 TFLOP/s (up to 128 / more directions).  E.g. 256 directions: packed
+// for complex64 (72 vs 80 ms), from the cube for complex128 (80 vs 82 ms);
+// 64 directions on complex128: from the cube (31 vs 52 ms).  Host-resident
+// cubes always take the packed route (pol 0 only crosses the bus).
+leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, uint32_t D) {
+    leap_options r = o;
+    if (!use_packed(o) || o.direction_block != 0 || !ms.vis_on_device) return r;
+    const double rc = double(T) * ms.num_baselines * ms.num_channels;  // (row, channel)s
+    const double s = ms.vis_dtype == LEAP_VIS_C64 ? 8.0 : 16.0;
+    const double flops = 8.0 * rc * D;
+    const double packed = rc * (ms.num_pols * s + 16.0) / 5.2e12 + flops / (D > 128 ? 55e12 : D > 64 ? 50e12 : 32e12);
+    const double direct = std::max(rc * ms.num_pols * s / 6.4e12, flops / (D > 128 ? 41e12 : 30e12));
+    if (packed > direct) r.direct_cube = 1;
+    return r;
+}
+
 // Whether a batched rotate takes the packed path: the caller's tile (or, when
 // tuning, the requested one) must be a packed tile.
 bool packed_route(const leap_options& opts, int32_t tile) {
@@ -659,7 +681,7 @@ void calibrate_loop_graph(leap_ctx* ctx, const leap_metadata& md, const leap_ms_
 
 // The Calibrate loop (CpuLeapCalibrator.cc:61-145) over prepared metadata.
 void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const double* dirs, uint32_t D,
-                    int32_t s_start, int32_t s_interval, int32_t s_end, const leap_options& opts,
+                    int32_t s_start, int32_t s_interval, int32_t s_end, const leap_options& opts_in,
                     leap_solution_cb cb, void* user) {
     validate_view(ms);
     check_metadata_matches(md, ms);
@@ -671,6 +693,7 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
     const uint32_t interval = uint32_t(range[1]);
     const uint32_t nbl = ms.num_baselines;
     const int32_t N = md.A.cols;
+    const leap_options opts = routed(opts_in, ms, interval, D);
     if (opts.graph) {
         calibrate_loop_graph(ctx, md, ms, dirs, D, solutions, interval, opts, cb, user);
         return;
@@ -1169,11 +1192,11 @@ int leap_hip_rotate_visibilities(leap_ctx* ctx, const leap_metadata* md, const l
                                  double* avg_out) {
     return guarded(ctx, [&] {
         require(ctx && md && ms && avg_out && (dirs || D == 0), LEAP_ERR_INVALID_ARGUMENT, "null argument");
-        const leap_options o = resolve(opts);
         validate_view(*ms);
         check_metadata_matches(*md, *ms);
         require(n_rows % ms->num_baselines == 0, LEAP_ERR_RANGE,
                 "integration rows must be a whole number of timesteps");
+        const leap_options o = routed(resolve(opts), *ms, n_rows / ms->num_baselines, D);
         DeviceScope scope(ctx->device);
         begin_timing(ctx);
         const DeviceRows rows = stage_rows(ctx, *ms, row_start, row_start + n_rows, !packed_route(o, 0));

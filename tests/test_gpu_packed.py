@@ -86,8 +86,11 @@ def test_packed_chunks_in_a_graph(leap, ctx):
         view = leap.view_from_arrays(ms, vis_device=vis_p, vis_dtype=leap.LEAP_VIS_C128, uvw_device=uvw_p)
         md = ctx.metadata(view)
         dirs = dirs_near(ms["phase_centre"], 40, 5)
-        eager = ctx.calibrate(view, dirs, (0, 12, -1), opts=leap.options(strategy=1, pack_timesteps=5), md=md)
-        g = leap.options(strategy=1, pack_timesteps=5, graph=1)
+        # an explicit packed tile: a tuned call this small would run straight from the cube
+        kw = dict(strategy=1, direction_block=64, lane_directions=1, pack_timesteps=5)
+        eager = ctx.calibrate(view, dirs, (0, 12, -1), opts=leap.options(**kw), md=md)
+        assert ctx.last_pack_times()[1] == 6  # 2 intervals of 12 timesteps, chunks of 5
+        g = leap.options(graph=1, **kw)
         ctx.calibrate(view, dirs, (0, 12, -1), opts=g, md=md)
         assert ctx.last_graph() == 1
         for rep in range(3):
@@ -98,3 +101,23 @@ def test_packed_chunks_in_a_graph(leap, ctx):
     finally:
         ctx.free(vis_p)
         ctx.free(uvw_p)
+
+
+def test_route_follows_the_cost_model(leap, ctx):
+    """Tuned calls on a device cube pack only when the FP64 work outweighs the pack pass."""
+    s = leap.SynthMS(n_antennas=16, n_timesteps=4, n_channels=24, seed=94)
+    for dtype, vd in ((np.complex64, leap.LEAP_VIS_C64), (np.complex128, leap.LEAP_VIS_C128)):
+        host = s.fill_host(dtype)
+        vis_p = ctx.malloc(host.nbytes)
+        try:
+            ctx.h2d(vis_p, host)
+            ms = s.host_dict(host)
+            view = leap.view_from_arrays(ms, vis_device=vis_p, vis_dtype=vd)
+            md = ctx.metadata(view)
+            c = ms["phase_centre"]
+            for D, packs in ((64, 0), (256, 1 if dtype == np.complex64 else 0)):
+                dirs = np.column_stack([np.full(D, c[0]), np.linspace(c[1] - 0.01, c[1] + 0.01, D)])
+                md.rotate(view, dirs, opts=leap.options(strategy=1))
+                assert ctx.last_pack_times()[1] == packs, (dtype, D)
+        finally:
+            ctx.free(vis_p)

@@ -39,9 +39,13 @@ def big(request, leap, ctx):
     ctx.free(uvw_p)
 
 
-@pytest.mark.parametrize("opts_kw", [dict(strategy=1), dict(strategy=1, pack_timesteps=30),
-                                     dict(strategy=1, precision=1), dict(strategy=1, direct_cube=1),
-                                     dict(strategy=0)],
+# Packed cases name their tile: a tuned call on the device cube would be routed straight from the cube
+# at this direction count, while a host cube always takes the packed route.
+PK = dict(strategy=1, direction_block=64, lane_directions=1)
+
+
+@pytest.mark.parametrize("opts_kw", [PK, dict(PK, pack_timesteps=30), dict(PK, precision=1),
+                                     dict(strategy=1, direct_cube=1), dict(strategy=0)],
                          ids=["packed", "packed-chunks", "packed-mixed", "direct-cube", "per-direction"])
 def test_host_cube_equals_device_cube(leap, ctx, big, opts_kw):
     s, ms, md, host_view, dev_view, elem = big
