@@ -144,6 +144,8 @@ def main():
     ap.add_argument("--direct-cube", action="store_true",
                     help="batched: run the Horner kernel straight from the cube (no pol-0 pack)")
     ap.add_argument("--pack-timesteps", type=int, default=0, help="timesteps per pack chunk (0 = automatic)")
+    ap.add_argument("--channel-sum", default="auto", choices=["auto", "horner", "nufft"],
+                    help="mixed precision: channel sums by FP64 Horner or the NUFFT (auto: NUFFT where supported)")
     ap.add_argument("--host-cube", action="store_true",
                     help="batched modes: the cube stays in (pageable) host memory and every step stages it "
                          "(PCIe-inclusive rate; not the headline value, which has the cube resident in HBM)")
@@ -193,6 +195,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     opts.lane_directions = args.lane_directions
     opts.direct_cube = 1 if args.direct_cube else 0
     opts.pack_timesteps = args.pack_timesteps
+    opts.channel_sum = {"auto": 0, "horner": 1, "nufft": 2}[args.channel_sum]
 
     cfg = c1_config()
     log(f"[{mode}] building synthetic measurement set (host columns)")

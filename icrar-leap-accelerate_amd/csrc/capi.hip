@@ -76,6 +76,14 @@ leap_options resolve(const leap_options* o) {
     require(r.pack_timesteps >= 0, LEAP_ERR_INVALID_ARGUMENT, "pack_timesteps must be >= 0");
     require(r.workgroups_per_cu >= -1 && r.workgroups_per_cu <= 64, LEAP_ERR_INVALID_ARGUMENT,
             "workgroups per CU must be -1 (one work item each), 0 (tuned) or 1 to 64");
+    require(r.channel_sum >= LEAP_CHANNEL_SUM_AUTO && r.channel_sum <= LEAP_CHANNEL_SUM_NUFFT,
+            LEAP_ERR_INVALID_ARGUMENT, "channel_sum must be 0 (automatic), 1 (Horner) or 2 (NUFFT)");
+    if (r.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+        require(r.precision == LEAP_PRECISION_MIXED, LEAP_ERR_INVALID_ARGUMENT,
+                "the NUFFT channel sums are a mixed-precision method");
+        require(!r.all_pols && !r.fp32_blocks && !r.direct_cube, LEAP_ERR_INVALID_ARGUMENT,
+                "the NUFFT channel sums run on packed pol 0 (no all_pols, fp32_blocks or direct_cube)");
+    }
     return r;
 }
 
@@ -222,6 +230,16 @@ bool use_packed(const leap_options& opts) {
 // cubes always take the packed route (pol 0 only crosses the bus).
 leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, uint32_t D) {
     leap_options r = o;
+    if (r.channel_sum == LEAP_CHANNEL_SUM_AUTO)
+        r.channel_sum = use_packed(r) && r.precision == LEAP_PRECISION_MIXED && nufft_supported(ms.num_channels) &&
+                                D >= kNufftMinDirections
+                            ? LEAP_CHANNEL_SUM_NUFFT
+                            : LEAP_CHANNEL_SUM_HORNER;
+    if (r.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+        require(nufft_supported(ms.num_channels), LEAP_ERR_INVALID_ARGUMENT,
+                "the NUFFT channel sums support 1.." + std::to_string(kNufftMaxChannels) + " channels");
+        return r;
+    }
     if (!use_packed(o) || o.direction_block != 0 || !ms.vis_on_device) return r;
     const double rc = double(T) * ms.num_baselines * ms.num_channels;  // (row, channel)s
     const double s = ms.vis_dtype == LEAP_VIS_C64 ? 8.0 : 16.0;
@@ -251,8 +269,9 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
         return opts.direction_block * 10 + (fp32 ? 1 : opts.lane_directions);
     if (opts.direction_block != 0 && opts.lane_directions == 0) return opts.direction_block * 10 + 1;
     const bool packed = use_packed(opts);
+    const bool nufft = packed && opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
     const TuneKey key{opts.precision * 4 + (opts.fp32_blocks ? 2 : 0) + (opts.lane_directions ? 1 : 0) +
-                          16 * opts.lane_directions + (packed ? kTunePackedBit : 0),
+                          16 * opts.lane_directions + (packed ? kTunePackedBit : 0) + (nufft ? kTuneNufftBit : 0),
                       ms.vis_dtype, opts.all_pols, D, T, md.channels, md.pols};
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) return it->second;
@@ -287,7 +306,11 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
         float best_of = 0.f;
         for (int rep = 0; rep < 2; ++rep) {  // the first repetition warms the code object
             LEAP_HIP(hipEventRecord(a, ctx->stream));
-            if (packed)
+            if (nufft)
+                launch_rotate_nufft(ctx->stream, tile, md.nufft, ctx->d_packed.as<double>(), rows.uvw,
+                                    ctx->d_dd.as<double>(), D, T, md.nbl, probe_bl, md.channels, md.pols,
+                                    md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
+            else if (packed)
                 launch_rotate_packed(ctx->stream, tile, ctx->d_packed.as<double>(), rows.uvw, ctx->d_dd.as<double>(),
                                      D, T, md.nbl, probe_bl, md.channels, md.pols, md.freq_start_hz, md.freq_inc_hz,
                                      ctx->d_avg.as<double>(), 0);
@@ -341,6 +364,8 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
     ctx->d_packed.reserve(buf_elems * (n_chunks > 1 ? 2 : 1) * 2 * sizeof(double));
     const size_t vis_row_bytes = vis_elem_bytes(ms) * md.channels;
     const bool host = rows.vis == nullptr;
+    const bool nufft = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
+    const double* scale = nufft ? md.d_nufft_scale.as<double>() : nullptr;
     double* buf[2] = {ctx->d_packed.as<double>(), ctx->d_packed.as<double>() + 2 * buf_elems};
     auto pack = [&](uint32_t k, hipStream_t st) {
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
@@ -348,19 +373,25 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
         if (host) {
             stage_pol0(ctx, static_cast<const char*>(rows.host_vis) + off, ms.vis_dtype, md.pols,
                        size_t(nt) * per_t, buf[k % 2]);
+            if (nufft) launch_scale_channels(ctx->stream, buf[k % 2], size_t(nt) * per_t, scale, md.channels);
             return;
         }
         record_pack_begin(ctx, st);
         launch_pack_pol0(st, static_cast<const char*>(rows.vis) + off, ms.vis_dtype, size_t(nt) * per_t, md.pols,
-                         buf[k % 2]);
+                         buf[k % 2], scale, md.channels);
         record_pack_end(ctx, st);
     };
     auto horner = [&](uint32_t k) {
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
         record_rotate_begin(ctx);
-        launch_rotate_packed(ctx->stream, tile, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
-                             ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
-                             md.freq_inc_hz, ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
+        if (nufft)
+            launch_rotate_nufft(ctx->stream, tile, md.nufft, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
+                                ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols,
+                                md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
+        else
+            launch_rotate_packed(ctx->stream, tile, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
+                                 ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols,
+                                 md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
         record_rotate_end(ctx);
     };
     pack(0, ctx->stream);
@@ -399,9 +430,11 @@ int32_t batched_tile(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view&
     if (!use_packed(opts)) return tuned_direction_block(ctx, md, ms, rows, T, D, opts);
     const uint32_t tc = std::min(T, pack_chunk(T, md.nbl, md.channels, opts));
     const size_t n = size_t(tc) * md.nbl * md.channels;
+    const double* scale = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT ? md.d_nufft_scale.as<double>() : nullptr;
     return tuned_direction_block(ctx, md, ms, rows, tc, D, opts, [&] {
         ctx->d_packed.reserve(n * 2 * sizeof(double));
-        launch_pack_pol0(ctx->stream, rows.vis, ms.vis_dtype, n, md.pols, ctx->d_packed.as<double>());
+        launch_pack_pol0(ctx->stream, rows.vis, ms.vis_dtype, n, md.pols, ctx->d_packed.as<double>(), scale,
+                         md.channels);
     });
 }
 
@@ -534,6 +567,12 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
     launch_assemble_ad(s, g1p, N, d_pairs1.as<int2>(), K1 - 1, md->A1.constraint, md->d_Ad1T.as<double>());
     md->d_AdT_planes.reserve(bf16_planes_bytes(uint32_t(N), K));
     launch_split_b(s, md->d_AdT.as<double>(), K, N, md->d_AdT_planes.ptr);
+    if (nufft_supported(md->channels)) {
+        md->nufft = make_nufft_plan(md->channels);
+        md->d_nufft_scale.reserve(md->nufft.scale.size() * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(md->d_nufft_scale.ptr, md->nufft.scale.data(), md->nufft.scale.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+    }
     LEAP_HIP(hipStreamSynchronize(s));
     return md.release();
 }
@@ -577,6 +616,7 @@ uint64_t key_ptr(const void* p) { return uint64_t(reinterpret_cast<uintptr_t>(p)
 void key_common(std::vector<uint64_t>& k, const leap_ctx* ctx, const leap_options& opts) {
     k.insert(k.end(), {uint64_t(opts.precision), uint64_t(opts.strategy), uint64_t(opts.all_pols),
                        uint64_t(opts.fp32_blocks), uint64_t(opts.direct_cube), uint64_t(opts.pack_timesteps),
+                       uint64_t(opts.channel_sum),
                        key_ptr(ctx->d_packed.ptr),
                        key_ptr(ctx->d_dd.ptr), key_ptr(ctx->d_avg.ptr),
                        key_ptr(ctx->d_phase.ptr), key_ptr(ctx->d_cal1.ptr), key_ptr(ctx->d_delta.ptr),
@@ -1420,6 +1460,20 @@ int leap_phasor64(const double* theta, uint32_t n, double* cos_out, double* sin_
             cos_out[i] = p.c;
             sin_out[i] = p.s;
         }
+    });
+}
+
+int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef, double* scale) {
+    return guarded(nullptr, [&] {
+        const NufftPlan p = make_nufft_plan(C);
+        if (N) *N = p.N;
+        if (W) *W = p.W;
+        if (h) *h = p.h;
+        if (beta) *beta = p.beta;
+        if (coef)
+            for (int j = 0; j < kNufftMaxWidth / 2; ++j)
+                for (int m = 0; m < kNufftMaxWidth; ++m) coef[j * kNufftMaxWidth + m] = p.coef[j][m];
+        if (scale) std::copy(p.scale.begin(), p.scale.end(), scale);
     });
 }
 

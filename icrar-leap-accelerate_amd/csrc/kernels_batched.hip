@@ -28,64 +28,15 @@
 // for finite and NaN inputs the two rules agree, and for an infinite input the
 // reference's result depends on the direction (inf * 0 = NaN), where here the
 // channel is always skipped.
-#include "leap_internal.h"
-#include "phasor.h"
+#include "batched_common.h"
 
 namespace leap {
 namespace {
 
+using namespace dev;
+
 constexpr int kThreads = 256;
 constexpr int kBlockCh = 32;  // Horner block length (channels)
-
-template <typename T>
-struct C2 {
-    T x, y;
-};
-
-// Complex product with the fused operations written out (no contraction), so
-// every kernel evaluates it the same way.
-template <typename T>
-__device__ inline C2<T> cmul(C2<T> a, C2<T> b) {
-#pragma clang fp contract(off)
-    return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)};
-}
-
-// a * z + v with fused multiply-adds.
-template <typename T>
-__device__ inline C2<T> horner_step(C2<T> a, C2<T> z, C2<T> v) {
-    C2<T> r;
-    r.x = fma(a.x, z.x, fma(-a.y, z.y, v.x));
-    r.y = fma(a.x, z.y, fma(a.y, z.x, v.y));
-    return r;
-}
-
-constexpr double kTwoPiHi = 6.283185307179586232;           // double(2 pi)
-constexpr double kTwoPiLo = 2.4492935982947064e-16;         // 2 pi - double(2 pi)
-constexpr double kInvTwoPi = 0.15915494309189533577;
-
-// exp(i theta) with the argument reduced modulo 2 pi in FP64 and the sincos in FP32.
-__device__ inline C2<float> phasor_mixed(double theta) {
-    const double n = rint(theta * kInvTwoPi);
-    double r = fma(-n, kTwoPiHi, theta);
-    r = fma(-n, kTwoPiLo, r);
-    float s, c;
-    sincosf(float(r), &s, &c);
-    return {c, s};
-}
-
-__device__ inline C2<double> phasor_f64(double theta) {
-    const Phasor64 p = phasor64(theta);
-    return {p.c, p.s};
-}
-
-// -2 pi ((dd . uvw).z - w) (CpuLeapCalibrator.cc:190) with the fused operations written out.
-__device__ inline double shift_factor(const double dz[3], double u, double v, double w) {
-#pragma clang fp contract(off)
-    double t = dz[0] * u;
-    t = fma(dz[1], v, t);
-    t = fma(dz[2], w, t);
-    return -kTwoPi * (t - w);
-}
 
 template <typename A>
 __device__ inline C2<A> phasor(double theta);
@@ -263,9 +214,6 @@ struct Horner64Smem {
     double red[(RP > 1) ? RP : 1][L][ND][NP][2];
 };
 
-// Global-memory pointers: pointers read from a descriptor table are otherwise
-// generic, and flat memory instructions share the LDS wait counter.
-#define LEAP_GLOBAL __attribute__((address_space(1)))
 
 // One (baseline, direction block) tile; every thread of the workgroup calls it.
 template <typename VT, int DB, int ND, int NP, int CC>
@@ -485,15 +433,39 @@ constexpr int kWaitScalarMem = 0xC07F;
 // The pol-0 visibility of every (row, channel) as FP64 complex, 0 where any
 // polarisation of that (row, channel) is not finite (the reference skips the
 // whole (row, channel) then, CpuLeapCalibrator.cc:201-214).  HBM-bound: reads
-// the cube once, writes 16 B per (row, channel).
-template <typename VT, int P>
+// the cube once, writes 16 B per (row, channel).  SCALED multiplies each value
+// by scale[channel] (the NUFFT precorrection); element i is channel i % C.
+template <typename VT, int P, bool SCALED>
 __global__ __launch_bounds__(kThreads) void pack_pol0_kernel(const VT* __restrict__ vis, size_t n,
-                                                             C2<double>* __restrict__ out) {
+                                                             C2<double>* __restrict__ out,
+                                                             const double* __restrict__ scale, uint32_t C) {
     // Four independent (row, channel)s per thread and step keep eight loads in
     // flight per thread; consecutive threads take consecutive elements.
     constexpr int U = 4;
     const size_t stride = size_t(gridDim.x) * kThreads;
     size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    // channel of element i and the channel step of one stride (SCALED only)
+    uint32_t ch = 0, ch_step = 0;
+    if constexpr (SCALED) {
+        ch = uint32_t(i % C);
+        ch_step = uint32_t(stride % C);
+    }
+    auto next_ch = [&](uint32_t c) {
+        c += ch_step;
+        return c >= C ? c - C : c;
+    };
+    auto value = [&](const VT* x, uint32_t c) {
+        bool finite = true;
+#pragma unroll
+        for (int k = 0; k < 2 * P; ++k) finite = finite && __builtin_isfinite(x[k]);
+        C2<double> r = finite ? C2<double>{double(x[0]), double(x[1])} : C2<double>{0.0, 0.0};
+        if constexpr (SCALED) {
+            const double f = scale[c];
+            r.x *= f;
+            r.y *= f;
+        }
+        return r;
+    };
     for (; i + (U - 1) * stride < n; i += U * stride) {
         VT x[U][2 * P];
 #pragma unroll
@@ -502,20 +474,34 @@ __global__ __launch_bounds__(kThreads) void pack_pol0_kernel(const VT* __restric
             for (int k = 0; k < 2 * P; ++k) x[u][k] = vis[(i + u * stride) * (2 * P) + k];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            bool finite = true;
-#pragma unroll
-            for (int k = 0; k < 2 * P; ++k) finite = finite && __builtin_isfinite(x[u][k]);
-            out[i + u * stride] = finite ? C2<double>{double(x[u][0]), double(x[u][1])} : C2<double>{0.0, 0.0};
+            out[i + u * stride] = value(x[u], ch);
+            if constexpr (SCALED) ch = next_ch(ch);
         }
     }
     for (; i < n; i += stride) {
         VT x[2 * P];
 #pragma unroll
         for (int k = 0; k < 2 * P; ++k) x[k] = vis[i * (2 * P) + k];
-        bool finite = true;
-#pragma unroll
-        for (int k = 0; k < 2 * P; ++k) finite = finite && __builtin_isfinite(x[k]);
-        out[i] = finite ? C2<double>{double(x[0]), double(x[1])} : C2<double>{0.0, 0.0};
+        out[i] = value(x, ch);
+        if constexpr (SCALED) ch = next_ch(ch);
+    }
+}
+
+// x[i] *= scale[i % C] over n FP64 complex values.
+__global__ __launch_bounds__(kThreads) void scale_channels_kernel(C2<double>* __restrict__ x, size_t n,
+                                                                  const double* __restrict__ scale, uint32_t C) {
+    const size_t stride = size_t(gridDim.x) * kThreads;
+    size_t i = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    uint32_t ch = uint32_t(i % C);
+    const uint32_t step = uint32_t(stride % C);
+    for (; i < n; i += stride) {
+        const double f = scale[ch];
+        C2<double> v = x[i];
+        v.x *= f;
+        v.y *= f;
+        x[i] = v;
+        ch += step;
+        if (ch >= C) ch -= C;
     }
 }
 
@@ -738,15 +724,22 @@ void launch_np(hipStream_t s, int db, bool all_pols, const void* vis, const doub
 
 }  // namespace
 
-void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t n, uint32_t P, double* out) {
+void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t n, uint32_t P, double* out,
+                      const double* scale, uint32_t C) {
     if (n == 0) return;
+    require(scale == nullptr || C > 0, LEAP_ERR_INTERNAL, "scaled pack needs the channel count");
     // 256 workgroups per CU: read/write streams reach ~5.2 TB/s on MI355X
     // (a device-to-device copy reaches 4.9 TB/s).
     const size_t want = (n + 4 * kThreads - 1) / (4 * kThreads);
     const uint32_t grid = uint32_t(std::min<size_t>(want, 256 * 256));
     C2<double>* o = reinterpret_cast<C2<double>*>(out);
 #define LEAP_PACK(VT, NP_)                                                                                    \
-    pack_pol0_kernel<VT, NP_><<<grid, kThreads, 0, s>>>(static_cast<const VT*>(vis), n, o)
+    do {                                                                                                      \
+        if (scale)                                                                                            \
+            pack_pol0_kernel<VT, NP_, true><<<grid, kThreads, 0, s>>>(static_cast<const VT*>(vis), n, o, scale, C); \
+        else                                                                                                  \
+            pack_pol0_kernel<VT, NP_, false><<<grid, kThreads, 0, s>>>(static_cast<const VT*>(vis), n, o, nullptr, 0); \
+    } while (0)
     const bool c64 = vis_dtype == LEAP_VIS_C64;
     switch (P) {
         case 1: if (c64) LEAP_PACK(float, 1); else LEAP_PACK(double, 1); break;
@@ -756,6 +749,13 @@ void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t 
         default: throw Error(LEAP_ERR_NOT_IMPLEMENTED, "polarisation counts above 4 are not supported");
     }
 #undef LEAP_PACK
+    LEAP_HIP(hipGetLastError());
+}
+
+void launch_scale_channels(hipStream_t s, double* x, size_t n, const double* scale, uint32_t C) {
+    if (n == 0) return;
+    const uint32_t grid = uint32_t(std::min<size_t>((n + kThreads - 1) / kThreads, 256 * 64));
+    scale_channels_kernel<<<grid, kThreads, 0, s>>>(reinterpret_cast<C2<double>*>(x), n, scale, C);
     LEAP_HIP(hipGetLastError());
 }
 

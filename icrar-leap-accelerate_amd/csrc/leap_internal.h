@@ -91,6 +91,23 @@ bool multi_tile_supported(int32_t tile);
 double multi_item_cost(uint32_t T, uint32_t C, int32_t tile);
 MultiPlan plan_multi(const std::vector<MultiShape>& sets, int32_t tile, uint32_t n_blocks);
 
+// NUFFT channel sums (nufft_plan.cpp).
+constexpr uint32_t kNufftMaxChannels = 512;  // grid N <= 1024
+constexpr int kNufftMaxWidth = 12;
+constexpr int kNufftQuadratureNodes = 100;
+constexpr uint32_t kNufftMinDirections = 16;  // automatic choice: fewer directions stay on Horner
+struct NufftPlan {
+    uint32_t N = 0;  // grid size (power of two, >= 2C, >= 256)
+    uint32_t W = 0;  // kernel width in grid points (10 or 12)
+    uint32_t h = 0;  // 64 * round(C / 128): frequency k = c - h
+    double beta = 0.0;
+    // coef[j][m]: coefficient of s^m in tap j's weight polynomial, j < W / 2
+    double coef[kNufftMaxWidth / 2][kNufftMaxWidth] = {};
+    std::vector<double> scale;  // C precorrection factors (2 pi / N) / psi^(c - h)
+};
+bool nufft_supported(uint32_t C);
+NufftPlan make_nufft_plan(uint32_t C);
+
 }  // namespace leap
 
 // Metadata resident on the device (cpu::MetaData + cuda::ConstantBuffer of the reference).
@@ -110,10 +127,14 @@ struct leap_metadata {
     leap::DeviceBuffer d_lambda;   // channels doubles: wavelength of each channel
     leap::DeviceBuffer d_Gp;       // scratch for Ad assembly
     leap::DeviceBuffer d_AdT_planes;  // split-3 x BF16 planes of AdT in MFMA operand layout
+    leap::NufftPlan nufft;             // N == 0 when the channel count is not supported
+    leap::DeviceBuffer d_nufft_scale;  // nufft.scale on the device
 };
 
-// TuneKey::precision holds the batched tuner's mode code; this bit marks the packed path.
+// TuneKey::precision holds the batched tuner's mode code; these bits mark the
+// packed path and its NUFFT channel sums.
 constexpr int32_t kTunePackedBit = 1024;
+constexpr int32_t kTuneNufftBit = 2048;
 
 struct TuneKey {
     int32_t precision, vis_dtype, all_pols;
@@ -222,8 +243,18 @@ void stage_copy(leap_ctx* ctx, const void* host_src, size_t bytes, void* device_
 // pol 0 crosses the bus (complex64 sources are widened on the device).
 void stage_pol0(leap_ctx* ctx, const void* host_vis, int32_t vis_dtype, uint32_t P, size_t n, double* device_dst);
 void launch_widen_c64(hipStream_t s, const void* in, size_t n, double* out);
-// Packed pol-0 path: pack the cube, then FP64 Horner fed by scalar loads.
-void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t n, uint32_t P, double* out);
+// Packed pol-0 path: pack the cube, then FP64 Horner fed by scalar loads.  With
+// scale non-null every element is multiplied by scale[channel] (n counts whole
+// rows of C channels).
+void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t n, uint32_t P, double* out,
+                      const double* scale = nullptr, uint32_t C = 0);
+// x[i] *= scale[i % C] for n packed FP64 complex values (whole rows).
+void launch_scale_channels(hipStream_t s, double* x, size_t n, const double* scale, uint32_t C);
+// NUFFT channel sums over precorrected packed pol-0 rows (kernels_nufft.hip);
+// tiles 641 / 1282 / 2564 as for launch_rotate_packed.
+void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* w, const double* uvw,
+                         const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C,
+                         uint32_t P, double f0, double df, double* avg, int32_t accumulate);
 bool packed_tile_supported(int32_t tile);
 void launch_rotate_packed(hipStream_t s, int32_t tile, const double* w, const double* uvw, const double* dd,
                           uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,

@@ -19,6 +19,9 @@ LEAP_VIS_C128 = 0
 LEAP_VIS_C64 = 1
 LEAP_PRECISION_FP64 = 0
 LEAP_PRECISION_MIXED = 1
+LEAP_CHANNEL_SUM_AUTO = 0
+LEAP_CHANNEL_SUM_HORNER = 1
+LEAP_CHANNEL_SUM_NUFFT = 2
 LEAP_STRATEGY_PER_DIRECTION = 0
 LEAP_STRATEGY_BATCHED = 1
 
@@ -80,12 +83,14 @@ class Options(ctypes.Structure):
         ("graph", ctypes.c_int32),
         ("direct_cube", ctypes.c_int32),
         ("pack_timesteps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("channel_sum", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
 def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION, direction_block=0, all_pols=0,
-            fp32_blocks=0, lane_directions=0, workgroups_per_cu=0, graph=0, direct_cube=0, pack_timesteps=0):
+            fp32_blocks=0, lane_directions=0, workgroups_per_cu=0, graph=0, direct_cube=0, pack_timesteps=0,
+            channel_sum=0):
     o = Options()
     o.precision = precision
     o.strategy = strategy
@@ -97,6 +102,7 @@ def options(precision=LEAP_PRECISION_FP64, strategy=LEAP_STRATEGY_PER_DIRECTION,
     o.graph = graph
     o.direct_cube = direct_cube
     o.pack_timesteps = pack_timesteps
+    o.channel_sum = channel_sum
     return o
 
 
@@ -169,7 +175,7 @@ EXPORTS = [
     "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes",
     "leap_hip_tuning_save", "leap_hip_tuning_load", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
-    "leap_multi_item_cost", "leap_phasor64",
+    "leap_multi_item_cost", "leap_phasor64", "leap_nufft_plan",
 ]
 
 
@@ -208,6 +214,8 @@ def lib():
                                       _u32p, _u32p, _u32p, _dp]
     L.leap_multi_item_cost.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
     L.leap_phasor64.argtypes = [_dp, ctypes.c_uint32, _dp, _dp]
+    _u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.leap_nufft_plan.argtypes = [ctypes.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
     L.leap_multi_item_cost.restype = ctypes.c_double
     L.leap_hip_metadata_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), ctypes.c_int32,
                                            ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
@@ -449,6 +457,20 @@ def phasor64(theta):
 
 def multi_item_cost(T, C, tile):
     return lib().leap_multi_item_cost(T, C, tile)
+
+
+def nufft_plan(C):
+    """The NUFFT plan the mixed path uses for C channels: dict of N, W, h, beta,
+    coef (W/2 x W weight-polynomial coefficients in s) and scale (C precorrection factors)."""
+    N, W, h = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    beta = ctypes.c_double()
+    coef = np.zeros(6 * 12)
+    scale = np.zeros(max(int(C), 1))
+    _check(lib().leap_nufft_plan(C, ctypes.byref(N), ctypes.byref(W), ctypes.byref(h), ctypes.byref(beta),
+                                 _dptr(coef), _dptr(scale)))
+    w = W.value
+    return {"N": N.value, "W": w, "h": h.value, "beta": beta.value,
+            "coef": coef.reshape(6, 12)[: w // 2, :w].copy(), "scale": scale[: int(C)].copy()}
 
 
 # ---------------------------------------------------------------------------

@@ -110,6 +110,17 @@ typedef enum leap_strategy {
     LEAP_STRATEGY_BATCHED = 1
 } leap_strategy;
 
+typedef enum leap_channel_sum {
+    /* Mixed precision with a supported channel count (1..512): the NUFFT; otherwise Horner. */
+    LEAP_CHANNEL_SUM_AUTO = 0,
+    /* FP64 Horner: four FMAs per channel and direction, exact to FP64 rounding. */
+    LEAP_CHANNEL_SUM_HORNER = 1,
+    /* Type-2 non-uniform FFT per row (one inverse FFT of 256..1024 points, then W = 10 or 12
+     * grid points per direction), relative error about 1e-11 of sum |v|; mixed precision,
+     * pol 0, packed path, 1..512 channels. */
+    LEAP_CHANNEL_SUM_NUFFT = 2
+} leap_channel_sum;
+
 typedef struct leap_options {
     int32_t precision;        /* leap_precision */
     int32_t strategy;         /* leap_strategy */
@@ -133,7 +144,8 @@ typedef struct leap_options {
     int32_t pack_timesteps;   /* packed path: timesteps per pack chunk; chunks are packed on a
                                  second stream while the Horner kernel runs the previous one.
                                  0 = automatic; >= the interval's timesteps = one chunk */
-    int32_t reserved[2];
+    int32_t channel_sum;      /* leap_channel_sum: how the batched pol-0 path sums channels */
+    int32_t reserved[1];
 } leap_options;
 
 /* One solution interval's result: cpu::Calibration (model/cpu/calibration/Calibration.h:42-111)
@@ -291,6 +303,15 @@ int leap_phase_matrix_pinv(const int32_t* a1, const int32_t* a2, const uint8_t* 
  * No reference counterpart (the reference calls std::exp on a complex,
  * CpuLeapCalibrator.cc:198); exported so the routine can be checked without a GPU. */
 int leap_phasor64(const double* theta, uint32_t n, double* cos_out, double* sin_out);
+
+/* The NUFFT plan of LEAP_CHANNEL_SUM_NUFFT for C channels (1..512): grid size
+ * N, kernel width W, frequency offset h (channel c is frequency c - h), the
+ * kernel's beta, the weight polynomials coef[j * 12 + m] (coefficient of s^m of
+ * tap j < W / 2; 6 x 12 doubles) and the C precorrection factors scale[c].
+ * Any output may be null.  No reference counterpart (the reference sums the
+ * channels directly, CpuLeapCalibrator.cc:181-217); exported so the plan can be
+ * checked without a GPU. */
+int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef, double* scale);
 
 /* ---- synthetic measurement sets ------------------------------------------- */
 typedef struct leap_synth_config {

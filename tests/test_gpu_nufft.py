@@ -1,0 +1,160 @@
+"""NUFFT channel sums of the mixed-precision batched path (LEAP_CHANNEL_SUM_NUFFT)
+against the oracle.
+
+Bars:
+- avgData: for every direction and baseline |got - oracle| <= 5e-10 * A, where
+  A = sum over the interval's rows and channels of |v| for that baseline (pol 0,
+  non-finite (row, channel)s excluded).  The NUFFT's error for one row and
+  direction is a fraction of sum_c |v_c| (1e-11 .. 1e-10 for the plans used,
+  tests/test_nufft_plan.py), so the bound is independent of size and data.
+- calibrations: within 1e-5 rad wrapped of the FP64 oracle (the mixed budget).
+- host-resident and device-resident cubes give identical results; chunked packs
+  agree with one chunk to summation order.
+"""
+import numpy as np
+import pytest
+
+import np_restatement as npr
+
+pytestmark = pytest.mark.gpu
+
+NUFFT = 2
+HORNER = 1
+
+
+def wrapped(a, b):
+    return np.abs(npr.wrap(np.asarray(a) - np.asarray(b)))
+
+
+def dirs_near(centre, n, seed, spread=0.02):
+    rng = np.random.default_rng(seed)
+    return np.column_stack([centre[0] + rng.uniform(-spread, spread, n), centre[1] + rng.uniform(-spread, spread, n)])
+
+
+def abs_mass(ms):
+    """sum over rows and channels of |pol 0| per baseline, skipping non-finite (row, channel)s."""
+    vis = ms["vis"]
+    ok = np.isfinite(vis.real).all(axis=2) & np.isfinite(vis.imag).all(axis=2)
+    a = np.where(ok, np.abs(np.nan_to_num(vis[:, :, 0].astype(np.complex128))), 0.0).sum(axis=1)
+    nbl = ms["num_baselines"]
+    return a.reshape(-1, nbl).sum(axis=0)
+
+
+def mixed(**kw):
+    return dict(precision=1, strategy=1, **kw)
+
+
+@pytest.mark.parametrize("tile", [641, 1282, 2564])
+@pytest.mark.parametrize("n_ch,n_dirs", [(48, 37), (77, 300), (130, 70), (256, 16), (384, 64), (500, 20)])
+def test_nufft_rotate_matches_oracle(leap, ctx, oracle, n_ch, n_dirs, tile):
+    print(f"nufft rotate: {n_ch} channels, {n_dirs} directions, tile {tile}", flush=True)
+    s = leap.SynthMS(n_antennas=20, n_timesteps=5, n_channels=n_ch, freq_start_hz=50e6, freq_inc_hz=781.25e3,
+                     layout=1, array_radius_m=20000.0, nan_fraction=0.003, seed=200 + n_ch)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    omd = oracle.metadata(ms)
+    dirs = dirs_near(ms["phase_centre"], n_dirs, 5, spread=0.1)
+    db, nd = tile // 10, tile % 10
+    got = md.rotate(view, dirs, opts=leap.options(**mixed(direction_block=db, lane_directions=nd, channel_sum=NUFFT)))
+    bound = 5e-10 * abs_mass(ms)
+    worst = 0.0
+    for d in range(n_dirs):
+        want = omd.rotate(dirs[d])[:, 0]
+        err = np.abs(got[d, :, 0] - want)
+        worst = max(worst, (err / bound).max())
+        assert np.all(err <= bound), d
+        assert not np.any(got[d, :, 1:])  # pols 1.. are not accumulated
+    print(f"  worst error / bound {worst:.3f}")
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_nufft_host_and_device_cubes_agree(leap, ctx, dtype):
+    s = leap.SynthMS(n_antennas=18, n_timesteps=6, n_channels=200, nan_fraction=0.004, seed=211)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(dtype))
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    dirs = dirs_near(ms["phase_centre"], 90, 6)
+    opts = leap.options(**mixed(channel_sum=NUFFT, direction_block=128, lane_directions=2))
+    host = md.rotate(view, dirs, opts=opts)
+    vis_p = ctx.malloc(ms["vis"].nbytes)
+    try:
+        ctx.h2d(vis_p, np.ascontiguousarray(ms["vis"]))
+        dview = leap.view_from_arrays(ms, vis_device=vis_p,
+                                      vis_dtype=leap.LEAP_VIS_C64 if dtype == np.complex64 else leap.LEAP_VIS_C128)
+        dev = md.rotate(dview, dirs, opts=opts)
+    finally:
+        ctx.free(vis_p)
+    assert np.array_equal(host, dev)
+
+
+@pytest.mark.parametrize("chunk", [3, 7])
+def test_nufft_chunks_agree_with_one_chunk(leap, ctx, chunk):
+    s = leap.SynthMS(n_antennas=16, n_timesteps=20, n_channels=96, nan_fraction=0.002, seed=212)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    dirs = dirs_near(ms["phase_centre"], 40, 7)
+    one = md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT)))
+    got = md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT, pack_timesteps=chunk)))
+    assert ctx.last_kernel_times()[1] == -(-20 // chunk)
+    assert np.abs(got - one).max() <= 1e-13 * np.abs(one).max()
+
+
+def test_nufft_calibration_error_budget(leap, ctx, oracle):
+    """The error-budget case of the mixed path (decorrelated long baselines,
+    where the inversion amplifies phase errors) on the NUFFT channel sums."""
+    s = leap.SynthMS(n_antennas=48, n_timesteps=8, n_channels=384, freq_start_hz=50e6, freq_inc_hz=781.25e3,
+                     layout=1, array_radius_m=20000.0, flag_fraction=0.02, seed=33)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    view = leap.view_from_arrays(ms)
+    dirs = dirs_near(ms["phase_centre"], 24, 6)
+    got, _ = ctx.calibrate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT)))
+    want, _ = oracle.calibrate(ms, dirs)
+    err = wrapped(got, want)
+    print(f"nufft calibrate: wrapped error max {err.max():.2e} rad, rms {np.sqrt((err ** 2).mean()):.2e} rad")
+    assert err.max() < 1e-5
+
+
+def test_nufft_is_the_automatic_mixed_choice(leap, ctx):
+    s = leap.SynthMS(n_antennas=12, n_timesteps=4, n_channels=64, seed=213)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    dirs = dirs_near(ms["phase_centre"], 32, 8)
+    auto = md.rotate(view, dirs, opts=leap.options(**mixed(direction_block=64, lane_directions=1)))
+    nufft = md.rotate(view, dirs, opts=leap.options(**mixed(direction_block=64, lane_directions=1, channel_sum=NUFFT)))
+    horner = md.rotate(view, dirs, opts=leap.options(**mixed(direction_block=64, lane_directions=1,
+                                                             channel_sum=HORNER)))
+    assert np.array_equal(auto, nufft)
+    assert not np.array_equal(auto, horner)
+    # few directions stay on Horner
+    few = md.rotate(view, dirs[:4], opts=leap.options(**mixed(direction_block=64, lane_directions=1)))
+    assert np.array_equal(few, horner[:4])
+
+
+def test_nufft_option_errors(leap, ctx):
+    s = leap.SynthMS(n_antennas=8, n_timesteps=2, n_channels=520, seed=214)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    dirs = dirs_near(ms["phase_centre"], 20, 9)
+    with pytest.raises(leap.LeapError, match="mixed-precision"):
+        md.rotate(view, dirs, opts=leap.options(strategy=1, channel_sum=NUFFT))
+    for bad in (dict(all_pols=1), dict(direct_cube=1), dict(fp32_blocks=1)):
+        with pytest.raises(leap.LeapError, match="packed pol 0"):
+            md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT, **bad)))
+    with pytest.raises(leap.LeapError, match="channel_sum"):
+        md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=3)))
+    with pytest.raises(leap.LeapError, match="1..512 channels"):
+        md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT)))
+    # 520 channels: the automatic choice stays on Horner
+    auto = md.rotate(view, dirs, opts=leap.options(**mixed()))
+    horner = md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=HORNER)))
+    assert np.array_equal(auto, horner)
