@@ -32,6 +32,9 @@ import leap_amd as la  # noqa: E402
 METRIC = "directions calibrated/s (256 st,128 t,384 ch) batched-mixed vs plain FP64; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 VALU_FP64_PEAK_TFLOPS = 78.6
+# NUFFT flops per row and direction besides the weights and taps: shift factor 7,
+# grid coordinate and offset 7, s^2 1, phasor 43, complex product 6, accumulation 2.
+NUFFT_DIR_FLOPS = 66
 
 
 def log(msg):
@@ -243,8 +246,36 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     value = steps * D / elapsed
     k_ms = float(np.mean(rot_ms))
     tile = ctx.last_direction_block()
-    packed = mode != "plain" and not args.fp32_blocks and not args.direct_cube and tile in (641, 1282, 2564)
-    if mode == "plain":
+    route_cs, route_dc = ctx.last_route() if mode != "plain" else (0, 0)
+    nufft = mode == "batched-mixed" and route_cs == la.LEAP_CHANNEL_SUM_NUFFT
+    packed = (mode != "plain" and not args.fp32_blocks and not route_dc and tile in (641, 1282, 2564))
+    if nufft:
+        # NUFFT channel sums: per row one N-point complex FFT (5 N log2 N flops,
+        # the conventional count) and per row and direction the weight
+        # polynomials (W^2 / 2 FMAs), the W-tap interpolation (2 W FMAs) and
+        # NUFFT_DIR_FLOPS of shift factor, grid coordinate, phasor and
+        # accumulation (FMA = 2 flops).
+        plan = la.nufft_plan(C)
+        n_grid, width = plan["N"], plan["W"]
+        per_row = 5 * n_grid * np.log2(n_grid)
+        per_dir = 2 * (width * width // 2 + 2 * width) + NUFFT_DIR_FLOPS
+        alg_flops = int(R * (per_row + D * per_dir))
+        achieved = alg_flops / (k_ms * 1e-3) / 1e12
+        src_bytes = R * C * P * elem if route_dc else R * C * 16
+        alg_bytes = src_bytes + R * 24 + D * nbl * P * 16
+        roofline = {"kernel": "rotate_average_nufft",
+                    "bound": "valu-fp64", "achieved": round(achieved, 2), "peak": VALU_FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / VALU_FP64_PEAK_TFLOPS, 4),
+                    "traffic": traffic_from_profile("rotate_average_nufft_c1_traffic.json"),
+                    "kernel_ms": round(k_ms, 4), "algorithmic_flops_per_launch": alg_flops,
+                    "nufft": {"grid": n_grid, "width": width, "fft_flops_per_row": int(per_row),
+                              "flops_per_row_direction": per_dir,
+                              "source": "complex64 cube" if route_dc else "packed pol-0 rows"},
+                    "hbm_gbs": round(alg_bytes / (k_ms * 1e-3) / 1e9, 1),
+                    "hbm_frac": round(alg_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    # the direct sum's 8 flops per (row, channel, direction) over the kernel time
+                    "direct_sum_equivalent_tflops": round(8 * R * C * D / (k_ms * 1e-3) / 1e12, 1)}
+    elif mode == "plain":
         alg_bytes = R * C * P * elem + R * 24 + D * nbl * P * 16 + C * 8
         achieved = alg_bytes / (k_ms * 1e-3) / 1e9
         roofline = {"kernel": "rotate_average_f64", "bound": "hbm", "achieved": round(achieved, 1),
@@ -270,14 +301,14 @@ def run_mode(args, mode, D, with_cpu, steps=None):
                     "traffic": traffic_from_profile(prof) if prof else None,
                     "kernel_ms": round(k_ms, 4), "algorithmic_flops_per_launch": alg_flops,
                     "hbm_gbs": round(alg_bytes / (k_ms * 1e-3) / 1e9, 1)}
-        if packed:
-            p_ms = float(np.mean(pack_ms))
-            pack_bytes = R * C * (P * elem + 16)  # read every pol once, write pol 0 as FP64 complex
-            roofline["pack"] = {"kernel": "pack_pol0_kernel", "bound": "hbm", "ms": round(p_ms, 4),
-                                "algorithmic_bytes": pack_bytes,
-                                "achieved": round(pack_bytes / (p_ms * 1e-3) / 1e9, 1) if p_ms > 0 else None,
-                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": round(pack_bytes / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if p_ms > 0 else None}
+    if packed and mode != "plain":
+        p_ms = float(np.mean(pack_ms))
+        pack_bytes = R * C * (P * elem + 16)  # read every pol once, write pol 0 as FP64 complex
+        roofline["pack"] = {"kernel": "pack_pol0_kernel", "bound": "hbm", "ms": round(p_ms, 4),
+                            "algorithmic_bytes": pack_bytes,
+                            "achieved": round(pack_bytes / (p_ms * 1e-3) / 1e9, 1) if p_ms > 0 else None,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(pack_bytes / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if p_ms > 0 else None}
 
     cpu = None
     if with_cpu:
@@ -314,7 +345,8 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if mode != "batched-mixed" else "c64 storage, f64 phase and Horner",
+        "dtype": "f64" if mode != "batched-mixed" else ("c64 storage, f64 phase and NUFFT" if nufft else
+                                                        "c64 storage, f64 phase and Horner"),
         "data": "synthetic (seeded point sources x per-antenna phase screens + noise, generated on the device)",
         "config": {
             "workload": workload + "; 256 stations / 32896 baselines (autos) / 128 t / 384 ch / 4 pol, "

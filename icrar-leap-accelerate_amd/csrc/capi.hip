@@ -81,8 +81,8 @@ leap_options resolve(const leap_options* o) {
     if (r.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
         require(r.precision == LEAP_PRECISION_MIXED, LEAP_ERR_INVALID_ARGUMENT,
                 "the NUFFT channel sums are a mixed-precision method");
-        require(!r.all_pols && !r.fp32_blocks && !r.direct_cube, LEAP_ERR_INVALID_ARGUMENT,
-                "the NUFFT channel sums run on packed pol 0 (no all_pols, fp32_blocks or direct_cube)");
+        require(!r.all_pols && !r.fp32_blocks, LEAP_ERR_INVALID_ARGUMENT,
+                "the NUFFT channel sums run on pol 0 (no all_pols or fp32_blocks)");
     }
     return r;
 }
@@ -238,6 +238,13 @@ leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, u
     if (r.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
         require(nufft_supported(ms.num_channels), LEAP_ERR_INVALID_ARGUMENT,
                 "the NUFFT channel sums support 1.." + std::to_string(kNufftMaxChannels) + " channels");
+        const bool cube_ok = nufft_cube_supported(ms.vis_dtype, ms.num_pols);
+        require(!r.direct_cube || cube_ok, LEAP_ERR_INVALID_ARGUMENT,
+                "the NUFFT kernel reads the cube directly only for complex64 with 1, 2 or 4 pols");
+        // A device-resident complex64 cube is read by the NUFFT kernel itself
+        // (no pack pass) unless pack chunks were asked for; host cubes are packed
+        // while they are staged (pol 0 only crosses the bus).
+        if (cube_ok && ms.vis_on_device && r.pack_timesteps == 0) r.direct_cube = 1;
         return r;
     }
     if (!use_packed(o) || o.direction_block != 0 || !ms.vis_on_device) return r;
@@ -269,7 +276,7 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
         return opts.direction_block * 10 + (fp32 ? 1 : opts.lane_directions);
     if (opts.direction_block != 0 && opts.lane_directions == 0) return opts.direction_block * 10 + 1;
     const bool packed = use_packed(opts);
-    const bool nufft = packed && opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
+    const bool nufft = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
     const TuneKey key{opts.precision * 4 + (opts.fp32_blocks ? 2 : 0) + (opts.lane_directions ? 1 : 0) +
                           16 * opts.lane_directions + (packed ? kTunePackedBit : 0) + (nufft ? kTuneNufftBit : 0),
                       ms.vis_dtype, opts.all_pols, D, T, md.channels, md.pols};
@@ -279,7 +286,7 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
     std::vector<int32_t> candidates;
     const int32_t all64[] = {321, 641, 642, 1281, 1282, 1284, 2561, 2562, 2564};
     const int32_t all32[] = {321, 641, 1281, 2561};
-    if (packed) {
+    if (packed || nufft) {
         // 64 lanes per row: 64, 128 or 256 directions per workgroup
         for (int32_t t : {641, 1282, 2564})
             if ((t == 641 || uint32_t(t / 10) <= 2 * D - 2) &&
@@ -302,14 +309,15 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
     LEAP_HIP(hipEventCreate(&b));
     for (int32_t tile : candidates) {
         const uint32_t db = uint32_t(tile / 10);
-        if (!packed && db > 32 && db >= 2 * D) continue;  // mostly idle lanes
+        if (!packed && !nufft && db > 32 && db >= 2 * D) continue;  // mostly idle lanes
         float best_of = 0.f;
         for (int rep = 0; rep < 2; ++rep) {  // the first repetition warms the code object
             LEAP_HIP(hipEventRecord(a, ctx->stream));
             if (nufft)
-                launch_rotate_nufft(ctx->stream, tile, md.nufft, ctx->d_packed.as<double>(), rows.uvw,
-                                    ctx->d_dd.as<double>(), D, T, md.nbl, probe_bl, md.channels, md.pols,
-                                    md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
+                launch_rotate_nufft(ctx->stream, tile, md.nufft, md.d_nufft.as<double>(),
+                                    packed ? ctx->d_packed.ptr : rows.vis, !packed, rows.uvw, ctx->d_dd.as<double>(),
+                                    D, T, md.nbl, probe_bl, md.channels, md.pols, md.freq_start_hz,
+                                    md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
             else if (packed)
                 launch_rotate_packed(ctx->stream, tile, ctx->d_packed.as<double>(), rows.uvw, ctx->d_dd.as<double>(),
                                      D, T, md.nbl, probe_bl, md.channels, md.pols, md.freq_start_hz, md.freq_inc_hz,
@@ -365,7 +373,7 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
     const size_t vis_row_bytes = vis_elem_bytes(ms) * md.channels;
     const bool host = rows.vis == nullptr;
     const bool nufft = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
-    const double* scale = nufft ? md.d_nufft_scale.as<double>() : nullptr;
+    const double* scale = nufft ? md.nufft_scale() : nullptr;
     double* buf[2] = {ctx->d_packed.as<double>(), ctx->d_packed.as<double>() + 2 * buf_elems};
     auto pack = [&](uint32_t k, hipStream_t st) {
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
@@ -385,9 +393,10 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
         record_rotate_begin(ctx);
         if (nufft)
-            launch_rotate_nufft(ctx->stream, tile, md.nufft, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
-                                ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols,
-                                md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
+            launch_rotate_nufft(ctx->stream, tile, md.nufft, md.d_nufft.as<double>(), buf[k % 2], false,
+                                rows.uvw + size_t(t0) * md.nbl * 3, ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl,
+                                md.channels, md.pols, md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(),
+                                k > 0 ? 1 : 0);
         else
             launch_rotate_packed(ctx->stream, tile, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
                                  ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols,
@@ -430,7 +439,7 @@ int32_t batched_tile(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view&
     if (!use_packed(opts)) return tuned_direction_block(ctx, md, ms, rows, T, D, opts);
     const uint32_t tc = std::min(T, pack_chunk(T, md.nbl, md.channels, opts));
     const size_t n = size_t(tc) * md.nbl * md.channels;
-    const double* scale = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT ? md.d_nufft_scale.as<double>() : nullptr;
+    const double* scale = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT ? md.nufft_scale() : nullptr;
     return tuned_direction_block(ctx, md, ms, rows, tc, D, opts, [&] {
         ctx->d_packed.reserve(n * 2 * sizeof(double));
         launch_pack_pol0(ctx->stream, rows.vis, ms.vis_dtype, n, md.pols, ctx->d_packed.as<double>(), scale,
@@ -448,6 +457,16 @@ void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_vie
         return;
     }
     require(rows.vis != nullptr, LEAP_ERR_INTERNAL, "visibilities were not staged");
+    if (opts.strategy == LEAP_STRATEGY_BATCHED && opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+        const int32_t db = tile ? tile : batched_tile(ctx, md, ms, rows, T, D, opts);
+        ctx->last_direction_block = db;
+        record_rotate_begin(ctx);
+        launch_rotate_nufft(ctx->stream, db, md.nufft, md.d_nufft.as<double>(), rows.vis, true, rows.uvw,
+                            ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
+                            md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
+        record_rotate_end(ctx);
+        return;
+    }
     if (opts.strategy == LEAP_STRATEGY_BATCHED) {
         const int32_t db = tile ? tile : batched_tile(ctx, md, ms, rows, T, D, opts);
         ctx->last_direction_block = db;
@@ -567,10 +586,13 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
     launch_assemble_ad(s, g1p, N, d_pairs1.as<int2>(), K1 - 1, md->A1.constraint, md->d_Ad1T.as<double>());
     md->d_AdT_planes.reserve(bf16_planes_bytes(uint32_t(N), K));
     launch_split_b(s, md->d_AdT.as<double>(), K, N, md->d_AdT_planes.ptr);
+    std::vector<double> nufft_tables;
     if (nufft_supported(md->channels)) {
         md->nufft = make_nufft_plan(md->channels);
-        md->d_nufft_scale.reserve(md->nufft.scale.size() * sizeof(double));
-        LEAP_HIP(hipMemcpyAsync(md->d_nufft_scale.ptr, md->nufft.scale.data(), md->nufft.scale.size() * sizeof(double),
+        nufft_tables.assign(&md->nufft.coef[0][0], &md->nufft.coef[0][0] + kNufftCoefDoubles);
+        nufft_tables.insert(nufft_tables.end(), md->nufft.scale.begin(), md->nufft.scale.end());
+        md->d_nufft.reserve(nufft_tables.size() * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(md->d_nufft.ptr, nufft_tables.data(), nufft_tables.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
     }
     LEAP_HIP(hipStreamSynchronize(s));
@@ -734,6 +756,8 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
     const uint32_t nbl = ms.num_baselines;
     const int32_t N = md.A.cols;
     const leap_options opts = routed(opts_in, ms, interval, D);
+    ctx->last_channel_sum = opts.channel_sum;
+    ctx->last_direct_cube = opts.direct_cube;
     if (opts.graph) {
         calibrate_loop_graph(ctx, md, ms, dirs, D, solutions, interval, opts, cb, user);
         return;
@@ -1237,6 +1261,8 @@ int leap_hip_rotate_visibilities(leap_ctx* ctx, const leap_metadata* md, const l
         require(n_rows % ms->num_baselines == 0, LEAP_ERR_RANGE,
                 "integration rows must be a whole number of timesteps");
         const leap_options o = routed(resolve(opts), *ms, n_rows / ms->num_baselines, D);
+        ctx->last_channel_sum = o.channel_sum;
+        ctx->last_direct_cube = o.direct_cube;
         DeviceScope scope(ctx->device);
         begin_timing(ctx);
         const DeviceRows rows = stage_rows(ctx, *ms, row_start, row_start + n_rows, !packed_route(o, 0));
@@ -1484,6 +1510,13 @@ double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
 }  // extern "C"
 
 extern "C" int leap_hip_last_direction_block(const leap_ctx* ctx) { return ctx ? ctx->last_direction_block : 0; }
+
+extern "C" int leap_hip_last_route(const leap_ctx* ctx, int32_t* channel_sum, int32_t* direct_cube) {
+    if (!ctx) return LEAP_ERR_INVALID_ARGUMENT;
+    if (channel_sum) *channel_sum = ctx->last_channel_sum;
+    if (direct_cube) *direct_cube = ctx->last_direct_cube;
+    return LEAP_OK;
+}
 extern "C" int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx) { return ctx ? ctx->last_workgroups_per_cu : 0; }
 extern "C" int leap_hip_last_graph(const leap_ctx* ctx) { return ctx ? ctx->last_graph : 0; }
 extern "C" uint64_t leap_hip_last_staged_bytes(const leap_ctx* ctx) { return ctx ? ctx->last_stage_bytes : 0; }

@@ -16,30 +16,43 @@
 //
 // Work decomposition: a workgroup is one wave.  It owns one baseline and a
 // block of 64 * NDL directions (NDL per lane) and walks the baseline's rows in
-// order, prefetching the next row's frequencies while it works on the current
+// order, prefetching the next row's channels while it works on the current
 // one.  The FFT is a Stockham transform whose butterflies run in registers
 // (radices 16, 16, 4 for N = 1024; 8, 8, 8 for 512; 4, 4, 4, 4 for 256), with
 // the exchanges between passes through the wave's LDS (rows of 16 elements
 // padded by one element against bank conflicts); the last pass writes the grid
 // in natural order plus W - 1 wrapped elements, so a direction's W grid points
-// are consecutive.  Wave-scope fences order the LDS exchanges; there are no
-// workgroup barriers and no atomics, and each direction's row sums are added
-// in row order, so results are reproducible.
+// are consecutive.  The transform's input is the channels in their own order
+// (channel c at index c; lane l owns channels l + 64 i, which are its register
+// slots i, the upper half being the zero padding); the frequency offset h of
+// the plan then multiplies output m by exp(-2 pi i h m / N), which for
+// h = 64 H depends on m mod (N / 64) = lane mod (N / 64) only and is folded into
+// the last pass's twiddles.  Wave-scope fences order the LDS exchanges; there
+// are no workgroup barriers and no atomics, and each direction's row sums are
+// added in row order, so results are reproducible.
 #include "batched_common.h"
+
+// No contraction anywhere in this file: every fused operation is one written
+// as fma(), so the kernel gives the same bits whether a row's values come
+// from the cube (precorrection multiplied here) or from packed rows
+// (multiplied by the pack kernel).
+#pragma clang fp contract(off)
 
 namespace leap {
 namespace {
 
 using namespace dev;
 using Z = C2<double>;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kWave = 64;
 
 struct NufftArgs {
-    double coef[kNufftMaxWidth / 2][kNufftMaxWidth];  // weight polynomial of taps j < W / 2, powers of s
-    double kappa_mid;                                 // (f0 + h df) / c_light
-    double dkappa;                                    // df / c_light
-    int32_t h;                                        // frequency offset, a multiple of 64
+    const double* coef;  // device: [W / 2][kNufftMaxWidth] weight polynomials of taps j < W / 2, powers of s
+    double kappa_mid;    // (f0 + h df) / c_light
+    double dkappa;       // df / c_light
+    int32_t h;           // frequency offset, a multiple of 64
 };
 
 __device__ inline Z zadd(Z a, Z b) { return {a.x + b.x, a.y + b.y}; }
@@ -59,20 +72,43 @@ __device__ inline void idft4(Z& a0, Z& a1, Z& a2, Z& a3) {
     a3 = zsub(t1, t3);
 }
 
-template <int R>
+// idft4 of (a0, a1, 0, 0).
+__device__ inline void idft4_half(Z& a0, Z& a1, Z& a2, Z& a3) {
+    const Z x0 = a0, t1 = zmuli(a1);
+    a0 = zadd(x0, a1);
+    a2 = zsub(x0, a1);
+    a3 = zsub(x0, t1);
+    a1 = zadd(x0, t1);
+}
+
+template <bool HALF>
+__device__ inline void idft4_any(Z& a0, Z& a1, Z& a2, Z& a3) {
+    if constexpr (HALF)
+        idft4_half(a0, a1, a2, a3);
+    else
+        idft4(a0, a1, a2, a3);
+}
+
+// HALF: the upper half of the inputs is zero (the first pass's padding).
+template <int R, bool HALF>
 __device__ inline void idft(Z* x);
 
 template <>
-__device__ inline void idft<4>(Z* x) {
+__device__ inline void idft<4, false>(Z* x) {
     idft4(x[0], x[1], x[2], x[3]);
 }
 
 template <>
-__device__ inline void idft<8>(Z* x) {
+__device__ inline void idft<4, true>(Z* x) {
+    idft4_half(x[0], x[1], x[2], x[3]);
+}
+
+template <bool HALF>
+__device__ inline void idft8(Z* x) {
     // even / odd halves, then X_k = E_k + w8^k O_k, X_{k+4} = E_k - w8^k O_k
     Z e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
-    idft4(e[0], e[1], e[2], e[3]);
-    idft4(o[0], o[1], o[2], o[3]);
+    idft4_any<HALF>(e[0], e[1], e[2], e[3]);
+    idft4_any<HALF>(o[0], o[1], o[2], o[3]);
     o[1] = cmul(o[1], Z{kCos4, kCos4});
     o[2] = zmuli(o[2]);
     o[3] = cmul(o[3], Z{-kCos4, kCos4});
@@ -84,7 +120,17 @@ __device__ inline void idft<8>(Z* x) {
 }
 
 template <>
-__device__ inline void idft<16>(Z* x) {
+__device__ inline void idft<8, false>(Z* x) {
+    idft8<false>(x);
+}
+
+template <>
+__device__ inline void idft<8, true>(Z* x) {
+    idft8<true>(x);
+}
+
+template <bool HALF>
+__device__ inline void idft16(Z* x) {
     // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1, twiddle w16^(n2 k1), DFT4 over n2
     Z y[4][4];
 #pragma unroll
@@ -93,7 +139,7 @@ __device__ inline void idft<16>(Z* x) {
         y[n2][1] = x[4 + n2];
         y[n2][2] = x[8 + n2];
         y[n2][3] = x[12 + n2];
-        idft4(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+        idft4_any<HALF>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
     }
     const Z w1{kCos8, kSin8}, w2{kCos4, kCos4}, w3{kSin8, kCos8}, w6{-kCos4, kCos4}, w9{-kCos8, -kSin8};
     y[1][1] = cmul(y[1][1], w1);
@@ -116,6 +162,16 @@ __device__ inline void idft<16>(Z* x) {
     }
 }
 
+template <>
+__device__ inline void idft<16, false>(Z* x) {
+    idft16<false>(x);
+}
+
+template <>
+__device__ inline void idft<16, true>(Z* x) {
+    idft16<true>(x);
+}
+
 // Orders the wave's LDS accesses across lanes (the workgroup is one wave).
 __device__ inline void wave_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -130,16 +186,26 @@ __device__ inline uint32_t padded(uint32_t e) { return e + (e >> 4); }
 // and span NS has B = N / (64 R) butterflies per lane: butterfly j = lane + 64 q
 // takes elements j + r N / R (slots q + B r), multiplies element r by
 // w^r, w = exp(2 pi i (j mod NS) / (NS R)), transforms, and its output r goes
-// to (j / NS) NS R + j mod NS + r NS.
-template <int N, int R, int NS>
-__device__ inline void pass_compute(Z* v, const Z* base) {
+// to (j / NS) NS R + j mod NS + r NS.  HALF: the upper half of the inputs is
+// zero (first pass).  PHI: every output of the lane is also multiplied by phi
+// (folded into the input twiddles: element r by phi w^r).
+template <int N, int R, int NS, bool HALF, bool PHI>
+__device__ inline void pass_compute(Z* v, const Z* base, Z phi) {
     constexpr int B = N / (kWave * R);
 #pragma unroll
     for (int q = 0; q < B; ++q) {
         Z x[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) x[r] = v[q + B * r];
-        if constexpr (NS > 1) {
+        if constexpr (PHI) {
+            Z t = phi;
+            x[0] = cmul(x[0], t);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                t = cmul(t, base[q]);
+                x[r] = cmul(x[r], t);
+            }
+        } else if constexpr (NS > 1) {
             Z p = base[q];
             x[1] = cmul(x[1], p);
 #pragma unroll
@@ -148,7 +214,7 @@ __device__ inline void pass_compute(Z* v, const Z* base) {
                 x[r] = cmul(x[r], p);
             }
         }
-        idft<R>(x);
+        idft<R, HALF>(x);
 #pragma unroll
         for (int r = 0; r < R; ++r) v[q + B * r] = x[r];
     }
@@ -202,6 +268,15 @@ __device__ inline void opaque(Z& z) {
 template <int N>
 struct Twiddles;
 
+// phi = exp(-2 pi i H (lane mod V) / V), V = N / 64: the frequency offset
+// h = 64 H applied to the lane's outputs (see the file comment).
+template <int N>
+__device__ inline Z offset_phase(uint32_t lane, int32_t H) {
+    constexpr int V = N / kWave;
+    const Phasor64 a = phasor64(-kTwoPi * double((uint32_t(H) * (lane % V)) % V) / double(V));
+    return {a.c, a.s};
+}
+
 template <>
 struct Twiddles<1024> {  // passes (16, 1), (16, 16), (4, 256)
     Z p2[1], p3[4];
@@ -254,107 +329,143 @@ struct Twiddles<256> {  // passes (4, 1), (4, 4), (4, 16), (4, 64)
     }
 };
 
-// g_m = sum_k f~_k exp(2 pi i k m / N) from the lane's slots (element lane + 64 m
-// = frequency index mod N) into lds[0, N + W - 1).
+// g_m = sum_k f~_k exp(2 pi i k m / N) into lds[0, N + W - 1), from the lane's
+// slots (slot i < N / 128: channel lane + 64 i; the rest zero) and the lane's
+// offset phase phi.
 template <int N, int W>
-__device__ inline void row_fft(Z* v, Z* lds, const Twiddles<N>& tw, uint32_t lane) {
+__device__ inline void row_fft(Z* v, Z* lds, const Twiddles<N>& tw, Z phi, uint32_t lane) {
     if constexpr (N == 1024) {
-        pass_compute<N, 16, 1>(v, nullptr);
+        pass_compute<N, 16, 1, true, false>(v, nullptr, phi);
         wave_fence();  // the previous row's interpolation has read the grid
         pass_store_padded<N, 16, 1>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 16, 16>(v, tw.p2);
+        pass_compute<N, 16, 16, false, false>(v, tw.p2, phi);
         wave_fence();
         pass_store_padded<N, 16, 16>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 256>(v, tw.p3);
+        pass_compute<N, 4, 256, false, true>(v, tw.p3, phi);
         wave_fence();
         store_grid<N, W>(lds, v, lane);
         wave_fence();
     } else if constexpr (N == 512) {
-        pass_compute<N, 8, 1>(v, nullptr);
+        pass_compute<N, 8, 1, true, false>(v, nullptr, phi);
         wave_fence();
         pass_store_padded<N, 8, 1>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 8, 8>(v, tw.p2);
+        pass_compute<N, 8, 8, false, false>(v, tw.p2, phi);
         wave_fence();
         pass_store_padded<N, 8, 8>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 8, 64>(v, tw.p3);
+        pass_compute<N, 8, 64, false, true>(v, tw.p3, phi);
         wave_fence();
         store_grid<N, W>(lds, v, lane);
         wave_fence();
     } else {
         static_assert(N == 256, "grid sizes 256, 512, 1024");
-        pass_compute<N, 4, 1>(v, nullptr);
+        pass_compute<N, 4, 1, true, false>(v, nullptr, phi);
         wave_fence();
         pass_store_padded<N, 4, 1>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 4>(v, tw.p2);
+        pass_compute<N, 4, 4, false, false>(v, tw.p2, phi);
         wave_fence();
         pass_store_padded<N, 4, 4>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 16>(v, tw.p3);
+        pass_compute<N, 4, 16, false, false>(v, tw.p3, phi);
         wave_fence();
         pass_store_padded<N, 4, 16>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 64>(v, tw.p4);
+        pass_compute<N, 4, 64, false, true>(v, tw.p4, phi);
         wave_fence();
         store_grid<N, W>(lds, v, lane);
         wave_fence();
     }
 }
 
-// Channels of one row owned by the lane: c = lane + 64 i, i < N / 128 (C <= N / 2),
-// zero beyond C.
-template <int N>
-__device__ inline void load_channels(Z* x, const LEAP_GLOBAL Z* __restrict__ wr, uint32_t lane, uint32_t C) {
+// Row sources.  The lane owns channels c = lane + 64 i, i < N / 128 (C <= N / 2);
+// load() fetches them for one row (channels past C read channel C - 1 and are
+// zeroed by freq()), freq() returns the precorrected pol-0 values f~.
+//   P == 0: packed rows of FP64 complex, already precorrected (pack kernel).
+//   P >= 1: the complex64 cube [row][channel][pol]: a channel whose 2P floats
+//           are not all finite is zero (CpuLeapCalibrator.cc:201-214), pol 0 is
+//           widened and multiplied by the channel's precorrection factor.
+template <int N, int P>
+struct Source {
+    static constexpr int CPL = N / (2 * kWave);
+    float raw[CPL][2 * P];
+    double sc[CPL];    // precorrection, 0 past C
+    uint32_t off[CPL];  // float offset of the channel (clamped) in a row
+    __device__ void init(const LEAP_GLOBAL double* scale, uint32_t lane, uint32_t C) {
 #pragma unroll
-    for (int i = 0; i < N / (2 * kWave); ++i) {
-        const uint32_t c = lane + kWave * i;
-        Z z{0.0, 0.0};
-        if (c < C) {
-            z.x = wr[c].x;
-            z.y = wr[c].y;
+        for (int i = 0; i < CPL; ++i) {
+            const uint32_t c = lane + kWave * i;
+            sc[i] = c < C ? scale[c] : 0.0;
+            off[i] = min(c, C - 1) * (2 * P);
         }
-        x[i] = z;
     }
-}
-
-// Slot m holds grid index n = lane + 64 m, frequency k = n (n < N / 2) or
-// n - N, channel c = k + h = lane + 64 i with i = m + H or m + H - N / 64
-// (h = 64 H): the lane's own channels, placed by a compile-time rotation.
-template <int N, int H>
-__device__ inline void place_channels_h(Z* v, const Z* x) {
-    constexpr int V = N / kWave;
+    __device__ void load(const void* base, size_t row, uint32_t, uint32_t C) {
+        const LEAP_GLOBAL float* r = (const LEAP_GLOBAL float*)base + row * C * (2 * P);
 #pragma unroll
-    for (int m = 0; m < V; ++m) {
-        const int i = (m < V / 2) ? m + H : m + H - V;
-        v[m] = (i >= 0 && i < V / 2) ? x[i] : Z{0.0, 0.0};
+        for (int i = 0; i < CPL; ++i) {
+            const LEAP_GLOBAL float* q = r + off[i];
+            if constexpr (P == 4 || P == 2) {
+#pragma unroll
+                for (int h = 0; h < P / 2; ++h) {
+                    const f32x4 f = ((const LEAP_GLOBAL f32x4*)q)[h];
+                    raw[i][4 * h] = f.x;
+                    raw[i][4 * h + 1] = f.y;
+                    raw[i][4 * h + 2] = f.z;
+                    raw[i][4 * h + 3] = f.w;
+                }
+            } else {
+                const f32x2 f = *(const LEAP_GLOBAL f32x2*)q;
+                raw[i][0] = f.x;
+                raw[i][1] = f.y;
+            }
+        }
     }
-}
+    __device__ void freq(Z* x, uint32_t, uint32_t) const {
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) {
+            bool finite = true;
+#pragma unroll
+            for (int k = 0; k < 2 * P; ++k) finite = finite && __builtin_isfinite(raw[i][k]);
+            const float re = finite ? raw[i][0] : 0.0f, im = finite ? raw[i][1] : 0.0f;
+            x[i] = Z{double(re) * sc[i], double(im) * sc[i]};
+        }
+    }
+};
 
 template <int N>
-__device__ inline void place_channels(Z* v, const Z* x, int32_t H) {
-    // h <= N / 4, so H <= N / 256
-    switch (H) {
-        case 0: place_channels_h<N, 0>(v, x); break;
-        case 1: place_channels_h<N, 1>(v, x); break;
-        case 2: if constexpr (N >= 512) place_channels_h<N, 2>(v, x); break;
-        case 3: if constexpr (N >= 1024) place_channels_h<N, 3>(v, x); break;
-        default: if constexpr (N >= 1024) place_channels_h<N, 4>(v, x); break;
+struct Source<N, 0> {
+    static constexpr int CPL = N / (2 * kWave);
+    Z raw[CPL];
+    __device__ void init(const LEAP_GLOBAL double*, uint32_t, uint32_t) {}
+    __device__ void load(const void* base, size_t row, uint32_t lane, uint32_t C) {
+        const LEAP_GLOBAL Z* r = (const LEAP_GLOBAL Z*)base + row * C;
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) {
+            const uint32_t c = min(lane + kWave * i, C - 1);
+            raw[i].x = r[c].x;
+            raw[i].y = r[c].y;
+        }
     }
-}
+    __device__ void freq(Z* x, uint32_t lane, uint32_t C) const {
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) x[i] = lane + kWave * i < C ? raw[i] : Z{0.0, 0.0};
+    }
+};
 
-template <int N, int W, int NDL>
-__global__ __launch_bounds__(kWave) void rotate_average_nufft(const LEAP_GLOBAL Z* __restrict__ w,
+// SP: the row source (0 packed rows, 1 / 2 / 4 a complex64 cube with that many pols).
+template <int N, int W, int NDL, int SP>
+__global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __restrict__ w,
+                                                              const LEAP_GLOBAL double* __restrict__ scale,
                                                               const LEAP_GLOBAL double* __restrict__ uvw,
                                                               const LEAP_GLOBAL double* __restrict__ dd, uint32_t D,
                                                               uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
@@ -381,53 +492,78 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const LEAP_GLOBAL 
     Z sum[NDL];
 #pragma unroll
     for (int j = 0; j < NDL; ++j) sum[j] = {0.0, 0.0};
-    const int32_t H = args.h / kWave;
-    Z next[V / 2];
-    if (T > 0) load_channels<N>(next, w + size_t(bl) * C, lane, C);
+    const Z phi = offset_phase<N>(lane, args.h / kWave);
+    // The next row's loads are issued right after the current row is converted
+    // and stay in flight while it is transformed and interpolated.  The load is
+    // unconditional (the last row reloads itself), so the row buffer keeps its
+    // registers from one iteration to the next.
+    Source<N, SP> src;
+    src.init(scale, lane, C);
+    if (T > 0) src.load(w, bl, lane, C);
     for (uint32_t t = 0; t < T; ++t) {
         Z v[V];
-        place_channels<N>(v, next, H);
+        src.freq(v, lane, C);
+#pragma unroll
+        for (int m = V / 2; m < V; ++m) v[m] = Z{0.0, 0.0};  // padding (not read by the first pass)
+        __builtin_amdgcn_sched_barrier(0);
         const size_t row = size_t(t) * nbl + bl;
-        if (t + 1 < T) load_channels<N>(next, w + (row + nbl) * C, lane, C);
+        src.load(w, size_t(t + 1 < T ? t + 1 : t) * nbl + bl, lane, C);
         const double u = uvw[3 * row], vv = uvw[3 * row + 1], ww = uvw[3 * row + 2];
         tw.hide();
-        row_fft<N, W>(v, s_g, tw, lane);
+        row_fft<N, W>(v, s_g, tw, phi, lane);
+        // Interpolation.  Per direction: b = sf dkappa, grid coordinate
+        // ug = N frac(b / 2 pi), first tap m0 = ceil(ug - W/2), offset
+        // s = ug - m0 - (W - 1)/2 in (-1/2, 1/2]; tap pairs (jp, W - 1 - jp) take
+        // weights P_jp(s), P_jp(-s), P_jp(s) = E(s^2) + s O(s^2).  The pairs
+        // are the outer loop so that one pair's coefficients are live at a time.
+        double sfv[NDL], sv[NDL], s2[NDL];
+        const Z* gp[NDL];
 #pragma unroll
         for (int j = 0; j < NDL; ++j) {
-            const double sf = shift_factor(dz[j], u, vv, ww);
-            double x = sf * args.dkappa * kInvTwoPi;  // turns of b
-            x -= floor(x);                           // [0, 1]
-            const double ug = x * double(N);          // grid coordinate in [0, N]
+            sfv[j] = shift_factor(dz[j], u, vv, ww);
+            double x = sfv[j] * args.dkappa * kInvTwoPi;  // turns of b
+            x -= floor(x);                                // [0, 1]
+            const double ug = x * double(N);               // [0, N]
             const double m0f = ceil(ug - 0.5 * W);
-            const double s = (ug - m0f) - 0.5 * (W - 1);  // (-1/2, 1/2]
+            sv[j] = (ug - m0f) - 0.5 * (W - 1);
+            s2[j] = sv[j] * sv[j];
             int32_t m0 = int32_t(m0f);
             if (m0 < 0) m0 += N;  // taps m0 .. m0 + W - 1 <= N + W - 2 (wrapped copy)
             m0 = min(max(m0, 0), N - 1);  // a non-finite uvw stays inside the grid
-            const Z* gp = s_g + m0;
-            const double s2 = s * s;
-            Z acc{0.0, 0.0};
+            gp[j] = s_g + m0;
+        }
+        Z acc[NDL];
 #pragma unroll
-            for (int jp = 0; jp < W / 2; ++jp) {
-                // P_jp(s) = E(s^2) + s O(s^2); tap W - 1 - jp has P_jp(-s)
-                double e = args.coef[jp][W - 2];
-                double o = args.coef[jp][W - 1];
+        for (int j = 0; j < NDL; ++j) acc[j] = {0.0, 0.0};
+#pragma unroll
+        for (int jp = 0; jp < W / 2; ++jp) {
+            // The pair's coefficients are read here (scalar loads), not kept in
+            // SGPRs across the loop: all W/2 pairs do not fit.
+            const double* cp = args.coef + jp * kNufftMaxWidth;
+            asm volatile("" : "+s"(cp));
+            const __attribute__((address_space(4))) double* cf = (const __attribute__((address_space(4))) double*)cp;
+#pragma unroll
+            for (int j = 0; j < NDL; ++j) {
+                double e = cf[W - 2];
+                double o = cf[W - 1];
 #pragma unroll
                 for (int k = W - 4; k >= 0; k -= 2) {
-                    e = fma(e, s2, args.coef[jp][k]);
-                    o = fma(o, s2, args.coef[jp][k + 1]);
+                    e = fma(e, s2[j], cf[k]);
+                    o = fma(o, s2[j], cf[k + 1]);
                 }
-                const double wl = fma(s, o, e), wh = fma(-s, o, e);
-                const Z g0 = gp[jp], g1 = gp[W - 1 - jp];
-                acc.x = fma(g0.x, wl, acc.x);
-                acc.y = fma(g0.y, wl, acc.y);
-                acc.x = fma(g1.x, wh, acc.x);
-                acc.y = fma(g1.y, wh, acc.y);
+                const double wl = fma(sv[j], o, e), wh = fma(-sv[j], o, e);
+                const Z g0 = gp[j][jp], g1 = gp[j][W - 1 - jp];
+                acc[j].x = fma(g0.x, wl, acc[j].x);
+                acc[j].y = fma(g0.y, wl, acc[j].y);
+                acc[j].x = fma(g1.x, wh, acc[j].x);
+                acc[j].y = fma(g1.y, wh, acc[j].y);
             }
-            const Z r = cmul(phasor_f64(sf * args.kappa_mid), acc);
+        }
+#pragma unroll
+        for (int j = 0; j < NDL; ++j) {
+            const Z r = cmul(phasor_f64(sfv[j] * args.kappa_mid), acc[j]);
             sum[j].x += r.x;
             sum[j].y += r.y;
-            // one direction's grid reads in flight at a time (registers)
-            __builtin_amdgcn_sched_barrier(0);
         }
     }
 #pragma unroll
@@ -446,51 +582,69 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const LEAP_GLOBAL 
     }
 }
 
-template <int N, int W>
-void launch_nw(hipStream_t s, int32_t tile, const dim3& grid_x, const Z* w, const double* uvw, const double* dd,
-               uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, const NufftArgs& a, double* avg,
-               int32_t accumulate) {
-    const auto* gw = (const LEAP_GLOBAL Z*)w;
+template <int N, int W, int SP>
+void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
+                const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, const NufftArgs& a,
+                double* avg, int32_t accumulate) {
+    const auto* gs = (const LEAP_GLOBAL double*)scale;
     const auto* gu = (const LEAP_GLOBAL double*)uvw;
     const auto* gd = (const LEAP_GLOBAL double*)dd;
     auto* ga = (LEAP_GLOBAL double*)avg;
     switch (tile) {
         case 641:
-            rotate_average_nufft<N, W, 1><<<dim3(grid_x.x, (D + 63) / 64), kWave, 0, s>>>(gw, gu, gd, D, T, nbl, C, P,
-                                                                                        a, ga, accumulate);
+            rotate_average_nufft<N, W, 1, SP><<<dim3(nbl_grid, (D + 63) / 64), kWave, 0, s>>>(
+                w, gs, gu, gd, D, T, nbl, C, P, a, ga, accumulate);
             break;
         case 1282:
-            rotate_average_nufft<N, W, 2><<<dim3(grid_x.x, (D + 127) / 128), kWave, 0, s>>>(gw, gu, gd, D, T, nbl, C,
-                                                                                          P, a, ga, accumulate);
+            rotate_average_nufft<N, W, 2, SP><<<dim3(nbl_grid, (D + 127) / 128), kWave, 0, s>>>(
+                w, gs, gu, gd, D, T, nbl, C, P, a, ga, accumulate);
             break;
         case 2564:
-            rotate_average_nufft<N, W, 4><<<dim3(grid_x.x, (D + 255) / 256), kWave, 0, s>>>(gw, gu, gd, D, T, nbl, C,
-                                                                                          P, a, ga, accumulate);
+            rotate_average_nufft<N, W, 4, SP><<<dim3(nbl_grid, (D + 255) / 256), kWave, 0, s>>>(
+                w, gs, gu, gd, D, T, nbl, C, P, a, ga, accumulate);
             break;
         default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported NUFFT tile " + std::to_string(tile));
     }
 }
 
+template <int N, int W>
+void launch_nw(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, int32_t src_pols, const double* scale,
+               const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
+               const NufftArgs& a, double* avg, int32_t accumulate) {
+    switch (src_pols) {
+        case 0: launch_nws<N, W, 0>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
+        case 1: launch_nws<N, W, 1>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
+        case 2: launch_nws<N, W, 2>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
+        case 4: launch_nws<N, W, 4>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
+        default: throw Error(LEAP_ERR_INTERNAL, "unsupported NUFFT row source");
+    }
+}
+
 }  // namespace
 
-void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* w, const double* uvw,
-                         const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C,
-                         uint32_t P, double f0, double df, double* avg, int32_t accumulate) {
+bool nufft_cube_supported(int32_t vis_dtype, uint32_t P) {
+    return vis_dtype == LEAP_VIS_C64 && (P == 1 || P == 2 || P == 4);
+}
+
+void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
+                         bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
+                         uint32_t nbl_grid, uint32_t C, uint32_t P, double f0, double df, double* avg,
+                         int32_t accumulate) {
     if (nbl_grid == 0 || D == 0) return;
     require(plan.N != 0 && plan.scale.size() == C, LEAP_ERR_INTERNAL, "NUFFT plan does not match the channel count");
+    require(!cube || (P == 1 || P == 2 || P == 4), LEAP_ERR_INTERNAL, "NUFFT cube source needs 1, 2 or 4 pols");
     NufftArgs a{};
-    for (int j = 0; j < kNufftMaxWidth / 2; ++j)
-        for (int m = 0; m < kNufftMaxWidth; ++m) a.coef[j][m] = plan.coef[j][m];
+    a.coef = tables;
     a.kappa_mid = (f0 + double(plan.h) * df) / kSpeedOfLight;
     a.dkappa = df / kSpeedOfLight;
     a.h = int32_t(plan.h);
-    const Z* wp = reinterpret_cast<const Z*>(w);
-    const dim3 gx(nbl_grid);
-#define LEAP_NUFFT(N_, W_)                                                                          \
-    if (plan.N == N_ && plan.W == W_) {                                                             \
-        launch_nw<N_, W_>(s, tile, gx, wp, uvw, dd, D, T, nbl, C, P, a, avg, accumulate);            \
-        LEAP_HIP(hipGetLastError());                                                                \
-        return;                                                                                     \
+    const double* scale = tables + kNufftCoefDoubles;
+    const int32_t sp = cube ? int32_t(P) : 0;
+#define LEAP_NUFFT(N_, W_)                                                                                   \
+    if (plan.N == N_ && plan.W == W_) {                                                                      \
+        launch_nw<N_, W_>(s, tile, nbl_grid, src, sp, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate);  \
+        LEAP_HIP(hipGetLastError());                                                                         \
+        return;                                                                                              \
     }
     LEAP_NUFFT(256, 10)
     LEAP_NUFFT(256, 12)

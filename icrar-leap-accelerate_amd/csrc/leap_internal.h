@@ -95,6 +95,7 @@ MultiPlan plan_multi(const std::vector<MultiShape>& sets, int32_t tile, uint32_t
 constexpr uint32_t kNufftMaxChannels = 512;  // grid N <= 1024
 constexpr int kNufftMaxWidth = 12;
 constexpr int kNufftQuadratureNodes = 100;
+constexpr int kNufftCoefDoubles = (kNufftMaxWidth / 2) * kNufftMaxWidth;
 constexpr uint32_t kNufftMinDirections = 16;  // automatic choice: fewer directions stay on Horner
 struct NufftPlan {
     uint32_t N = 0;  // grid size (power of two, >= 2C, >= 256)
@@ -127,8 +128,9 @@ struct leap_metadata {
     leap::DeviceBuffer d_lambda;   // channels doubles: wavelength of each channel
     leap::DeviceBuffer d_Gp;       // scratch for Ad assembly
     leap::DeviceBuffer d_AdT_planes;  // split-3 x BF16 planes of AdT in MFMA operand layout
-    leap::NufftPlan nufft;             // N == 0 when the channel count is not supported
-    leap::DeviceBuffer d_nufft_scale;  // nufft.scale on the device
+    leap::NufftPlan nufft;       // N == 0 when the channel count is not supported
+    leap::DeviceBuffer d_nufft;  // nufft.coef (kNufftCoefDoubles) then nufft.scale (channels) on the device
+    const double* nufft_scale() const { return d_nufft.as<double>() + leap::kNufftCoefDoubles; }
 };
 
 // TuneKey::precision holds the batched tuner's mode code; these bits mark the
@@ -204,6 +206,7 @@ struct leap_ctx {
     // autotuned direction blocks of the batched kernel
     std::map<TuneKey, int32_t> tuned;
     int32_t last_direction_block = 0;
+    int32_t last_channel_sum = 0, last_direct_cube = 0;  // route of the last batched call
     // multi-set launches
     std::vector<std::unique_ptr<leap::SetBuffers>> batch_sets;
     leap::DeviceBuffer d_tables;     // descriptor tables of the last multi-set launch
@@ -250,11 +253,17 @@ void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t 
                       const double* scale = nullptr, uint32_t C = 0);
 // x[i] *= scale[i % C] for n packed FP64 complex values (whole rows).
 void launch_scale_channels(hipStream_t s, double* x, size_t n, const double* scale, uint32_t C);
-// NUFFT channel sums over precorrected packed pol-0 rows (kernels_nufft.hip);
-// tiles 641 / 1282 / 2564 as for launch_rotate_packed.
-void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* w, const double* uvw,
-                         const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C,
-                         uint32_t P, double f0, double df, double* avg, int32_t accumulate);
+// NUFFT channel sums (kernels_nufft.hip) over precorrected packed pol-0 rows
+// (cube false) or straight from a complex64 cube with 1, 2 or 4 pols (cube
+// true; scale = the plan's precorrection factors on the device); tiles
+// 641 / 1282 / 2564 as for launch_rotate_packed.
+// tables: the plan on the device, kNufftCoefDoubles coefficients (coef[j][m])
+// then the C precorrection factors.
+bool nufft_cube_supported(int32_t vis_dtype, uint32_t P);
+void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
+                         bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
+                         uint32_t nbl_grid, uint32_t C, uint32_t P, double f0, double df, double* avg,
+                         int32_t accumulate);
 bool packed_tile_supported(int32_t tile);
 void launch_rotate_packed(hipStream_t s, int32_t tile, const double* w, const double* uvw, const double* dd,
                           uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,

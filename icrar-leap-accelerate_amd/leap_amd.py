@@ -171,7 +171,8 @@ EXPORTS = [
     "leap_synth_create", "leap_synth_destroy", "leap_synth_view", "leap_synth_sky", "leap_synth_fill_vis_host",
     "leap_hip_synth_fill_vis_device",
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
-    "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_workgroups_per_cu",
+    "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_route",
+    "leap_hip_last_workgroups_per_cu",
     "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes",
     "leap_hip_tuning_save", "leap_hip_tuning_load", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
@@ -247,6 +248,7 @@ def lib():
     L.leap_hip_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.leap_hip_last_kernel_times.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp]
     L.leap_hip_last_direction_block.argtypes = [ctypes.c_void_p]
+    L.leap_hip_last_route.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.leap_hip_last_workgroups_per_cu.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_graph.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_pack_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int32)]
@@ -586,6 +588,12 @@ class Context:
 
     def last_direction_block(self):
         return int(lib().leap_hip_last_direction_block(self.h))
+
+    def last_route(self):
+        """(channel_sum, direct_cube) of the last batched call: 1 Horner / 2 NUFFT, 1 = read the cube directly."""
+        cs, dc = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(lib().leap_hip_last_route(self.h, ctypes.byref(cs), ctypes.byref(dc)))
+        return cs.value, dc.value
 
     def last_workgroups_per_cu(self):
         return int(lib().leap_hip_last_workgroups_per_cu(self.h))

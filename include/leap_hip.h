@@ -373,6 +373,10 @@ int leap_hip_last_kernel_times(const leap_ctx* ctx, double* rotate_ms_avg, int32
 /* Tile the batched kernel used in the last call: directions per workgroup * 10
  * + directions per lane (0 if none). */
 int leap_hip_last_direction_block(const leap_ctx* ctx);
+/* Route of the last batched calibrate / rotate call: the channel sums used
+ * (LEAP_CHANNEL_SUM_HORNER or _NUFFT) and whether the kernel read the cube
+ * directly (1) or packed pol-0 rows (0).  Any output may be null. */
+int leap_hip_last_route(const leap_ctx* ctx, int32_t* channel_sum, int32_t* direct_cube);
 /* Workgroups per CU of the last multi-set launch (-1: one work item per
  * workgroup; 0 if none). */
 int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx);

@@ -8,8 +8,9 @@ Bars:
   direction is a fraction of sum_c |v_c| (1e-11 .. 1e-10 for the plans used,
   tests/test_nufft_plan.py), so the bound is independent of size and data.
 - calibrations: within 1e-5 rad wrapped of the FP64 oracle (the mixed budget).
-- host-resident and device-resident cubes give identical results; chunked packs
-  agree with one chunk to summation order.
+- host-resident and device-resident cubes give identical results, whether the
+  NUFFT kernel reads the complex64 cube itself or packed pol-0 rows; chunked
+  packs agree with one chunk to summation order.
 """
 import numpy as np
 import pytest
@@ -90,6 +91,39 @@ def test_nufft_host_and_device_cubes_agree(leap, ctx, dtype):
     assert np.array_equal(host, dev)
 
 
+@pytest.mark.parametrize("n_pols", [1, 2, 3, 4])
+def test_nufft_cube_kernel_equals_packed(leap, ctx, oracle, n_pols):
+    """A device-resident complex64 cube with 1, 2 or 4 pols is read by the NUFFT
+    kernel itself (mask and precorrection in registers); packing first (asked
+    for through pack_timesteps, and always for 3 pols) gives the same bits."""
+    s = leap.SynthMS(n_antennas=16, n_timesteps=6, n_channels=150, n_pols=n_pols, nan_fraction=0.01, seed=215)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    dirs = dirs_near(ms["phase_centre"], 50, 10, spread=0.1)
+    vis_p = ctx.malloc(ms["vis"].nbytes)
+    try:
+        ctx.h2d(vis_p, np.ascontiguousarray(ms["vis"]))
+        dview = leap.view_from_arrays(ms, vis_device=vis_p, vis_dtype=leap.LEAP_VIS_C64)
+        md = ctx.metadata(dview)
+        direct = md.rotate(dview, dirs, opts=leap.options(**mixed(channel_sum=NUFFT)))
+        assert ctx.last_pack_times()[1] == (1 if n_pols == 3 else 0)
+        packed = md.rotate(dview, dirs, opts=leap.options(**mixed(channel_sum=NUFFT, pack_timesteps=6)))
+        assert ctx.last_pack_times()[1] == 1
+        assert np.array_equal(direct, packed)
+        if n_pols != 3:
+            forced = md.rotate(dview, dirs, opts=leap.options(**mixed(channel_sum=NUFFT, direct_cube=1)))
+            assert np.array_equal(forced, direct)
+        else:
+            with pytest.raises(leap.LeapError, match="reads the cube directly"):
+                md.rotate(dview, dirs, opts=leap.options(**mixed(channel_sum=NUFFT, direct_cube=1)))
+    finally:
+        ctx.free(vis_p)
+    omd = oracle.metadata(ms)
+    bound = 5e-10 * abs_mass(ms)
+    for d in (0, 25, 49):
+        assert np.all(np.abs(direct[d, :, 0] - omd.rotate(dirs[d])[:, 0]) <= bound)
+
+
 @pytest.mark.parametrize("chunk", [3, 7])
 def test_nufft_chunks_agree_with_one_chunk(leap, ctx, chunk):
     s = leap.SynthMS(n_antennas=16, n_timesteps=20, n_channels=96, nan_fraction=0.002, seed=212)
@@ -147,8 +181,8 @@ def test_nufft_option_errors(leap, ctx):
     dirs = dirs_near(ms["phase_centre"], 20, 9)
     with pytest.raises(leap.LeapError, match="mixed-precision"):
         md.rotate(view, dirs, opts=leap.options(strategy=1, channel_sum=NUFFT))
-    for bad in (dict(all_pols=1), dict(direct_cube=1), dict(fp32_blocks=1)):
-        with pytest.raises(leap.LeapError, match="packed pol 0"):
+    for bad in (dict(all_pols=1), dict(fp32_blocks=1)):
+        with pytest.raises(leap.LeapError, match="run on pol 0"):
             md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT, **bad)))
     with pytest.raises(leap.LeapError, match="channel_sum"):
         md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=3)))
