@@ -232,7 +232,7 @@ leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, u
     leap_options r = o;
     if (r.channel_sum == LEAP_CHANNEL_SUM_AUTO)
         r.channel_sum = use_packed(r) && r.precision == LEAP_PRECISION_MIXED && nufft_supported(ms.num_channels) &&
-                                D >= kNufftMinDirections
+                                ms.num_channels >= kNufftMinChannels && D >= kNufftMinDirections
                             ? LEAP_CHANNEL_SUM_NUFFT
                             : LEAP_CHANNEL_SUM_HORNER;
     if (r.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {

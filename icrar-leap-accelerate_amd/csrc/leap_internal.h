@@ -96,7 +96,10 @@ constexpr uint32_t kNufftMaxChannels = 512;  // grid N <= 1024
 constexpr int kNufftMaxWidth = 12;
 constexpr int kNufftQuadratureNodes = 100;
 constexpr int kNufftCoefDoubles = (kNufftMaxWidth / 2) * kNufftMaxWidth;
-constexpr uint32_t kNufftMinDirections = 16;  // automatic choice: fewer directions stay on Horner
+// Automatic choice: fewer directions or channels stay on Horner (per row the FFT
+// costs about what 4 C FMAs per lane cost at C = 64..96).
+constexpr uint32_t kNufftMinDirections = 16;
+constexpr uint32_t kNufftMinChannels = 96;
 struct NufftPlan {
     uint32_t N = 0;  // grid size (power of two, >= 2C, >= 256)
     uint32_t W = 0;  // kernel width in grid points (10 or 12)

@@ -111,7 +111,8 @@ typedef enum leap_strategy {
 } leap_strategy;
 
 typedef enum leap_channel_sum {
-    /* Mixed precision with a supported channel count (1..512): the NUFFT; otherwise Horner. */
+    /* Mixed precision, pol 0, 96..512 channels and at least 16 directions: the NUFFT;
+     * otherwise Horner. */
     LEAP_CHANNEL_SUM_AUTO = 0,
     /* FP64 Horner: four FMAs per channel and direction, exact to FP64 rounding. */
     LEAP_CHANNEL_SUM_HORNER = 1,

@@ -91,9 +91,13 @@ def test_batched_fp64_calibrate_matches_oracle(leap, ctx, oracle, small, slice_)
     assert wrapped(got, want).max() < FP64_TOL
 
 
-@pytest.mark.parametrize("fp32_blocks", [0, 1], ids=["fp64-horner", "fp32-blocks"])
+ROUTES = {"nufft": dict(channel_sum=2), "horner": dict(channel_sum=1), "fp32-blocks": dict(fp32_blocks=1)}
+
+
+@pytest.mark.parametrize("route", list(ROUTES))
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
-def test_mixed_precision_calibrate_error_budget(leap, ctx, oracle, dtype, fp32_blocks):
+def test_mixed_precision_calibrate_error_budget(leap, ctx, oracle, dtype, route):
+    fp32_blocks = route == "fp32-blocks"
     s = leap.SynthMS(n_antennas=48, n_timesteps=8, n_channels=384, freq_start_hz=50e6, freq_inc_hz=781.25e3,
                      layout=1, array_radius_m=20000.0, flag_fraction=0.02, seed=33)
     s.fill_host(np.complex128)
@@ -103,14 +107,13 @@ def test_mixed_precision_calibrate_error_budget(leap, ctx, oracle, dtype, fp32_b
     view = leap.view_from_arrays(ms)
     dirs = dirs_near(ms["phase_centre"], 24, 6)
     got, _ = ctx.calibrate(view, dirs, opts=leap.options(precision=leap.LEAP_PRECISION_MIXED,
-                                                         strategy=leap.LEAP_STRATEGY_BATCHED,
-                                                         fp32_blocks=fp32_blocks))
+                                                         strategy=leap.LEAP_STRATEGY_BATCHED, **ROUTES[route]))
     # against the FP64 oracle on the same stored visibilities
     want, _ = oracle.calibrate(ms, dirs)
     err = wrapped(got, want)
     per_dir_max = err.max(axis=(0, 2))
     per_dir_rms = np.sqrt((err ** 2).mean(axis=(0, 2)))
-    print(f"mixed {np.dtype(dtype).name} fp32_blocks={fp32_blocks}: wrapped error max {per_dir_max.max():.2e} rad, "
+    print(f"mixed {np.dtype(dtype).name} {route}: wrapped error max {per_dir_max.max():.2e} rad, "
           f"rms {per_dir_rms.max():.2e} rad (worst direction)")
     # The FP32 blocked variant is kept for comparison; it does not meet the 1e-5 rad budget.
     assert per_dir_max.max() < (MIXED_TOL if not fp32_blocks else 1e-2)

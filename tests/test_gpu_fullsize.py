@@ -6,8 +6,8 @@ baselines, 128 timesteps, 384 channels, 4 pols (the complex128 cube is
   and packed paths, against the oracle on the same rows (<= 1e-9 rad wrapped).
 - The whole interval: the batched FP64 paths against the plain per-direction
   path (two independent kernels; <= 1e-9 rad wrapped).
-- Mixed precision on the complex64 cube against FP64 on the same values
-  (<= 1e-5 rad wrapped; max and RMS printed).
+- Mixed precision on the complex64 cube against FP64 on the same values, on the
+  NUFFT and both FP64 Horner routes (<= 1e-5 rad wrapped; max and RMS printed).
 """
 import os
 import sys
@@ -78,10 +78,14 @@ def test_full_size_mixed(leap, ctx):
     try:
         md = ctx.metadata(view)
         c = (view.phase_centre_ra_rad, view.phase_centre_dec_rad)
-        dirs = bench.directions(c, 8, seed=12)
-        ref, _ = ctx.calibrate(view, dirs, opts=leap.options(), md=md)  # FP64 on the same complex64 values
-        for name, kw in (("packed", dict(direction_block=256, lane_directions=4)), ("direct", dict(direct_cube=1))):
+        dirs = bench.directions(c, 32, seed=12)
+        # FP64 (batched, one pass) on the same complex64 values
+        ref, _ = ctx.calibrate(view, dirs, opts=leap.options(strategy=1), md=md)
+        for name, kw in (("nufft", dict()),
+                         ("horner packed", dict(channel_sum=1, direction_block=256, lane_directions=4)),
+                         ("horner direct", dict(channel_sum=1, direct_cube=1))):
             got, _ = ctx.calibrate(view, dirs, opts=leap.options(precision=1, strategy=1, **kw), md=md)
+            assert ctx.last_route()[0] == (2 if name == "nufft" else 1)
             err = wrapped(got, ref)
             print(f"mixed ({name}) vs FP64, 256 stations: max {err.max():.2e} rad, rms {np.sqrt((err ** 2).mean()):.2e}")
             assert err.max() < 1e-5

@@ -155,7 +155,7 @@ def test_nufft_calibration_error_budget(leap, ctx, oracle):
 
 
 def test_nufft_is_the_automatic_mixed_choice(leap, ctx):
-    s = leap.SynthMS(n_antennas=12, n_timesteps=4, n_channels=64, seed=213)
+    s = leap.SynthMS(n_antennas=12, n_timesteps=4, n_channels=128, seed=213)
     s.fill_host(np.complex128)
     ms = s.host_dict(s.vis.astype(np.complex64))
     view = leap.view_from_arrays(ms)
@@ -167,9 +167,18 @@ def test_nufft_is_the_automatic_mixed_choice(leap, ctx):
                                                              channel_sum=HORNER)))
     assert np.array_equal(auto, nufft)
     assert not np.array_equal(auto, horner)
+    assert ctx.last_route() == (HORNER, 0)
     # few directions stay on Horner
     few = md.rotate(view, dirs[:4], opts=leap.options(**mixed(direction_block=64, lane_directions=1)))
+    assert ctx.last_route()[0] == HORNER
     assert np.array_equal(few, horner[:4])
+    # and so do few channels
+    s2 = leap.SynthMS(n_antennas=12, n_timesteps=4, n_channels=80, seed=216)
+    s2.fill_host(np.complex128)
+    ms2 = s2.host_dict(s2.vis.astype(np.complex64))
+    view2 = leap.view_from_arrays(ms2)
+    ctx.metadata(view2).rotate(view2, dirs, opts=leap.options(**mixed()))
+    assert ctx.last_route()[0] == HORNER
 
 
 def test_nufft_option_errors(leap, ctx):
