@@ -7,7 +7,7 @@
 //  "multi": [{"key": [k0, k1, ...], "tile": t, "workgroups_per_cu": w}, ...]}
 //
 // "mode" is the batched tuner's key code (precision, FP32 blocks, lane-direction
-// request and packed path); "key" the multi-set tuner's key (requested tile and
+// request, packed path and NUFFT channel sums); "key" the multi-set tuner's key (requested tile and
 // grid depth, then nbl, T, C, D and storage type of each set).
 #include <cctype>
 #include <cstdint>
@@ -206,6 +206,8 @@ void tuning_from_json(leap_ctx& ctx, const char* text) {
                     "tuning JSON: unknown batched tile " + std::to_string(tile));
             require(!(k.precision & kTunePackedBit) || packed_tile_supported(tile), LEAP_ERR_INVALID_ARGUMENT,
                     "tuning JSON: tile " + std::to_string(tile) + " is not a packed-path tile");
+            require(!(k.precision & kTuneNufftBit) || packed_tile_supported(tile), LEAP_ERR_INVALID_ARGUMENT,
+                    "tuning JSON: tile " + std::to_string(tile) + " is not a NUFFT tile");
             require(!(k.precision & 2) || tile % 10 == 1, LEAP_ERR_INVALID_ARGUMENT,
                     "tuning JSON: the FP32 blocked kernel runs one direction per lane");
             batched[k] = tile;

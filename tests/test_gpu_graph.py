@@ -56,21 +56,22 @@ class DeviceSet:
         self.ctx.free(self.uvw_p)
 
 
-CASES = [  # precision, strategy, direction block
-    (0, 0, 0), (0, 1, 0), (1, 1, 0), (0, 1, 64),
+CASES = [  # precision, strategy, direction block, channel sums
+    (0, 0, 0, 0), (0, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 2), (0, 1, 64, 0),
 ]
 
 
-@pytest.mark.parametrize("precision,strategy,db", CASES, ids=["fp64-per-dir", "fp64-batched", "mixed", "fp64-db64"])
-def test_graph_replays_equal_eager(leap, ctx, precision, strategy, db):
+@pytest.mark.parametrize("precision,strategy,db,cs", CASES,
+                         ids=["fp64-per-dir", "fp64-batched", "mixed", "mixed-nufft", "fp64-db64"])
+def test_graph_replays_equal_eager(leap, ctx, precision, strategy, db, cs):
     a = DeviceSet(leap, ctx, np.complex64 if precision else np.complex128, n_antennas=24, n_timesteps=9,
                   n_channels=50, seed=81, nan_fraction=0.002)
     b = DeviceSet(leap, ctx, n_antennas=10, n_timesteps=3, n_channels=7, seed=82)
     try:
         dirs = dirs_near(a.ms["phase_centre"], 12, 3)
         slice_ = (0, 3, -1)
-        eager = ctx.calibrate(a.view, dirs, slice_, opts=leap.options(precision, strategy, db), md=a.md)
-        g = leap.options(precision, strategy, db, graph=1)
+        eager = ctx.calibrate(a.view, dirs, slice_, opts=leap.options(precision, strategy, db, channel_sum=cs), md=a.md)
+        g = leap.options(precision, strategy, db, graph=1, channel_sum=cs)
         first = ctx.calibrate(a.view, dirs, slice_, opts=g, md=a.md)
         assert ctx.last_graph() == 1
         assert np.array_equal(first[0], eager[0]) and np.array_equal(first[1], eager[1])
@@ -83,7 +84,7 @@ def test_graph_replays_equal_eager(leap, ctx, precision, strategy, db):
         # new contents of the same buffers and new directions replay the same graph
         a.refill(leap, 99)
         dirs2 = dirs_near(a.ms["phase_centre"], 12, 4)
-        eager2 = ctx.calibrate(a.view, dirs2, slice_, opts=leap.options(precision, strategy, db), md=a.md)
+        eager2 = ctx.calibrate(a.view, dirs2, slice_, opts=leap.options(precision, strategy, db, channel_sum=cs), md=a.md)
         replay2 = ctx.calibrate(a.view, dirs2, slice_, opts=g, md=a.md)
         assert ctx.last_graph() == 2
         assert np.array_equal(replay2[0], eager2[0])

@@ -70,6 +70,33 @@ def test_nufft_rotate_matches_oracle(leap, ctx, oracle, n_ch, n_dirs, tile):
     print(f"  worst error / bound {worst:.3f}")
 
 
+@pytest.mark.parametrize("n_ch,n_dirs,n_t,n_pols", [(1, 1, 3, 4), (7, 5, 1, 2), (64, 65, 2, 1), (512, 3, 2, 4)])
+def test_nufft_edge_shapes_match_oracle(leap, ctx, oracle, n_ch, n_dirs, n_t, n_pols):
+    """Smallest and largest channel counts, one timestep, one direction and
+    direction counts one past a block, on the device cube and on packed rows."""
+    print(f"nufft edge: {n_ch} ch, {n_dirs} dirs, {n_t} t, {n_pols} pols", flush=True)
+    s = leap.SynthMS(n_antennas=10, n_timesteps=n_t, n_channels=n_ch, n_pols=n_pols, nan_fraction=0.01, seed=217)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    dirs = dirs_near(ms["phase_centre"], n_dirs, 11, spread=0.1)
+    omd = oracle.metadata(ms)
+    bound = 5e-10 * abs_mass(ms)
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    packed = md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT)))
+    vis_p = ctx.malloc(ms["vis"].nbytes)
+    try:
+        ctx.h2d(vis_p, np.ascontiguousarray(ms["vis"]))
+        dview = leap.view_from_arrays(ms, vis_device=vis_p, vis_dtype=leap.LEAP_VIS_C64)
+        cube = md.rotate(dview, dirs, opts=leap.options(**mixed(channel_sum=NUFFT)))
+        assert ctx.last_route() == (NUFFT, 1)
+    finally:
+        ctx.free(vis_p)
+    assert np.array_equal(packed, cube)
+    for d in range(n_dirs):
+        assert np.all(np.abs(cube[d, :, 0] - omd.rotate(dirs[d])[:, 0]) <= bound + 1e-300), d
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_nufft_host_and_device_cubes_agree(leap, ctx, dtype):
     s = leap.SynthMS(n_antennas=18, n_timesteps=6, n_channels=200, nan_fraction=0.004, seed=211)
