@@ -3,16 +3,17 @@
 // For row (t, baseline) and direction d the reference sums
 //   sum_c vis[c][0] exp(i sf_d (f0 + c df) / c_light)     (CpuLeapCalibrator.cc:181-217)
 // over the channels.  With b = sf_d df / c_light this is
-// exp(i sf_d kappa_mid) * sum_{k=-h}^{C-1-h} v_{k+h} exp(i k b), h = C / 2,
-// kappa_mid = (f0 + h df) / c_light, and the trigonometric polynomial is
-// evaluated by a type-2 non-uniform FFT (the plan and its error are described
-// in nufft_plan.cpp): the packed rows hold v_c already multiplied by the
-// precorrection factor of channel c, one N-point inverse FFT per row puts the
-// polynomial on a uniform grid of N points over one period, and each direction
-// interpolates W grid points with the "exponential of semicircle" kernel.  Per
-// row and direction this costs about 130 FP64 operations instead of the 4 C
-// FMAs of the Horner kernel; the FFT's cost is shared by every direction of
-// the row.
+// exp(i sf_d kappa_mid) * sum_{k=-h}^{C-1-h} v_{k+h} exp(i k b), h the plan's
+// frequency offset (C / 2 rounded to a multiple of 64), kappa_mid =
+// (f0 + h df) / c_light, and the trigonometric polynomial is evaluated by a
+// type-2 non-uniform FFT (the plan and its error are described in
+// nufft_plan.cpp): each channel's value is multiplied by its precorrection
+// factor (here, from the cube, or by the pack kernel), one N-point inverse FFT
+// per row puts the polynomial on a uniform grid of N points over one period,
+// and each direction interpolates W grid points with the "exponential of
+// semicircle" kernel.  Per row and direction this costs about 130 FP64
+// operations instead of the 4 C FMAs of the Horner kernel; the FFT's cost is
+// shared by every direction of the row.
 //
 // Work decomposition: a workgroup is one wave.  It owns one baseline and a
 // block of 64 * NDL directions (NDL per lane) and walks the baseline's rows in
