@@ -506,13 +506,15 @@ void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const
                              cal);
         return;
     }
-    launch_inversion_head(ctx->stream, md, avg, D, ctx->d_phase.as<double>(),
-                          ctx->d_cal1.as<double>(), ctx->d_delta.as<double>());
-    if (opts.precision == LEAP_PRECISION_MIXED) {
+    const bool mixed = opts.precision == LEAP_PRECISION_MIXED;
+    launch_inversion_head(ctx->stream, md, avg, D, ctx->d_phase.as<double>(), ctx->d_cal1.as<double>(),
+                          mixed ? nullptr : ctx->d_delta.as<double>());
+    if (mixed) {
         ctx->d_a_planes.reserve(bf16_planes_bytes((D + 63) / 64 * 64, K));
         ctx->d_partial.reserve(mfma_partial_floats(K, D, N) * sizeof(float));
-        launch_gemm_bf16x3(ctx->stream, md.d_AdT_planes.ptr, K, N, ctx->d_delta.as<double>(), D, ctx->d_a_planes.ptr,
-                           ctx->d_cal1.as<double>(), static_cast<float*>(ctx->d_partial.ptr), cal);
+        const DeltaSource from{avg, md.nbl, md.pols, md.d_pairs.as<int2>()};
+        launch_gemm_bf16x3(ctx->stream, md.d_AdT_planes.ptr, K, N, nullptr, D, ctx->d_a_planes.ptr,
+                           ctx->d_cal1.as<double>(), static_cast<float*>(ctx->d_partial.ptr), cal, &from);
     } else {
         ctx->d_partial.reserve(gemm_f64_partial_doubles(K, N, D) * sizeof(double));
         launch_gemm_f64(ctx->stream, md.d_AdT.as<double>(), K, N, ctx->d_delta.as<double>(), D,

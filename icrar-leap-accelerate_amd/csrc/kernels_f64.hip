@@ -10,6 +10,7 @@
 // c64); uvw [row][3] f64; avgData [direction][baseline][pol] complex128;
 // Ad stored transposed as AdT [K][N] (row k = column k of Ad) so the GEMV
 // reads whole rows; cal [direction][N].
+#include "inversion_common.h"
 #include "leap_internal.h"
 
 namespace leap {
@@ -198,35 +199,44 @@ __global__ void phase_i1_kernel(const double* __restrict__ avg, uint32_t nbl, ui
     phase[size_t(d) * K1 + k] = v;
 }
 
-__global__ void cal1_kernel(const double* __restrict__ Ad1T, int32_t K1, int32_t N,
-                            const double* __restrict__ phase, double* __restrict__ cal1) {
-    const uint32_t d = blockIdx.y;
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int32_t k = 0; k < K1; ++k) s += Ad1T[size_t(k) * N + i] * phase[size_t(d) * K1 + k];
-        cal1[size_t(d) * N + i] = s;
+// cal1[d][i] = sum_k Ad1T[k][i] phase[d][k], summed in k order.  One wave per
+// (kCal1Dirs directions, 64 antennas): lane = antenna, the wave's phases are
+// wave-uniform (scalar loads), every Ad1T element read feeds kCal1Dirs chains,
+// and the loop is unrolled so several rows of Ad1T are in flight.
+constexpr int kCal1Dirs = 4;
+
+__global__ __launch_bounds__(64) void cal1_kernel(const double* __restrict__ Ad1T, int32_t K1, int32_t N,
+                                                  const double* __restrict__ phase, uint32_t D,
+                                                  double* __restrict__ cal1) {
+    const int32_t i = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t d0 = blockIdx.y * kCal1Dirs;
+    const int32_t ic = i < N ? i : N - 1;
+    const double* ph[kCal1Dirs];
+#pragma unroll
+    for (int j = 0; j < kCal1Dirs; ++j) ph[j] = phase + size_t(d0 + j < D ? d0 + j : D - 1) * K1;
+    double acc[kCal1Dirs];
+#pragma unroll
+    for (int j = 0; j < kCal1Dirs; ++j) acc[j] = 0.0;
+#pragma unroll 8
+    for (int32_t k = 0; k < K1; ++k) {
+        const double b = Ad1T[size_t(k) * N + ic];
+#pragma unroll
+        for (int j = 0; j < kCal1Dirs; ++j) acc[j] = fma(b, ph[j][k], acc[j]);
+    }
+    if (i < N) {
+#pragma unroll
+        for (int j = 0; j < kCal1Dirs; ++j)
+            if (d0 + j < D) cal1[size_t(d0 + j) * N + i] = acc[j];
     }
 }
 
 __global__ void delta_kernel(const double* __restrict__ avg, uint32_t nbl, uint32_t P,
                              const int2* __restrict__ pairs, int32_t K, int32_t N,
                              const double* __restrict__ cal1, double* __restrict__ delta) {
-#pragma clang fp contract(off)
     const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t d = blockIdx.y;
     if (n >= K) return;
-    double v = 0.0;
-    if (n < K - 1) {
-        const int2 p = pairs[n];
-        const double acal1 = cal1[size_t(d) * N + p.x] - cal1[size_t(d) * N + p.y];
-        double sn, cs;
-        sincos(-kTwoPi * acal1, &sn, &cs);
-        const double* a = avg + (size_t(d) * nbl + n) * (2 * P);
-        const double re = cs * a[0] - sn * a[1];
-        const double im = cs * a[1] + sn * a[0];
-        v = atan2(im, re);
-    }
-    delta[size_t(d) * K + n] = v;
+    delta[size_t(d) * K + n] = dev::delta_phase(avg, nbl, P, pairs, K, N, cal1, d, n);
 }
 
 // Split-K GEMV: partial[d][chunk][i] = sum over the chunk's k of AdT[k][i] * delta[d][k].
@@ -305,7 +315,8 @@ void launch_inversion_f64(hipStream_t s, const leap_metadata& md, const double* 
     const uint32_t nbl = md.nbl, P = md.pols;
     phase_i1_kernel<<<dim3((K1 + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_I1.as<int32_t>(), K1, phase);
     LEAP_HIP(hipGetLastError());
-    cal1_kernel<<<dim3((N + 255) / 256, D), 256, 0, s>>>(md.d_Ad1T.as<double>(), K1, N, phase, cal1);
+    cal1_kernel<<<dim3((N + 63) / 64, (D + kCal1Dirs - 1) / kCal1Dirs), 64, 0, s>>>(md.d_Ad1T.as<double>(), K1, N,
+                                                                                phase, D, cal1);
     LEAP_HIP(hipGetLastError());
     delta_kernel<<<dim3((K + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_pairs.as<int2>(), K, N, cal1, delta);
     LEAP_HIP(hipGetLastError());
@@ -323,10 +334,13 @@ void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double*
     const uint32_t nbl = md.nbl, P = md.pols;
     phase_i1_kernel<<<dim3((K1 + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_I1.as<int32_t>(), K1, phase);
     LEAP_HIP(hipGetLastError());
-    cal1_kernel<<<dim3((N + 255) / 256, D), 256, 0, s>>>(md.d_Ad1T.as<double>(), K1, N, phase, cal1);
+    cal1_kernel<<<dim3((N + 63) / 64, (D + kCal1Dirs - 1) / kCal1Dirs), 64, 0, s>>>(md.d_Ad1T.as<double>(), K1, N,
+                                                                                phase, D, cal1);
     LEAP_HIP(hipGetLastError());
-    delta_kernel<<<dim3((K + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_pairs.as<int2>(), K, N, cal1, delta);
-    LEAP_HIP(hipGetLastError());
+    if (delta) {
+        delta_kernel<<<dim3((K + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_pairs.as<int2>(), K, N, cal1, delta);
+        LEAP_HIP(hipGetLastError());
+    }
 }
 
 size_t inversion_partial_doubles(int32_t K, int32_t N, uint32_t D) {

@@ -14,6 +14,7 @@
 // 2^-24 relative.  The split planes are stored in the operand layout of
 // v_mfma_f32_16x16x32_bf16 (8 consecutive k of one row or column per lane), so
 // each lane loads 16 contiguous bytes per plane and K-step.
+#include "inversion_common.h"
 #include "leap_internal.h"
 
 namespace leap {
@@ -93,6 +94,37 @@ __global__ void split_a_planes(const double* __restrict__ X, uint32_t M, int32_t
         out[(1 * total + idx) * 8 + j] = md;
         out[(2 * total + idx) * 8 + j] = l;
     }
+}
+
+// The A operand's planes straight from avgData: X[m][k] = delta_phase(d = m,
+// n = k) (inversion_common.h), split like split_a_planes, without the FP64
+// delta matrix in between.  Each thread's 8 consecutive k of one row are one
+// 16-byte store per plane.
+__global__ void delta_split_a_planes(const double* __restrict__ avg, uint32_t nbl, uint32_t P,
+                                     const int2* __restrict__ pairs, int32_t N, const double* __restrict__ cal1,
+                                     uint32_t M, int32_t K, uint32_t Mt, int32_t Ks, bf16x8* __restrict__ out) {
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;  // one (mt, ks, lane)
+    const size_t total = size_t(Mt) * Ks * 64;
+    if (idx >= total) return;
+    const uint32_t lane = idx % 64;
+    const size_t t = idx / 64;
+    const int32_t ks = int32_t(t % Ks);
+    const uint32_t mt = uint32_t(t / Ks);
+    const uint32_t m = mt * 16 + lane % 16;
+    const int32_t kb = ks * 32 + int32_t(lane / 16) * 8;
+    bf16x8 h, md, l;
+    for (int j = 0; j < 8; ++j) {
+        const int32_t k = kb + j;
+        const double x = (m < M && k < K) ? dev::delta_phase(avg, nbl, P, pairs, K, N, cal1, m, k) : 0.0;
+        __bf16 a, b, c;
+        split3(x, a, b, c);
+        h[j] = a;
+        md[j] = b;
+        l[j] = c;
+    }
+    out[0 * total + idx] = h;
+    out[1 * total + idx] = md;
+    out[2 * total + idx] = l;
 }
 
 // Planes of a row-major K x N matrix B (the B operand, columns = output columns):
@@ -213,12 +245,16 @@ size_t mfma_partial_floats(int32_t K, uint32_t D, int32_t N) {
 }
 
 void launch_gemm_bf16x3(hipStream_t s, const void* b_planes, int32_t K, int32_t N, const double* delta, uint32_t D,
-                        void* a_planes, const double* cal1, float* partial, double* cal) {
+                        void* a_planes, const double* cal1, float* partial, double* cal, const DeltaSource* from) {
     const uint32_t Mt = (D + 63) / 64 * 4;  // whole 4-wave groups
     const int32_t Nt = (N + 15) / 16, Ks = (K + 31) / 32;
     const size_t a_total = size_t(Mt) * Ks * 64;
-    split_a_planes<<<dim3(uint32_t((a_total + 255) / 256)), 256, 0, s>>>(delta, D, K, Mt, Ks,
-                                                                         static_cast<__bf16*>(a_planes));
+    if (from)
+        delta_split_a_planes<<<dim3(uint32_t((a_total + 255) / 256)), 256, 0, s>>>(
+            from->avg, from->nbl, from->pols, from->pairs, N, cal1, D, K, Mt, Ks, static_cast<bf16x8*>(a_planes));
+    else
+        split_a_planes<<<dim3(uint32_t((a_total + 255) / 256)), 256, 0, s>>>(delta, D, K, Mt, Ks,
+                                                                             static_cast<__bf16*>(a_planes));
     LEAP_HIP(hipGetLastError());
     const int32_t n_split = mfma_n_split(K, D, N);
     const int32_t ks_per = (Ks + n_split - 1) / n_split;

@@ -288,6 +288,8 @@ void launch_rotate_batched(hipStream_t s, int32_t precision, int32_t direction_b
                            double df, double* avg, int32_t fp32_blocks);
 size_t inversion_partial_doubles(int32_t K, int32_t N, uint32_t D);
 // Batched inversion pieces: cal1 / delta for D directions, then one GEMM.
+// phase and cal1 for D directions, and delta unless it is null (the mixed GEMM
+// computes its operand from avgData itself).
 void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D, double* phase,
                            double* cal1, double* delta);
 size_t gemm_f64_partial_doubles(int32_t K, int32_t N, uint32_t D);
@@ -296,8 +298,16 @@ void launch_gemm_f64(hipStream_t s, const double* AdT, int32_t K, int32_t N, con
 size_t bf16_planes_bytes(uint32_t rows, int32_t K);
 void launch_split_b(hipStream_t s, const double* B, int32_t K, int32_t N, void* planes);
 size_t mfma_partial_floats(int32_t K, uint32_t D, int32_t N);
+// Where the mixed GEMM's A operand comes from when it is computed in place of
+// the FP64 delta matrix: avgData and the antenna pairs of A's rows.
+struct DeltaSource {
+    const double* avg;
+    uint32_t nbl, pols;
+    const int2* pairs;
+};
 void launch_gemm_bf16x3(hipStream_t s, const void* b_planes, int32_t K, int32_t N, const double* delta, uint32_t D,
-                        void* a_planes, const double* cal1, float* partial, double* cal);
+                        void* a_planes, const double* cal1, float* partial, double* cal,
+                        const DeltaSource* from = nullptr);
 void launch_perturb_uvw(hipStream_t s, const double* uvw0, const double* M, const int2* bl_ants, const double* dp,
                         uint32_t nbl, uint32_t rows, double* uvw);
 void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, const double* uvw, const double* dd,
