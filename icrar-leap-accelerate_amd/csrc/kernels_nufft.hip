@@ -49,6 +49,8 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kWave = 64;
 
+using AS4double = __attribute__((address_space(4))) double;
+
 struct NufftArgs {
     const double* coef;  // device: [W / 2][kNufftMaxWidth] weight polynomials of taps j < W / 2, powers of s
     double kappa_mid;    // (f0 + h df) / c_light
@@ -207,12 +209,23 @@ __device__ inline void pass_compute(Z* v, const Z* base, Z phi) {
                 x[r] = cmul(x[r], t);
             }
         } else if constexpr (NS > 1) {
-            Z p = base[q];
-            x[1] = cmul(x[1], p);
+            // Odd and even powers as two interleaved chains (step w^2): half
+            // the dependent depth of one chain, three values live.
+            const Z w1 = base[q];
+            const Z w2 = cmul(w1, w1);
+            Z po = w1, pe = w2;
+            x[1] = cmul(x[1], po);
+            x[2] = cmul(x[2], pe);
 #pragma unroll
-            for (int r = 2; r < R; ++r) {
-                p = cmul(p, base[q]);
-                x[r] = cmul(x[r], p);
+            for (int r = 3; r + 1 < R; r += 2) {
+                po = cmul(po, w2);
+                pe = cmul(pe, w2);
+                x[r] = cmul(x[r], po);
+                x[r + 1] = cmul(x[r + 1], pe);
+            }
+            if constexpr (R % 2 == 0) {
+                po = cmul(po, w2);
+                x[R - 1] = cmul(x[R - 1], po);
             }
         }
         idft<R, HALF>(x);
@@ -516,19 +529,27 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
         // ug = N frac(b / 2 pi), first tap m0 = ceil(ug - W/2), offset
         // s = ug - m0 - (W - 1)/2 in (-1/2, 1/2]; tap pairs (jp, W - 1 - jp) take
         // weights P_jp(s), P_jp(-s), P_jp(s) = E(s^2) + s O(s^2).  The pairs
-        // are the outer loop so that one pair's coefficients are live at a time.
-        double sfv[NDL], sv[NDL], s2[NDL];
+        // are the outer loop so that at most two pairs' coefficients are live.
+        // The NDL directions' steps are interleaved (independent chains side by
+        // side) with each direction's operations unchanged.
+        double sfv[NDL], sv[NDL], s2[NDL], x[NDL], m0f[NDL];
         const Z* gp[NDL];
 #pragma unroll
+        for (int j = 0; j < NDL; ++j) sfv[j] = shift_factor(dz[j], u, vv, ww);
+#pragma unroll
+        for (int j = 0; j < NDL; ++j) x[j] = sfv[j] * args.dkappa * kInvTwoPi;  // turns of b
+#pragma unroll
+        for (int j = 0; j < NDL; ++j) x[j] -= floor(x[j]);  // [0, 1]
+#pragma unroll
         for (int j = 0; j < NDL; ++j) {
-            sfv[j] = shift_factor(dz[j], u, vv, ww);
-            double x = sfv[j] * args.dkappa * kInvTwoPi;  // turns of b
-            x -= floor(x);                                // [0, 1]
-            const double ug = x * double(N);               // [0, N]
-            const double m0f = ceil(ug - 0.5 * W);
-            sv[j] = (ug - m0f) - 0.5 * (W - 1);
+            x[j] = x[j] * double(N);  // grid coordinate ug in [0, N]
+            m0f[j] = ceil(x[j] - 0.5 * W);
+        }
+#pragma unroll
+        for (int j = 0; j < NDL; ++j) {
+            sv[j] = (x[j] - m0f[j]) - 0.5 * (W - 1);
             s2[j] = sv[j] * sv[j];
-            int32_t m0 = int32_t(m0f);
+            int32_t m0 = int32_t(m0f[j]);
             if (m0 < 0) m0 += N;  // taps m0 .. m0 + W - 1 <= N + W - 2 (wrapped copy)
             m0 = min(max(m0, 0), N - 1);  // a non-finite uvw stays inside the grid
             gp[j] = s_g + m0;
@@ -536,33 +557,77 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
         Z acc[NDL];
 #pragma unroll
         for (int j = 0; j < NDL; ++j) acc[j] = {0.0, 0.0};
-#pragma unroll
-        for (int jp = 0; jp < W / 2; ++jp) {
-            // The pair's coefficients are read here (scalar loads), not kept in
-            // SGPRs across the loop: all W/2 pairs do not fit.
-            const double* cp = args.coef + jp * kNufftMaxWidth;
+        // A pair's coefficients are read with scalar loads one pair ahead and
+        // not kept across the loop (all W/2 pairs do not fit in SGPRs).  Scalar
+        // and LDS loads share one wait counter and scalar loads return out of
+        // order, so every wait is for all of them: the pair's 2 NDL grid reads
+        // are issued together with the next pair's coefficients, ahead of the
+        // weight polynomials that hide their latency, and waited for once.
+        double cc[W];
+        {
+            const double* cp = args.coef;
             asm volatile("" : "+s"(cp));
-            const __attribute__((address_space(4))) double* cf = (const __attribute__((address_space(4))) double*)cp;
+            const AS4double* cf = (const AS4double*)cp;
 #pragma unroll
-            for (int j = 0; j < NDL; ++j) {
-                double e = cf[W - 2];
-                double o = cf[W - 1];
-#pragma unroll
-                for (int k = W - 4; k >= 0; k -= 2) {
-                    e = fma(e, s2[j], cf[k]);
-                    o = fma(o, s2[j], cf[k + 1]);
-                }
-                const double wl = fma(sv[j], o, e), wh = fma(-sv[j], o, e);
-                const Z g0 = gp[j][jp], g1 = gp[j][W - 1 - jp];
-                acc[j].x = fma(g0.x, wl, acc[j].x);
-                acc[j].y = fma(g0.y, wl, acc[j].y);
-                acc[j].x = fma(g1.x, wh, acc[j].x);
-                acc[j].y = fma(g1.y, wh, acc[j].y);
-            }
+            for (int k = 0; k < W; ++k) cc[k] = cf[k];
         }
 #pragma unroll
+        for (int jp = 0; jp < W / 2; ++jp) {
+            Z ga[NDL], gb[NDL];
+#pragma unroll
+            for (int j = 0; j < NDL; ++j) {
+                ga[j] = gp[j][jp];
+                gb[j] = gp[j][W - 1 - jp];
+            }
+            double cn[W];
+            if (jp + 1 < W / 2) {
+                const double* cp = args.coef + (jp + 1) * kNufftMaxWidth;
+                asm volatile("" : "+s"(cp));
+                const AS4double* cf = (const AS4double*)cp;
+#pragma unroll
+                for (int k = 0; k < W; ++k) cn[k] = cf[k];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            double wl[NDL], wh[NDL], e[NDL], o[NDL];
+#pragma unroll
+            for (int j = 0; j < NDL; ++j) {
+                e[j] = cc[W - 2];
+                o[j] = cc[W - 1];
+            }
+#pragma unroll
+            for (int k = W - 4; k >= 0; k -= 2) {
+#pragma unroll
+                for (int j = 0; j < NDL; ++j) {
+                    e[j] = fma(e[j], s2[j], cc[k]);
+                    o[j] = fma(o[j], s2[j], cc[k + 1]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NDL; ++j) {
+                wl[j] = fma(sv[j], o[j], e[j]);
+                wh[j] = fma(-sv[j], o[j], e[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < NDL; ++j) {
+                acc[j].x = fma(ga[j].x, wl[j], acc[j].x);
+                acc[j].y = fma(ga[j].y, wl[j], acc[j].y);
+                acc[j].x = fma(gb[j].x, wh[j], acc[j].x);
+                acc[j].y = fma(gb[j].y, wh[j], acc[j].y);
+            }
+            if (jp + 1 < W / 2) {
+#pragma unroll
+                for (int k = 0; k < W; ++k) cc[k] = cn[k];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        double th[NDL];
+        Phasor64 ph[NDL];
+#pragma unroll
+        for (int j = 0; j < NDL; ++j) th[j] = sfv[j] * args.kappa_mid;
+        phasor64_n<NDL>(th, ph);
+#pragma unroll
         for (int j = 0; j < NDL; ++j) {
-            const Z r = cmul(phasor_f64(sfv[j] * args.kappa_mid), acc[j]);
+            const Z r = cmul(Z{ph[j].c, ph[j].s}, acc[j]);
             sum[j].x += r.x;
             sum[j].y += r.y;
         }

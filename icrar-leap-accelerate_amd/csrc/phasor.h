@@ -60,4 +60,50 @@ __host__ __device__ inline Phasor64 phasor64(double theta) {
     return out;
 }
 
+// phasor64 of M arguments with the steps of the M evaluations interleaved
+// (independent chains side by side); the same operations as phasor64 for
+// each argument, so the same bits.
+template <int M>
+__device__ inline void phasor64_n(const double* theta, Phasor64* out) {
+#pragma clang fp contract(off)
+    constexpr double kTwoOverPi = 0.63661977236758134308;
+    constexpr double kPiO2Hi = 1.5707963267948965580;
+    constexpr double kPiO2Lo = 6.1232339957367658e-17;
+    constexpr double ks[8] = {-1.0 / 1307674368000.0, 1.0 / 6227020800.0, -1.0 / 39916800.0, 1.0 / 362880.0,
+                              -1.0 / 5040.0,          1.0 / 120.0,        -1.0 / 6.0,        0.0};
+    constexpr double kc[8] = {-1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0,
+                              -1.0 / 720.0,         1.0 / 24.0,        -0.5,             0.0};
+    double n[M], r[M], r2[M], ps[M], pc[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) n[j] = rint(theta[j] * kTwoOverPi);
+#pragma unroll
+    for (int j = 0; j < M; ++j) r[j] = fma(-n[j], kPiO2Hi, theta[j]);
+#pragma unroll
+    for (int j = 0; j < M; ++j) r[j] = fma(-n[j], kPiO2Lo, r[j]);
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        r2[j] = r[j] * r[j];
+        ps[j] = 1.0 / 355687428096000.0;
+        pc[j] = 1.0 / 20922789888000.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            ps[j] = fma(ps[j], r2[j], ks[k]);
+            pc[j] = fma(pc[j], r2[j], kc[k]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const double sn = fma(r[j] * r2[j], ps[j], r[j]);
+        const double cs = fma(r2[j], pc[j], 1.0);
+        const int q = int(static_cast<int64_t>(n[j]) & 3);
+        out[j].c = (q & 1) ? sn : cs;
+        out[j].s = (q & 1) ? cs : sn;
+        if (q == 1 || q == 2) out[j].c = -out[j].c;
+        if (q >= 2) out[j].s = -out[j].s;
+    }
+}
+
 }  // namespace leap
