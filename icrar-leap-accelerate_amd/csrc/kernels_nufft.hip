@@ -632,9 +632,13 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
             sum[j].y += r.y;
         }
     }
+    // The output indices are recomputed here from an opaque copy of the lane,
+    // so they are not kept in registers across the row loop.
+    uint32_t ln = lane;
+    asm volatile("" : "+v"(ln));
 #pragma unroll
     for (int j = 0; j < NDL; ++j) {
-        const uint32_t d = d0 + lane + kWave * j;
+        const uint32_t d = d0 + ln + kWave * j;
         if (d >= D) continue;
         LEAP_GLOBAL double* o = avg + (size_t(d) * nbl + bl) * P * 2;
         if (accumulate) {
