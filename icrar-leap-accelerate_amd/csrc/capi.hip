@@ -316,7 +316,7 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
             if (nufft)
                 launch_rotate_nufft(ctx->stream, tile, md.nufft, md.d_nufft.as<double>(),
                                     packed ? ctx->d_packed.ptr : rows.vis, !packed, rows.uvw, ctx->d_dd.as<double>(),
-                                    D, T, md.nbl, probe_bl, md.channels, md.pols, md.freq_start_hz,
+                                    D, T, md.nbl, probe_bl, md.channels, md.pols, md.pols, md.freq_start_hz,
                                     md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
             else if (packed)
                 launch_rotate_packed(ctx->stream, tile, ctx->d_packed.as<double>(), rows.uvw, ctx->d_dd.as<double>(),
@@ -364,7 +364,7 @@ uint32_t pack_chunk(uint32_t T, uint32_t nbl, uint32_t C, const leap_options& op
 // packed by the host staging instead (pol 0 only crosses the bus), which
 // orders itself on the context stream.
 void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
-                            uint32_t T, uint32_t D, const leap_options& opts, int32_t tile) {
+                            uint32_t T, uint32_t D, const leap_options& opts, int32_t tile, uint32_t avg_pols) {
     const uint32_t tc = pack_chunk(T, md.nbl, md.channels, opts);
     const uint32_t n_chunks = (T + tc - 1) / tc;
     const size_t per_t = size_t(md.nbl) * md.channels;  // (row, channel)s per timestep
@@ -395,8 +395,8 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
         if (nufft)
             launch_rotate_nufft(ctx->stream, tile, md.nufft, md.d_nufft.as<double>(), buf[k % 2], false,
                                 rows.uvw + size_t(t0) * md.nbl * 3, ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl,
-                                md.channels, md.pols, md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(),
-                                k > 0 ? 1 : 0);
+                                md.channels, md.pols, avg_pols, md.freq_start_hz, md.freq_inc_hz,
+                                ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
         else
             launch_rotate_packed(ctx->stream, tile, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
                                  ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols,
@@ -447,14 +447,20 @@ int32_t batched_tile(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view&
     });
 }
 
-// avgData of D directions for T timesteps starting at the staged rows, into ctx->d_avg.
-void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
-                       uint32_t T, uint32_t D, const leap_options& opts, int32_t tile = 0) {
-    const size_t per_dir = size_t(md.nbl) * md.pols * 2;
+// avgData of D directions for T timesteps starting at the staged rows, into
+// ctx->d_avg; returns the pols it holds per (direction, baseline).  With
+// pol0_only (calibrations: only pol 0 reaches cal, CpuLeapCalibrator.cc:161,171)
+// the NUFFT route writes pol 0 alone, a quarter of the bytes for 4 pols.
+uint32_t rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
+                           uint32_t T, uint32_t D, const leap_options& opts, int32_t tile = 0,
+                           bool pol0_only = false) {
+    const bool nufft_route = opts.strategy == LEAP_STRATEGY_BATCHED && opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
+    const uint32_t avg_pols = pol0_only && nufft_route ? 1u : md.pols;
+    const size_t per_dir = size_t(md.nbl) * md.pols * 2;  // room for the full layout (autotune probes)
     ctx->d_avg.reserve(per_dir * D * sizeof(double));
     if (opts.strategy == LEAP_STRATEGY_BATCHED && packed_route(opts, tile)) {
-        rotate_packed_pipeline(ctx, md, ms, rows, T, D, opts, tile);
-        return;
+        rotate_packed_pipeline(ctx, md, ms, rows, T, D, opts, tile, avg_pols);
+        return avg_pols;
     }
     require(rows.vis != nullptr, LEAP_ERR_INTERNAL, "visibilities were not staged");
     if (opts.strategy == LEAP_STRATEGY_BATCHED && opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
@@ -462,10 +468,10 @@ void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_vie
         ctx->last_direction_block = db;
         record_rotate_begin(ctx);
         launch_rotate_nufft(ctx->stream, db, md.nufft, md.d_nufft.as<double>(), rows.vis, true, rows.uvw,
-                            ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
-                            md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
+                            ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, avg_pols,
+                            md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
         record_rotate_end(ctx);
-        return;
+        return avg_pols;
     }
     if (opts.strategy == LEAP_STRATEGY_BATCHED) {
         const int32_t db = tile ? tile : batched_tile(ctx, md, ms, rows, T, D, opts);
@@ -475,7 +481,7 @@ void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_vie
                               ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, md.freq_start_hz,
                               md.freq_inc_hz, ctx->d_avg.as<double>(), opts.fp32_blocks);
         record_rotate_end(ctx);
-        return;
+        return avg_pols;
     }
     for (uint32_t d = 0; d < D; ++d) {
         record_rotate_begin(ctx);
@@ -484,11 +490,14 @@ void rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_vie
                           ctx->d_avg.as<double>() + per_dir * d);
         record_rotate_end(ctx);
     }
+    return avg_pols;
 }
 
-// cal[D][N] of D directions from avgData at avg (ctx->d_avg unless given).
+// cal[D][N] of D directions from avgData at avg (ctx->d_avg unless given),
+// which holds avg_pols pols per (direction, baseline) (md.pols when 0).
 void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const leap_options& opts,
-                       const double* avg = nullptr, double* cal = nullptr) {
+                       const double* avg = nullptr, double* cal = nullptr, uint32_t avg_pols = 0) {
+    if (avg_pols == 0) avg_pols = md.pols;
     const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
     ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
     ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
@@ -507,12 +516,12 @@ void invert_directions(leap_ctx* ctx, const leap_metadata& md, uint32_t D, const
         return;
     }
     const bool mixed = opts.precision == LEAP_PRECISION_MIXED;
-    launch_inversion_head(ctx->stream, md, avg, D, ctx->d_phase.as<double>(), ctx->d_cal1.as<double>(),
+    launch_inversion_head(ctx->stream, md, avg, avg_pols, D, ctx->d_phase.as<double>(), ctx->d_cal1.as<double>(),
                           mixed ? nullptr : ctx->d_delta.as<double>());
     if (mixed) {
         ctx->d_a_planes.reserve(bf16_planes_bytes((D + 63) / 64 * 64, K));
         ctx->d_partial.reserve(mfma_partial_floats(K, D, N) * sizeof(float));
-        const DeltaSource from{avg, md.nbl, md.pols, md.d_pairs.as<int2>()};
+        const DeltaSource from{avg, md.nbl, avg_pols, md.d_pairs.as<int2>()};
         launch_gemm_bf16x3(ctx->stream, md.d_AdT_planes.ptr, K, N, nullptr, D, ctx->d_a_planes.ptr,
                            ctx->d_cal1.as<double>(), static_cast<float*>(ctx->d_partial.ptr), cal, &from);
     } else {
@@ -714,8 +723,8 @@ void calibrate_loop_graph(leap_ctx* ctx, const leap_metadata& md, const leap_ms_
                 for (int32_t s = 0; s < solutions; ++s) {
                     const DeviceRows rows = stage_rows(ctx, ms, uint32_t(size_t(s) * interval * nbl),
                                                        uint32_t(size_t(s + 1) * interval * nbl));
-                    rotate_directions(ctx, md, ms, rows, interval, D, opts, tile);
-                    invert_directions(ctx, md, D, opts, nullptr, seq->d_cal.as<double>() + per_sol * size_t(s));
+                    const uint32_t ap = rotate_directions(ctx, md, ms, rows, interval, D, opts, tile, true);
+                    invert_directions(ctx, md, D, opts, nullptr, seq->d_cal.as<double>() + per_sol * size_t(s), ap);
                 }
             });
             ctx->last_graph = 1;
@@ -782,8 +791,8 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
         sol.calibrations = ctx->h_cal.data();
         if (D > 0) {
             const DeviceRows rows = stage_rows(ctx, ms, row_begin, row_end, !packed_route(opts, 0));
-            rotate_directions(ctx, md, ms, rows, interval, D, opts);
-            invert_directions(ctx, md, D, opts);
+            const uint32_t ap = rotate_directions(ctx, md, ms, rows, interval, D, opts, 0, true);
+            invert_directions(ctx, md, D, opts, nullptr, nullptr, ap);
             LEAP_HIP(hipMemcpyAsync(ctx->h_cal.data(), ctx->d_cal.ptr, size_t(N) * D * sizeof(double),
                                     hipMemcpyDeviceToHost, ctx->stream));
             LEAP_HIP(hipStreamSynchronize(ctx->stream));

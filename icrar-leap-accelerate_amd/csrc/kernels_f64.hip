@@ -327,11 +327,11 @@ void launch_inversion_f64(hipStream_t s, const leap_metadata& md, const double* 
     LEAP_HIP(hipGetLastError());
 }
 
-void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D, double* phase,
-                           double* cal1, double* delta) {
+void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t avg_pols, uint32_t D,
+                           double* phase, double* cal1, double* delta) {
     if (D == 0) return;
     const int32_t K = md.A.rows, N = md.A.cols, K1 = md.A1.rows;
-    const uint32_t nbl = md.nbl, P = md.pols;
+    const uint32_t nbl = md.nbl, P = avg_pols;
     phase_i1_kernel<<<dim3((K1 + 255) / 256, D), 256, 0, s>>>(avg, nbl, P, md.d_I1.as<int32_t>(), K1, phase);
     LEAP_HIP(hipGetLastError());
     cal1_kernel<<<dim3((N + 63) / 64, (D + kCal1Dirs - 1) / kCal1Dirs), 64, 0, s>>>(md.d_Ad1T.as<double>(), K1, N,

@@ -482,7 +482,7 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
                                                               const LEAP_GLOBAL double* __restrict__ scale,
                                                               const LEAP_GLOBAL double* __restrict__ uvw,
                                                               const LEAP_GLOBAL double* __restrict__ dd, uint32_t D,
-                                                              uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
+                                                              uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
                                                               NufftArgs args, LEAP_GLOBAL double* __restrict__ avg,
                                                               int32_t accumulate) {
 #pragma clang fp contract(off)
@@ -640,21 +640,21 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     for (int j = 0; j < NDL; ++j) {
         const uint32_t d = d0 + ln + kWave * j;
         if (d >= D) continue;
-        LEAP_GLOBAL double* o = avg + (size_t(d) * nbl + bl) * P * 2;
+        LEAP_GLOBAL double* o = avg + (size_t(d) * nbl + bl) * Pout * 2;
         if (accumulate) {
             o[0] += sum[j].x;
             o[1] += sum[j].y;
         } else {
             o[0] = sum[j].x;
             o[1] = sum[j].y;
-            for (uint32_t q = 2; q < 2 * P; ++q) o[q] = 0.0;  // pols not accumulated
+            for (uint32_t q = 2; q < 2 * Pout; ++q) o[q] = 0.0;  // pols not accumulated
         }
     }
 }
 
 template <int N, int W, int SP>
 void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
-                const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, const NufftArgs& a,
+                const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout, const NufftArgs& a,
                 double* avg, int32_t accumulate) {
     const auto* gs = (const LEAP_GLOBAL double*)scale;
     const auto* gu = (const LEAP_GLOBAL double*)uvw;
@@ -663,15 +663,15 @@ void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, c
     switch (tile) {
         case 641:
             rotate_average_nufft<N, W, 1, SP><<<dim3(nbl_grid, (D + 63) / 64), kWave, 0, s>>>(
-                w, gs, gu, gd, D, T, nbl, C, P, a, ga, accumulate);
+                w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate);
             break;
         case 1282:
             rotate_average_nufft<N, W, 2, SP><<<dim3(nbl_grid, (D + 127) / 128), kWave, 0, s>>>(
-                w, gs, gu, gd, D, T, nbl, C, P, a, ga, accumulate);
+                w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate);
             break;
         case 2564:
             rotate_average_nufft<N, W, 4, SP><<<dim3(nbl_grid, (D + 255) / 256), kWave, 0, s>>>(
-                w, gs, gu, gd, D, T, nbl, C, P, a, ga, accumulate);
+                w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate);
             break;
         default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported NUFFT tile " + std::to_string(tile));
     }
@@ -679,13 +679,13 @@ void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, c
 
 template <int N, int W>
 void launch_nw(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, int32_t src_pols, const double* scale,
-               const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
+               const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
                const NufftArgs& a, double* avg, int32_t accumulate) {
     switch (src_pols) {
-        case 0: launch_nws<N, W, 0>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
-        case 1: launch_nws<N, W, 1>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
-        case 2: launch_nws<N, W, 2>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
-        case 4: launch_nws<N, W, 4>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate); break;
+        case 0: launch_nws<N, W, 0>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
+        case 1: launch_nws<N, W, 1>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
+        case 2: launch_nws<N, W, 2>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
+        case 4: launch_nws<N, W, 4>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
         default: throw Error(LEAP_ERR_INTERNAL, "unsupported NUFFT row source");
     }
 }
@@ -698,9 +698,10 @@ bool nufft_cube_supported(int32_t vis_dtype, uint32_t P) {
 
 void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
                          bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
-                         uint32_t nbl_grid, uint32_t C, uint32_t P, double f0, double df, double* avg,
-                         int32_t accumulate) {
+                         uint32_t nbl_grid, uint32_t C, uint32_t P, uint32_t P_out, double f0, double df,
+                         double* avg, int32_t accumulate) {
     if (nbl_grid == 0 || D == 0) return;
+    require(P_out >= 1, LEAP_ERR_INTERNAL, "NUFFT output needs at least pol 0");
     require(plan.N != 0 && plan.scale.size() == C, LEAP_ERR_INTERNAL, "NUFFT plan does not match the channel count");
     require(!cube || (P == 1 || P == 2 || P == 4), LEAP_ERR_INTERNAL, "NUFFT cube source needs 1, 2 or 4 pols");
     NufftArgs a{};
@@ -712,7 +713,8 @@ void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, con
     const int32_t sp = cube ? int32_t(P) : 0;
 #define LEAP_NUFFT(N_, W_)                                                                                   \
     if (plan.N == N_ && plan.W == W_) {                                                                      \
-        launch_nw<N_, W_>(s, tile, nbl_grid, src, sp, scale, uvw, dd, D, T, nbl, C, P, a, avg, accumulate);  \
+        launch_nw<N_, W_>(s, tile, nbl_grid, src, sp, scale, uvw, dd, D, T, nbl, C, P_out, a, avg,           \
+                          accumulate);                                                                       \
         LEAP_HIP(hipGetLastError());                                                                         \
         return;                                                                                              \
     }

@@ -263,10 +263,12 @@ void launch_scale_channels(hipStream_t s, double* x, size_t n, const double* sca
 // tables: the plan on the device, kNufftCoefDoubles coefficients (coef[j][m])
 // then the C precorrection factors.
 bool nufft_cube_supported(int32_t vis_dtype, uint32_t P);
+// avg holds P_out pols per (direction, baseline) (pol 0 summed, the others
+// set to zero unless accumulating); P is the cube's pol count.
 void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
                          bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
-                         uint32_t nbl_grid, uint32_t C, uint32_t P, double f0, double df, double* avg,
-                         int32_t accumulate);
+                         uint32_t nbl_grid, uint32_t C, uint32_t P, uint32_t P_out, double f0, double df,
+                         double* avg, int32_t accumulate);
 bool packed_tile_supported(int32_t tile);
 void launch_rotate_packed(hipStream_t s, int32_t tile, const double* w, const double* uvw, const double* dd,
                           uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,
@@ -290,8 +292,9 @@ size_t inversion_partial_doubles(int32_t K, int32_t N, uint32_t D);
 // Batched inversion pieces: cal1 / delta for D directions, then one GEMM.
 // phase and cal1 for D directions, and delta unless it is null (the mixed GEMM
 // computes its operand from avgData itself).
-void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D, double* phase,
-                           double* cal1, double* delta);
+// avg holds avg_pols pols per (direction, baseline); pol 0 is used.
+void launch_inversion_head(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t avg_pols, uint32_t D,
+                           double* phase, double* cal1, double* delta);
 size_t gemm_f64_partial_doubles(int32_t K, int32_t N, uint32_t D);
 void launch_gemm_f64(hipStream_t s, const double* AdT, int32_t K, int32_t N, const double* delta, uint32_t D,
                      const double* cal1, double* partial, double* cal);
