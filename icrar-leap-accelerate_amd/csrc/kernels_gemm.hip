@@ -191,13 +191,145 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_mfma(const bf16x8* __restrict
     }
 }
 
+// Whole-output variant for M, N <= 256 (Mt, Nt <= 16 tiles): a workgroup of 8
+// waves takes one K-split for all output tiles, so every operand byte comes
+// from HBM once per split.  Wave w owns m-tiles 2w, 2w + 1 and all n-tiles (32
+// tiles, 128 FP32 accumulators per lane).  Per K-step the workgroup stages the
+// B operand (all n-tiles, 3 planes, 48 KB) in LDS, double-buffered, while each
+// wave holds its A tiles in registers; the next K-step's A and B loads are in
+// flight during the current K-step's products.  Per tile and K-step the six
+// products go in the order of gemm_bf16x3_mfma.
+constexpr int kFullTM = 2, kFullWaves = 8, kFullThreads = 64 * kFullWaves;
+constexpr int kFullMaxTiles = 16;                         // per dimension: 256 rows or columns
+constexpr int kFullBChunks = kFullMaxTiles * 3 * 64;      // 16-byte chunks of one K-step's B
+constexpr int kFullBPerThread = kFullBChunks / kFullThreads;
+constexpr int kFullSplits = 160;                          // K-splits of the whole-output kernel
+static_assert(kFullBChunks % kFullThreads == 0 && kFullWaves * kFullTM == kFullMaxTiles, "tiling");
+
+__global__ __launch_bounds__(kFullThreads) void gemm_bf16x3_full(const bf16x8* __restrict__ Ap,
+                                                                 const bf16x8* __restrict__ Bp, uint32_t Mt,
+                                                                 int32_t Nt, int32_t Ks, int32_t ks_per_split,
+                                                                 float* __restrict__ partial) {
+    __shared__ bf16x8 s_b[2][kFullBChunks];  // [buffer][(n-tile * 3 + plane) * 64 + lane]
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid % 64;
+    const uint32_t m0 = (tid / 64) * kFullTM;
+    const bool rows = m0 < Mt;  // wave-uniform; a wave without rows still stages B and meets the barriers
+    const int32_t split = blockIdx.x;
+    const size_t a_total = size_t(Mt) * Ks * 64;
+    const size_t b_total = size_t(Nt) * Ks * 64;
+    const int32_t ks0 = split * ks_per_split;
+    const int32_t ks1 = min(Ks, ks0 + ks_per_split);
+    f32x4 acc[kFullTM][kFullMaxTiles];
+#pragma unroll
+    for (int i = 0; i < kFullTM; ++i)
+#pragma unroll
+        for (int j = 0; j < kFullMaxTiles; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // This thread's B chunks c = tid + kFullThreads r: n-tile c / 192, plane (c / 64) % 3, lane c % 64.
+    auto load_b = [&](int32_t ks, bf16x8* b) {
+#pragma unroll
+        for (int r = 0; r < kFullBPerThread; ++r) {
+            const uint32_t c = tid + kFullThreads * r;
+            const uint32_t nt = c / 192, p = (c / 64) % 3, l = c % 64;
+            if (int32_t(nt) < Nt) b[r] = Bp[p * b_total + (size_t(nt) * Ks + ks) * 64 + l];
+        }
+    };
+    auto store_b = [&](int buf, const bf16x8* b) {
+#pragma unroll
+        for (int r = 0; r < kFullBPerThread; ++r) {
+            const uint32_t c = tid + kFullThreads * r;
+            if (int32_t(c / 192) < Nt) s_b[buf][c] = b[r];
+        }
+    };
+    auto load_a = [&](int32_t ks, bf16x8 (*a)[3]) {
+        if (!rows) return;
+#pragma unroll
+        for (int i = 0; i < kFullTM; ++i) {
+            const bf16x8* t = Ap + (size_t(m0 + i) * Ks + ks) * 64;  // m0 + kFullTM <= Mt (a multiple of 4)
+            a[i][0] = t[lane];
+            a[i][1] = t[a_total + lane];
+            a[i][2] = t[2 * a_total + lane];
+        }
+    };
+    bf16x8 a[kFullTM][3], an[kFullTM][3], b[kFullBPerThread];
+    if (ks0 < ks1) {
+        load_b(ks0, b);
+        load_a(ks0, a);
+        store_b(0, b);
+    }
+    __syncthreads();
+    for (int32_t ks = ks0; ks < ks1; ++ks) {
+        const int buf = (ks - ks0) & 1;
+        const bool more = ks + 1 < ks1;  // uniform over the workgroup
+        if (more) {
+            load_b(ks + 1, b);
+            load_a(ks + 1, an);
+        }
+        if (rows) {
+#pragma unroll
+            for (int j = 0; j < kFullMaxTiles; ++j) {
+                if (j < Nt) {  // uniform
+                    const bf16x8 bh = s_b[buf][(j * 3) * 64 + lane];
+                    const bf16x8 bm = s_b[buf][(j * 3 + 1) * 64 + lane];
+                    const bf16x8 bl = s_b[buf][(j * 3 + 2) * 64 + lane];
+                    // the kFullTM tiles' accumulation chains interleaved
+#pragma unroll
+                    for (int i = 0; i < kFullTM; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], bm, acc[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < kFullTM; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], bl, acc[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < kFullTM; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], bh, acc[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < kFullTM; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], bm, acc[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < kFullTM; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], bh, acc[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < kFullTM; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], bh, acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        if (more) {
+            store_b(buf ^ 1, b);  // that buffer was last read before the previous barrier
+#pragma unroll
+            for (int i = 0; i < kFullTM; ++i)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) a[i][q] = an[i][q];
+        }
+        __syncthreads();
+    }
+    if (!rows) return;
+    const uint32_t M16 = Mt * 16;
+    const uint32_t N16 = uint32_t(Nt) * 16;
+#pragma unroll
+    for (int i = 0; i < kFullTM; ++i) {
+#pragma unroll
+        for (int j = 0; j < kFullMaxTiles; ++j) {
+            if (j < Nt) {
+                const uint32_t col = uint32_t(j) * 16 + lane % 16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t row = (m0 + i) * 16 + (lane / 16) * 4 + r;
+                    partial[(size_t(split) * M16 + row) * N16 + col] = acc[i][j][r];
+                }
+            }
+        }
+    }
+}
+
 __global__ void mfma_reduce(const float* __restrict__ partial, int32_t n_split, uint32_t D, int32_t N, uint32_t M16,
                             uint32_t N16, const double* __restrict__ cal1, double* __restrict__ cal) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(D) * N) return;
     const uint32_t d = uint32_t(i / N), n = uint32_t(i % N);
     double s = 0.0;
-    for (int32_t c = 0; c < n_split; ++c) s += double(partial[(size_t(c) * M16 + d) * N16 + n]);
+#pragma unroll 8
+    for (int32_t c = 0; c < n_split; ++c) s += double(partial[(size_t(c) * M16 + d) * N16 + n]);  // split order
     cal[i] = s + cal1[i];
 }
 
@@ -231,8 +363,13 @@ void launch_split_b(hipStream_t s, const double* B, int32_t K, int32_t N, void* 
     LEAP_HIP(hipGetLastError());
 }
 
+bool mfma_full_output(uint32_t D, int32_t N) {
+    return (D + 63) / 64 * 4 <= uint32_t(kFullMaxTiles) && (N + 15) / 16 <= kFullMaxTiles;
+}
+
 int32_t mfma_n_split(int32_t K, uint32_t D, int32_t N) {
     const int32_t Ks = (K + 31) / 32;
+    if (mfma_full_output(D, N)) return std::min(Ks, kFullSplits);  // about one workgroup per CU
     const int64_t tiles = int64_t((D + 63) / 64) * ((N + 15) / 16);
     int32_t split = int32_t(std::max<int64_t>(1, 2048 / std::max<int64_t>(1, tiles)));
     return std::min(split, Ks);
@@ -256,11 +393,17 @@ void launch_gemm_bf16x3(hipStream_t s, const void* b_planes, int32_t K, int32_t 
         split_a_planes<<<dim3(uint32_t((a_total + 255) / 256)), 256, 0, s>>>(delta, D, K, Mt, Ks,
                                                                              static_cast<__bf16*>(a_planes));
     LEAP_HIP(hipGetLastError());
-    const int32_t n_split = mfma_n_split(K, D, N);
+    int32_t n_split = mfma_n_split(K, D, N);
     const int32_t ks_per = (Ks + n_split - 1) / n_split;
-    gemm_bf16x3_mfma<<<dim3(Nt, Mt / 4, n_split), 256, 0, s>>>(static_cast<const bf16x8*>(a_planes),
-                                                               static_cast<const bf16x8*>(b_planes), Mt, Nt, Ks,
-                                                               ks_per, partial);
+    n_split = (Ks + ks_per - 1) / ks_per;  // no empty splits
+    if (mfma_full_output(D, N))
+        gemm_bf16x3_full<<<dim3(n_split), kFullThreads, 0, s>>>(static_cast<const bf16x8*>(a_planes),
+                                                                    static_cast<const bf16x8*>(b_planes), Mt, Nt, Ks,
+                                                                    ks_per, partial);
+    else
+        gemm_bf16x3_mfma<<<dim3(Nt, Mt / 4, n_split), 256, 0, s>>>(static_cast<const bf16x8*>(a_planes),
+                                                                   static_cast<const bf16x8*>(b_planes), Mt, Nt, Ks,
+                                                                   ks_per, partial);
     LEAP_HIP(hipGetLastError());
     const size_t n = size_t(D) * N;
     mfma_reduce<<<dim3(uint32_t((n + 255) / 256)), 256, 0, s>>>(partial, n_split, D, N, Mt * 16, uint32_t(Nt) * 16,
