@@ -262,7 +262,8 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         alg_flops = int(R * (per_row + D * per_dir))
         achieved = alg_flops / (k_ms * 1e-3) / 1e12
         src_bytes = R * C * P * elem if route_dc else R * C * 16
-        alg_bytes = src_bytes + R * 24 + D * nbl * P * 16
+        # calibrations on this route keep pol 0 of avgData only (16 B per direction and baseline)
+        alg_bytes = src_bytes + R * 24 + D * nbl * 16
         roofline = {"kernel": "rotate_average_nufft",
                     "bound": "valu-fp64", "achieved": round(achieved, 2), "peak": VALU_FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / VALU_FP64_PEAK_TFLOPS, 4),
