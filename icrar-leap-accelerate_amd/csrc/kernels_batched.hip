@@ -42,8 +42,6 @@ template <typename A>
 __device__ inline C2<A> phasor(double theta);
 template <>
 __device__ inline C2<float> phasor<float>(double theta) { return phasor_mixed(theta); }
-template <>
-__device__ inline C2<double> phasor<double>(double theta) { return phasor_f64(theta); }
 
 // Running sum: plain for FP64, Kahan-compensated for FP32.
 template <typename A>
