@@ -266,6 +266,10 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         alg_bytes = src_bytes + R * 24 + D * nbl * 16
         roofline = {"kernel": "rotate_average_nufft",
                     "bound": "valu-fp64", "achieved": round(achieved, 2), "peak": VALU_FP64_PEAK_TFLOPS,
+                    # measured with rocprofv3 counters, not live: the CU's LDS (FFT exchanges and
+                    # the interpolation's gathers) is the tighter limit
+                    "limiter": "LDS 80% busy, FP64 VALU 71% of issue slots "
+                               "(profiles/rotate_average_nufft_c1.md)",
                     "unit": "TFLOP/s", "frac": round(achieved / VALU_FP64_PEAK_TFLOPS, 4),
                     "traffic": traffic_from_profile("rotate_average_nufft_c1_traffic.json"),
                     "kernel_ms": round(k_ms, 4), "algorithmic_flops_per_launch": alg_flops,
