@@ -151,9 +151,11 @@ def test_mixed_requires_batched(leap, ctx, small):
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 2e-6)], ids=["fp64-gemm", "bf16x3-mfma"])
-@pytest.mark.parametrize("n_dirs", [1, 16, 70])
+@pytest.mark.parametrize("n_dirs", [1, 16, 70, 300])
 def test_batched_inversion_gemm(leap, ctx, oracle, precision, tol, n_dirs):
-    """cal = Ad * dphi + cal1 as one GEMM (FP64) or split-3 x BF16 MFMA (mixed) against the oracle."""
+    """cal = Ad * dphi + cal1 as one GEMM (FP64) or split-3 x BF16 MFMA (mixed) against the oracle.
+    Up to 256 directions and antennas the MFMA product runs as whole-output workgroups; 300
+    directions take the one-wave-per-tile kernel."""
     s = leap.SynthMS(n_antennas=61, autocorrelations=0, n_timesteps=2, n_channels=4, flag_fraction=0.03, seed=40)
     s.fill_host()
     ms = s.host_dict()
