@@ -112,10 +112,29 @@ __global__ void delta_split_a_planes(const double* __restrict__ avg, uint32_t nb
     const uint32_t mt = uint32_t(t / Ks);
     const uint32_t m = mt * 16 + lane % 16;
     const int32_t kb = ks * 32 + int32_t(lane / 16) * 8;
+    // The 8 elements' loads (antenna pairs, then cal1 and avgData) all go out
+    // before any element is computed; indices past the matrix are clamped to
+    // valid ones and their results replaced by 0.
+    const uint32_t mc = m < M ? m : M - 1;
+    double acal1[8], re[8], im[8];
+    if (K > 1) {  // uniform; with K = 1 there are no baselines and every element is 0
+        int2 pr[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) pr[j] = pairs[min(kb + j, K - 2)];  // K - 1 pairs, K - 1 <= nbl
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int32_t a1 = min(max(pr[j].x, 0), N - 1), a2 = min(max(pr[j].y, 0), N - 1);
+            acal1[j] = cal1[size_t(mc) * N + a1] - cal1[size_t(mc) * N + a2];
+            const double* a = avg + (size_t(mc) * nbl + min(kb + j, K - 2)) * (2 * P);
+            re[j] = a[0];
+            im[j] = a[1];
+        }
+    }
     bf16x8 h, md, l;
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int32_t k = kb + j;
-        const double x = (m < M && k < K) ? dev::delta_phase(avg, nbl, P, pairs, K, N, cal1, m, k) : 0.0;
+        const double x = (m < M && k < K - 1) ? dev::delta_phase_of(acal1[j], re[j], im[j]) : 0.0;
         __bf16 a, b, c;
         split3(x, a, b, c);
         h[j] = a;
