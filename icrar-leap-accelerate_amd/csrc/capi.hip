@@ -374,6 +374,7 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
     const bool host = rows.vis == nullptr;
     const bool nufft = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
     const double* scale = nufft ? md.nufft_scale() : nullptr;
+    if (nufft) ctx->d_nufft_tail.reserve(nufft_tail_bytes(D));
     double* buf[2] = {ctx->d_packed.as<double>(), ctx->d_packed.as<double>() + 2 * buf_elems};
     auto pack = [&](uint32_t k, hipStream_t st) {
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
@@ -396,7 +397,8 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
             launch_rotate_nufft(ctx->stream, tile, md.nufft, md.d_nufft.as<double>(), buf[k % 2], false,
                                 rows.uvw + size_t(t0) * md.nbl * 3, ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl,
                                 md.channels, md.pols, avg_pols, md.freq_start_hz, md.freq_inc_hz,
-                                ctx->d_avg.as<double>(), k > 0 ? 1 : 0);
+                                ctx->d_avg.as<double>(), k > 0 ? 1 : 0, ctx->d_nufft_tail.as<double>(),
+                                nufft_tail_bytes(D));
         else
             launch_rotate_packed(ctx->stream, tile, buf[k % 2], rows.uvw + size_t(t0) * md.nbl * 3,
                                  ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl, md.channels, md.pols,
@@ -467,9 +469,11 @@ uint32_t rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms
         const int32_t db = tile ? tile : batched_tile(ctx, md, ms, rows, T, D, opts);
         ctx->last_direction_block = db;
         record_rotate_begin(ctx);
+        ctx->d_nufft_tail.reserve(nufft_tail_bytes(D));
         launch_rotate_nufft(ctx->stream, db, md.nufft, md.d_nufft.as<double>(), rows.vis, true, rows.uvw,
                             ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, avg_pols,
-                            md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
+                            md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), 0,
+                            ctx->d_nufft_tail.as<double>(), nufft_tail_bytes(D));
         record_rotate_end(ctx);
         return avg_pols;
     }

@@ -51,6 +51,20 @@ constexpr int kWave = 64;
 
 using AS4double = __attribute__((address_space(4))) double;
 
+// Which rows a workgroup sums: chunks == 1: workgroup x takes baseline
+// bl_base + x, all T rows; chunks > 1: workgroup x takes baseline
+// bl_base + x / chunks, rows [c T / chunks, (c + 1) T / chunks) with c = x mod
+// chunks (output mode kNufftPartial).
+struct NufftSpan {
+    uint32_t bl_base;
+    uint32_t chunks;
+};
+
+// Output modes (the kernel's mode argument): set avgData (pol 0, the other
+// pols zero), add to avgData, or write the row sums to avg[x][direction]
+// (complex; the partial-sum array of a chunked launch, summed by nufft_tail_sum).
+constexpr int32_t kNufftSet = 0, kNufftAccumulate = 1, kNufftPartial = 2;
+
 struct NufftArgs {
     const double* coef;  // device: [W / 2][kNufftMaxWidth] weight polynomials of taps j < W / 2, powers of s
     double kappa_mid;    // (f0 + h df) / c_light
@@ -423,6 +437,10 @@ struct Source {
             off[i] = min(c, C - 1) * (2 * P);
         }
     }
+    // base advanced by `rows` rows of C channels
+    __device__ static const void* advance(const void* base, size_t rows, uint32_t C) {
+        return (const float*)base + rows * C * (2 * P);
+    }
     __device__ void load(const void* base, size_t row, uint32_t, uint32_t C) {
         const LEAP_GLOBAL float* r = (const LEAP_GLOBAL float*)base + row * C * (2 * P);
 #pragma unroll
@@ -461,6 +479,9 @@ struct Source<N, 0> {
     static constexpr int CPL = N / (2 * kWave);
     Z raw[CPL];
     __device__ void init(const LEAP_GLOBAL double*, uint32_t, uint32_t) {}
+    __device__ static const void* advance(const void* base, size_t rows, uint32_t C) {
+        return (const Z*)base + rows * C;
+    }
     __device__ void load(const void* base, size_t row, uint32_t lane, uint32_t C) {
         const LEAP_GLOBAL Z* r = (const LEAP_GLOBAL Z*)base + row * C;
 #pragma unroll
@@ -484,13 +505,23 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
                                                               const LEAP_GLOBAL double* __restrict__ dd, uint32_t D,
                                                               uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
                                                               NufftArgs args, LEAP_GLOBAL double* __restrict__ avg,
-                                                              int32_t accumulate) {
+                                                              int32_t mode, NufftSpan span) {
 #pragma clang fp contract(off)
     constexpr int V = N / kWave;
     static_assert(W % 2 == 0 && W <= kNufftMaxWidth && N / 16 >= W - 1, "kernel width");
     __shared__ Z s_g[N + N / 16];
     const uint32_t lane = threadIdx.x;
-    const uint32_t bl = blockIdx.x;
+    // The span is resolved here into the row source, uvw and output pointers and
+    // a row count, so the row loop holds no more values than an unchunked one.
+    const uint32_t item = blockIdx.x;
+    const uint32_t chunk = span.chunks > 1 ? item % span.chunks : 0;
+    const uint32_t bl = span.bl_base + (span.chunks > 1 ? item / span.chunks : item);
+    const uint32_t t_begin = uint32_t(uint64_t(chunk) * T / max(span.chunks, 1u));
+    const uint32_t t_count = uint32_t(uint64_t(chunk + 1) * T / max(span.chunks, 1u)) - t_begin;
+    const void* wr = Source<N, SP>::advance(w, size_t(t_begin) * nbl, C);
+    const LEAP_GLOBAL double* uvr = uvw + size_t(t_begin) * nbl * 3;
+    LEAP_GLOBAL double* out = mode == kNufftPartial ? avg + size_t(item) * D * 2 : avg + size_t(bl) * Pout * 2;
+    const uint32_t dstride = mode == kNufftPartial ? 2u : nbl * Pout * 2;  // doubles between directions
     const uint32_t d0 = blockIdx.y * (kWave * NDL);
     double dz[NDL][3];
 #pragma unroll
@@ -513,16 +544,16 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     // registers from one iteration to the next.
     Source<N, SP> src;
     src.init(scale, lane, C);
-    if (T > 0) src.load(w, bl, lane, C);
-    for (uint32_t t = 0; t < T; ++t) {
+    if (t_count > 0) src.load(wr, bl, lane, C);
+    for (uint32_t t = 0; t < t_count; ++t) {
         Z v[V];
         src.freq(v, lane, C);
 #pragma unroll
         for (int m = V / 2; m < V; ++m) v[m] = Z{0.0, 0.0};  // padding (not read by the first pass)
         __builtin_amdgcn_sched_barrier(0);
         const size_t row = size_t(t) * nbl + bl;
-        src.load(w, size_t(t + 1 < T ? t + 1 : t) * nbl + bl, lane, C);
-        const double u = uvw[3 * row], vv = uvw[3 * row + 1], ww = uvw[3 * row + 2];
+        src.load(wr, size_t(t + 1 < t_count ? t + 1 : t) * nbl + bl, lane, C);
+        const double u = uvr[3 * row], vv = uvr[3 * row + 1], ww = uvr[3 * row + 2];
         tw.hide();
         row_fft<N, W>(v, s_g, tw, phi, lane);
         // Interpolation.  Per direction: b = sf dkappa, grid coordinate
@@ -640,8 +671,11 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     for (int j = 0; j < NDL; ++j) {
         const uint32_t d = d0 + ln + kWave * j;
         if (d >= D) continue;
-        LEAP_GLOBAL double* o = avg + (size_t(d) * nbl + bl) * Pout * 2;
-        if (accumulate) {
+        LEAP_GLOBAL double* o = out + size_t(d) * dstride;
+        if (mode == kNufftPartial) {
+            o[0] = sum[j].x;
+            o[1] = sum[j].y;
+        } else if (mode == kNufftAccumulate) {
             o[0] += sum[j].x;
             o[1] += sum[j].y;
         } else {
@@ -652,26 +686,89 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     }
 }
 
+// avgData of the chunked baselines: the chunks' partial sums added in chunk
+// order (each chunk summed its rows in row order).
+__global__ void nufft_tail_sum(const double* __restrict__ partial, uint32_t rem, uint32_t chunks, uint32_t D,
+                               uint32_t nbl, uint32_t bl_base, uint32_t Pout, double* __restrict__ avg) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(rem) * D) return;
+    const uint32_t r = uint32_t(i / D), d = uint32_t(i % D);
+    double x = 0.0, y = 0.0;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const double* p = partial + ((size_t(r) * chunks + c) * D + d) * 2;
+        x += p[0];
+        y += p[1];
+    }
+    double* o = avg + (size_t(d) * nbl + bl_base + r) * Pout * 2;
+    o[0] = x;
+    o[1] = y;
+    for (uint32_t q = 2; q < 2 * Pout; ++q) o[q] = 0.0;
+}
+
+// Workgroups of one NUFFT kernel resident on the whole device at once.
+template <class K>
+uint32_t resident_workgroups(K kernel) {
+    int per_cu = 0, dev = 0, cus = 0;
+    LEAP_HIP(hipGetDevice(&dev));
+    LEAP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    LEAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0));
+    return uint32_t(std::max(1, per_cu) * std::max(1, cus));
+}
+
+// One launch of rotate_average_nufft<N, W, NDL, SP> over nbl_grid baselines.
+// When a single direction block covers D and the baselines do not fill whole
+// rounds of resident workgroups, the whole rounds run as usual and the
+// remaining baselines' rows are split into chunks so that the last round is
+// full (and short): their partial sums go to the tail scratch and are added by
+// nufft_tail_sum.  Not for accumulating launches.
+template <int N, int W, int NDL, int SP>
+void launch_nufft_tile(hipStream_t s, uint32_t nbl_grid, const void* w, const LEAP_GLOBAL double* gs,
+                       const LEAP_GLOBAL double* gu, const LEAP_GLOBAL double* gd, uint32_t D, uint32_t T, uint32_t nbl,
+                       uint32_t C, uint32_t Pout, const NufftArgs& a, LEAP_GLOBAL double* ga, int32_t accumulate,
+                       double* tail, size_t tail_doubles) {
+    auto* kernel = rotate_average_nufft<N, W, NDL, SP>;
+    const uint32_t ny = (D + kWave * NDL - 1) / (kWave * NDL);
+    if (tail && !accumulate && ny == 1 && T >= 2) {
+        static const uint32_t slots = resident_workgroups(kernel);
+        const uint32_t full = nbl_grid / slots * slots, rem = nbl_grid - full;
+        const uint32_t chunks = rem ? std::min(T, (slots + rem - 1) / rem) : 0;
+        if (rem && chunks >= 2 && size_t(rem) * chunks * D * 2 <= tail_doubles) {
+            if (full)
+                kernel<<<dim3(full, 1), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, kNufftSet,
+                                                       NufftSpan{0, 1});
+            kernel<<<dim3(rem * chunks, 1), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a,
+                                                           (LEAP_GLOBAL double*)tail, kNufftPartial,
+                                                           NufftSpan{full, chunks});
+            const size_t n = size_t(rem) * D;
+            nufft_tail_sum<<<dim3(uint32_t((n + 255) / 256)), 256, 0, s>>>(tail, rem, chunks, D, nbl, full, Pout,
+                                                                        (double*)ga);
+            return;
+        }
+    }
+    kernel<<<dim3(nbl_grid, ny), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a, ga,
+                                               accumulate ? kNufftAccumulate : kNufftSet, NufftSpan{0, 1});
+}
+
 template <int N, int W, int SP>
 void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
                 const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout, const NufftArgs& a,
-                double* avg, int32_t accumulate) {
+                double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
     const auto* gs = (const LEAP_GLOBAL double*)scale;
     const auto* gu = (const LEAP_GLOBAL double*)uvw;
     const auto* gd = (const LEAP_GLOBAL double*)dd;
     auto* ga = (LEAP_GLOBAL double*)avg;
     switch (tile) {
         case 641:
-            rotate_average_nufft<N, W, 1, SP><<<dim3(nbl_grid, (D + 63) / 64), kWave, 0, s>>>(
-                w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate);
+            launch_nufft_tile<N, W, 1, SP>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
+                                           tail_doubles);
             break;
         case 1282:
-            rotate_average_nufft<N, W, 2, SP><<<dim3(nbl_grid, (D + 127) / 128), kWave, 0, s>>>(
-                w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate);
+            launch_nufft_tile<N, W, 2, SP>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
+                                           tail_doubles);
             break;
         case 2564:
-            rotate_average_nufft<N, W, 4, SP><<<dim3(nbl_grid, (D + 255) / 256), kWave, 0, s>>>(
-                w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate);
+            launch_nufft_tile<N, W, 4, SP>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
+                                           tail_doubles);
             break;
         default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported NUFFT tile " + std::to_string(tile));
     }
@@ -680,17 +777,29 @@ void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, c
 template <int N, int W>
 void launch_nw(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, int32_t src_pols, const double* scale,
                const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
-               const NufftArgs& a, double* avg, int32_t accumulate) {
+               const NufftArgs& a, double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
     switch (src_pols) {
-        case 0: launch_nws<N, W, 0>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
-        case 1: launch_nws<N, W, 1>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
-        case 2: launch_nws<N, W, 2>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
-        case 4: launch_nws<N, W, 4>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate); break;
+        case 0: launch_nws<N, W, 0>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
+                break;
+        case 1: launch_nws<N, W, 1>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
+                break;
+        case 2: launch_nws<N, W, 2>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
+                break;
+        case 4: launch_nws<N, W, 4>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
+                break;
         default: throw Error(LEAP_ERR_INTERNAL, "unsupported NUFFT row source");
     }
 }
 
 }  // namespace
+
+size_t nufft_tail_bytes(uint32_t D) {
+    int dev = 0, cus = 0;
+    LEAP_HIP(hipGetDevice(&dev));
+    LEAP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // resident workgroups <= 16 per CU for these kernels; the chunked round holds < 2x that
+    return 2 * size_t(16) * size_t(std::max(1, cus)) * D * 2 * sizeof(double);
+}
 
 bool nufft_cube_supported(int32_t vis_dtype, uint32_t P) {
     return vis_dtype == LEAP_VIS_C64 && (P == 1 || P == 2 || P == 4);
@@ -699,7 +808,7 @@ bool nufft_cube_supported(int32_t vis_dtype, uint32_t P) {
 void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
                          bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
                          uint32_t nbl_grid, uint32_t C, uint32_t P, uint32_t P_out, double f0, double df,
-                         double* avg, int32_t accumulate) {
+                         double* avg, int32_t accumulate, double* tail, size_t tail_bytes) {
     if (nbl_grid == 0 || D == 0) return;
     require(P_out >= 1, LEAP_ERR_INTERNAL, "NUFFT output needs at least pol 0");
     require(plan.N != 0 && plan.scale.size() == C, LEAP_ERR_INTERNAL, "NUFFT plan does not match the channel count");
@@ -714,7 +823,7 @@ void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, con
 #define LEAP_NUFFT(N_, W_)                                                                                   \
     if (plan.N == N_ && plan.W == W_) {                                                                      \
         launch_nw<N_, W_>(s, tile, nbl_grid, src, sp, scale, uvw, dd, D, T, nbl, C, P_out, a, avg,           \
-                          accumulate);                                                                       \
+                          accumulate, tail, tail_bytes / sizeof(double));                                    \
         LEAP_HIP(hipGetLastError());                                                                         \
         return;                                                                                              \
     }

@@ -203,6 +203,7 @@ struct leap_ctx {
     leap::DeviceBuffer d_widen[2];  // complex64 pol-0 chunks from the host, widened into d_packed
     leap::DeviceBuffer d_uvw, d_vis_stage, d_dirs, d_dd, d_avg, d_phase, d_cal1, d_delta, d_cal,
         d_partial, d_a_planes;
+    leap::DeviceBuffer d_nufft_tail;  // partial sums of the NUFFT launch's chunked last round
     std::vector<double> h_avg, h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;
@@ -268,7 +269,10 @@ bool nufft_cube_supported(int32_t vis_dtype, uint32_t P);
 void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
                          bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
                          uint32_t nbl_grid, uint32_t C, uint32_t P, uint32_t P_out, double f0, double df,
-                         double* avg, int32_t accumulate);
+                         double* avg, int32_t accumulate, double* tail = nullptr, size_t tail_bytes = 0);
+// Scratch for launch_rotate_nufft's chunked last round (D directions, a single
+// direction block): at most twice the resident workgroups x D complex sums.
+size_t nufft_tail_bytes(uint32_t D);
 bool packed_tile_supported(int32_t tile);
 void launch_rotate_packed(hipStream_t s, int32_t tile, const double* w, const double* uvw, const double* dd,
                           uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,
