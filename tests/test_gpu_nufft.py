@@ -70,6 +70,31 @@ def test_nufft_rotate_matches_oracle(leap, ctx, oracle, n_ch, n_dirs, tile):
     print(f"  worst error / bound {worst:.3f}")
 
 
+def test_nufft_whole_rounds_and_chunked_tail(leap, ctx, oracle):
+    # 64 antennas with autocorrelations = 2,080 baselines: more than one round of
+    # resident workgroups on a 256-CU part (2,048 at two waves per SIMD), so the
+    # launch runs whole rounds of baselines plus a few baselines split into row
+    # chunks whose partial sums are added afterwards.
+    print("nufft rotate: 64 antennas (2,080 baselines), 6 timesteps, 160 channels", flush=True)
+    s = leap.SynthMS(n_antennas=64, n_timesteps=6, n_channels=160, freq_start_hz=50e6, freq_inc_hz=781.25e3,
+                     layout=1, array_radius_m=20000.0, nan_fraction=0.002, seed=218)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(np.complex64))
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    omd = oracle.metadata(ms)
+    n_dirs = 9
+    dirs = dirs_near(ms["phase_centre"], n_dirs, 6, spread=0.1)
+    got = md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=NUFFT)))
+    assert got.shape[1] == 2080
+    bound = 5e-10 * abs_mass(ms)
+    for d in range(n_dirs):
+        want = omd.rotate(dirs[d])[:, 0]
+        err = np.abs(got[d, :, 0] - want)
+        assert np.all(err <= bound), (d, int(err.argmax()), (err / bound).max())
+        assert not np.any(got[d, :, 1:])
+
+
 @pytest.mark.parametrize("n_ch,n_dirs,n_t,n_pols", [(1, 1, 3, 4), (7, 5, 1, 2), (64, 65, 2, 1), (512, 3, 2, 4)])
 def test_nufft_edge_shapes_match_oracle(leap, ctx, oracle, n_ch, n_dirs, n_t, n_pols):
     """Smallest and largest channel counts, one timestep, one direction and
