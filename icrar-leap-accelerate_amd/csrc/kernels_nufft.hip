@@ -733,12 +733,17 @@ void launch_nufft_tile(hipStream_t s, uint32_t nbl_grid, const void* w, const LE
         const uint32_t full = nbl_grid / slots * slots, rem = nbl_grid - full;
         const uint32_t chunks = rem ? std::min(T, (slots + rem - 1) / rem) : 0;
         if (rem && chunks >= 2 && size_t(rem) * chunks * D * 2 <= tail_doubles) {
-            if (full)
+            // Each launch is checked on its own: a later successful launch may
+            // reset the error state of an earlier failed one.
+            if (full) {
                 kernel<<<dim3(full, 1), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, kNufftSet,
                                                        NufftSpan{0, 1});
+                LEAP_HIP(hipGetLastError());
+            }
             kernel<<<dim3(rem * chunks, 1), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a,
                                                            (LEAP_GLOBAL double*)tail, kNufftPartial,
                                                            NufftSpan{full, chunks});
+            LEAP_HIP(hipGetLastError());
             const size_t n = size_t(rem) * D;
             nufft_tail_sum<<<dim3(uint32_t((n + 255) / 256)), 256, 0, s>>>(tail, rem, chunks, D, nbl, full, Pout,
                                                                         (double*)ga);
