@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "icrar/leap-accelerate/algorithm/LeapCalibratorFactory.h"
+#include "icrar/leap-accelerate/common/SphericalDirection.h"
 #include "icrar/leap-accelerate/ms/MeasurementSet.h"
 #include "icrar/leap-accelerate/ms/RawMeasurementSet.h"
 
@@ -63,6 +64,26 @@ static int selftest() {
         LeapCalibratorFactory::Create(ComputeImplementation::cuda);
         ++bad;
     } catch (const invalid_argument_exception&) {
+    }
+    // ParseDirections accept / reject cases of leap-accelerate-cli/tests/JSONHelperTests.cc:59-90.
+    // The one-direction case there compares against a casacore MVDirection round trip of
+    // the same two numbers; here the parsed values are compared with the numbers themselves.
+    if (!ParseDirections("[]").empty()) ++bad;
+    for (const char* reject : {"{}", "[[]]", "[[1.0]]", "[[1.0,1.0,1.0]]", "[1.0,2.0]", "[[\"a\",1.0]]"}) {
+        try {
+            ParseDirections(std::string(reject));
+            std::cerr << "ParseDirections accepted " << reject << "\n";
+            ++bad;
+        } catch (const icrar::exception&) {
+        }
+    }
+    {
+        const auto one = ParseDirections("[[-0.4606549305661674,-0.29719233792392513]]");
+        if (one.size() != 1 || !(one[0] == SphericalDirection(-0.4606549305661674, -0.29719233792392513))) ++bad;
+        const auto five = ParseDirections("[[0.0,0.0],[1.0,1.0],[2.0,2.0],[3.0,3.0],[4.0,4.0]]");
+        if (five.size() != 5) ++bad;
+        for (size_t i = 0; i < five.size(); ++i)
+            if (!(five[i] == SphericalDirection(double(i), double(i)))) ++bad;
     }
     cpu::CalibrationCollection cc({cpu::Calibration(1.0, 2.0, {cpu::BeamCalibration({0.5, -0.25}, {0.0, 1.5})})});
     std::ostringstream os;
