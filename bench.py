@@ -133,10 +133,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (the headline's batched leg runs steps/4)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="headline", choices=["headline", "plain", "batched-fp64", "batched-mixed"],
+    ap.add_argument("--mode", default="headline",
+                    choices=["headline", "plain", "batched-fp64", "batched-mixed", "adjoint"],
                     help="headline: batched-mixed (value) measured beside plain; plain: FP64 per direction on "
                          "the complex128 cube; batched-fp64: FP64, one pass per block of directions on the "
-                         "complex128 cube; batched-mixed: mixed precision on the complex64 cube")
+                         "complex128 cube; batched-mixed: mixed precision on the complex64 cube; adjoint: "
+                         "calibration-residual loss and its gradients (128 stations, 16 directions)")
     ap.add_argument("--directions", type=int, default=0, help="directions per step (default 1 plain, 64 "
                     "batched-fp64, 256 batched-mixed)")
     ap.add_argument("--direction-block", type=int, default=0, help="0 = autotuned")
@@ -165,6 +167,9 @@ def main():
               "changing a GPU's kernel).", file=sys.stderr)
         return 2
 
+    if args.mode == "adjoint":
+        print(json.dumps(run_adjoint(args)), flush=True)
+        return 0
     if args.mode == "headline":
         plain = run_mode(args, "plain", 1, with_cpu=False)
         mixed = run_mode(args, "batched-mixed", args.directions or 256, with_cpu=args.cpu_timesteps > 0,
@@ -363,6 +368,88 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
         **({"host_cube": host_note} if host_note else {}),
+    }
+
+
+def run_adjoint(args):
+    """Loss + gradient evaluations of leap_hip_residual_loss_grad: 128 stations without
+    autocorrelations (8128 baselines), 64 timesteps of 2 s, 192 channels, complex64 cube generated
+    on the device, 16 directions, direction offsets N(0, 1e-4 rad), antenna position corrections
+    N(0, 0.01 m).  A step is one forward (rotate, average, inversion, loss) and backward (adjoint
+    of the inversion, then of rotate/average with the phasors recomputed) pass."""
+    cfg = dict(n_antennas=128, autocorrelations=0, n_timesteps=64, n_channels=192, seed=4)
+    D = args.directions or 16
+    synth = la.SynthMS(**cfg)
+    nbl, R, C, P = synth.num_baselines, synth.num_rows, synth.num_channels, synth.num_pols
+    ctx = la.Context(0)
+    vis_ptr = ctx.malloc(R * C * P * 8)
+    uvw_ptr = ctx.malloc(R * 3 * 8)
+    ctx.synth_fill_device(synth, la.LEAP_VIS_C64, vis_ptr, uvw_ptr)
+    view = la.view_from_arrays(synth.host_dict(), vis_device=vis_ptr, vis_dtype=la.LEAP_VIS_C64, uvw_device=uvw_ptr)
+    rng = np.random.default_rng(4)
+    centre = (view.phase_centre_ra_rad, view.phase_centre_dec_rad)
+    dirs = directions(centre, D, seed=4)
+    off = rng.normal(0, 1e-4, (D, 2))
+    md = ctx.metadata(view)
+    dp = rng.normal(0, 0.01, (md.N, 3))
+    M = synth.enu_to_uvw
+    for i in range(args.warmup):
+        md.loss_grad(view, dirs, M, off, dp)
+        log(f"[adjoint] warmup step {i + 1}/{args.warmup} done")
+    steps = args.steps
+    t = time.perf_counter()
+    for _ in range(steps):
+        md.loss_grad(view, dirs, M, off, dp)  # returns host values: every step ends synchronised
+    elapsed = time.perf_counter() - t
+    t = time.perf_counter()
+    for _ in range(steps):
+        md.loss_grad(view, dirs, M, off, dp, want_grad=False)
+    fwd = time.perf_counter() - t
+    log(f"[adjoint] {steps} timed steps in {elapsed:.3f} s (forward only {fwd:.3f} s)")
+    cpu = None
+    if args.cpu_timesteps > 0:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        from oracle_bindings import Oracle  # the checker, used only for this reported baseline
+
+        ts = args.cpu_timesteps
+        rows = nbl * ts
+        host = np.zeros((rows, C, P), dtype=np.complex64)
+        ctx.d2h(host, vis_ptr)
+        ms = dict(synth.host_dict(host))
+        ms["num_rows"] = rows
+        ms["uvw"] = synth.uvw[:rows]
+        ms["time"] = synth.time[:rows]
+        t = time.perf_counter()
+        Oracle().loss_grad(ms, dirs, M, off, dp)
+        t_cpu = time.perf_counter() - t
+        scale = cfg["n_timesteps"] / ts
+        cpu = {"value": 1.0 / (t_cpu * scale), "unit": "evaluations/s", "cores": 1, "kind": "port",
+               "sample": (f"oracle loss and explicit gradient (FP64 C++ restatement, 1 thread) on {ts} of "
+                          f"{cfg['n_timesteps']} timesteps, {D} directions, metadata included: {t_cpu:.2f} s "
+                          f"scaled x{scale:g} in T")}
+    ctx.free(vis_ptr)
+    ctx.free(uvw_ptr)
+    ctx.close()
+    return {
+        "metric": "calibration-residual loss + gradient evaluations/s (128 st, 8128 bl, 64 t, 192 ch, 16 dirs)",
+        "value": round(steps / elapsed, 3),
+        "unit": "evaluations/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 storage, f64 arithmetic",
+        "data": "synthetic (seeded point sources x per-antenna phase screens + noise, generated on the device)",
+        "config": {"workload": f"loss and gradients w.r.t. {D} direction offsets and 128 x 3 antenna positions; "
+                               "128 stations / 8128 baselines (no autos) / 64 t / 192 ch / 4 pol, complex64 cube "
+                               "resident in HBM",
+                   "stations": 128, "baselines": nbl, "timesteps": 64, "channels": C, "pols": P,
+                   "directions": D, "forward_only_ms": round(fwd / steps * 1e3, 3)},
+        "roofline": None,
+        "cpu_baseline": cpu,
     }
 
 

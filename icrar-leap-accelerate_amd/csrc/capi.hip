@@ -4,8 +4,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <sstream>
+#include <thread>
 
 #include "leap_internal.h"
 #include "phasor.h"
@@ -1586,20 +1588,35 @@ void dz_derivs(double ra0, double dec0, double ra, double dec, double dra[3], do
     ddec[2] = -c3 * c2 * s1 - s3 * c1;
 }
 
-// Ad(i, k) from the Gram pseudo-inverse: column k of pinv(A) = G+ (A row k)^T.
-struct AdView {
-    const leap::PhaseMatrix& pm;
-    const std::vector<double>& Gp;
-    int32_t N;
-    const double* col_a(int32_t k) const {
-        return Gp.data() + size_t(k + 1 < pm.rows ? pm.ant1[k] : pm.constraint) * N;
+// Products with Ad = pinv(A) = G+ A^T (G+ the N x N column-major pseudo-inverse of the Gram
+// matrix A^T A): Ad v = G+ (A^T v) and Ad^T g = A (G+^T g), O(K + N^2) each instead of the
+// O(K N) of a dense Ad.  A row k < K - 1 is +1 at ant1[k] and -1 at ant2[k]; the last row is +1
+// at the constraint column.
+void add_ad_times(const leap::PhaseMatrix& pm, const std::vector<double>& Gp, const double* v, double* out) {
+    const int32_t N = pm.cols, K = pm.rows;
+    std::vector<double> x(size_t(N), 0.0);
+    for (int32_t k = 0; k + 1 < K; ++k) {
+        x[pm.ant1[k]] += v[k];
+        x[pm.ant2[k]] -= v[k];
     }
-    const double* col_b(int32_t k) const { return k + 1 < pm.rows ? Gp.data() + size_t(pm.ant2[k]) * N : nullptr; }
-    double at(int32_t i, int32_t k) const {
-        const double* b = col_b(k);
-        return col_a(k)[i] - (b ? b[i] : 0.0);
+    if (K > 0) x[pm.constraint] += v[K - 1];
+    for (int32_t a = 0; a < N; ++a) {
+        const double* col = Gp.data() + size_t(a) * N;
+        for (int32_t i = 0; i < N; ++i) out[i] += col[i] * x[a];
     }
-};
+}
+
+// y = G+^T g: (Ad^T g)[k] = y[ant1[k]] - y[ant2[k]] for k < K - 1.
+std::vector<double> gram_t_times(const std::vector<double>& Gp, int32_t N, const double* g) {
+    std::vector<double> y(size_t(N), 0.0);
+    for (int32_t a = 0; a < N; ++a) {
+        const double* col = Gp.data() + size_t(a) * N;
+        double t = 0.0;
+        for (int32_t i = 0; i < N; ++i) t += col[i] * g[i];
+        y[a] = t;
+    }
+    return y;
+}
 
 // Forward inversion, loss and its adjoint for one direction (pol 0 of avgData).
 // Returns the loss; abar[b] receives dL/dRe avg + i dL/dIm avg.
@@ -1607,18 +1624,15 @@ double inversion_loss_adjoint(const leap_metadata& md, const cplx* avg, cplx* ab
     const leap::PhaseMatrix& A = md.A;
     const leap::PhaseMatrix& A1 = md.A1;
     const int32_t N = A.cols, K = A.rows, K1 = A1.rows, nI = K - 1;
-    const AdView Ad{A, md.Gp, N}, Ad1{A1, md.G1p, N};
     std::vector<double> phi1(size_t(K1), 0.0), cal1(size_t(N), 0.0);
     for (int32_t k = 0; k + 1 < K1; ++k) phi1[k] = std::arg(avg[A1.I[k]]);
-    for (int32_t k = 0; k < K1; ++k)
-        for (int32_t i = 0; i < N; ++i) cal1[i] += Ad1.at(i, k) * phi1[k];
+    add_ad_times(A1, md.G1p, phi1.data(), cal1.data());
     std::vector<double> dphi(size_t(K), 0.0), cal(cal1);
     for (int32_t n = 0; n < nI; ++n) {
         const double acal1 = cal1[A.ant1[n]] - cal1[A.ant2[n]];
         dphi[n] = std::arg(std::exp(cplx(0.0, -kTwoPi * acal1)) * avg[n]);  // row n of avgData
     }
-    for (int32_t k = 0; k < K; ++k)
-        for (int32_t i = 0; i < N; ++i) cal[i] += Ad.at(i, k) * dphi[k];
+    add_ad_times(A, md.Gp, dphi.data(), cal.data());
     double loss = 0.0;
     std::vector<double> res(static_cast<size_t>(nI));
     for (int32_t n = 0; n < nI; ++n) {
@@ -1634,19 +1648,16 @@ double inversion_loss_adjoint(const leap_metadata& md, const cplx* avg, cplx* ab
         g_cal[A.ant2[n]] -= 2.0 * res[n];
     }
     std::vector<double> g_cal1(g_cal);
+    const std::vector<double> y = gram_t_times(md.Gp, N, g_cal.data());
     for (int32_t n = 0; n < nI; ++n) {
-        double g = 0.0;  // (Ad^T g_cal)[n]
-        for (int32_t i = 0; i < N; ++i) g += Ad.at(i, n) * g_cal[i];
+        const double g = y[A.ant1[n]] - y[A.ant2[n]];  // (Ad^T g_cal)[n]
         g_arg[n] += g;
         const double g_acal1 = -kTwoPi * g;
         g_cal1[A.ant1[n]] += g_acal1;
         g_cal1[A.ant2[n]] -= g_acal1;
     }
-    for (int32_t k = 0; k + 1 < K1; ++k) {
-        double g = 0.0;
-        for (int32_t i = 0; i < N; ++i) g += Ad1.at(i, k) * g_cal1[i];
-        g_arg[A1.I[k]] += g;
-    }
+    const std::vector<double> y1 = gram_t_times(md.G1p, N, g_cal1.data());
+    for (int32_t k = 0; k + 1 < K1; ++k) g_arg[A1.I[k]] += y1[A1.ant1[k]] - y1[A1.ant2[k]];
     for (uint32_t b = 0; b < nbl; ++b)
         abar[b] = (avg[b] == cplx(0, 0)) ? cplx(0, 0) : cplx(0.0, g_arg[b]) / std::conj(avg[b]);
     return loss;
@@ -1698,19 +1709,41 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         std::vector<double> h_avg(per_dir * D);
         LEAP_HIP(hipMemcpyAsync(h_avg.data(), ctx->d_avg.ptr, h_avg.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         LEAP_HIP(hipStreamSynchronize(s));
-        // inversion, loss and its adjoint on the host (O(D K N))
-        std::vector<double> h_abar(size_t(D) * nbl * 2);
-        std::vector<cplx> avg0(nbl), ab(nbl);
-        double loss = 0.0;
-        for (uint32_t d = 0; d < D; ++d) {
-            for (uint32_t b = 0; b < nbl; ++b)
-                avg0[b] = cplx(h_avg[d * per_dir + b * md->pols * 2], h_avg[d * per_dir + b * md->pols * 2 + 1]);
-            loss += inversion_loss_adjoint(*md, avg0.data(), ab.data());
-            for (uint32_t b = 0; b < nbl; ++b) {
-                h_abar[2 * (size_t(d) * nbl + b)] = ab[b].real();
-                h_abar[2 * (size_t(d) * nbl + b) + 1] = ab[b].imag();
+        // inversion, loss and its adjoint on the host, directions split over threads; the
+        // per-direction losses are added in direction order
+        std::vector<double> h_abar(size_t(D) * nbl * 2), dir_loss(D, 0.0);
+        auto invert_range = [&](uint32_t d_begin, uint32_t d_end) {
+            std::vector<cplx> avg0(nbl), ab(nbl);
+            for (uint32_t d = d_begin; d < d_end; ++d) {
+                for (uint32_t b = 0; b < nbl; ++b)
+                    avg0[b] = cplx(h_avg[d * per_dir + b * md->pols * 2], h_avg[d * per_dir + b * md->pols * 2 + 1]);
+                dir_loss[d] = inversion_loss_adjoint(*md, avg0.data(), ab.data());
+                for (uint32_t b = 0; b < nbl; ++b) {
+                    h_abar[2 * (size_t(d) * nbl + b)] = ab[b].real();
+                    h_abar[2 * (size_t(d) * nbl + b) + 1] = ab[b].imag();
+                }
             }
+        };
+        const uint32_t workers = std::min<uint32_t>(D, std::clamp(std::thread::hardware_concurrency(), 1u, 16u));
+        if (workers <= 1) {
+            invert_range(0, D);
+        } else {
+            std::vector<std::thread> pool;
+            std::vector<std::exception_ptr> errors(workers);
+            for (uint32_t w = 0; w < workers; ++w)
+                pool.emplace_back([&, w] {
+                    try {
+                        invert_range(uint32_t(uint64_t(w) * D / workers), uint32_t(uint64_t(w + 1) * D / workers));
+                    } catch (...) {
+                        errors[w] = std::current_exception();
+                    }
+                });
+            for (auto& t : pool) t.join();
+            for (auto& e : errors)
+                if (e) std::rethrow_exception(e);
         }
+        double loss = 0.0;
+        for (uint32_t d = 0; d < D; ++d) loss += dir_loss[d];
         *loss_out = loss;
         if (!grad_offsets && !grad_antenna) return;
         // adjoint of rotate -> average on the device

@@ -159,13 +159,21 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
     }
 }
 
-// g_dz[d][k] = sum over baselines of g_dz_partial[b][d][k]
-__global__ void reduce_gdz_kernel(const double* __restrict__ part, uint32_t nbl, uint32_t D, double* __restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= D * 3) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < nbl; ++b) s += part[size_t(b) * D * 3 + i];
-    out[i] = s;
+// g_dz[d][k] = sum over baselines of g_dz_partial[b][d][k]: one workgroup per output
+// i = 3 d + k, each lane a strided slice of the baselines, then a fixed-order tree in LDS.
+__global__ __launch_bounds__(256) void reduce_gdz_kernel(const double* __restrict__ part, uint32_t nbl, uint32_t D,
+                                                         double* __restrict__ out) {
+    __shared__ double s_red[256];
+    const uint32_t i = blockIdx.x, n_out = 3 * D;
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < nbl; b += 256) acc += part[size_t(b) * n_out + i];
+    s_red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) s_red[threadIdx.x] += s_red[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = s_red[0];
 }
 
 // One workgroup per antenna: sum over its baselines (sign +1 as antenna2, -1 as
@@ -230,7 +238,7 @@ void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, co
 }
 
 void launch_reduce_gdz(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double* out) {
-    reduce_gdz_kernel<<<(D * 3 + 255) / 256, 256, 0, s>>>(part, nbl, D, out);
+    reduce_gdz_kernel<<<D * 3, 256, 0, s>>>(part, nbl, D, out);
     LEAP_HIP(hipGetLastError());
 }
 
