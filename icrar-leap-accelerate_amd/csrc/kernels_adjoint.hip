@@ -18,7 +18,6 @@ namespace leap {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kCC = 256;  // channels per LDS chunk
 
 struct Cx {
     double x, y;
@@ -51,13 +50,16 @@ __global__ void perturb_uvw_kernel(const double* __restrict__ uvw0, const double
     }
 }
 
-// One workgroup per baseline, one lane per direction (D <= 256), 256/DL rows side by side.
+// One workgroup per baseline, one lane per direction (D <= 256), 256/DL rows side by side,
+// channels staged through LDS kCC at a time (at most 32 KB of visibilities per workgroup).
+// The Horner recurrences run over the chunks in order, so the chunk size does not change the result.
 template <int DL>
 __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
     const void* __restrict__ vis, int32_t c64, const double* __restrict__ uvw, const double* __restrict__ dd,
     const double* __restrict__ abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0,
     double dkappa, double* __restrict__ g_uvw, double* __restrict__ g_dz_partial) {
     constexpr int RP = kThreads / DL;
+    constexpr int kCC = DL == 16 ? 128 : 256;  // channels per LDS chunk
     __shared__ Cx s_v[RP][kCC];
     __shared__ double s_g[RP][DL][3];
     const uint32_t bl = blockIdx.x;
@@ -222,7 +224,10 @@ void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, co
     const int32_t c64 = vis_dtype == LEAP_VIS_C64 ? 1 : 0;
     require(D >= 1 && D <= 256, LEAP_ERR_NOT_IMPLEMENTED, "the adjoint handles 1 to 256 directions per call");
     const dim3 grid(nbl);
-    if (D <= 32)
+    if (D <= 16)
+        rotate_adjoint_kernel<16><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, dkappa,
+                                                            g_uvw, g_dz_partial);
+    else if (D <= 32)
         rotate_adjoint_kernel<32><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, dkappa,
                                                             g_uvw, g_dz_partial);
     else if (D <= 64)

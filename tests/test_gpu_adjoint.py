@@ -24,8 +24,11 @@ def problem(leap, n_ant, T, C, D, seed, autos=1, dtype=np.complex128):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-def test_adjoint_matches_oracle_gradient(leap, ctx, oracle, dtype):
-    s, ms, dirs, off, dp = problem(leap, 12, 4, 40, 5, 21, dtype=dtype)
+@pytest.mark.parametrize("n_dirs,n_ch", [(5, 40), (16, 300), (20, 300), (70, 130)])
+def test_adjoint_matches_oracle_gradient(leap, ctx, oracle, dtype, n_dirs, n_ch):
+    # direction lanes per row 16 / 16 / 32 / 128; several LDS channel chunks at 130 and 300 channels
+    print(f"adjoint vs oracle: {n_dirs} directions, {n_ch} channels, {np.dtype(dtype).name}", flush=True)
+    s, ms, dirs, off, dp = problem(leap, 12, 4, n_ch, n_dirs, 21, dtype=dtype)
     view = leap.view_from_arrays(ms)
     md = ctx.metadata(view)
     L, g_off, g_ant = md.loss_grad(view, dirs, s.enu_to_uvw, off, dp)
