@@ -1683,7 +1683,7 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         upload_directions(ctx, *md, dirs.data(), D);
         const DeviceRows rows = stage_rows(ctx, *ms, 0, R);
         // perturbed uvw
-        DeviceBuffer d_M, d_ants, d_dp, d_uvwp;
+        DeviceBuffer &d_M = ctx->d_adj_M, &d_ants = ctx->d_adj_ants, &d_dp = ctx->d_adj_dp, &d_uvwp = ctx->d_adj_uvw;
         d_M.reserve(size_t(T) * 9 * sizeof(double));
         LEAP_HIP(hipMemcpyAsync(d_M.ptr, enu_to_uvw, size_t(T) * 9 * sizeof(double), hipMemcpyHostToDevice, s));
         const double* uvw = rows.uvw;
@@ -1706,8 +1706,12 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         launch_rotate_batched(s, LEAP_PRECISION_FP64, tile, false, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(),
                               D, T, nbl, nbl, md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
                               ctx->d_avg.as<double>(), 0);
-        std::vector<double> h_avg(per_dir * D);
-        LEAP_HIP(hipMemcpyAsync(h_avg.data(), ctx->d_avg.ptr, h_avg.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        // pol 0 of avgData, gathered on the device (the avgData adjoint buffer is free until the backward)
+        DeviceBuffer& d_avg0 = ctx->d_adj_abar;
+        d_avg0.reserve(size_t(D) * nbl * 2 * sizeof(double));
+        launch_gather_pol0(s, ctx->d_avg.as<double>(), D * nbl, md->pols, d_avg0.as<double>());
+        std::vector<double> h_avg0(size_t(D) * nbl * 2);
+        LEAP_HIP(hipMemcpyAsync(h_avg0.data(), d_avg0.ptr, h_avg0.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         LEAP_HIP(hipStreamSynchronize(s));
         // inversion, loss and its adjoint on the host, directions split over threads; the
         // per-direction losses are added in direction order
@@ -1716,7 +1720,7 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
             std::vector<cplx> avg0(nbl), ab(nbl);
             for (uint32_t d = d_begin; d < d_end; ++d) {
                 for (uint32_t b = 0; b < nbl; ++b)
-                    avg0[b] = cplx(h_avg[d * per_dir + b * md->pols * 2], h_avg[d * per_dir + b * md->pols * 2 + 1]);
+                    avg0[b] = cplx(h_avg0[2 * (size_t(d) * nbl + b)], h_avg0[2 * (size_t(d) * nbl + b) + 1]);
                 dir_loss[d] = inversion_loss_adjoint(*md, avg0.data(), ab.data());
                 for (uint32_t b = 0; b < nbl; ++b) {
                     h_abar[2 * (size_t(d) * nbl + b)] = ab[b].real();
@@ -1747,7 +1751,8 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         *loss_out = loss;
         if (!grad_offsets && !grad_antenna) return;
         // adjoint of rotate -> average on the device
-        DeviceBuffer d_abar, d_guvw, d_gdzp, d_gdz;
+        DeviceBuffer &d_abar = ctx->d_adj_abar, &d_guvw = ctx->d_adj_guvw, &d_gdzp = ctx->d_adj_gdzp,
+                     &d_gdz = ctx->d_adj_gdz;
         d_abar.reserve(h_abar.size() * sizeof(double));
         d_guvw.reserve(size_t(R) * 3 * sizeof(double));
         d_gdzp.reserve(size_t(nbl) * D * 3 * sizeof(double));
@@ -1775,7 +1780,8 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
                     sgn.push_back(e.second);
                 }
             }
-            DeviceBuffer d_off, d_bls, d_sgn, d_gant;
+            DeviceBuffer &d_off = ctx->d_adj_off, &d_bls = ctx->d_adj_bls, &d_sgn = ctx->d_adj_sgn,
+                         &d_gant = ctx->d_adj_gant;
             d_off.reserve(off.size() * sizeof(int32_t));
             d_bls.reserve(std::max<size_t>(1, bls.size()) * sizeof(int32_t));
             d_sgn.reserve(std::max<size_t>(1, sgn.size()) * sizeof(int32_t));

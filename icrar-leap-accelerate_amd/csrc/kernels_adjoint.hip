@@ -178,6 +178,14 @@ __global__ __launch_bounds__(256) void reduce_gdz_kernel(const double* __restric
     if (threadIdx.x == 0) out[i] = s_red[0];
 }
 
+// out[i] = pol 0 of avg[i] for the n = D nbl (direction, baseline) entries of P pols each.
+__global__ void gather_pol0_kernel(const double* __restrict__ avg, uint32_t n, uint32_t P, double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[2 * size_t(i)] = avg[size_t(i) * P * 2];
+    out[2 * size_t(i) + 1] = avg[size_t(i) * P * 2 + 1];
+}
+
 // One workgroup per antenna: sum over its baselines (sign +1 as antenna2, -1 as
 // antenna1) and timesteps of M_t^T g_uvw(t, b).
 __global__ __launch_bounds__(256) void antenna_grad_kernel(const double* __restrict__ g_uvw,
@@ -239,6 +247,12 @@ void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, co
     else
         rotate_adjoint_kernel<256><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0,
                                                              dkappa, g_uvw, g_dz_partial);
+    LEAP_HIP(hipGetLastError());
+}
+
+void launch_gather_pol0(hipStream_t s, const double* avg, uint32_t n, uint32_t P, double* out) {
+    if (n == 0) return;
+    gather_pol0_kernel<<<(n + 255) / 256, 256, 0, s>>>(avg, n, P, out);
     LEAP_HIP(hipGetLastError());
 }
 

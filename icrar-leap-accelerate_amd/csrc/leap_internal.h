@@ -204,6 +204,10 @@ struct leap_ctx {
     leap::DeviceBuffer d_uvw, d_vis_stage, d_dirs, d_dd, d_avg, d_phase, d_cal1, d_delta, d_cal,
         d_partial, d_a_planes;
     leap::DeviceBuffer d_nufft_tail;  // partial sums of the NUFFT launch's chunked last round
+    // residual-loss gradient: ENU->UVW rotations, baseline antennas, antenna corrections, perturbed
+    // uvw, avgData adjoint, uvw / dz gradients, the antennas' baseline lists and their gradient
+    leap::DeviceBuffer d_adj_M, d_adj_ants, d_adj_dp, d_adj_uvw, d_adj_abar, d_adj_guvw, d_adj_gdzp, d_adj_gdz,
+        d_adj_off, d_adj_bls, d_adj_sgn, d_adj_gant;
     std::vector<double> h_avg, h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;
@@ -321,6 +325,7 @@ void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, co
                            const double* abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
                            double df, double* g_uvw, double* g_dz_partial);
 void launch_reduce_gdz(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double* out);
+void launch_gather_pol0(hipStream_t s, const double* avg, uint32_t n, uint32_t P, double* out);
 void launch_antenna_grad(hipStream_t s, const double* g_uvw, const double* M, const int32_t* csr_off,
                          const int32_t* csr_bl, const int32_t* csr_sign, int32_t n_ant, uint32_t nbl, uint32_t T,
                          double* out);
