@@ -1569,10 +1569,9 @@ namespace {
 
 using cplx = std::complex<double>;
 
+// x wrapped into [-pi, pi)
 double wrap_pi(double x) {
-    double r = std::fmod(x + kPi, 2.0 * kPi);
-    if (r < 0) r += 2.0 * kPi;
-    return r - kPi;
+    return x - kTwoPi * std::floor((x + kPi) / kTwoPi);
 }
 
 // Row 2 of the DD matrix and its derivatives in ra and dec (MetaData.cc:166-203 differentiated).
@@ -1618,25 +1617,29 @@ std::vector<double> gram_t_times(const std::vector<double>& Gp, int32_t N, const
     return y;
 }
 
-// Forward inversion, loss and its adjoint for one direction (pol 0 of avgData).
+// Forward inversion, loss and its adjoint for one direction (pol 0 of avgData, with
+// ph[b] = arg avg[b] computed on the device).  The phase of exp(-i theta) avg is taken as
+// wrap(arg avg - theta), equal to it modulo 2 pi; where avg is 0 the product itself is formed.
 // Returns the loss; abar[b] receives dL/dRe avg + i dL/dIm avg.
-double inversion_loss_adjoint(const leap_metadata& md, const cplx* avg, cplx* abar) {
+double inversion_loss_adjoint(const leap_metadata& md, const cplx* avg, const double* ph, cplx* abar) {
     const leap::PhaseMatrix& A = md.A;
     const leap::PhaseMatrix& A1 = md.A1;
     const int32_t N = A.cols, K = A.rows, K1 = A1.rows, nI = K - 1;
     std::vector<double> phi1(size_t(K1), 0.0), cal1(size_t(N), 0.0);
-    for (int32_t k = 0; k + 1 < K1; ++k) phi1[k] = std::arg(avg[A1.I[k]]);
+    for (int32_t k = 0; k + 1 < K1; ++k) phi1[k] = ph[A1.I[k]];
     add_ad_times(A1, md.G1p, phi1.data(), cal1.data());
     std::vector<double> dphi(size_t(K), 0.0), cal(cal1);
     for (int32_t n = 0; n < nI; ++n) {
         const double acal1 = cal1[A.ant1[n]] - cal1[A.ant2[n]];
-        dphi[n] = std::arg(std::exp(cplx(0.0, -kTwoPi * acal1)) * avg[n]);  // row n of avgData
+        // row n of avgData; a zero entry keeps the signed-zero phase of the rotated product
+        dphi[n] = avg[n] == cplx(0, 0) ? std::arg(std::exp(cplx(0.0, -kTwoPi * acal1)) * avg[n])
+                                       : wrap_pi(ph[n] - kTwoPi * acal1);
     }
     add_ad_times(A, md.Gp, dphi.data(), cal.data());
     double loss = 0.0;
     std::vector<double> res(static_cast<size_t>(nI));
     for (int32_t n = 0; n < nI; ++n) {
-        res[n] = wrap_pi(cal[A.ant1[n]] - cal[A.ant2[n]] - std::arg(avg[A.I[n]]));
+        res[n] = wrap_pi(cal[A.ant1[n]] - cal[A.ant2[n]] - ph[A.I[n]]);
         loss += res[n] * res[n];
     }
     // adjoint
@@ -1658,8 +1661,11 @@ double inversion_loss_adjoint(const leap_metadata& md, const cplx* avg, cplx* ab
     }
     const std::vector<double> y1 = gram_t_times(md.G1p, N, g_cal1.data());
     for (int32_t k = 0; k + 1 < K1; ++k) g_arg[A1.I[k]] += y1[A1.ant1[k]] - y1[A1.ant2[k]];
-    for (uint32_t b = 0; b < nbl; ++b)
-        abar[b] = (avg[b] == cplx(0, 0)) ? cplx(0, 0) : cplx(0.0, g_arg[b]) / std::conj(avg[b]);
+    // i g / conj(z) = i g z / |z|^2
+    for (uint32_t b = 0; b < nbl; ++b) {
+        const double x = avg[b].real(), y = avg[b].imag(), m = x * x + y * y;
+        abar[b] = m == 0.0 ? cplx(0, 0) : cplx(-g_arg[b] * y / m, g_arg[b] * x / m);
+    }
     return loss;
 }
 
@@ -1708,9 +1714,9 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
                               ctx->d_avg.as<double>(), 0);
         // pol 0 of avgData, gathered on the device (the avgData adjoint buffer is free until the backward)
         DeviceBuffer& d_avg0 = ctx->d_adj_abar;
-        d_avg0.reserve(size_t(D) * nbl * 2 * sizeof(double));
+        d_avg0.reserve(size_t(D) * nbl * 3 * sizeof(double));
         launch_gather_pol0(s, ctx->d_avg.as<double>(), D * nbl, md->pols, d_avg0.as<double>());
-        std::vector<double> h_avg0(size_t(D) * nbl * 2);
+        std::vector<double> h_avg0(size_t(D) * nbl * 3);  // D nbl complex values, then their arguments
         LEAP_HIP(hipMemcpyAsync(h_avg0.data(), d_avg0.ptr, h_avg0.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         LEAP_HIP(hipStreamSynchronize(s));
         // inversion, loss and its adjoint on the host, directions split over threads; the
@@ -1721,7 +1727,8 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
             for (uint32_t d = d_begin; d < d_end; ++d) {
                 for (uint32_t b = 0; b < nbl; ++b)
                     avg0[b] = cplx(h_avg0[2 * (size_t(d) * nbl + b)], h_avg0[2 * (size_t(d) * nbl + b) + 1]);
-                dir_loss[d] = inversion_loss_adjoint(*md, avg0.data(), ab.data());
+                dir_loss[d] = inversion_loss_adjoint(*md, avg0.data(), h_avg0.data() + 2 * size_t(D) * nbl + size_t(d) * nbl,
+                                                     ab.data());
                 for (uint32_t b = 0; b < nbl; ++b) {
                     h_abar[2 * (size_t(d) * nbl + b)] = ab[b].real();
                     h_abar[2 * (size_t(d) * nbl + b) + 1] = ab[b].imag();

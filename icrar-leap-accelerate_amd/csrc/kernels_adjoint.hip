@@ -178,12 +178,15 @@ __global__ __launch_bounds__(256) void reduce_gdz_kernel(const double* __restric
     if (threadIdx.x == 0) out[i] = s_red[0];
 }
 
-// out[i] = pol 0 of avg[i] for the n = D nbl (direction, baseline) entries of P pols each.
+// For the n = D nbl (direction, baseline) entries of P pols each: out[2 i], out[2 i + 1] = pol 0
+// of avg[i], and out[2 n + i] = its argument atan2(im, re).
 __global__ void gather_pol0_kernel(const double* __restrict__ avg, uint32_t n, uint32_t P, double* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[2 * size_t(i)] = avg[size_t(i) * P * 2];
-    out[2 * size_t(i) + 1] = avg[size_t(i) * P * 2 + 1];
+    const double re = avg[size_t(i) * P * 2], im = avg[size_t(i) * P * 2 + 1];
+    out[2 * size_t(i)] = re;
+    out[2 * size_t(i) + 1] = im;
+    out[2 * size_t(n) + i] = atan2(im, re);
 }
 
 // One workgroup per antenna: sum over its baselines (sign +1 as antenna2, -1 as

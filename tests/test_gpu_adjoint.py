@@ -39,6 +39,24 @@ def test_adjoint_matches_oracle_gradient(leap, ctx, oracle, dtype, n_dirs, n_ch)
     assert np.abs(g_ant - go_ant).max() <= 1e-7 * np.abs(go_ant).max()
 
 
+def test_adjoint_zero_avgdata_matches_oracle(leap, ctx, oracle):
+    """Baselines whose every (row, channel) holds a NaN have zero avgData; the loss and gradients
+    still follow the oracle's arg(exp(-i theta) avg) for them (a signed-zero phase)."""
+    s, ms, dirs, off, dp = problem(leap, 12, 4, 40, 6, 23)
+    vis = ms["vis"]
+    nbl = ms["num_baselines"]
+    for b in (3, 17, nbl - 2):
+        vis[b::nbl, :, 1] = np.nan  # pol 1 of every channel of every row of baseline b
+    view = leap.view_from_arrays(ms)
+    md = ctx.metadata(view)
+    L, g_off, g_ant = md.loss_grad(view, dirs, s.enu_to_uvw, off, dp)
+    Lo, go_off, go_ant = oracle.loss_grad(ms, dirs, s.enu_to_uvw, off, dp)
+    print(f"loss {L:.12e} vs oracle {Lo:.12e}")
+    assert abs(L - Lo) <= 1e-11 * abs(Lo)
+    assert np.abs(g_off - go_off).max() <= 1e-7 * np.abs(go_off).max()
+    assert np.abs(g_ant - go_ant).max() <= 1e-7 * np.abs(go_ant).max()
+
+
 def test_adjoint_matches_central_differences_128_stations(leap, ctx):
     """128 stations without autocorrelations (8128 baselines), 64 t, 192 channels, complex64 generated on the
     device, 16 directions, antenna perturbations N(0, 0.01 m): device gradient vs central differences."""
