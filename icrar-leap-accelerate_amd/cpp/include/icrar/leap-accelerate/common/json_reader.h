@@ -6,6 +6,7 @@
 
 #include <cctype>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -141,13 +142,36 @@ class JsonReader {
             return v;
         }
         if (*m_p == '-' || std::isdigit(static_cast<unsigned char>(*m_p))) {
-            char* end = nullptr;
-            v.num = std::strtod(m_p, &end);
-            if (end == m_p) Fail("bad number");
+            // RFC 8259 number grammar: -?(0|[1-9][0-9]*)(.[0-9]+)?([eE][+-]?[0-9]+)?; the text
+            // is checked before strtod, which would also take hex, "inf" or a bare "5.".
+            auto digit = [](char c) { return std::isdigit(static_cast<unsigned char>(c)) != 0; };
+            const char* q = m_p;
+            if (*q == '-') ++q;
+            if (*q == '0') {
+                ++q;
+            } else if (digit(*q)) {
+                while (digit(*q)) ++q;
+            } else {
+                Fail("bad number");
+            }
             v.integral = true;
-            for (const char* q = m_p; q < end; ++q)
-                if (*q == '.' || *q == 'e' || *q == 'E') v.integral = false;
-            m_p = end;
+            if (*q == '.') {
+                ++q;
+                if (!digit(*q)) Fail("bad number");
+                while (digit(*q)) ++q;
+                v.integral = false;
+            }
+            if (*q == 'e' || *q == 'E') {
+                ++q;
+                if (*q == '+' || *q == '-') ++q;
+                if (!digit(*q)) Fail("bad number");
+                while (digit(*q)) ++q;
+                v.integral = false;
+            }
+            const std::string text(m_p, q);
+            v.num = std::strtod(text.c_str(), nullptr);
+            if (!std::isfinite(v.num)) Fail("number too big");  // as rapidjson's kParseErrorNumberTooBig
+            m_p = q;
             v.kind = JsonValue::Number;
             return v;
         }

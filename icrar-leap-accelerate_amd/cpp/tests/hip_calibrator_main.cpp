@@ -69,7 +69,11 @@ static int selftest() {
     // The one-direction case there compares against a casacore MVDirection round trip of
     // the same two numbers; here the parsed values are compared with the numbers themselves.
     if (!ParseDirections("[]").empty()) ++bad;
-    for (const char* reject : {"{}", "[[]]", "[[1.0]]", "[[1.0,1.0,1.0]]", "[1.0,2.0]", "[[\"a\",1.0]]"}) {
+    // JSON number grammar as rapidjson reads it: no hex, no bare "5." or ".5", no leading zeros,
+    // and no overflow to infinity
+    for (const char* reject : {"{}", "[[]]", "[[1.0]]", "[[1.0,1.0,1.0]]", "[1.0,2.0]", "[[\"a\",1.0]]",
+                               "[[0x10,2]]", "[[5.,2]]", "[[.5,2]]", "[[01,2]]", "[[1e999,2]]", "[[-,2]]",
+                               "[[1e,2]]", "[[inf,2]]"}) {
         try {
             ParseDirections(std::string(reject));
             std::cerr << "ParseDirections accepted " << reject << "\n";
@@ -80,6 +84,10 @@ static int selftest() {
     {
         const auto one = ParseDirections("[[-0.4606549305661674,-0.29719233792392513]]");
         if (one.size() != 1 || !(one[0] == SphericalDirection(-0.4606549305661674, -0.29719233792392513))) ++bad;
+        const auto num = ParseDirections("[[-0.5e-3,2E+2],[0,-0.0],[1e-400,12.5E-1]]");
+        if (num.size() != 3 || !(num[0] == SphericalDirection(-0.0005, 200.0)) ||
+            !(num[1] == SphericalDirection(0.0, 0.0)) || !(num[2] == SphericalDirection(0.0, 1.25)))
+            ++bad;
         const auto five = ParseDirections("[[0.0,0.0],[1.0,1.0],[2.0,2.0],[3.0,3.0],[4.0,4.0]]");
         if (five.size() != 5) ++bad;
         for (size_t i = 0; i < five.size(); ++i)
