@@ -628,6 +628,8 @@ void reserve_solution(leap_ctx* ctx, const leap_metadata& md, uint32_t D, uint32
         const uint32_t tc = pack_chunk(T, md.nbl, md.channels, opts);
         ctx->d_packed.reserve(size_t(tc) * md.nbl * md.channels * (T > tc ? 2 : 1) * 2 * sizeof(double));
     }
+    if (opts.strategy == LEAP_STRATEGY_BATCHED && opts.precision == LEAP_PRECISION_MIXED)
+        ctx->d_nufft_tail.reserve(nufft_tail_bytes(D));  // the NUFFT route's chunked last round
     ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
     ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
     ctx->d_delta.reserve(size_t(K) * D * sizeof(double));
