@@ -97,39 +97,45 @@ def test_graph_replays_equal_eager(leap, ctx, precision, strategy, db, cs):
         b.free()
 
 
-def test_graph_capture_first_on_a_fresh_context(leap):
-    """A captured mixed-precision (NUFFT) calibration as the first call of a new context: every
-    buffer the captured sequence uses, the NUFFT's chunked-round scratch included, is reserved
-    before the capture, and the result equals the eager one bit for bit."""
+@pytest.mark.parametrize("precision,strategy,db,cs", CASES,
+                         ids=["fp64-per-dir", "fp64-batched", "mixed", "mixed-nufft", "fp64-db64"])
+def test_graph_capture_first_on_a_fresh_context(leap, precision, strategy, db, cs):
+    """A captured calibration as the first call of a new context: every buffer the captured
+    sequence uses (the NUFFT's chunked-round scratch included) is reserved before the capture,
+    and the result equals the eager one bit for bit."""
     c2 = leap.Context(0)
-    a = DeviceSet(leap, c2, np.complex64, n_antennas=24, n_timesteps=9, n_channels=130, seed=83)
+    a = DeviceSet(leap, c2, np.complex64 if precision else np.complex128, n_antennas=24, n_timesteps=9,
+                  n_channels=130, seed=83)
     try:
         dirs = dirs_near(a.ms["phase_centre"], 40, 5)
         slice_ = (0, 3, -1)
-        first = c2.calibrate(a.view, dirs, slice_, opts=leap.options(1, 1, 0, graph=1, channel_sum=2), md=a.md)
+        first = c2.calibrate(a.view, dirs, slice_, opts=leap.options(precision, strategy, db, graph=1, channel_sum=cs),
+                             md=a.md)
         assert c2.last_graph() == 1
-        eager = c2.calibrate(a.view, dirs, slice_, opts=leap.options(1, 1, 0, channel_sum=2), md=a.md)
+        eager = c2.calibrate(a.view, dirs, slice_, opts=leap.options(precision, strategy, db, channel_sum=cs), md=a.md)
         assert np.array_equal(first[0], eager[0]) and np.array_equal(first[1], eager[1])
     finally:
         a.free()
         c2.close()
-    # the same for a multi-set capture, FP64 and mixed, each on a new context
-    for precision in (0, 1):
-        c3 = leap.Context(0)
-        sets = [DeviceSet(leap, c3, n_antennas=12, autocorrelations=0, n_timesteps=6, n_channels=40, seed=88),
-                DeviceSet(leap, c3, np.complex64, n_antennas=20, n_timesteps=4, n_channels=9, seed=89)]
-        try:
-            arg = [(st.md, st.view, dirs_near(st.ms["phase_centre"], n, n), sl)
-                   for st, n, sl in zip(sets, (5, 33), ((0, 2, -1), (0, -1, -1)))]
-            got = c3.calibrate_batch(arg, leap.options(precision, 1, graph=1))
-            assert c3.last_graph() == 1
-            eager = c3.calibrate_batch(arg, leap.options(precision, 1))
-            for (gc, ge), (ec, ee) in zip(got, eager):
-                assert np.array_equal(gc, ec) and np.array_equal(ge, ee)
-        finally:
-            for st in sets:
-                st.free()
-            c3.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multi_set_graph_capture_first_on_a_fresh_context(leap, precision):
+    c3 = leap.Context(0)
+    sets = [DeviceSet(leap, c3, n_antennas=12, autocorrelations=0, n_timesteps=6, n_channels=40, seed=88),
+            DeviceSet(leap, c3, np.complex64, n_antennas=20, n_timesteps=4, n_channels=9, seed=89)]
+    try:
+        arg = [(st.md, st.view, dirs_near(st.ms["phase_centre"], n, n), sl)
+               for st, n, sl in zip(sets, (5, 33), ((0, 2, -1), (0, -1, -1)))]
+        got = c3.calibrate_batch(arg, leap.options(precision, 1, graph=1))
+        assert c3.last_graph() == 1
+        eager = c3.calibrate_batch(arg, leap.options(precision, 1))
+        for (gc, ge), (ec, ee) in zip(got, eager):
+            assert np.array_equal(gc, ec) and np.array_equal(ge, ee)
+    finally:
+        for st in sets:
+            st.free()
+        c3.close()
 
 
 def test_graph_matches_oracle(leap, ctx, oracle):
