@@ -240,7 +240,9 @@ __global__ void delta_kernel(const double* __restrict__ avg, uint32_t nbl, uint3
 }
 
 // Split-K GEMV: partial[d][chunk][i] = sum over the chunk's k of AdT[k][i] * delta[d][k].
-constexpr int32_t kGemvChunk = 256;
+// 64 rows per chunk: about two workgroups per CU at 32,896 baselines, and a short dependent
+// chain per thread.
+constexpr int32_t kGemvChunk = 64;
 
 __global__ __launch_bounds__(kBlock) void gemv_partial_kernel(
     const double* __restrict__ AdT, int32_t K, int32_t N, const double* __restrict__ delta,
@@ -259,14 +261,23 @@ __global__ __launch_bounds__(kBlock) void gemv_partial_kernel(
     }
 }
 
-__global__ void gemv_reduce_kernel(const double* __restrict__ partial, int32_t n_chunks, int32_t N,
-                                   const double* __restrict__ cal1, double* __restrict__ cal) {
+// cal[d][i] = sum over chunks of partial[d][chunk][i] + cal1[d][i]: one workgroup per output,
+// each lane a strided slice of the chunks, then a fixed-order tree in LDS.
+__global__ __launch_bounds__(kBlock) void gemv_reduce_kernel(const double* __restrict__ partial, int32_t n_chunks,
+                                                             int32_t N, const double* __restrict__ cal1,
+                                                             double* __restrict__ cal) {
+    __shared__ double s_red[kBlock];
+    const int32_t i = blockIdx.x;
     const uint32_t d = blockIdx.y;
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int32_t c = 0; c < n_chunks; ++c) s += partial[(size_t(d) * n_chunks + c) * N + i];
-        cal[size_t(d) * N + i] = s + cal1[size_t(d) * N + i];
+    double acc = 0.0;
+    for (int32_t c = threadIdx.x; c < n_chunks; c += kBlock) acc += partial[(size_t(d) * n_chunks + c) * N + i];
+    s_red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t stride = kBlock / 2; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) s_red[threadIdx.x] += s_red[threadIdx.x + stride];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) cal[size_t(d) * N + i] = s_red[0] + cal1[size_t(d) * N + i];
 }
 
 template <typename T>
@@ -323,7 +334,7 @@ void launch_inversion_f64(hipStream_t s, const leap_metadata& md, const double* 
     const int32_t chunks = (K + kGemvChunk - 1) / kGemvChunk;
     gemv_partial_kernel<<<dim3(chunks, D), kBlock, 0, s>>>(md.d_AdT.as<double>(), K, N, delta, partial);
     LEAP_HIP(hipGetLastError());
-    gemv_reduce_kernel<<<dim3((N + 255) / 256, D), 256, 0, s>>>(partial, chunks, N, cal1, cal);
+    gemv_reduce_kernel<<<dim3(N, D), kBlock, 0, s>>>(partial, chunks, N, cal1, cal);
     LEAP_HIP(hipGetLastError());
 }
 
