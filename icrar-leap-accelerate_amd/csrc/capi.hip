@@ -1718,18 +1718,22 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         DeviceBuffer& d_avg0 = ctx->d_adj_abar;
         d_avg0.reserve(size_t(D) * nbl * 3 * sizeof(double));
         launch_gather_pol0(s, ctx->d_avg.as<double>(), D * nbl, md->pols, d_avg0.as<double>());
-        std::vector<double> h_avg0(size_t(D) * nbl * 3);  // D nbl complex values, then their arguments
-        LEAP_HIP(hipMemcpyAsync(h_avg0.data(), d_avg0.ptr, h_avg0.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        ctx->h_adj_avg.reserve(size_t(D) * nbl * 3 * sizeof(double));  // D nbl complex values, then their arguments
+        const double* h_avg0 = ctx->h_adj_avg.as<double>();
+        LEAP_HIP(hipMemcpyAsync(ctx->h_adj_avg.ptr, d_avg0.ptr, size_t(D) * nbl * 3 * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
         LEAP_HIP(hipStreamSynchronize(s));
         // inversion, loss and its adjoint on the host, directions split over threads; the
         // per-direction losses are added in direction order
-        std::vector<double> h_abar(size_t(D) * nbl * 2), dir_loss(D, 0.0);
+        ctx->h_adj_abar.reserve(size_t(D) * nbl * 2 * sizeof(double));
+        double* h_abar = ctx->h_adj_abar.as<double>();
+        std::vector<double> dir_loss(D, 0.0);
         auto invert_range = [&](uint32_t d_begin, uint32_t d_end) {
             std::vector<cplx> avg0(nbl), ab(nbl);
             for (uint32_t d = d_begin; d < d_end; ++d) {
                 for (uint32_t b = 0; b < nbl; ++b)
                     avg0[b] = cplx(h_avg0[2 * (size_t(d) * nbl + b)], h_avg0[2 * (size_t(d) * nbl + b) + 1]);
-                dir_loss[d] = inversion_loss_adjoint(*md, avg0.data(), h_avg0.data() + 2 * size_t(D) * nbl + size_t(d) * nbl,
+                dir_loss[d] = inversion_loss_adjoint(*md, avg0.data(), h_avg0 + 2 * size_t(D) * nbl + size_t(d) * nbl,
                                                      ab.data());
                 for (uint32_t b = 0; b < nbl; ++b) {
                     h_abar[2 * (size_t(d) * nbl + b)] = ab[b].real();
@@ -1762,11 +1766,11 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         // adjoint of rotate -> average on the device
         DeviceBuffer &d_abar = ctx->d_adj_abar, &d_guvw = ctx->d_adj_guvw, &d_gdzp = ctx->d_adj_gdzp,
                      &d_gdz = ctx->d_adj_gdz;
-        d_abar.reserve(h_abar.size() * sizeof(double));
+        d_abar.reserve(size_t(D) * nbl * 2 * sizeof(double));
         d_guvw.reserve(size_t(R) * 3 * sizeof(double));
         d_gdzp.reserve(size_t(nbl) * D * 3 * sizeof(double));
         d_gdz.reserve(size_t(D) * 3 * sizeof(double));
-        LEAP_HIP(hipMemcpyAsync(d_abar.ptr, h_abar.data(), h_abar.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        LEAP_HIP(hipMemcpyAsync(d_abar.ptr, h_abar, size_t(D) * nbl * 2 * sizeof(double), hipMemcpyHostToDevice, s));
         launch_rotate_adjoint(s, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(), d_abar.as<double>(), D, T, nbl,
                               md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz, d_guvw.as<double>(),
                               d_gdzp.as<double>());

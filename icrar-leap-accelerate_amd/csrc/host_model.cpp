@@ -34,6 +34,22 @@ void DeviceBuffer::reserve(size_t n) {
     bytes = n;
 }
 
+PinnedBuffer::~PinnedBuffer() {
+    if (ptr) (void)hipHostFree(ptr);
+}
+
+void PinnedBuffer::reserve(size_t n) {
+    if (n <= bytes && ptr) return;
+    if (ptr) {
+        LEAP_HIP(hipHostFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+    }
+    if (n == 0) return;
+    LEAP_HIP(hipHostMalloc(&ptr, n, hipHostMallocDefault));
+    bytes = n;
+}
+
 std::vector<double> PhaseMatrix::dense() const {
     std::vector<double> a(size_t(rows) * cols, 0.0);
     for (int32_t k = 0; k + 1 < rows; ++k) {

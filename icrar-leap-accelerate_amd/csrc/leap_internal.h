@@ -45,6 +45,20 @@ struct DeviceBuffer {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Pinned host buffer (hipHostMalloc) owned by a context: grows, never shrinks.
+struct PinnedBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer();
+    // Grows to at least n bytes; contents are not kept.
+    void reserve(size_t n);
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
 // Host-side phase matrices, in the reference's layout conventions.
 struct PhaseMatrix {
     int32_t rows = 0;  // K: baselines used + the constraint row
@@ -208,6 +222,7 @@ struct leap_ctx {
     // uvw, avgData adjoint, uvw / dz gradients, the antennas' baseline lists and their gradient
     leap::DeviceBuffer d_adj_M, d_adj_ants, d_adj_dp, d_adj_uvw, d_adj_abar, d_adj_guvw, d_adj_gdzp, d_adj_gdz,
         d_adj_off, d_adj_bls, d_adj_sgn, d_adj_gant;
+    leap::PinnedBuffer h_adj_avg, h_adj_abar;  // pol 0 of avgData with its arguments; the avgData adjoint
     std::vector<double> h_avg, h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;
