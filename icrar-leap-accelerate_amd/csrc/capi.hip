@@ -1710,7 +1710,7 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         // forward avgData (pol 0), FP64
         const size_t per_dir = size_t(nbl) * md->pols * 2;
         ctx->d_avg.reserve(per_dir * D * sizeof(double));
-        const int32_t tile = D <= 16 ? 161 : D <= 32 ? 321 : D <= 64 ? 641 : D <= 128 ? 1282 : 2562;
+        const int32_t tile = D <= 16 ? 162 : D <= 32 ? 321 : D <= 64 ? 641 : D <= 128 ? 1282 : 2562;
         launch_rotate_batched(s, LEAP_PRECISION_FP64, tile, false, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(),
                               D, T, nbl, nbl, md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
                               ctx->d_avg.as<double>(), 0);

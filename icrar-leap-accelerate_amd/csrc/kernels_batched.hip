@@ -657,9 +657,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(ND == 
 template <typename VT, typename A, int DB, int ND, int NP>
 void launch_one(hipStream_t s, const void* vis, const double* uvw, const double* dd, uint32_t D, uint32_t T,
                 uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double kappa0, double dkappa, double* avg) {
-    // Channels per LDS chunk (32 KB of pol-0 visibilities at DB 16 or 32).  The recurrences
-    // run over the chunks in order, so the chunk size does not change the result.
-    constexpr int CC = (NP == 1) ? (DB == 16 ? 128 : 256) : 64;
+    // Channels per LDS chunk: at most 32 KB of pol-0 visibilities for the tile's RP rows (256
+    // channels from 8 rows up).  The recurrences run over the chunks in order, so the chunk
+    // size does not change the result.
+    constexpr int RP = kThreads / (DB / ND);
+    constexpr int CC = (NP == 1) ? (2048 / RP < 256 ? 2048 / RP : 256) : 64;
     const dim3 grid(nbl_grid, (D + DB - 1) / DB);
     if constexpr (sizeof(A) == 8) {
         rotate_average_horner64<VT, DB, ND, NP, CC><<<grid, kThreads, 0, s>>>(
@@ -682,8 +684,13 @@ void launch_db(hipStream_t s, int tile, const void* vis, const double* uvw, cons
         return;
     if constexpr (sizeof(A) == 8) {
         if constexpr (NP == 1) {
-            if (tile == 161) {  // 16 directions per workgroup (internal: the residual-loss forward pass)
+            // 16 directions per workgroup (internal: the residual-loss forward pass)
+            if (tile == 161) {
                 launch_one<VT, A, 16, 1, NP>(s, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
+                return;
+            }
+            if (tile == 162) {
+                launch_one<VT, A, 16, 2, NP>(s, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
                 return;
             }
         }
