@@ -50,27 +50,40 @@ __global__ void perturb_uvw_kernel(const double* __restrict__ uvw0, const double
     }
 }
 
-// One workgroup per baseline, one lane per direction (D <= 256), 256/DL rows side by side,
-// channels staged through LDS kCC at a time (at most 32 KB of visibilities per workgroup).
-// The Horner recurrences run over the chunks in order, so the chunk size does not change the result.
-template <int DL>
+// One workgroup per baseline; DL lanes per row with ND directions each (direction lane + DL j,
+// D <= DL ND), 256/DL rows side by side, channels staged through LDS kCC at a time (at most 32 KB
+// of visibilities per workgroup).  The Horner recurrences run over the chunks in order, so the
+// chunk size does not change the result.
+template <int DL, int ND>
 __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
     const void* __restrict__ vis, int32_t c64, const double* __restrict__ uvw, const double* __restrict__ dd,
     const double* __restrict__ abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0,
     double dkappa, double* __restrict__ g_uvw, double* __restrict__ g_dz_partial) {
     constexpr int RP = kThreads / DL;
-    constexpr int kCC = DL == 16 ? 128 : 256;  // channels per LDS chunk
+    constexpr int kCC = 2048 / RP < 256 ? 2048 / RP : 256;  // channels per LDS chunk
+    constexpr int DW = DL * ND;                               // directions of the workgroup
     __shared__ Cx s_v[RP][kCC];
-    __shared__ double s_g[RP][DL][3];
+    __shared__ double s_g[RP][DW][3];
     const uint32_t bl = blockIdx.x;
     const uint32_t lane_d = threadIdx.x % DL;
     const uint32_t rsub = threadIdx.x / DL;
-    const bool dir_ok = lane_d < D;
-    const uint32_t dc = dir_ok ? lane_d : 0;
-    const double dz0 = dd[9 * dc + 6], dz1 = dd[9 * dc + 7], dz2 = dd[9 * dc + 8];
-    Cx ab = {0.0, 0.0};
-    if (dir_ok) ab = {abar[2 * (size_t(dc) * nbl + bl)], abar[2 * (size_t(dc) * nbl + bl) + 1]};
-    double gdz[3] = {0.0, 0.0, 0.0};
+    bool dir_ok[ND];
+    bool any_dir = false;
+    double dz[ND][3], gdz[ND][3];
+    Cx ab[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        const uint32_t dj = lane_d + DL * j;
+        dir_ok[j] = dj < D;
+        any_dir = any_dir || dir_ok[j];
+        const uint32_t dc = dir_ok[j] ? dj : 0;
+        dz[j][0] = dd[9 * dc + 6];
+        dz[j][1] = dd[9 * dc + 7];
+        dz[j][2] = dd[9 * dc + 8];
+        ab[j] = Cx{0.0, 0.0};
+        if (dir_ok[j]) ab[j] = {abar[2 * (size_t(dc) * nbl + bl)], abar[2 * (size_t(dc) * nbl + bl) + 1]};
+        gdz[j][0] = gdz[j][1] = gdz[j][2] = 0.0;
+    }
     const uint32_t n_chunks = (C + kCC - 1) / kCC;
 
     for (uint32_t tb = 0; tb < T; tb += RP) {
@@ -83,9 +96,15 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
             v = uvw[3 * row + 1];
             w = uvw[3 * row + 2];
         }
-        const double sf = -kTwoPi * ((dz0 * u + dz1 * v + dz2 * w) - w);
-        const Cx z = expi(sf * dkappa);
-        Cx p = {0.0, 0.0}, dp = {0.0, 0.0};
+        double sf[ND];
+        Cx z[ND], p[ND], dp[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            sf[j] = -kTwoPi * ((dz[j][0] * u + dz[j][1] * v + dz[j][2] * w) - w);
+            z[j] = expi(sf[j] * dkappa);
+            p[j] = {0.0, 0.0};
+            dp[j] = {0.0, 0.0};
+        }
         for (int ch = int(n_chunks) - 1; ch >= 0; --ch) {
             const uint32_t c0 = uint32_t(ch) * kCC;
             const uint32_t cn = min(uint32_t(kCC), C - c0);
@@ -118,45 +137,57 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
                 s_v[r][c] = val;
             }
             __syncthreads();
-            if (!row_ok || !dir_ok) continue;
+            if (!row_ok || !any_dir) continue;
             for (int k = int(cn) - 1; k >= 0; --k) {
-                dp = hstep(dp, z, p);  // P' <- P' z + P (uses P before its update)
-                p = hstep(p, z, s_v[rsub][k]);
+                const Cx x = s_v[rsub][k];
+#pragma unroll
+                for (int j = 0; j < ND; ++j) {
+                    dp[j] = hstep(dp[j], z[j], p[j]);  // P' <- P' z + P (uses P before its update)
+                    p[j] = hstep(p[j], z[j], x);
+                }
             }
         }
-        double gsf = 0.0;
-        if (row_ok && dir_ok) {
-            const Cx e0 = expi(sf * kappa0);
-            const Cx zdp = cmulx(z, dp);
-            const Cx s1 = cmulx(e0, Cx{kappa0 * p.x + dkappa * zdp.x, kappa0 * p.y + dkappa * zdp.y});
-            // Re(conj(ab) * i * s1) with conj(ab) = (ab.x, -ab.y) and i * s1 = (-s1.y, s1.x)
-            gsf = -ab.x * s1.y + ab.y * s1.x;
-            gdz[0] += -kTwoPi * gsf * u;
-            gdz[1] += -kTwoPi * gsf * v;
-            gdz[2] += -kTwoPi * gsf * w;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            double gsf = 0.0;
+            if (row_ok && dir_ok[j]) {
+                const Cx e0 = expi(sf[j] * kappa0);
+                const Cx zdp = cmulx(z[j], dp[j]);
+                const Cx s1 = cmulx(e0, Cx{kappa0 * p[j].x + dkappa * zdp.x, kappa0 * p[j].y + dkappa * zdp.y});
+                // Re(conj(ab) * i * s1) with conj(ab) = (ab.x, -ab.y) and i * s1 = (-s1.y, s1.x)
+                gsf = -ab[j].x * s1.y + ab[j].y * s1.x;
+                gdz[j][0] += -kTwoPi * gsf * u;
+                gdz[j][1] += -kTwoPi * gsf * v;
+                gdz[j][2] += -kTwoPi * gsf * w;
+            }
+            // g_uvw(row) = sum over directions of -2 pi gsf (dz - e_z), reduced in LDS.
+            s_g[rsub][lane_d + DL * j][0] = -kTwoPi * gsf * dz[j][0];
+            s_g[rsub][lane_d + DL * j][1] = -kTwoPi * gsf * dz[j][1];
+            s_g[rsub][lane_d + DL * j][2] = -kTwoPi * gsf * (dz[j][2] - 1.0);
         }
-        // g_uvw(row) = sum over directions of -2 pi gsf (dz - e_z), reduced in LDS.
-        s_g[rsub][lane_d][0] = -kTwoPi * gsf * dz0;
-        s_g[rsub][lane_d][1] = -kTwoPi * gsf * dz1;
-        s_g[rsub][lane_d][2] = -kTwoPi * gsf * (dz2 - 1.0);
         __syncthreads();
         if (lane_d < 3 && row_ok) {
             double s = 0.0;
-            for (int j = 0; j < DL; ++j) s += s_g[rsub][j][lane_d];
+            for (int j = 0; j < DW; ++j) s += s_g[rsub][j][lane_d];  // direction order
             g_uvw[3 * (size_t(t) * nbl + bl) + lane_d] = s;
         }
     }
     // Partials of this baseline's rows per rsub lane; the RP lanes are summed in order below.
     __syncthreads();
-    s_g[rsub][lane_d][0] = dir_ok ? gdz[0] : 0.0;
-    s_g[rsub][lane_d][1] = dir_ok ? gdz[1] : 0.0;
-    s_g[rsub][lane_d][2] = dir_ok ? gdz[2] : 0.0;
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+        for (int k = 0; k < 3; ++k) s_g[rsub][lane_d + DL * j][k] = dir_ok[j] ? gdz[j][k] : 0.0;
     __syncthreads();
-    if (rsub == 0 && dir_ok) {
-        for (int k = 0; k < 3; ++k) {
-            double s = 0.0;
-            for (int r = 0; r < RP; ++r) s += s_g[r][lane_d][k];
-            g_dz_partial[(size_t(bl) * D + lane_d) * 3 + k] = s;
+    if (rsub == 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            if (!dir_ok[j]) continue;
+            const uint32_t dj = lane_d + DL * j;
+            for (int k = 0; k < 3; ++k) {
+                double s = 0.0;
+                for (int r = 0; r < RP; ++r) s += s_g[r][dj][k];
+                g_dz_partial[(size_t(bl) * D + dj) * 3 + k] = s;
+            }
         }
     }
 }
@@ -235,21 +266,20 @@ void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, co
     const int32_t c64 = vis_dtype == LEAP_VIS_C64 ? 1 : 0;
     require(D >= 1 && D <= 256, LEAP_ERR_NOT_IMPLEMENTED, "the adjoint handles 1 to 256 directions per call");
     const dim3 grid(nbl);
+#define LEAP_ADJ(DL_, ND_)                                                                                  \
+    rotate_adjoint_kernel<DL_, ND_><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, \
+                                                               dkappa, g_uvw, g_dz_partial)
     if (D <= 16)
-        rotate_adjoint_kernel<16><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, dkappa,
-                                                            g_uvw, g_dz_partial);
+        LEAP_ADJ(8, 2);
     else if (D <= 32)
-        rotate_adjoint_kernel<32><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, dkappa,
-                                                            g_uvw, g_dz_partial);
+        LEAP_ADJ(32, 1);
     else if (D <= 64)
-        rotate_adjoint_kernel<64><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, dkappa,
-                                                            g_uvw, g_dz_partial);
+        LEAP_ADJ(64, 1);
     else if (D <= 128)
-        rotate_adjoint_kernel<128><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0,
-                                                             dkappa, g_uvw, g_dz_partial);
+        LEAP_ADJ(128, 1);
     else
-        rotate_adjoint_kernel<256><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0,
-                                                             dkappa, g_uvw, g_dz_partial);
+        LEAP_ADJ(256, 1);
+#undef LEAP_ADJ
     LEAP_HIP(hipGetLastError());
 }
 
