@@ -23,6 +23,8 @@ struct Cx {
     double x, y;
 };
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 __device__ inline Cx cmulx(Cx a, Cx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ inline Cx hstep(Cx a, Cx z, Cx v) {
     return {fma(a.x, z.x, fma(-a.y, z.y, v.x)), fma(a.x, z.y, fma(a.y, z.x, v.y))};
@@ -109,6 +111,36 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
             const uint32_t c0 = uint32_t(ch) * kCC;
             const uint32_t cn = min(uint32_t(kCC), C - c0);
             __syncthreads();
+            if (c64 && P == 4) {
+                // complex64, 4 pols: every element of the thread is loaded (two 16-byte loads
+                // each) before any is checked, so all of them are in flight together.  Slots past
+                // the rows or channels read a valid element (row 0 of the baseline, channel 0) and
+                // store zero.
+                constexpr int E = RP * kCC / kThreads;
+                f32x4 raw[E][2];
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t i = threadIdx.x + kThreads * e;
+                    const uint32_t r = i / kCC, c = i - r * kCC;
+                    const bool ok = tb + r < T && c < cn;
+                    const size_t el = ok ? (size_t(tb + r) * nbl + bl) * C + c0 + c : size_t(bl) * C;
+                    const f32x4* q = static_cast<const f32x4*>(vis) + 2 * el;
+                    raw[e][0] = q[0];
+                    raw[e][1] = q[1];
+                }
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t i = threadIdx.x + kThreads * e;
+                    const uint32_t r = i / kCC, c = i - r * kCC;
+                    const bool ok = tb + r < T && c < cn;
+                    bool finite = true;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) finite = finite && __builtin_isfinite(raw[e][h][k]);
+                    s_v[r][c] = (ok && finite) ? Cx{double(raw[e][0].x), double(raw[e][0].y)} : Cx{0.0, 0.0};
+                }
+            } else
             for (uint32_t i = threadIdx.x; i < RP * uint32_t(kCC); i += kThreads) {
                 const uint32_t r = i / kCC, c = i - r * kCC;
                 const uint32_t tt = tb + r;

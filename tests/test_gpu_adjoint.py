@@ -39,10 +39,12 @@ def test_adjoint_matches_oracle_gradient(leap, ctx, oracle, dtype, n_dirs, n_ch)
     assert np.abs(g_ant - go_ant).max() <= 1e-7 * np.abs(go_ant).max()
 
 
-def test_adjoint_zero_avgdata_matches_oracle(leap, ctx, oracle):
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_adjoint_zero_avgdata_matches_oracle(leap, ctx, oracle, dtype):
     """Baselines whose every (row, channel) holds a NaN have zero avgData; the loss and gradients
-    still follow the oracle's arg(exp(-i theta) avg) for them (a signed-zero phase)."""
-    s, ms, dirs, off, dp = problem(leap, 12, 4, 40, 6, 23)
+    still follow the oracle's arg(exp(-i theta) avg) for them (a signed-zero phase).  The NaNs
+    sit in pol 1, so the whole-channel finiteness test of the adjoint's staging is exercised."""
+    s, ms, dirs, off, dp = problem(leap, 12, 4, 40, 6, 23, dtype=dtype)
     vis = ms["vis"]
     nbl = ms["num_baselines"]
     for b in (3, 17, nbl - 2):
