@@ -33,6 +33,8 @@
 namespace leap {
 namespace {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 using namespace dev;
 
 constexpr int kThreads = 256;
@@ -277,6 +279,42 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
             const uint32_t c0 = uint32_t(ch) * CC;
             const uint32_t cn = min(uint32_t(CC), C - c0);
             __syncthreads();
+            // 16-direction tiles on a complex64 4-pol cube (the residual-loss forward pass): every
+            // element of the thread is loaded (two 16-byte loads each) before any is checked, so
+            // all are in flight together; slots past the rows or channels read a valid element
+            // (row 0 of the baseline, channel 0) and store zero.  Other tiles stage element by element.
+            bool staged = false;
+            if constexpr (DB == 16 && NP == 1 && sizeof(VT) == 4) {
+                if (P == 4) {
+                    constexpr int E = RP * CC / kThreads;
+                    f32x4 raw[E][2];
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        const uint32_t i = threadIdx.x + kThreads * e;
+                        const uint32_t r = i / CC, c = i - r * CC;
+                        const bool ok = tb + r < T && c < cn;
+                        const size_t el = ok ? (size_t(tb + r) * nbl + bl) * C + c0 + c : size_t(bl) * C;
+                        const LEAP_GLOBAL f32x4* q = reinterpret_cast<const LEAP_GLOBAL f32x4*>(vis) + 2 * el;
+                        raw[e][0] = q[0];
+                        raw[e][1] = q[1];
+                    }
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        const uint32_t i = threadIdx.x + kThreads * e;
+                        const uint32_t r = i / CC, c = i - r * CC;
+                        const bool ok = tb + r < T && c < cn;
+                        bool finite = true;
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) finite = finite && __builtin_isfinite(raw[e][h][k]);
+                        s_v[r][0][c] = (ok && finite) ? C2<double>{double(raw[e][0].x), double(raw[e][0].y)}
+                                                      : C2<double>{0.0, 0.0};
+                    }
+                    staged = true;
+                }
+            }
+            if (!staged)
             for (uint32_t i = threadIdx.x; i < RP * uint32_t(CC); i += kThreads) {
                 const uint32_t r = i / CC, c = i - r * CC;
                 const uint32_t tt = tb + r;
