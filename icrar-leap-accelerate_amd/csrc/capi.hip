@@ -9,6 +9,7 @@
 #include <sstream>
 #include <thread>
 
+#include "capi_common.h"
 #include "leap_internal.h"
 #include "phasor.h"
 #include "synth.h"
@@ -16,44 +17,6 @@
 using namespace leap;
 
 namespace {
-
-thread_local std::string g_error;  // failures without a context
-
-void set_error(const leap_ctx* ctx, const std::string& msg) {
-    if (ctx)
-        const_cast<leap_ctx*>(ctx)->error = msg;
-    g_error = msg;
-}
-
-template <typename F>
-int guarded(const leap_ctx* ctx, F&& f) {
-    try {
-        f();
-        return LEAP_OK;
-    } catch (const Error& e) {
-        set_error(ctx, e.what());
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        set_error(ctx, "host allocation failed");
-        return LEAP_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        set_error(ctx, e.what());
-        return LEAP_ERR_INTERNAL;
-    }
-}
-
-// Makes the context's device current for the scope of a call.
-struct DeviceScope {
-    int prev = -1;
-    explicit DeviceScope(int dev) {
-        LEAP_HIP(hipGetDevice(&prev));
-        if (prev != dev) LEAP_HIP(hipSetDevice(dev));
-    }
-    ~DeviceScope() {
-        int cur = -1;
-        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
-    }
-};
 
 leap_options resolve(const leap_options* o) {
     leap_options r{};
@@ -1161,6 +1124,7 @@ int leap_hip_create(leap_ctx** out, int32_t device) {
         auto ctx = std::make_unique<leap_ctx>();
         ctx->device = device;
         DeviceScope scope(device);
+        noalloc_init(device);
         LEAP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         LEAP_HIP(hipStreamCreateWithFlags(&ctx->pack_stream, hipStreamNonBlocking));
         LEAP_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));

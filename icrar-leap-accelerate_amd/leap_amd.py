@@ -165,7 +165,7 @@ class SynthConfig(ctypes.Structure):
 # Exported symbols of include/leap_hip.h (checked by the CPU test-suite).
 EXPORTS = [
     "leap_hip_create", "leap_hip_destroy", "leap_hip_last_error", "leap_hip_version",
-    "leap_hip_calibrate", "leap_hip_calibrate_md",
+    "leap_hip_calibrate", "leap_hip_calibrate_md", "leap_hip_calibrate_md_noalloc",
     "leap_hip_metadata_create", "leap_hip_metadata_destroy", "leap_hip_metadata_dims", "leap_hip_metadata_copy",
     "leap_hip_rotate_visibilities", "leap_hip_apply_inversion", "leap_hip_dd_matrix",
     "leap_synth_create", "leap_synth_destroy", "leap_synth_view", "leap_synth_sky", "leap_synth_fill_vis_host",
@@ -208,6 +208,9 @@ def lib():
     L.leap_hip_calibrate_md.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp,
                                         ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.POINTER(Options), SOLUTION_CB, ctypes.c_void_p]
+    L.leap_hip_calibrate_md_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView), _dp,
+                                                ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_void_p, _dp, _i32p]
     L.leap_hip_calibrate_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchSet), ctypes.c_uint32,
                                            ctypes.POINTER(Options), BATCH_CB, ctypes.c_void_p]
     _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -554,6 +557,18 @@ class Context:
                                              ctypes.byref(o), cbf, None)
         self._chk(rc)
         return np.array(cals), np.array(epochs)
+
+    def calibrate_noalloc(self, view, directions, md, cal_device, slice_=(0, -1, -1)):
+        """leap_hip_calibrate_md_noalloc: cal into the device buffer cal_device
+        ([solutions][directions][N] doubles, the caller's); returns epochs [solutions][2]."""
+        dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)
+        n_sol = ctypes.c_int32()
+        self._chk(lib().leap_hip_calibrate_md_noalloc(self.h, md.h, ctypes.byref(view), _dptr(dirs), len(dirs),
+                                                      *slice_, None, None, ctypes.byref(n_sol)))
+        epochs = np.zeros((n_sol.value, 2))
+        self._chk(lib().leap_hip_calibrate_md_noalloc(self.h, md.h, ctypes.byref(view), _dptr(dirs), len(dirs),
+                                                      *slice_, cal_device, _dptr(epochs), ctypes.byref(n_sol)))
+        return epochs
 
     def calibrate_batch(self, sets, opts=None):
         """sets: [(md, view, directions, slice_)]; returns per set (cal [solutions][D][N], epochs [solutions][2])."""

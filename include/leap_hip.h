@@ -193,6 +193,32 @@ int leap_hip_calibrate_md(leap_ctx* ctx, const leap_metadata* md, const leap_ms_
                           int32_t slice_interval, int32_t slice_end, const leap_options* opts,
                           leap_solution_cb callback, void* user);
 
+/* Bytes of the device scratch array behind leap_hip_calibrate_md_noalloc (one
+ * per GPU, held until the process ends). */
+#define LEAP_NOALLOC_SCRATCH_BYTES (16u << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): leap_hip_calibrate_md on its FP64 per-direction route,
+ * allocating nothing.  The direction table, dd matrices, one direction's
+ * avgData and the inversion's phases, cal1, dphi and GEMV partial sums live in
+ * a fixed device array of LEAP_NOALLOC_SCRATCH_BYTES per GPU; the results are
+ * bit-identical to leap_hip_calibrate_md with opts NULL.
+ *   ms         visibilities and uvw device-resident (vis_on_device = 1,
+ *              uvw_device set)
+ *   cal_out    device memory, [n_solutions][n_directions][n_antennas] doubles;
+ *              NULL only reports n_solutions_out and computes nothing
+ *   epochs_out host memory, [n_solutions][2] (start, end), or NULL
+ * Returns LEAP_ERR_RANGE when one solution's working set (16 B per direction
+ * and 72 B of dd, 16 B per baseline and pol, 8 B per row of A and A1 and per
+ * antenna, plus the GEMV partials) exceeds the scratch.  Every call on a GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (other threads,
+ * other contexts on the device) race.  Synchronous on return. */
+int leap_hip_calibrate_md_noalloc(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
+                                  const double* directions, uint32_t n_directions, int32_t slice_start,
+                                  int32_t slice_interval, int32_t slice_end, double* cal_out,
+                                  double* epochs_out, int32_t* n_solutions_out);
+
 /* ---- several measurement sets in one launch -------------------------------- */
 /* One measurement set of leap_hip_calibrate_batch: the metadata built for it by
  * leap_hip_metadata_create, its directions and its Slice (as leap_hip_calibrate). */
