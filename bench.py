@@ -63,6 +63,81 @@ def traffic_from_profile(name):
         return None
 
 
+def host_cpu():
+    """CPU model and core counts of this host (for the CPU baseline's label)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "nproc": usable, "logical_cpus": os.cpu_count()}
+
+
+def c0_config():
+    """The 128-antenna MWA-like set: 1.5 km disk, 8256 baselines with autos, 14 timesteps of 2 s, 48 channels
+    from 169.6 MHz at 40 kHz, 4 pols, complex128, 1% of baselines flagged."""
+    return dict(n_antennas=128, layout=0, array_radius_m=750.0, autocorrelations=1, n_timesteps=14,
+                integration_s=2.0, n_channels=48, freq_start_hz=169.6e6, freq_inc_hz=40e3, n_pols=4,
+                flag_fraction=0.01, seed=20201)
+
+
+def run_c0(args):
+    """The 128-antenna set in full: the oracle (1 thread) and the GPU (plain FP64 per direction, batched FP64),
+    1 and 64 directions, the cube resident in host memory for the oracle and in HBM for the GPU."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_bindings import Oracle  # the checker, used only for this reported baseline
+
+    synth = la.SynthMS(**c0_config())
+    synth.fill_host()
+    ms = synth.host_dict()
+    centre = ms["phase_centre"]
+    o = Oracle()
+    t = time.perf_counter()
+    omd = o.metadata(ms)
+    t_md = time.perf_counter() - t
+    out = {"workload": "128 antennas / 8256 baselines (autos) / 14 t / 48 ch / 4 pol, complex128 (355 MB), "
+                       "1 solution interval",
+           "cpu": {"kind": "port", "threads": 1, "host": host_cpu(), "metadata_s": round(t_md, 3)}, "gpu": {}}
+    for D in (1, 64):
+        dirs = directions(centre, D, seed=55)
+        t = time.perf_counter()
+        for d in dirs:  # the Calibrate loop (CpuLeapCalibrator.cc:132-139) over prebuilt metadata
+            omd.apply_inversion(omd.rotate(d))
+        dt = time.perf_counter() - t
+        out["cpu"][f"directions_per_s_{D}"] = round(D / dt, 3)
+        log(f"[c0] oracle {D} direction(s): {dt:.2f} s after metadata")
+    ctx = la.Context(0)
+    vis_ptr = ctx.malloc(synth.num_rows * synth.num_channels * synth.num_pols * 16)
+    uvw_ptr = ctx.malloc(synth.num_rows * 24)
+    ctx.synth_fill_device(synth, la.LEAP_VIS_C128, vis_ptr, uvw_ptr)
+    view = la.view_from_arrays(synth.host_dict(), vis_device=vis_ptr, vis_dtype=la.LEAP_VIS_C128, uvw_device=uvw_ptr)
+    md = ctx.metadata(view)
+    for name, D, opts in (("plain_fp64", 1, la.options()),
+                          ("plain_fp64", 64, la.options()),
+                          ("batched_fp64", 64, la.options(strategy=la.LEAP_STRATEGY_BATCHED))):
+        dirs = directions(centre, D, seed=56)
+        for _ in range(2):
+            ctx.calibrate(view, dirs, md=md, opts=opts)
+        reps = 10
+        t = time.perf_counter()
+        for _ in range(reps):
+            ctx.calibrate(view, dirs, md=md, opts=opts)
+        dt = (time.perf_counter() - t) / reps
+        out["gpu"][f"{name}_directions_per_s_{D}"] = round(D / dt, 1)
+    ctx.free(vis_ptr)
+    ctx.free(uvw_ptr)
+    ctx.close()
+    return out
+
+
 def cpu_baseline(ctx, synth, vis_ptr, vis_dtype, timesteps):
     """The oracle (single thread) on the first `timesteps` timesteps of the same cube, scaled in T."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -95,7 +170,8 @@ def cpu_baseline(ctx, synth, vis_ptr, vis_dtype, timesteps):
         "unit": "directions/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle (FP64 C++ restatement, 1 thread, {platform.processor() or platform.machine()}) on "
+        "host": host_cpu(),
+        "sample": (f"oracle (FP64 C++ restatement, 1 thread of {host_cpu()['model']}) on "
                    f"{timesteps} of {T} timesteps of the same cube, 1 direction; RotateVisibilities time "
                    f"{t_rot:.2f} s scaled x{T // timesteps} in T plus inversion {t_inv * 1e3:.1f} ms; "
                    f"one-off metadata (SVD pseudo-inverse) {t_md:.1f} s not counted"),
@@ -131,7 +207,7 @@ def copy_probe(gb):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed steps (the headline's batched leg runs steps/4)")
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of each leg")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="headline",
                     choices=["headline", "plain", "batched-fp64", "batched-mixed", "adjoint"],
@@ -154,6 +230,10 @@ def main():
     ap.add_argument("--host-cube", action="store_true",
                     help="batched modes: the cube stays in (pageable) host memory and every step stages it "
                          "(PCIe-inclusive rate; not the headline value, which has the cube resident in HBM)")
+    ap.add_argument("--no-error-check", dest="error_check", action="store_false",
+                    help="batched-mixed: skip the per-direction error against the plain FP64 path")
+    ap.add_argument("--no-c0", dest="c0", action="store_false",
+                    help="headline: skip the 128-antenna set timed in full on the oracle and the GPU")
     ap.add_argument("--cpu-timesteps", type=int, default=2, help="timesteps in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--copy-probe-gb", type=float, default=0.0,
                     help="only run device-to-device copies of this size (calibrates rocprofv3 FETCH_SIZE / "
@@ -172,14 +252,15 @@ def main():
         return 0
     if args.mode == "headline":
         plain = run_mode(args, "plain", 1, with_cpu=False)
-        mixed = run_mode(args, "batched-mixed", args.directions or 256, with_cpu=args.cpu_timesteps > 0,
-                         steps=max(1, args.steps // 4))
+        mixed = run_mode(args, "batched-mixed", args.directions or 256, with_cpu=args.cpu_timesteps > 0)
         out = mixed
         out["config"]["plain_fp64"] = {
             "directions_per_s": plain["value"], "ms_per_step": plain["ms_per_step"],
             "rotate_kernel_ms": plain["roofline"]["kernel_ms"], "roofline": plain["roofline"],
         }
         out["config"]["speedup_vs_plain_fp64"] = round(mixed["value"] / plain["value"], 2)
+        if args.c0:
+            out["config"]["c0"] = run_c0(args)
         out["config"]["workload"] = ("batched mixed-precision calibration (complex64 cube 51.7 GB resident in "
                                      f"HBM, {out['config']['directions_per_step']} directions per step, one pass "
                                      "over the cube) vs plain FP64 per-direction calibration (complex128 cube "
@@ -235,11 +316,13 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     for i in range(args.warmup):
         ctx.calibrate(view, dirs[i * D:(i + 1) * D], md=md, opts=opts)
         log(f"[{mode}] warmup step {i + 1}/{args.warmup} done")
-    rot_ms, pack_ms, staged = [], [], []
+    rot_ms, pack_ms, staged, step_s = [], [], [], []
     t0 = time.perf_counter()
     for i in range(steps):
         j = args.warmup + i
-        ctx.calibrate(view, dirs[j * D:(j + 1) * D], md=md, opts=opts)
+        ts = time.perf_counter()
+        ctx.calibrate(view, dirs[j * D:(j + 1) * D], md=md, opts=opts)  # returns after a stream sync
+        step_s.append(time.perf_counter() - ts)
         avg_ms, launches, _ = ctx.last_kernel_times()
         # device time of the rotate/average launches of one step (the plain path launches one per direction)
         rot_ms.append(avg_ms if mode == "plain" else avg_ms * launches)
@@ -320,6 +403,26 @@ def run_mode(args, mode, D, with_cpu, steps=None):
                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(pack_bytes / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if p_ms > 0 else None}
 
+    error = None
+    if mode == "batched-mixed" and args.error_check:
+        # Outside the timed region: the first timed step's directions again, against the plain FP64 path
+        # (the reference's arithmetic, held to 1e-9 rad of the oracle by the parity tests) on the same cube.
+        j = args.warmup
+        sel = dirs[j * D:(j + 1) * D]
+        got, _ = ctx.calibrate(view, sel, md=md, opts=opts)
+        t = time.perf_counter()
+        want, _ = ctx.calibrate(view, sel, md=md, opts=la.options())
+        log(f"[{mode}] FP64 reference of {D} directions in {time.perf_counter() - t:.2f} s")
+        err = np.abs((got[0] - want[0] + np.pi) % (2 * np.pi) - np.pi)  # [D][N]
+        per_max = err.max(axis=1)
+        per_rms = np.sqrt((err ** 2).mean(axis=1))
+        error = {"against": "plain FP64 GPU path on the same complex64 cube (1e-9 rad of the FP64 oracle in the "
+                            "parity tests)",
+                 "directions": D, "max_rad": float(per_max.max()), "rms_rad": float(np.sqrt((err ** 2).mean())),
+                 "per_direction_max_rad": [float(f"{x:.3g}") for x in per_max],
+                 "per_direction_rms_rad": [float(f"{x:.3g}") for x in per_rms]}
+        log(f"[{mode}] wrapped phase error vs FP64: max {error['max_rad']:.3e} rad, rms {error['rms_rad']:.3e} rad")
+
     cpu = None
     if with_cpu:
         log(f"[{mode}] CPU baseline: oracle on {args.cpu_timesteps} timesteps")
@@ -352,6 +455,9 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": round(elapsed / steps * 1e3, 3),
+        "step_ms": {"median": round(float(np.median(step_s)) * 1e3, 3), "min": round(min(step_s) * 1e3, 3),
+                    "max": round(max(step_s) * 1e3, 3), "mean": round(float(np.mean(step_s)) * 1e3, 3)},
+        "value_median": round(D / float(np.median(step_s)), 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -364,6 +470,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
             "stations": 256, "baselines": nbl, "timesteps": 128, "channels": C, "pols": P, "rows": R,
             "vis_dtype": "complex64" if elem == 8 else "complex128", "directions_per_step": D, "mode": mode,
             "tile": tile, "metadata_s": round(t_md, 3),
+            **({"error_vs_fp64": error} if error else {}),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <cctype>
+#include <cstdint>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -51,6 +52,50 @@ class JsonReader {
         if (*m_p != c) Fail(std::string("expected '") + c + "'");
         ++m_p;
     }
+    // Four hex digits after "\\u" (m_p at the 'u'); leaves m_p after them.
+    uint32_t ParseHex4() {
+        uint32_t v = 0;
+        for (int i = 1; i <= 4; ++i) {
+            const char c = m_p[i];
+            uint32_t d;
+            if (c >= '0' && c <= '9') d = uint32_t(c - '0');
+            else if (c >= 'a' && c <= 'f') d = uint32_t(c - 'a' + 10);
+            else if (c >= 'A' && c <= 'F') d = uint32_t(c - 'A' + 10);
+            else Fail("invalid \\u escape");
+            v = v * 16 + d;
+        }
+        m_p += 5;
+        return v;
+    }
+    // A \\u escape (m_p at the 'u'), with a UTF-16 surrogate pair joined into
+    // one code point; a lone surrogate is an error.
+    uint32_t ParseEscapedCodePoint() {
+        const uint32_t hi = ParseHex4();
+        if (hi >= 0xDC00 && hi <= 0xDFFF) Fail("lone low surrogate in \\u escape");
+        if (hi < 0xD800 || hi > 0xDBFF) return hi;
+        if (m_p[0] != '\\' || m_p[1] != 'u') Fail("high surrogate without a low surrogate");
+        ++m_p;
+        const uint32_t lo = ParseHex4();
+        if (lo < 0xDC00 || lo > 0xDFFF) Fail("high surrogate without a low surrogate");
+        return 0x10000 + ((hi - 0xD800) << 10) + (lo - 0xDC00);
+    }
+    static void AppendUtf8(std::string& out, uint32_t cp) {
+        if (cp < 0x80) {
+            out += char(cp);
+        } else if (cp < 0x800) {
+            out += char(0xC0 | (cp >> 6));
+            out += char(0x80 | (cp & 0x3F));
+        } else if (cp < 0x10000) {
+            out += char(0xE0 | (cp >> 12));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        } else {
+            out += char(0xF0 | (cp >> 18));
+            out += char(0x80 | ((cp >> 12) & 0x3F));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        }
+    }
     std::string ParseString() {
         Expect('"');
         std::string out;
@@ -66,9 +111,12 @@ class JsonReader {
                     case 'n': out += '\n'; break;
                     case 'r': out += '\r'; break;
                     case 't': out += '\t'; break;
+                    case 'u': AppendUtf8(out, ParseEscapedCodePoint()); continue;
                     default: Fail("unsupported escape");
                 }
                 ++m_p;
+            } else if (static_cast<unsigned char>(*m_p) < 0x20) {
+                Fail("unescaped control character in string");
             } else {
                 out += *m_p++;
             }

@@ -93,6 +93,39 @@ static int selftest() {
         for (size_t i = 0; i < five.size(); ++i)
             if (!(five[i] == SphericalDirection(double(i), double(i)))) ++bad;
     }
+    // String escapes as RapidJSON reads them: \uXXXX (surrogate pairs joined) decoded to UTF-8;
+    // lone surrogates, bad hex digits and raw control characters rejected.
+    {
+        const std::pair<const char*, std::string> accept[] = {
+            {"\"a\\u0041\"", "aA"},
+            {"\"\\u00e9t\\u00C9\"", "\xC3\xA9t\xC3\x89"},
+            {"\"\\u20AC\"", "\xE2\x82\xAC"},
+            {"\"\\ud83d\\ude00\"", "\xF0\x9F\x98\x80"},
+            {"\"/data/\\u6e2c\\u5b9a.ms\"", "/data/\xE6\xB8\xAC\xE5\xAE\x9A.ms"},
+            {"\"\\u0000x\"", std::string("\0x", 2)},
+        };
+        for (const auto& c : accept) {
+            try {
+                const JsonValue v = JsonReader::Parse(c.first);
+                if (v.kind != JsonValue::String || v.str != c.second) {
+                    std::cerr << "string escape " << c.first << " decoded wrongly\n";
+                    ++bad;
+                }
+            } catch (const icrar::exception&) {
+                std::cerr << "string escape " << c.first << " rejected\n";
+                ++bad;
+            }
+        }
+        for (const char* reject : {"\"\\u12\"", "\"\\u12G4\"", "\"\\ud83d\"", "\"\\ud83dx\"",
+                                   "\"\\ud83d\\u0041\"", "\"\\ude00\"", "\"a\tb\"", "\"\\x41\""}) {
+            try {
+                JsonReader::Parse(reject);
+                std::cerr << "string escape accepted " << reject << "\n";
+                ++bad;
+            } catch (const icrar::exception&) {
+            }
+        }
+    }
     cpu::CalibrationCollection cc({cpu::Calibration(1.0, 2.0, {cpu::BeamCalibration({0.5, -0.25}, {0.0, 1.5})})});
     std::ostringstream os;
     cc.Serialize(os);

@@ -368,10 +368,14 @@ def view_from_arrays(ms, vis=None, vis_device=None, vis_dtype=None, uvw_device=N
     v.freq_inc_hz = ms["freq_inc_hz"]
     v.phase_centre_ra_rad, v.phase_centre_dec_rad = ms["phase_centre"]
     keep = {}
+    nbl, rows = int(ms["num_baselines"]), int(ms["num_rows"])
+    need = {"antenna1": nbl, "antenna2": nbl, "flag": nbl, "time": rows, "uvw": 3 * rows}
     for name, dt, ct in (("antenna1", np.int32, ctypes.c_int32), ("antenna2", np.int32, ctypes.c_int32),
                          ("time", np.float64, ctypes.c_double), ("flag", np.uint8, ctypes.c_uint8),
                          ("uvw", np.float64, ctypes.c_double)):
         a = np.ascontiguousarray(ms[name], dtype=dt)
+        if a.size < need[name]:
+            raise ValueError(f"{name} holds {a.size} values, the view reads {need[name]}")
         keep[name] = a
         setattr(v, name, a.ctypes.data_as(ctypes.POINTER(ct)))
     if vis_device is not None:
@@ -381,6 +385,12 @@ def view_from_arrays(ms, vis=None, vis_device=None, vis_dtype=None, uvw_device=N
     else:
         vis = ms["vis"] if vis is None else vis
         vis = np.ascontiguousarray(vis)
+        if vis.dtype not in (np.complex64, np.complex128):
+            raise ValueError(f"visibilities must be complex64 or complex128, not {vis.dtype}")
+        want = int(ms["num_rows"]) * int(ms["num_channels"]) * int(ms["num_pols"])
+        if vis.size != want:
+            raise ValueError(f"visibilities hold {vis.size} values, the view needs num_rows x num_channels x "
+                             f"num_pols = {want}")
         keep["vis"] = vis
         v.vis = vis.ctypes.data
         v.vis_on_device = 0

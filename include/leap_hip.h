@@ -94,11 +94,13 @@ typedef enum leap_precision {
     /* FP64 phase, FP64 sincos, FP64 sums: the reference's arithmetic. */
     LEAP_PRECISION_FP64 = 0,
     /* Visibilities stored as complex64 (or converted on load), the phase
-     * argument and its reduction in FP64, the channel sums as a full-row FP64
-     * Horner evaluation (FP64 FMA issues at the rate of non-packed FP32 on
-     * gfx950), Ad * dphi as split-3 x BF16 MFMA with FP32 accumulation.  The
-     * fp32_blocks option selects FP32 sincos / Horner / Kahan sums instead.
-     * Needs the batched strategy. */
+     * argument and its reduction in FP64, Ad * dphi as split-3 x BF16 MFMA
+     * with FP32 accumulation.  The channel sums of pol 0 follow channel_sum:
+     * by default (LEAP_CHANNEL_SUM_AUTO) the NUFFT for 96..512 channels and at
+     * least 16 directions, else a full-row FP64 Horner evaluation (FP64 FMA
+     * issues at the rate of non-packed FP32 on gfx950).  The fp32_blocks
+     * option selects FP32 sincos / Horner / Kahan sums instead of the FP64
+     * Horner.  Needs the batched strategy. */
     LEAP_PRECISION_MIXED = 1
 } leap_precision;
 
@@ -117,8 +119,11 @@ typedef enum leap_channel_sum {
     /* FP64 Horner: four FMAs per channel and direction, exact to FP64 rounding. */
     LEAP_CHANNEL_SUM_HORNER = 1,
     /* Type-2 non-uniform FFT per row (one inverse FFT of 256..1024 points, then W = 10 or 12
-     * grid points per direction), relative error about 1e-11 of sum |v|; mixed precision,
-     * pol 0, packed path, 1..512 channels. */
+     * grid points per direction); mixed precision, pol 0, 1..512 channels.  A device-resident
+     * complex64 cube with 1, 2 or 4 pols is read by the NUFFT kernel itself; other sources
+     * (host-resident or complex128 cubes, 3 pols) are first packed to pol-0 rows.  Error of a
+     * row's channel sum: about 1e-11 of sum |v| (the tests bound avgData by 5e-10 of the sum
+     * of |v| over the interval's rows and channels). */
     LEAP_CHANNEL_SUM_NUFFT = 2
 } leap_channel_sum;
 

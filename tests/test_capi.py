@@ -151,3 +151,19 @@ def test_synth_enu_to_uvw_is_the_uvw_geometry(leap):
     for t in range(5):
         assert np.abs(uvw[t] - enu_d @ M[t].T).max() < 1e-9 * np.abs(uvw).max()
     assert np.abs(enu_d[:, 2]).max() < 1e-9  # a flat array
+
+
+def test_view_from_arrays_checks_dtype_and_sizes(leap):
+    """Arrays the library would misread or read past are refused before any call."""
+    s = leap.SynthMS(n_antennas=6, n_timesteps=2, n_channels=3, seed=1)
+    s.fill_host()
+    ms = s.host_dict()
+    leap.view_from_arrays(ms)
+    for bad in (ms["vis"].real.copy(), ms["vis"].astype(np.clongdouble), ms["vis"][:-1]):
+        with pytest.raises(ValueError, match="visibilities"):
+            leap.view_from_arrays(ms, vis=bad)
+    for name in ("antenna1", "flag", "time", "uvw"):
+        short = dict(ms)
+        short[name] = ms[name][:-1]
+        with pytest.raises(ValueError, match=name):
+            leap.view_from_arrays(short)
