@@ -225,7 +225,7 @@ def main():
     ap.add_argument("--direct-cube", action="store_true",
                     help="batched: run the Horner kernel straight from the cube (no pol-0 pack)")
     ap.add_argument("--pack-timesteps", type=int, default=0, help="timesteps per pack chunk (0 = automatic)")
-    ap.add_argument("--channel-sum", default="auto", choices=["auto", "horner", "nufft"],
+    ap.add_argument("--channel-sum", default="auto", choices=["auto", "horner", "nufft", "nufft-fp32"],
                     help="mixed precision: channel sums by FP64 Horner or the NUFFT (auto: NUFFT where supported)")
     ap.add_argument("--host-cube", action="store_true",
                     help="batched modes: the cube stays in (pageable) host memory and every step stages it "
@@ -284,7 +284,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     opts.lane_directions = args.lane_directions
     opts.direct_cube = 1 if args.direct_cube else 0
     opts.pack_timesteps = args.pack_timesteps
-    opts.channel_sum = {"auto": 0, "horner": 1, "nufft": 2}[args.channel_sum]
+    opts.channel_sum = {"auto": 0, "horner": 1, "nufft": 2, "nufft-fp32": 3}[args.channel_sum]
 
     cfg = c1_config()
     log(f"[{mode}] building synthetic measurement set (host columns)")
@@ -335,37 +335,41 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     k_ms = float(np.mean(rot_ms))
     tile = ctx.last_direction_block()
     route_cs, route_dc = ctx.last_route() if mode != "plain" else (0, 0)
-    nufft = mode == "batched-mixed" and route_cs == la.LEAP_CHANNEL_SUM_NUFFT
+    nufft = mode == "batched-mixed" and route_cs in (la.LEAP_CHANNEL_SUM_NUFFT, la.LEAP_CHANNEL_SUM_NUFFT_FP32)
+    nufft32 = nufft and route_cs == la.LEAP_CHANNEL_SUM_NUFFT_FP32
     packed = (mode != "plain" and not args.fp32_blocks and not route_dc and tile in (641, 1282, 2564))
     if nufft:
         # NUFFT channel sums: per row one N-point complex FFT (5 N log2 N flops,
         # the conventional count) and per row and direction the weight
         # polynomials (W^2 / 2 FMAs), the W-tap interpolation (2 W FMAs) and
         # NUFFT_DIR_FLOPS of shift factor, grid coordinate, phasor and
-        # accumulation (FMA = 2 flops).
-        plan = la.nufft_plan(C)
+        # accumulation (FMA = 2 flops).  The roofline's bound is HBM: the
+        # algorithmic bytes (the cube once, uvw, pol 0 of avgData) over the
+        # kernel time against 8 TB/s; the vector-ALU rate is reported beside it.
+        plan = la.nufft_plan(C, fp32=nufft32)
         n_grid, width = plan["N"], plan["W"]
         per_row = 5 * n_grid * np.log2(n_grid)
         per_dir = 2 * (width * width // 2 + 2 * width) + NUFFT_DIR_FLOPS
         alg_flops = int(R * (per_row + D * per_dir))
-        achieved = alg_flops / (k_ms * 1e-3) / 1e12
+        tflops = alg_flops / (k_ms * 1e-3) / 1e12
         src_bytes = R * C * P * elem if route_dc else R * C * 16
         # calibrations on this route keep pol 0 of avgData only (16 B per direction and baseline)
         alg_bytes = src_bytes + R * 24 + D * nbl * 16
-        roofline = {"kernel": "rotate_average_nufft",
-                    "bound": "valu-fp64", "achieved": round(achieved, 2), "peak": VALU_FP64_PEAK_TFLOPS,
-                    # measured with rocprofv3 counters, not live: the CU's LDS (FFT exchanges and
-                    # the interpolation's gathers) is the tighter limit
-                    "limiter": "LDS 80% busy, FP64 VALU 71% of issue slots "
-                               "(profiles/rotate_average_nufft_c1.md)",
-                    "unit": "TFLOP/s", "frac": round(achieved / VALU_FP64_PEAK_TFLOPS, 4),
-                    "traffic": traffic_from_profile("rotate_average_nufft_c1_traffic.json"),
-                    "kernel_ms": round(k_ms, 4), "algorithmic_flops_per_launch": alg_flops,
-                    "nufft": {"grid": n_grid, "width": width, "fft_flops_per_row": int(per_row),
-                              "flops_per_row_direction": per_dir,
+        gbs = alg_bytes / (k_ms * 1e-3) / 1e9
+        kname = "rotate_average_nufft<float>" if nufft32 else "rotate_average_nufft<double>"
+        prof = "rotate_average_nufft32_c1" if nufft32 else "rotate_average_nufft_c1"
+        roofline = {"kernel": kname, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profile(prof + "_traffic.json"),
+                    "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": alg_bytes,
+                    # measured with rocprofv3 counters, not live
+                    "limiter": f"vector ALU and LDS (profiles/{prof}.md)",
+                    "valu": {"achieved_tflops": round(tflops, 2), "peak_tflops": VALU_FP64_PEAK_TFLOPS,
+                             "frac": round(tflops / VALU_FP64_PEAK_TFLOPS, 4),
+                             "algorithmic_flops_per_launch": alg_flops},
+                    "nufft": {"transform": "fp32" if nufft32 else "fp64", "grid": n_grid, "width": width,
+                              "fft_flops_per_row": int(per_row), "flops_per_row_direction": per_dir,
                               "source": "complex64 cube" if route_dc else "packed pol-0 rows"},
-                    "hbm_gbs": round(alg_bytes / (k_ms * 1e-3) / 1e9, 1),
-                    "hbm_frac": round(alg_bytes / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     # the direct sum's 8 flops per (row, channel, direction) over the kernel time
                     "direct_sum_equivalent_tflops": round(8 * R * C * D / (k_ms * 1e-3) / 1e12, 1)}
     elif mode == "plain":
@@ -461,7 +465,8 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if mode != "batched-mixed" else ("c64 storage, f64 phase and NUFFT" if nufft else
+        "dtype": "f64" if mode != "batched-mixed" else (("c64 storage, f64 phase, f32 NUFFT transform" if nufft32
+                                                         else "c64 storage, f64 phase and NUFFT") if nufft else
                                                         "c64 storage, f64 phase and Horner"),
         "data": "synthetic (seeded point sources x per-antenna phase screens + noise, generated on the device)",
         "config": {

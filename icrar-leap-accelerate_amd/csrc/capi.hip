@@ -41,9 +41,10 @@ leap_options resolve(const leap_options* o) {
     require(r.pack_timesteps >= 0, LEAP_ERR_INVALID_ARGUMENT, "pack_timesteps must be >= 0");
     require(r.workgroups_per_cu >= -1 && r.workgroups_per_cu <= 64, LEAP_ERR_INVALID_ARGUMENT,
             "workgroups per CU must be -1 (one work item each), 0 (tuned) or 1 to 64");
-    require(r.channel_sum >= LEAP_CHANNEL_SUM_AUTO && r.channel_sum <= LEAP_CHANNEL_SUM_NUFFT,
-            LEAP_ERR_INVALID_ARGUMENT, "channel_sum must be 0 (automatic), 1 (Horner) or 2 (NUFFT)");
-    if (r.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+    require(r.channel_sum >= LEAP_CHANNEL_SUM_AUTO && r.channel_sum <= LEAP_CHANNEL_SUM_NUFFT_FP32,
+            LEAP_ERR_INVALID_ARGUMENT,
+            "channel_sum must be 0 (automatic), 1 (Horner), 2 (NUFFT) or 3 (NUFFT, FP32 transform)");
+    if (is_nufft(r.channel_sum)) {
         require(r.precision == LEAP_PRECISION_MIXED, LEAP_ERR_INVALID_ARGUMENT,
                 "the NUFFT channel sums are a mixed-precision method");
         require(!r.all_pols && !r.fp32_blocks, LEAP_ERR_INVALID_ARGUMENT,
@@ -200,7 +201,7 @@ leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, u
                                 ms.num_channels >= kNufftMinChannels && D >= kNufftMinDirections
                             ? LEAP_CHANNEL_SUM_NUFFT
                             : LEAP_CHANNEL_SUM_HORNER;
-    if (r.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+    if (is_nufft(r.channel_sum)) {
         require(nufft_supported(ms.num_channels), LEAP_ERR_INVALID_ARGUMENT,
                 "the NUFFT channel sums support 1.." + std::to_string(kNufftMaxChannels) + " channels");
         const bool cube_ok = nufft_cube_supported(ms.vis_dtype, ms.num_pols);
@@ -241,9 +242,10 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
         return opts.direction_block * 10 + (fp32 ? 1 : opts.lane_directions);
     if (opts.direction_block != 0 && opts.lane_directions == 0) return opts.direction_block * 10 + 1;
     const bool packed = use_packed(opts);
-    const bool nufft = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
+    const bool nufft = is_nufft(opts.channel_sum);
     const TuneKey key{opts.precision * 4 + (opts.fp32_blocks ? 2 : 0) + (opts.lane_directions ? 1 : 0) +
-                          16 * opts.lane_directions + (packed ? kTunePackedBit : 0) + (nufft ? kTuneNufftBit : 0),
+                          16 * opts.lane_directions + (packed ? kTunePackedBit : 0) + (nufft ? kTuneNufftBit : 0) +
+                          (opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? kTuneNufft32Bit : 0),
                       ms.vis_dtype, opts.all_pols, D, T, md.channels, md.pols};
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) return it->second;
@@ -279,7 +281,8 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
         for (int rep = 0; rep < 2; ++rep) {  // the first repetition warms the code object
             LEAP_HIP(hipEventRecord(a, ctx->stream));
             if (nufft)
-                launch_rotate_nufft(ctx->stream, tile, md.nufft, md.d_nufft.as<double>(),
+                launch_rotate_nufft(ctx->stream, tile, md.nufft_plan(opts.channel_sum),
+                                    md.nufft_tables(opts.channel_sum),
                                     packed ? ctx->d_packed.ptr : rows.vis, !packed, rows.uvw, ctx->d_dd.as<double>(),
                                     D, T, md.nbl, probe_bl, md.channels, md.pols, md.pols, md.freq_start_hz,
                                     md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
@@ -337,8 +340,8 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
     ctx->d_packed.reserve(buf_elems * (n_chunks > 1 ? 2 : 1) * 2 * sizeof(double));
     const size_t vis_row_bytes = vis_elem_bytes(ms) * md.channels;
     const bool host = rows.vis == nullptr;
-    const bool nufft = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
-    const double* scale = nufft ? md.nufft_scale() : nullptr;
+    const bool nufft = is_nufft(opts.channel_sum);
+    const double* scale = nufft ? md.nufft_scale(opts.channel_sum) : nullptr;
     if (nufft) ctx->d_nufft_tail.reserve(nufft_tail_bytes(D));
     double* buf[2] = {ctx->d_packed.as<double>(), ctx->d_packed.as<double>() + 2 * buf_elems};
     auto pack = [&](uint32_t k, hipStream_t st) {
@@ -359,7 +362,8 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
         const uint32_t t0 = k * tc, nt = std::min(tc, T - t0);
         record_rotate_begin(ctx);
         if (nufft)
-            launch_rotate_nufft(ctx->stream, tile, md.nufft, md.d_nufft.as<double>(), buf[k % 2], false,
+            launch_rotate_nufft(ctx->stream, tile, md.nufft_plan(opts.channel_sum),
+                                md.nufft_tables(opts.channel_sum), buf[k % 2], false,
                                 rows.uvw + size_t(t0) * md.nbl * 3, ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl,
                                 md.channels, md.pols, avg_pols, md.freq_start_hz, md.freq_inc_hz,
                                 ctx->d_avg.as<double>(), k > 0 ? 1 : 0, ctx->d_nufft_tail.as<double>(),
@@ -406,7 +410,7 @@ int32_t batched_tile(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view&
     if (!use_packed(opts)) return tuned_direction_block(ctx, md, ms, rows, T, D, opts);
     const uint32_t tc = std::min(T, pack_chunk(T, md.nbl, md.channels, opts));
     const size_t n = size_t(tc) * md.nbl * md.channels;
-    const double* scale = opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT ? md.nufft_scale() : nullptr;
+    const double* scale = is_nufft(opts.channel_sum) ? md.nufft_scale(opts.channel_sum) : nullptr;
     return tuned_direction_block(ctx, md, ms, rows, tc, D, opts, [&] {
         ctx->d_packed.reserve(n * 2 * sizeof(double));
         launch_pack_pol0(ctx->stream, rows.vis, ms.vis_dtype, n, md.pols, ctx->d_packed.as<double>(), scale,
@@ -421,7 +425,7 @@ int32_t batched_tile(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view&
 uint32_t rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& ms, const DeviceRows& rows,
                            uint32_t T, uint32_t D, const leap_options& opts, int32_t tile = 0,
                            bool pol0_only = false) {
-    const bool nufft_route = opts.strategy == LEAP_STRATEGY_BATCHED && opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT;
+    const bool nufft_route = opts.strategy == LEAP_STRATEGY_BATCHED && is_nufft(opts.channel_sum);
     const uint32_t avg_pols = pol0_only && nufft_route ? 1u : md.pols;
     const size_t per_dir = size_t(md.nbl) * md.pols * 2;  // room for the full layout (autotune probes)
     ctx->d_avg.reserve(per_dir * D * sizeof(double));
@@ -430,12 +434,13 @@ uint32_t rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms
         return avg_pols;
     }
     require(rows.vis != nullptr, LEAP_ERR_INTERNAL, "visibilities were not staged");
-    if (opts.strategy == LEAP_STRATEGY_BATCHED && opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+    if (opts.strategy == LEAP_STRATEGY_BATCHED && is_nufft(opts.channel_sum)) {
         const int32_t db = tile ? tile : batched_tile(ctx, md, ms, rows, T, D, opts);
         ctx->last_direction_block = db;
         record_rotate_begin(ctx);
         ctx->d_nufft_tail.reserve(nufft_tail_bytes(D));
-        launch_rotate_nufft(ctx->stream, db, md.nufft, md.d_nufft.as<double>(), rows.vis, true, rows.uvw,
+        launch_rotate_nufft(ctx->stream, db, md.nufft_plan(opts.channel_sum), md.nufft_tables(opts.channel_sum),
+                            rows.vis, true, rows.uvw,
                             ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, avg_pols,
                             md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), 0,
                             ctx->d_nufft_tail.as<double>(), nufft_tail_bytes(D));
@@ -505,6 +510,74 @@ void check_metadata_matches(const leap_metadata& md, const leap_ms_view& ms) {
             LEAP_ERR_INVALID_ARGUMENT, "metadata was built for a different measurement set");
 }
 
+// Copies a host vector into a device buffer (at least one element is reserved).
+template <typename T>
+void upload(hipStream_t s, DeviceBuffer& buf, const std::vector<T>& v) {
+    buf.reserve(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty()) LEAP_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// Each antenna's rows of a phase matrix (baseline rows only, in row order):
+// entries 2 k + 1 where the antenna is antenna2 of row k, 2 k where antenna1.
+void phase_matrix_csr(const PhaseMatrix& pm, std::vector<int32_t>& off, std::vector<int32_t>& ent) {
+    const int32_t N = pm.cols, nI = pm.rows - 1;
+    off.assign(size_t(N) + 1, 0);
+    ent.assign(size_t(2) * size_t(std::max(nI, 0)), 0);
+    for (int32_t k = 0; k < nI; ++k) {
+        ++off[size_t(pm.ant1[k]) + 1];
+        ++off[size_t(pm.ant2[k]) + 1];
+    }
+    for (int32_t a = 0; a < N; ++a) off[a + 1] += off[a];
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int32_t k = 0; k < nI; ++k) {
+        ent[size_t(fill[pm.ant1[k]]++)] = 2 * k;
+        ent[size_t(fill[pm.ant2[k]]++)] = 2 * k + 1;
+    }
+}
+
+// G+ = pinv(A^T A) of a phase matrix into d_Gp (N x N, device), from the device
+// copies of its row pairs and per-antenna rows; the components come from the host.
+void gram_pinv_device(leap_ctx* ctx, const PhaseMatrix& pm, const int2* d_pairs, const int32_t* d_off,
+                      const int32_t* d_ent, double* d_Gp) {
+    hipStream_t s = ctx->stream;
+    const int32_t N = pm.cols;
+    const GramComponents comps = gram_components(pm);
+    std::vector<int64_t> ws_off(comps.has_c.size() + 1, 0);
+    for (size_t c = 0; c < comps.has_c.size(); ++c) {
+        const int64_t n = comps.off[c + 1] - comps.off[c];
+        ws_off[c + 1] = ws_off[c] + 2 * n * n;
+    }
+    DeviceBuffer d_comp_off, d_members, d_has_c, d_ws_off, d_ws, d_G, d_status;
+    upload(s, d_comp_off, comps.off);
+    upload(s, d_members, comps.members);
+    upload(s, d_has_c, comps.has_c);
+    upload(s, d_ws_off, ws_off);
+    d_ws.reserve(std::max<size_t>(1, size_t(ws_off.back())) * sizeof(double));
+    d_G.reserve(size_t(N) * N * sizeof(double));
+    const std::vector<int32_t> zero(1, 0);
+    upload(s, d_status, zero);
+    GramPinv g{};
+    g.N = N;
+    g.constraint = pm.constraint;
+    g.csr_off = d_off;
+    g.csr_ent = d_ent;
+    g.pairs = d_pairs;
+    g.n_components = int32_t(comps.has_c.size());
+    g.comp_off = d_comp_off.as<int32_t>();
+    g.members = d_members.as<int32_t>();
+    g.has_c = d_has_c.as<uint8_t>();
+    g.ws_off = d_ws_off.as<int64_t>();
+    g.ws = d_ws.as<double>();
+    g.G = d_G.as<double>();
+    g.Gp = d_Gp;
+    g.status = d_status.as<int32_t>();
+    launch_gram_pinv(s, g);
+    int32_t status = 0;
+    LEAP_HIP(hipMemcpyAsync(&status, d_status.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LEAP_HIP(hipStreamSynchronize(s));  // before the scratch buffers go
+    require(status == 0, LEAP_ERR_INTERNAL, "phase matrix Gram block is not positive definite");
+}
+
 leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t ref_ant, double thr, int use_cache) {
     validate_view(ms);
     auto md = std::make_unique<leap_metadata>();
@@ -522,57 +595,77 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
     const int32_t n = int32_t(ms.num_baselines);
     md->A1 = phase_matrix(ms.antenna1, ms.antenna2, fg.data(), n, int32_t(md->reference_antenna));  // :92
     md->A = phase_matrix(ms.antenna1, ms.antenna2, fg.data(), n, -1);                                // :97
-    md->G1p = gram_pinv(md->A1);                                                                      // :111
-    if (use_cache) {                                                                                  // :112-120
-        const uint64_t key = hash_phase_matrix(md->A);
-        auto it = ctx->gram_cache.find(key);
-        if (it == ctx->gram_cache.end()) it = ctx->gram_cache.emplace(key, gram_pinv(md->A)).first;
-        md->Gp = it->second;
-    } else {
-        md->Gp = gram_pinv(md->A);                                                                    // :123
+    md->ad_degenerate = null_components(md->A) > 0;    // :129-132
+    md->ad1_degenerate = null_components(md->A1) > 0;  // :133-136
+    if (ctx->log) {
+        if (md->ad_degenerate) ctx->log(ctx->log_user, LEAP_LOG_WARNING, "Ad is degenerate");
+        if (md->ad1_degenerate) ctx->log(ctx->log_user, LEAP_LOG_WARNING, "Ad1 is degenerate");
     }
     const int32_t K = md->A.rows, N = md->A.cols, K1 = md->A1.rows;
-    // Device copies.
+    // Device copies: antenna pairs and baselines of the rows of A and A1, each
+    // antenna's rows (CSR), wavelengths.
     std::vector<int2> pairs(size_t(K - 1)), pairs1(size_t(K1 - 1));
     for (int32_t k = 0; k + 1 < K; ++k) pairs[k] = make_int2(md->A.ant1[k], md->A.ant2[k]);
     for (int32_t k = 0; k + 1 < K1; ++k) pairs1[k] = make_int2(md->A1.ant1[k], md->A1.ant2[k]);
+    std::vector<int32_t> off, ent, off1, ent1;
+    phase_matrix_csr(md->A, off, ent);
+    phase_matrix_csr(md->A1, off1, ent1);
     std::vector<double> lambda(md->channels);
     for (uint32_t c = 0; c < md->channels; ++c)
         lambda[c] = kSpeedOfLight / (md->freq_start_hz + int32_t(c) * md->freq_inc_hz);  // MetaData.h:95-98
-    md->d_pairs.reserve(std::max<size_t>(1, pairs.size()) * sizeof(int2));
-    md->d_I1.reserve(std::max<size_t>(1, md->A1.I.size()) * sizeof(int32_t));
-    md->d_lambda.reserve(std::max<size_t>(1, lambda.size()) * sizeof(double));
-    md->d_AdT.reserve(size_t(K) * N * sizeof(double));
-    md->d_Ad1T.reserve(size_t(K1) * N * sizeof(double));
-    md->d_Gp.reserve(size_t(N) * N * sizeof(double) * 2);
-    DeviceBuffer d_pairs1;
-    d_pairs1.reserve(std::max<size_t>(1, pairs1.size()) * sizeof(int2));
     hipStream_t s = ctx->stream;
-    if (!pairs.empty())
-        LEAP_HIP(hipMemcpyAsync(md->d_pairs.ptr, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-    if (!pairs1.empty())
-        LEAP_HIP(hipMemcpyAsync(d_pairs1.ptr, pairs1.data(), pairs1.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-    if (!md->A1.I.empty())
-        LEAP_HIP(hipMemcpyAsync(md->d_I1.ptr, md->A1.I.data(), md->A1.I.size() * sizeof(int32_t),
-                                hipMemcpyHostToDevice, s));
-    if (!lambda.empty())
-        LEAP_HIP(hipMemcpyAsync(md->d_lambda.ptr, lambda.data(), lambda.size() * sizeof(double),
-                                hipMemcpyHostToDevice, s));
+    upload(s, md->d_pairs, pairs);
+    upload(s, md->d_pairs1, pairs1);
+    upload(s, md->d_I, md->A.I);
+    upload(s, md->d_I1, md->A1.I);
+    upload(s, md->d_csr_off, off);
+    upload(s, md->d_csr_ent, ent);
+    DeviceBuffer d_off1, d_ent1;
+    upload(s, d_off1, off1);
+    upload(s, d_ent1, ent1);
+    upload(s, md->d_lambda, lambda);
+    // The pseudo-inverses (MetaData.cc:111, :123) as G+ = pinv(A^T A), Ad = G+ A^T, on the
+    // device; with use_cache A's G+ is kept in the context by a hash of A (:112-120).
+    md->d_Gp.reserve(size_t(N) * N * sizeof(double) * 2);
     double* gp = md->d_Gp.as<double>();
     double* g1p = gp + size_t(N) * N;
-    LEAP_HIP(hipMemcpyAsync(gp, md->Gp.data(), size_t(N) * N * sizeof(double), hipMemcpyHostToDevice, s));
-    LEAP_HIP(hipMemcpyAsync(g1p, md->G1p.data(), size_t(N) * N * sizeof(double), hipMemcpyHostToDevice, s));
+    gram_pinv_device(ctx, md->A1, md->d_pairs1.as<int2>(), d_off1.as<int32_t>(), d_ent1.as<int32_t>(), g1p);
+    if (use_cache) {
+        const uint64_t key = hash_phase_matrix(md->A);
+        auto it = ctx->gram_cache.find(key);
+        if (it == ctx->gram_cache.end()) {
+            gram_pinv_device(ctx, md->A, md->d_pairs.as<int2>(), md->d_csr_off.as<int32_t>(),
+                             md->d_csr_ent.as<int32_t>(), gp);
+            std::vector<double> host(size_t(N) * N);
+            LEAP_HIP(hipMemcpyAsync(host.data(), gp, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            LEAP_HIP(hipStreamSynchronize(s));
+            ctx->gram_cache.emplace(key, std::move(host));
+        } else {
+            LEAP_HIP(hipMemcpyAsync(gp, it->second.data(), it->second.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    s));
+        }
+    } else {
+        gram_pinv_device(ctx, md->A, md->d_pairs.as<int2>(), md->d_csr_off.as<int32_t>(), md->d_csr_ent.as<int32_t>(),
+                         gp);
+    }
+    md->d_AdT.reserve(size_t(K) * N * sizeof(double));
+    md->d_Ad1T.reserve(size_t(K1) * N * sizeof(double));
     launch_assemble_ad(s, gp, N, md->d_pairs.as<int2>(), K - 1, md->A.constraint, md->d_AdT.as<double>());
-    launch_assemble_ad(s, g1p, N, d_pairs1.as<int2>(), K1 - 1, md->A1.constraint, md->d_Ad1T.as<double>());
+    launch_assemble_ad(s, g1p, N, md->d_pairs1.as<int2>(), K1 - 1, md->A1.constraint, md->d_Ad1T.as<double>());
+    LEAP_HIP(hipStreamSynchronize(s));  // the host vectors above end with this function
     md->d_AdT_planes.reserve(bf16_planes_bytes(uint32_t(N), K));
     launch_split_b(s, md->d_AdT.as<double>(), K, N, md->d_AdT_planes.ptr);
-    std::vector<double> nufft_tables;
+    std::vector<double> tables32, tables64;
     if (nufft_supported(md->channels)) {
-        md->nufft = make_nufft_plan(md->channels);
-        nufft_tables.assign(&md->nufft.coef[0][0], &md->nufft.coef[0][0] + kNufftCoefDoubles);
-        nufft_tables.insert(nufft_tables.end(), md->nufft.scale.begin(), md->nufft.scale.end());
-        md->d_nufft.reserve(nufft_tables.size() * sizeof(double));
-        LEAP_HIP(hipMemcpyAsync(md->d_nufft.ptr, nufft_tables.data(), nufft_tables.size() * sizeof(double),
+        md->nufft32 = make_nufft_plan(md->channels, true);
+        md->nufft64 = make_nufft_plan(md->channels, false);
+        tables32 = nufft_tables(md->nufft32);
+        tables64 = nufft_tables(md->nufft64);
+        md->d_nufft32.reserve(tables32.size() * sizeof(double));
+        md->d_nufft64.reserve(tables64.size() * sizeof(double));
+        LEAP_HIP(hipMemcpyAsync(md->d_nufft32.ptr, tables32.data(), tables32.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+        LEAP_HIP(hipMemcpyAsync(md->d_nufft64.ptr, tables64.data(), tables64.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
     }
     LEAP_HIP(hipStreamSynchronize(s));
@@ -1192,6 +1285,22 @@ int leap_hip_metadata_create(leap_ctx* ctx, const leap_ms_view* ms, int32_t ref_
     });
 }
 
+int leap_hip_set_log_callback(leap_ctx* ctx, leap_log_cb cb, void* user) {
+    return guarded(ctx, [&] {
+        require(ctx != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null context");
+        ctx->log = cb;
+        ctx->log_user = user;
+    });
+}
+
+int leap_hip_metadata_diagnostics(const leap_metadata* md, int32_t* ad_degenerate, int32_t* ad1_degenerate) {
+    return guarded(nullptr, [&] {
+        require(md != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null metadata");
+        if (ad_degenerate) *ad_degenerate = md->ad_degenerate ? 1 : 0;
+        if (ad1_degenerate) *ad1_degenerate = md->ad1_degenerate ? 1 : 0;
+    });
+}
+
 void leap_hip_metadata_destroy(leap_metadata* md) {
     if (!md) return;
     int prev = -1;
@@ -1472,9 +1581,11 @@ int leap_phasor64(const double* theta, uint32_t n, double* cos_out, double* sin_
     });
 }
 
-int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef, double* scale) {
+namespace {
+int nufft_plan_out(uint32_t C, bool fp32, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
+                   double* scale) {
     return guarded(nullptr, [&] {
-        const NufftPlan p = make_nufft_plan(C);
+        const NufftPlan p = make_nufft_plan(C, fp32);
         if (N) *N = p.N;
         if (W) *W = p.W;
         if (h) *h = p.h;
@@ -1484,6 +1595,16 @@ int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* b
                 for (int m = 0; m < kNufftMaxWidth; ++m) coef[j * kNufftMaxWidth + m] = p.coef[j][m];
         if (scale) std::copy(p.scale.begin(), p.scale.end(), scale);
     });
+}
+}  // namespace
+
+int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef, double* scale) {
+    return nufft_plan_out(C, false, N, W, h, beta, coef, scale);
+}
+
+int leap_nufft_plan32(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
+                      double* scale) {
+    return nufft_plan_out(C, true, N, W, h, beta, coef, scale);
 }
 
 double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {
@@ -1529,114 +1650,10 @@ extern "C" int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg
 }
 
 // ---------------------------------------------------------------------------
-// Calibration-residual loss and its gradient.
+// Calibration-residual loss and its gradient: forward rotate/average and
+// inversion, the loss, the inversion's adjoint and the adjoint of
+// rotate/average, all on the device; the host sums the per-direction losses.
 // ---------------------------------------------------------------------------
-namespace {
-
-using cplx = std::complex<double>;
-
-// x wrapped into [-pi, pi)
-double wrap_pi(double x) {
-    return x - kTwoPi * std::floor((x + kPi) / kTwoPi);
-}
-
-// Row 2 of the DD matrix and its derivatives in ra and dec (MetaData.cc:166-203 differentiated).
-void dz_derivs(double ra0, double dec0, double ra, double dec, double dra[3], double ddec[3]) {
-    const double s1 = std::sin(kPi / 2.0 - dec0), c1 = std::cos(kPi / 2.0 - dec0);
-    const double s2 = std::sin(ra - ra0), c2 = std::cos(ra - ra0);
-    const double s3 = std::sin(-kPi / 2.0 + dec), c3 = std::cos(-kPi / 2.0 + dec);
-    dra[0] = -s3 * c2;
-    dra[1] = -s3 * s2 * c1;
-    dra[2] = s3 * s2 * s1;
-    ddec[0] = -c3 * s2;
-    ddec[1] = c3 * c2 * c1 - s3 * s1;
-    ddec[2] = -c3 * c2 * s1 - s3 * c1;
-}
-
-// Products with Ad = pinv(A) = G+ A^T (G+ the N x N column-major pseudo-inverse of the Gram
-// matrix A^T A): Ad v = G+ (A^T v) and Ad^T g = A (G+^T g), O(K + N^2) each instead of the
-// O(K N) of a dense Ad.  A row k < K - 1 is +1 at ant1[k] and -1 at ant2[k]; the last row is +1
-// at the constraint column.
-void add_ad_times(const leap::PhaseMatrix& pm, const std::vector<double>& Gp, const double* v, double* out) {
-    const int32_t N = pm.cols, K = pm.rows;
-    std::vector<double> x(size_t(N), 0.0);
-    for (int32_t k = 0; k + 1 < K; ++k) {
-        x[pm.ant1[k]] += v[k];
-        x[pm.ant2[k]] -= v[k];
-    }
-    if (K > 0) x[pm.constraint] += v[K - 1];
-    for (int32_t a = 0; a < N; ++a) {
-        const double* col = Gp.data() + size_t(a) * N;
-        for (int32_t i = 0; i < N; ++i) out[i] += col[i] * x[a];
-    }
-}
-
-// y = G+^T g: (Ad^T g)[k] = y[ant1[k]] - y[ant2[k]] for k < K - 1.
-std::vector<double> gram_t_times(const std::vector<double>& Gp, int32_t N, const double* g) {
-    std::vector<double> y(size_t(N), 0.0);
-    for (int32_t a = 0; a < N; ++a) {
-        const double* col = Gp.data() + size_t(a) * N;
-        double t = 0.0;
-        for (int32_t i = 0; i < N; ++i) t += col[i] * g[i];
-        y[a] = t;
-    }
-    return y;
-}
-
-// Forward inversion, loss and its adjoint for one direction (pol 0 of avgData, with
-// ph[b] = arg avg[b] computed on the device).  The phase of exp(-i theta) avg is taken as
-// wrap(arg avg - theta), equal to it modulo 2 pi; where avg is 0 the product itself is formed.
-// Returns the loss; abar[b] receives dL/dRe avg + i dL/dIm avg.
-double inversion_loss_adjoint(const leap_metadata& md, const cplx* avg, const double* ph, cplx* abar) {
-    const leap::PhaseMatrix& A = md.A;
-    const leap::PhaseMatrix& A1 = md.A1;
-    const int32_t N = A.cols, K = A.rows, K1 = A1.rows, nI = K - 1;
-    std::vector<double> phi1(size_t(K1), 0.0), cal1(size_t(N), 0.0);
-    for (int32_t k = 0; k + 1 < K1; ++k) phi1[k] = ph[A1.I[k]];
-    add_ad_times(A1, md.G1p, phi1.data(), cal1.data());
-    std::vector<double> dphi(size_t(K), 0.0), cal(cal1);
-    for (int32_t n = 0; n < nI; ++n) {
-        const double acal1 = cal1[A.ant1[n]] - cal1[A.ant2[n]];
-        // row n of avgData; a zero entry keeps the signed-zero phase of the rotated product
-        dphi[n] = avg[n] == cplx(0, 0) ? std::arg(std::exp(cplx(0.0, -kTwoPi * acal1)) * avg[n])
-                                       : wrap_pi(ph[n] - kTwoPi * acal1);
-    }
-    add_ad_times(A, md.Gp, dphi.data(), cal.data());
-    double loss = 0.0;
-    std::vector<double> res(static_cast<size_t>(nI));
-    for (int32_t n = 0; n < nI; ++n) {
-        res[n] = wrap_pi(cal[A.ant1[n]] - cal[A.ant2[n]] - ph[A.I[n]]);
-        loss += res[n] * res[n];
-    }
-    // adjoint
-    const uint32_t nbl = md.nbl;
-    std::vector<double> g_arg(nbl, 0.0), g_cal(size_t(N), 0.0);
-    for (int32_t n = 0; n < nI; ++n) {
-        g_arg[A.I[n]] -= 2.0 * res[n];
-        g_cal[A.ant1[n]] += 2.0 * res[n];
-        g_cal[A.ant2[n]] -= 2.0 * res[n];
-    }
-    std::vector<double> g_cal1(g_cal);
-    const std::vector<double> y = gram_t_times(md.Gp, N, g_cal.data());
-    for (int32_t n = 0; n < nI; ++n) {
-        const double g = y[A.ant1[n]] - y[A.ant2[n]];  // (Ad^T g_cal)[n]
-        g_arg[n] += g;
-        const double g_acal1 = -kTwoPi * g;
-        g_cal1[A.ant1[n]] += g_acal1;
-        g_cal1[A.ant2[n]] -= g_acal1;
-    }
-    const std::vector<double> y1 = gram_t_times(md.G1p, N, g_cal1.data());
-    for (int32_t k = 0; k + 1 < K1; ++k) g_arg[A1.I[k]] += y1[A1.ant1[k]] - y1[A1.ant2[k]];
-    // i g / conj(z) = i g z / |z|^2
-    for (uint32_t b = 0; b < nbl; ++b) {
-        const double x = avg[b].real(), y = avg[b].imag(), m = x * x + y * y;
-        abar[b] = m == 0.0 ? cplx(0, 0) : cplx(-g_arg[b] * y / m, g_arg[b] * x / m);
-    }
-    return loss;
-}
-
-}  // namespace
-
 extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
                                            const double* directions, const double* offsets, uint32_t D,
                                            const double* antenna_delta, const double* enu_to_uvw, double* loss_out,
@@ -1678,69 +1695,78 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         launch_rotate_batched(s, LEAP_PRECISION_FP64, tile, false, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(),
                               D, T, nbl, nbl, md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
                               ctx->d_avg.as<double>(), 0);
-        // pol 0 of avgData, gathered on the device (the avgData adjoint buffer is free until the backward)
-        DeviceBuffer& d_avg0 = ctx->d_adj_abar;
+        // the inversion (CpuLeapCalibrator.cc:159-178) by the calibration's own kernels
+        leap_options inv{};
+        invert_directions(ctx, *md, D, inv);
+        // pol 0 of avgData and its arguments
+        DeviceBuffer& d_avg0 = ctx->d_adj_avg0;
         d_avg0.reserve(size_t(D) * nbl * 3 * sizeof(double));
         launch_gather_pol0(s, ctx->d_avg.as<double>(), D * nbl, md->pols, d_avg0.as<double>());
-        ctx->h_adj_avg.reserve(size_t(D) * nbl * 3 * sizeof(double));  // D nbl complex values, then their arguments
-        const double* h_avg0 = ctx->h_adj_avg.as<double>();
-        LEAP_HIP(hipMemcpyAsync(ctx->h_adj_avg.ptr, d_avg0.ptr, size_t(D) * nbl * 3 * sizeof(double),
-                                hipMemcpyDeviceToHost, s));
-        LEAP_HIP(hipStreamSynchronize(s));
-        // inversion, loss and its adjoint on the host, directions split over threads; the
-        // per-direction losses are added in direction order
-        ctx->h_adj_abar.reserve(size_t(D) * nbl * 2 * sizeof(double));
-        double* h_abar = ctx->h_adj_abar.as<double>();
-        std::vector<double> dir_loss(D, 0.0);
-        auto invert_range = [&](uint32_t d_begin, uint32_t d_end) {
-            std::vector<cplx> avg0(nbl), ab(nbl);
-            for (uint32_t d = d_begin; d < d_end; ++d) {
-                for (uint32_t b = 0; b < nbl; ++b)
-                    avg0[b] = cplx(h_avg0[2 * (size_t(d) * nbl + b)], h_avg0[2 * (size_t(d) * nbl + b) + 1]);
-                dir_loss[d] = inversion_loss_adjoint(*md, avg0.data(), h_avg0 + 2 * size_t(D) * nbl + size_t(d) * nbl,
-                                                     ab.data());
-                for (uint32_t b = 0; b < nbl; ++b) {
-                    h_abar[2 * (size_t(d) * nbl + b)] = ab[b].real();
-                    h_abar[2 * (size_t(d) * nbl + b) + 1] = ab[b].imag();
-                }
-            }
+        // loss and the adjoint of the inversion, on the device
+        const int32_t K = md->A.rows, K1 = md->A1.rows, nI = K - 1;
+        const bool want_grad = grad_offsets || grad_antenna;
+        const size_t rb = size_t(std::max(1, (nI + 255) / 256));
+        const size_t n_inv = size_t(D) * (size_t(std::max(nI, 1)) * 2 + rb + 1 + size_t(nbl) + 4 * size_t(N));
+        ctx->d_adj_inv.reserve(n_inv * sizeof(double));
+        ctx->d_adj_abar.reserve(size_t(D) * nbl * 2 * sizeof(double));
+        double* w = ctx->d_adj_inv.as<double>();
+        auto take = [&](size_t n) {
+            double* p = w;
+            w += n;
+            return p;
         };
-        const uint32_t workers = std::min<uint32_t>(D, std::clamp(std::thread::hardware_concurrency(), 1u, 16u));
-        if (workers <= 1) {
-            invert_range(0, D);
-        } else {
-            std::vector<std::thread> pool;
-            std::vector<std::exception_ptr> errors(workers);
-            for (uint32_t w = 0; w < workers; ++w)
-                pool.emplace_back([&, w] {
-                    try {
-                        invert_range(uint32_t(uint64_t(w) * D / workers), uint32_t(uint64_t(w + 1) * D / workers));
-                    } catch (...) {
-                        errors[w] = std::current_exception();
-                    }
-                });
-            for (auto& t : pool) t.join();
-            for (auto& e : errors)
-                if (e) std::rethrow_exception(e);
-        }
+        InversionAdjoint ia{};
+        ia.D = D;
+        ia.nbl = nbl;
+        ia.K = K;
+        ia.N = N;
+        ia.K1 = K1;
+        ia.cal = ctx->d_cal.as<double>();
+        ia.avg0 = d_avg0.as<double>();
+        ia.ph = d_avg0.as<double>() + 2 * size_t(D) * nbl;
+        ia.pairs = md->d_pairs.as<int2>();
+        ia.I = md->d_I.as<int32_t>();
+        ia.pairs1 = md->d_pairs1.as<int2>();
+        ia.I1 = md->d_I1.as<int32_t>();
+        ia.csr_off = md->d_csr_off.as<int32_t>();
+        ia.csr_ent = md->d_csr_ent.as<int32_t>();
+        ia.Gp = md->d_Gp.as<double>();
+        ia.G1p = md->d_Gp.as<double>() + size_t(N) * N;
+        ia.res = take(size_t(D) * std::max(nI, 1));
+        ia.part = take(size_t(D) * rb);
+        ia.loss = take(D);
+        ia.g_arg = take(size_t(D) * nbl);
+        ia.g_cal = take(size_t(D) * N);
+        ia.y = take(size_t(D) * N);
+        ia.gacal1 = take(size_t(D) * std::max(nI, 1));
+        ia.g_cal1 = take(size_t(D) * N);
+        ia.y1 = take(size_t(D) * N);
+        ia.abar = want_grad ? ctx->d_adj_abar.as<double>() : nullptr;
+        launch_inversion_adjoint(s, ia);
+        ctx->h_adj_out.reserve(size_t(D) * 3 * sizeof(double));
+        double* h_out = ctx->h_adj_out.as<double>();
+        LEAP_HIP(hipMemcpyAsync(h_out, ia.loss, size_t(D) * sizeof(double), hipMemcpyDeviceToHost, s));
+        LEAP_HIP(hipStreamSynchronize(s));
         double loss = 0.0;
-        for (uint32_t d = 0; d < D; ++d) loss += dir_loss[d];
+        for (uint32_t d = 0; d < D; ++d) loss += h_out[d];  // directions in order
         *loss_out = loss;
-        if (!grad_offsets && !grad_antenna) return;
+        if (!want_grad) return;
         // adjoint of rotate -> average on the device
         DeviceBuffer &d_abar = ctx->d_adj_abar, &d_guvw = ctx->d_adj_guvw, &d_gdzp = ctx->d_adj_gdzp,
                      &d_gdz = ctx->d_adj_gdz;
-        d_abar.reserve(size_t(D) * nbl * 2 * sizeof(double));
         d_guvw.reserve(size_t(R) * 3 * sizeof(double));
         d_gdzp.reserve(size_t(nbl) * D * 3 * sizeof(double));
-        d_gdz.reserve(size_t(D) * 3 * sizeof(double));
-        LEAP_HIP(hipMemcpyAsync(d_abar.ptr, h_abar, size_t(D) * nbl * 2 * sizeof(double), hipMemcpyHostToDevice, s));
+        d_gdz.reserve(size_t(D) * 5 * sizeof(double));  // g_dz (3 D), then dL/d(ra, dec) (2 D)
         launch_rotate_adjoint(s, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(), d_abar.as<double>(), D, T, nbl,
                               md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz, d_guvw.as<double>(),
                               d_gdzp.as<double>());
-        launch_reduce_gdz(s, d_gdzp.as<double>(), nbl, D, d_gdz.as<double>());
-        std::vector<double> g_dz(size_t(D) * 3);
-        LEAP_HIP(hipMemcpyAsync(g_dz.data(), d_gdz.ptr, g_dz.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        // dL/d(dz row of dd) reduced over baselines and mapped to dL/d(ra, dec) (MetaData.cc:166-203
+        // differentiated) on the device
+        launch_reduce_gdz_offsets(s, d_gdzp.as<double>(), nbl, D, md->phase_centre_ra, md->phase_centre_dec,
+                                  ctx->d_dirs.as<double>(), d_gdz.as<double>());
+        if (grad_offsets)
+            LEAP_HIP(hipMemcpyAsync(h_out + D, d_gdz.as<double>() + 3 * size_t(D), size_t(D) * 2 * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
         if (grad_antenna) {
             // baselines of each antenna: +1 where it is antenna2, -1 where antenna1
             std::vector<int32_t> off(size_t(N) + 1, 0), bls, sgn;
@@ -1774,13 +1800,6 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
             LEAP_HIP(hipStreamSynchronize(s));
         }
         LEAP_HIP(hipStreamSynchronize(s));
-        if (grad_offsets) {
-            for (uint32_t d = 0; d < D; ++d) {
-                double dra[3], ddec[3];
-                dz_derivs(md->phase_centre_ra, md->phase_centre_dec, dirs[2 * d], dirs[2 * d + 1], dra, ddec);
-                grad_offsets[2 * d] = g_dz[3 * d] * dra[0] + g_dz[3 * d + 1] * dra[1] + g_dz[3 * d + 2] * dra[2];
-                grad_offsets[2 * d + 1] = g_dz[3 * d] * ddec[0] + g_dz[3 * d + 1] * ddec[1] + g_dz[3 * d + 2] * ddec[2];
-            }
-        }
+        if (grad_offsets) std::copy(h_out + D, h_out + 3 * size_t(D), grad_offsets);
     });
 }

@@ -206,6 +206,47 @@ std::vector<double> gram_pinv(const PhaseMatrix& pm) {
     return Gp;
 }
 
+GramComponents gram_components(const PhaseMatrix& pm) {
+    const int32_t N = pm.cols;
+    UnionFind uf(N);
+    std::vector<int32_t> degree(static_cast<size_t>(N), 0);
+    for (size_t k = 0; k < pm.ant1.size(); ++k) {
+        uf.unite(pm.ant1[k], pm.ant2[k]);
+        ++degree[pm.ant1[k]];
+        ++degree[pm.ant2[k]];
+    }
+    std::vector<std::vector<int32_t>> comps(static_cast<size_t>(N));
+    for (int32_t a = 0; a < N; ++a) comps[uf.find(a)].push_back(a);
+    GramComponents out;
+    out.off.push_back(0);
+    for (const auto& S : comps) {
+        if (S.empty()) continue;
+        const bool has_c = std::find(S.begin(), S.end(), pm.constraint) != S.end();
+        if (S.size() == 1 && !has_c && degree[S[0]] == 0) continue;  // unused antenna: G+ = 0
+        out.members.insert(out.members.end(), S.begin(), S.end());
+        out.off.push_back(int32_t(out.members.size()));
+        out.has_c.push_back(has_c ? 1 : 0);
+    }
+    return out;
+}
+
+// Components of the baseline graph that do not hold the constraint antenna
+// (an unused antenna is one of its own): the dimension of the null space of A.
+// Ad A is the projection I - sum_S 1_S 1_S^T / |S| over them, so
+// ||Ad A - I||_F = sqrt(k) and ||Ad A||_F = sqrt(N - k): the reference's test
+// (Ad A).isApprox(I, 0.001) (model/cpu/MetaData.cc:129-136) fails exactly when
+// k >= 1 (for N < 1e6 k).
+int32_t null_components(const PhaseMatrix& pm) {
+    const int32_t N = pm.cols;
+    UnionFind uf(N);
+    for (size_t k = 0; k < pm.ant1.size(); ++k) uf.unite(pm.ant1[k], pm.ant2[k]);
+    const int32_t c = uf.find(pm.constraint);
+    int32_t k = 0;
+    for (int32_t a = 0; a < N; ++a)
+        if (uf.find(a) == a && a != c) ++k;
+    return k;
+}
+
 // MeasurementSet::GetFilteredBaselines (ms/MeasurementSet.cc:181-240).
 std::vector<uint8_t> filtered_baselines(const leap_ms_view& ms, double threshold) {
     const uint32_t nbl = ms.num_baselines;

@@ -12,6 +12,8 @@
 //   dL/d dz_d   += -2 pi dL/dsf(r, d) uvw(r)            (per baseline, reduced later)
 //   dL/d uvw(r)  = sum_d -2 pi dL/dsf(r, d) (dz_d - e_z)
 // and the antenna gradient is dL/d dp_a = sum_t M_t^T (sum_{b: a2 = a} g_uvw - sum_{b: a1 = a} g_uvw).
+#include <algorithm>
+
 #include "leap_internal.h"
 
 namespace leap {
@@ -241,6 +243,25 @@ __global__ __launch_bounds__(256) void reduce_gdz_kernel(const double* __restric
     if (threadIdx.x == 0) out[i] = s_red[0];
 }
 
+// dL/d(ra, dec) of direction d from g_dz[d] = dL/d(row 2 of its DD matrix): the
+// derivatives of dd row 2 = R_x(-pi/2 + dec) R_z(ra - ra0) R_x(pi/2 - dec0) row 2
+// (MetaData.cc:166-203) in ra and dec; dirs[d] = (ra, dec).  One thread per direction.
+__global__ void gdz_offsets_kernel(const double* __restrict__ g_dz, uint32_t D, double ra0, double dec0,
+                                   const double* __restrict__ dirs, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    double s1, c1, s2, c2, s3, c3;
+    sincos(kPi / 2.0 - dec0, &s1, &c1);
+    sincos(dirs[2 * d] - ra0, &s2, &c2);
+    sincos(-kPi / 2.0 + dirs[2 * d + 1], &s3, &c3);
+    const double dra[3] = {-s3 * c2, -s3 * s2 * c1, s3 * s2 * s1};
+    const double ddec[3] = {-c3 * s2, c3 * c2 * c1 - s3 * s1, -c3 * c2 * s1 - s3 * c1};
+    const double* g = g_dz + 3 * size_t(d);
+    out[2 * size_t(d)] = g[0] * dra[0] + g[1] * dra[1] + g[2] * dra[2];
+    out[2 * size_t(d) + 1] = g[0] * ddec[0] + g[1] * ddec[1] + g[2] * ddec[2];
+}
+
 // For the n = D nbl (direction, baseline) entries of P pols each: out[2 i], out[2 i + 1] = pol 0
 // of avg[i], and out[2 n + i] = its argument atan2(im, re).
 __global__ void gather_pol0_kernel(const double* __restrict__ avg, uint32_t n, uint32_t P, double* __restrict__ out) {
@@ -283,7 +304,192 @@ __global__ __launch_bounds__(256) void antenna_grad_kernel(const double* __restr
     if (threadIdx.x < 3) out[3 * a + threadIdx.x] = s_red[0][threadIdx.x];
 }
 
+// ---------------------------------------------------------------------------
+// The inversion, loss and their adjoint for D directions (layouts [d][...]).
+//
+// Forward (CpuLeapCalibrator.cc:159-178, then the loss): with ph[b] = arg avg[b]
+// (pol 0), cal1 = Ad1 phi1, dphi, cal = Ad [dphi; 0] + cal1 from the inversion
+// kernels, the residual of row n < K - 1 is
+//   res[n] = wrap(cal[a1(n)] - cal[a2(n)] - ph[I(n)]),   loss = sum_n res[n]^2.
+// Backward, with Ad = G+ A^T (G+ the pseudo-inverse of the Gram matrix A^T A):
+//   g_arg[I(n)] = -2 res[n];  g_cal = A_top^T (2 res)
+//   y = G+ g_cal;  g_arg[n] += (A y)[n] = y[a1(n)] - y[a2(n)]       (row n of avgData)
+//   g_cal1 = g_cal + A_top^T (-2 pi (A y))
+//   y1 = G1+ g_cal1;  g_arg[I1(k)] += y1[a1'(k)] - y1[a2'(k)]
+//   abar[b] = i g_arg[b] / conj(avg[b])   (d arg z = Im(dz / z))
+// The antenna sums A^T v are gathers over each antenna's rows (CSR, in row
+// order), so every sum has a fixed order and no atomics are used.
+// ---------------------------------------------------------------------------
+__device__ inline double wrap_pi(double x) {
+#pragma clang fp contract(off)
+    return x - kTwoPi * floor((x + kPi) / kTwoPi);
+}
+
+// res[d][n] and per-workgroup sums of res^2 (fixed-order tree) into part[d][block].
+__global__ __launch_bounds__(256) void inv_residual_kernel(const double* __restrict__ cal, int32_t N,
+                                                           const int2* __restrict__ pairs, const int32_t* __restrict__ I,
+                                                           int32_t nI, const double* __restrict__ ph, uint32_t nbl,
+                                                           double* __restrict__ res, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double s_red[256];
+    const int32_t n = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    double r2 = 0.0;
+    if (n < nI) {
+        const int2 p = pairs[n];
+        const double* c = cal + size_t(d) * N;
+        const double r = wrap_pi(c[p.x] - c[p.y] - ph[size_t(d) * nbl + I[n]]);
+        res[size_t(d) * nI + n] = r;
+        r2 = r * r;
+    }
+    s_red[threadIdx.x] = r2;
+    __syncthreads();
+    for (uint32_t stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) s_red[threadIdx.x] += s_red[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[size_t(d) * gridDim.x + blockIdx.x] = s_red[0];
+}
+
+// loss[d] = sum of the workgroup sums in block order.
+__global__ void inv_loss_kernel(const double* __restrict__ part, uint32_t blocks, uint32_t D,
+                                double* __restrict__ loss) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    double s = 0.0;
+    for (uint32_t k = 0; k < blocks; ++k) s += part[size_t(d) * blocks + k];
+    loss[d] = s;
+}
+
+// g_arg[d][b] = 0 for every baseline, then -2 res[n] at b = I(n) (I is one-to-one).
+__global__ void inv_garg_init_kernel(double* __restrict__ g_arg, uint32_t nbl) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbl) g_arg[size_t(blockIdx.y) * nbl + b] = 0.0;
+}
+
+__global__ void inv_garg_res_kernel(const double* __restrict__ res, const int32_t* __restrict__ I, int32_t nI,
+                                    uint32_t nbl, double* __restrict__ g_arg) {
+#pragma clang fp contract(off)
+    const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    if (n >= nI) return;
+    double& g = g_arg[size_t(d) * nbl + I[n]];
+    g -= 2.0 * res[size_t(d) * nI + n];
+}
+
+// out[d][a] = (base ? base[d][a] : 0) + sum over antenna a's rows e (row order) of
+// sign_e * scale * src[d][row_e]; csr_ent[e] = 2 row + (sign < 0).
+__global__ void inv_antenna_sum_kernel(const double* __restrict__ src, int32_t nI, double scale,
+                                       const int32_t* __restrict__ csr_off, const int32_t* __restrict__ csr_ent,
+                                       int32_t N, const double* __restrict__ base, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    if (a >= N) return;
+    double acc = base ? base[size_t(d) * N + a] : 0.0;
+    const double* x = src + size_t(d) * nI;
+    for (int32_t e = csr_off[a]; e < csr_off[a + 1]; ++e) {
+        const int32_t v = csr_ent[e];
+        const double t = scale * x[v >> 1];
+        if (v & 1)
+            acc -= t;
+        else
+            acc += t;
+    }
+    out[size_t(d) * N + a] = acc;
+}
+
+// y[d][a] = sum_i Gp[a N + i] g[d][i] (i in order; Gp is symmetric, so column a is row a).
+__global__ void inv_gram_gemv_kernel(const double* __restrict__ Gp, int32_t N, const double* __restrict__ g,
+                                     double* __restrict__ y) {
+#pragma clang fp contract(off)
+    const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    if (a >= N) return;
+    const double* col = Gp + size_t(a) * N;
+    const double* x = g + size_t(d) * N;
+    double t = 0.0;
+    for (int32_t i = 0; i < N; ++i) t += col[i] * x[i];
+    y[size_t(d) * N + a] = t;
+}
+
+// g = y[a1(n)] - y[a2(n)]: g_arg[d][n] += g (row n of avgData), gacal1[d][n] = -2 pi g.
+__global__ void inv_rows_kernel(const double* __restrict__ y, int32_t N, const int2* __restrict__ pairs, int32_t nI,
+                                uint32_t nbl, double* __restrict__ g_arg, double* __restrict__ gacal1) {
+#pragma clang fp contract(off)
+    const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    if (n >= nI) return;
+    const int2 p = pairs[n];
+    const double g = y[size_t(d) * N + p.x] - y[size_t(d) * N + p.y];
+    g_arg[size_t(d) * nbl + n] += g;
+    gacal1[size_t(d) * nI + n] = -kTwoPi * g;
+}
+
+// g_arg[d][I1(k)] += y1[a1'(k)] - y1[a2'(k)] (I1 is one-to-one).
+__global__ void inv_rows1_kernel(const double* __restrict__ y1, int32_t N, const int2* __restrict__ pairs1,
+                                 const int32_t* __restrict__ I1, int32_t nI1, uint32_t nbl,
+                                 double* __restrict__ g_arg) {
+#pragma clang fp contract(off)
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    if (k >= nI1) return;
+    const int2 p = pairs1[k];
+    int32_t b = I1[k];
+    if (b < 0) b += int32_t(nbl);
+    g_arg[size_t(d) * nbl + b] += y1[size_t(d) * N + p.x] - y1[size_t(d) * N + p.y];
+}
+
+// abar[d][b] = i g / conj(z) = (-g y / |z|^2, g x / |z|^2), 0 where z = 0; z = avg0[d][b].
+__global__ void inv_abar_kernel(const double* __restrict__ avg0, const double* __restrict__ g_arg, uint32_t n,
+                                double* __restrict__ abar) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = avg0[2 * size_t(i)], y = avg0[2 * size_t(i) + 1], m = x * x + y * y;
+    const double g = g_arg[i];
+    abar[2 * size_t(i)] = m == 0.0 ? 0.0 : -g * y / m;
+    abar[2 * size_t(i) + 1] = m == 0.0 ? 0.0 : g * x / m;
+}
+
 }  // namespace
+
+void launch_inversion_adjoint(hipStream_t s, const InversionAdjoint& a) {
+    const int32_t nI = a.K - 1, nI1 = a.K1 - 1;
+    const uint32_t D = a.D, nbl = a.nbl;
+    const uint32_t rb = uint32_t(std::max(1, (nI + 255) / 256));
+    inv_residual_kernel<<<dim3(rb, D), 256, 0, s>>>(a.cal, a.N, a.pairs, a.I, nI, a.ph, nbl, a.res, a.part);
+    LEAP_HIP(hipGetLastError());
+    inv_loss_kernel<<<(D + 63) / 64, 64, 0, s>>>(a.part, rb, D, a.loss);
+    LEAP_HIP(hipGetLastError());
+    if (!a.abar) return;
+    inv_garg_init_kernel<<<dim3((nbl + 255) / 256, D), 256, 0, s>>>(a.g_arg, nbl);
+    LEAP_HIP(hipGetLastError());
+    const dim3 gn((std::max(1, nI) + 255) / 256, D), ga((a.N + 63) / 64, D);
+    if (nI > 0) {
+        inv_garg_res_kernel<<<gn, 256, 0, s>>>(a.res, a.I, nI, nbl, a.g_arg);
+        LEAP_HIP(hipGetLastError());
+    }
+    inv_antenna_sum_kernel<<<ga, 64, 0, s>>>(a.res, nI, 2.0, a.csr_off, a.csr_ent, a.N, nullptr, a.g_cal);
+    LEAP_HIP(hipGetLastError());
+    inv_gram_gemv_kernel<<<ga, 64, 0, s>>>(a.Gp, a.N, a.g_cal, a.y);
+    LEAP_HIP(hipGetLastError());
+    if (nI > 0) {
+        inv_rows_kernel<<<gn, 256, 0, s>>>(a.y, a.N, a.pairs, nI, nbl, a.g_arg, a.gacal1);
+        LEAP_HIP(hipGetLastError());
+    }
+    inv_antenna_sum_kernel<<<ga, 64, 0, s>>>(a.gacal1, nI, 1.0, a.csr_off, a.csr_ent, a.N, a.g_cal, a.g_cal1);
+    LEAP_HIP(hipGetLastError());
+    inv_gram_gemv_kernel<<<ga, 64, 0, s>>>(a.G1p, a.N, a.g_cal1, a.y1);
+    LEAP_HIP(hipGetLastError());
+    if (nI1 > 0) {
+        inv_rows1_kernel<<<dim3((nI1 + 255) / 256, D), 256, 0, s>>>(a.y1, a.N, a.pairs1, a.I1, nI1, nbl, a.g_arg);
+        LEAP_HIP(hipGetLastError());
+    }
+    const uint32_t n = D * nbl;
+    inv_abar_kernel<<<(n + 255) / 256, 256, 0, s>>>(a.avg0, a.g_arg, n, a.abar);
+    LEAP_HIP(hipGetLastError());
+}
 
 void launch_perturb_uvw(hipStream_t s, const double* uvw0, const double* M, const int2* bl_ants, const double* dp,
                         uint32_t nbl, uint32_t rows, double* uvw) {
@@ -321,8 +527,11 @@ void launch_gather_pol0(hipStream_t s, const double* avg, uint32_t n, uint32_t P
     LEAP_HIP(hipGetLastError());
 }
 
-void launch_reduce_gdz(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double* out) {
+void launch_reduce_gdz_offsets(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double ra0, double dec0,
+                               const double* dirs, double* out) {
     reduce_gdz_kernel<<<D * 3, 256, 0, s>>>(part, nbl, D, out);
+    LEAP_HIP(hipGetLastError());
+    gdz_offsets_kernel<<<(D + 63) / 64, 64, 0, s>>>(out, D, ra0, dec0, dirs, out + 3 * size_t(D));
     LEAP_HIP(hipGetLastError());
 }
 

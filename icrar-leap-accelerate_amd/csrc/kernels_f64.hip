@@ -280,6 +280,106 @@ __global__ __launch_bounds__(kBlock) void gemv_reduce_kernel(const double* __res
     if (threadIdx.x == 0) cal[size_t(d) * N + i] = s_red[0] + cal1[size_t(d) * N + i];
 }
 
+// ---------------------------------------------------------------------------
+// Moore-Penrose inverse of the Gram matrix G = A^T A (N x N, column-major), for
+// A the baseline rows e_a1 - e_a2 plus the constraint row e_c.  G is a graph
+// Laplacian plus e_c e_c^T, formed exactly (small integers).  Its null space is
+// spanned by the indicators of the baseline graph's components without c, so
+// per component S (members sorted, from the host's union-find):
+//   c in S:      G+_SS = G_SS^-1                      (positive definite)
+//   c not in S:  G+_SS = (G_SS + J / |S|)^-1 - J / |S|  (L_S 1 = 0)
+// and G+ is zero between components and on unused antennas.  Each block is
+// inverted as M = L L^T (left-looking Cholesky), W = L^-1 (forward
+// substitution), M^-1 = W^T W, every sum in the order of the host routine.
+// ---------------------------------------------------------------------------
+__global__ void fill_zero_kernel(double* __restrict__ x, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+// Column a of G: the degree of antenna a (+1 at the constraint) on the diagonal,
+// -1 per row joining a and b at row b.  csr_ent holds 2 k + (a is antenna2 of row k).
+__global__ void gram_columns_kernel(const int32_t* __restrict__ csr_off, const int32_t* __restrict__ csr_ent,
+                                    const int2* __restrict__ pairs, int32_t N, int32_t constraint,
+                                    double* __restrict__ G) {
+    const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    double* col = G + size_t(a) * N;
+    col[a] = double(csr_off[a + 1] - csr_off[a]) + (a == constraint ? 1.0 : 0.0);
+    for (int32_t e = csr_off[a]; e < csr_off[a + 1]; ++e) {
+        const int32_t v = csr_ent[e];
+        const int2 p = pairs[v >> 1];
+        col[(v & 1) ? p.x : p.y] -= 1.0;
+    }
+}
+
+// One workgroup per component; ws holds 2 n^2 doubles per component (M, then W).
+// *status becomes 1 when a pivot is not positive (the block is not positive definite).
+__global__ __launch_bounds__(kBlock) void gram_block_pinv_kernel(const double* __restrict__ G, int32_t N,
+                                                                 const int32_t* __restrict__ comp_off,
+                                                                 const int32_t* __restrict__ members,
+                                                                 const uint8_t* __restrict__ has_c,
+                                                                 const int64_t* __restrict__ ws_off,
+                                                                 double* __restrict__ ws, double* __restrict__ Gp,
+                                                                 int32_t* __restrict__ status) {
+#pragma clang fp contract(off)
+    const int32_t c = blockIdx.x;
+    const int32_t n = comp_off[c + 1] - comp_off[c];
+    const int32_t* S = members + comp_off[c];
+    double* M = ws + ws_off[c];
+    double* Wm = M + size_t(n) * n;
+    auto at = [&](int32_t i, int32_t j) -> double& { return M[size_t(j) * n + i]; };
+    auto w = [&](int32_t i, int32_t j) -> double& { return Wm[size_t(j) * n + i]; };
+    const double inv_n = 1.0 / double(n);
+    const bool shifted = !has_c[c];
+    for (int32_t idx = threadIdx.x; idx < n * n; idx += kBlock) {
+        const int32_t i = idx % n, j = idx / n;
+        double x = G[size_t(S[j]) * N + S[i]];
+        if (shifted) x += inv_n;
+        M[idx] = x;
+        Wm[idx] = 0.0;
+    }
+    __syncthreads();
+    __shared__ double s_ljj;
+    for (int32_t j = 0; j < n; ++j) {  // L in the lower triangle
+        if (threadIdx.x == 0) {
+            double d = at(j, j);
+            for (int32_t k = 0; k < j; ++k) d -= at(j, k) * at(j, k);
+            if (!(d > 0.0)) {
+                *status = 1;
+                d = 1.0;
+            }
+            s_ljj = sqrt(d);
+            at(j, j) = s_ljj;
+        }
+        __syncthreads();
+        const double ljj = s_ljj;
+        for (int32_t i = j + 1 + int32_t(threadIdx.x); i < n; i += kBlock) {
+            double sum = at(i, j);
+            for (int32_t k = 0; k < j; ++k) sum -= at(i, k) * at(j, k);
+            at(i, j) = sum / ljj;
+        }
+        __syncthreads();
+    }
+    for (int32_t j = threadIdx.x; j < n; j += kBlock) {  // W = L^-1, one column per thread
+        w(j, j) = 1.0 / at(j, j);
+        for (int32_t i = j + 1; i < n; ++i) {
+            double sum = 0.0;
+            for (int32_t k = j; k < i; ++k) sum -= at(i, k) * w(k, j);
+            w(i, j) = sum / at(i, i);
+        }
+    }
+    __syncthreads();
+    for (int32_t idx = threadIdx.x; idx < n * n; idx += kBlock) {  // M^-1 = W^T W, scattered into G+
+        const int32_t i = idx % n, j = idx / n;
+        const int32_t lo = i > j ? i : j, a = i > j ? i : j, b = i > j ? j : i;
+        double sum = 0.0;
+        for (int32_t k = lo; k < n; ++k) sum += w(k, a) * w(k, b);
+        if (shifted) sum -= inv_n;
+        Gp[size_t(S[j]) * N + S[i]] = sum;
+    }
+}
+
 template <typename T>
 void launch_rotate_typed(hipStream_t s, const T* vis, const double* uvw, const double* dd,
                          uint32_t row_start, uint32_t T_steps, uint32_t nbl, uint32_t C, uint32_t P,
@@ -311,6 +411,21 @@ void launch_rotate_f64(hipStream_t s, const void* vis, int32_t vis_dtype, const 
         launch_rotate_typed(s, static_cast<const float*>(vis), uvw, dd, row_start, T_steps, nbl, C, P, lambda, avg);
     else
         launch_rotate_typed(s, static_cast<const double*>(vis), uvw, dd, row_start, T_steps, nbl, C, P, lambda, avg);
+}
+
+void launch_gram_pinv(hipStream_t s, const GramPinv& g) {
+    const size_t nn = size_t(g.N) * g.N;
+    fill_zero_kernel<<<uint32_t((nn + 255) / 256), 256, 0, s>>>(g.G, nn);
+    LEAP_HIP(hipGetLastError());
+    fill_zero_kernel<<<uint32_t((nn + 255) / 256), 256, 0, s>>>(g.Gp, nn);
+    LEAP_HIP(hipGetLastError());
+    gram_columns_kernel<<<(g.N + 63) / 64, 64, 0, s>>>(g.csr_off, g.csr_ent, g.pairs, g.N, g.constraint, g.G);
+    LEAP_HIP(hipGetLastError());
+    if (g.n_components > 0) {
+        gram_block_pinv_kernel<<<g.n_components, kBlock, 0, s>>>(g.G, g.N, g.comp_off, g.members, g.has_c, g.ws_off,
+                                                                 g.ws, g.Gp, g.status);
+        LEAP_HIP(hipGetLastError());
+    }
 }
 
 void launch_assemble_ad(hipStream_t s, const double* Gp, int32_t N, const int2* pairs, int32_t n_pairs,

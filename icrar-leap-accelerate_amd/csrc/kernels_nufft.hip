@@ -1,802 +1,28 @@
-// NUFFT channel sums of the mixed-precision batched path on gfx950.
-//
-// For row (t, baseline) and direction d the reference sums
-//   sum_c vis[c][0] exp(i sf_d (f0 + c df) / c_light)     (CpuLeapCalibrator.cc:181-217)
-// over the channels.  With b = sf_d df / c_light this is
-// exp(i sf_d kappa_mid) * sum_{k=-h}^{C-1-h} v_{k+h} exp(i k b), h the plan's
-// frequency offset (C / 2 rounded to a multiple of 64), kappa_mid =
-// (f0 + h df) / c_light, and the trigonometric polynomial is evaluated by a
-// type-2 non-uniform FFT (the plan and its error are described in
-// nufft_plan.cpp): each channel's value is multiplied by its precorrection
-// factor (here, from the cube, or by the pack kernel), one N-point inverse FFT
-// per row puts the polynomial on a uniform grid of N points over one period,
-// and each direction interpolates W grid points with the "exponential of
-// semicircle" kernel.  Per row and direction this costs about 130 FP64
-// operations instead of the 4 C FMAs of the Horner kernel; the FFT's cost is
-// shared by every direction of the row.
-//
-// Work decomposition: a workgroup is one wave.  It owns one baseline and a
-// block of 64 * NDL directions (NDL per lane) and walks the baseline's rows in
-// order, prefetching the next row's channels while it works on the current
-// one.  The FFT is a Stockham transform whose butterflies run in registers
-// (radices 16, 16, 4 for N = 1024; 8, 8, 8 for 512; 4, 4, 4, 4 for 256), with
-// the exchanges between passes through the wave's LDS (rows of 16 elements
-// padded by one element against bank conflicts); the last pass writes the grid
-// in natural order plus W - 1 wrapped elements, so a direction's W grid points
-// are consecutive.  The transform's input is the channels in their own order
-// (channel c at index c; lane l owns channels l + 64 i, which are its register
-// slots i, the upper half being the zero padding); the frequency offset h of
-// the plan then multiplies output m by exp(-2 pi i h m / N), which for
-// h = 64 H depends on m mod (N / 64) = lane mod (N / 64) only and is folded into
-// the last pass's twiddles.  Wave-scope fences order the LDS exchanges; there
-// are no workgroup barriers and no atomics, and each direction's row sums are
-// added in row order, so results are reproducible.
-#include "batched_common.h"
-
-// No contraction anywhere in this file: every fused operation is one written
-// as fma(), so the kernel gives the same bits whether a row's values come
-// from the cube (precorrection multiplied here) or from packed rows
-// (multiplied by the pack kernel).
-#pragma clang fp contract(off)
+// NUFFT channel sums with an FP64 transform, and the entry point that picks
+// the transform precision of a plan.  The kernels are in nufft_impl.h.
+#include "nufft_impl.h"
 
 namespace leap {
-namespace {
 
-using namespace dev;
-using Z = C2<double>;
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-
-constexpr int kWave = 64;
-
-using AS4double = __attribute__((address_space(4))) double;
-
-// Which rows a workgroup sums: chunks == 1: workgroup x takes baseline
-// bl_base + x, all T rows; chunks > 1: workgroup x takes baseline
-// bl_base + x / chunks, rows [c T / chunks, (c + 1) T / chunks) with c = x mod
-// chunks (output mode kNufftPartial).
-struct NufftSpan {
-    uint32_t bl_base;
-    uint32_t chunks;
-};
-
-// Output modes (the kernel's mode argument): set avgData (pol 0, the other
-// pols zero), add to avgData, or write the row sums to avg[x][direction]
-// (complex; the partial-sum array of a chunked launch, summed by nufft_tail_sum).
-constexpr int32_t kNufftSet = 0, kNufftAccumulate = 1, kNufftPartial = 2;
-
-struct NufftArgs {
-    const double* coef;  // device: [W / 2][kNufftMaxWidth] weight polynomials of taps j < W / 2, powers of s
-    double kappa_mid;    // (f0 + h df) / c_light
-    double dkappa;       // df / c_light
-    int32_t h;           // frequency offset, a multiple of 64
-};
-
-__device__ inline Z zadd(Z a, Z b) { return {a.x + b.x, a.y + b.y}; }
-__device__ inline Z zsub(Z a, Z b) { return {a.x - b.x, a.y - b.y}; }
-__device__ inline Z zmuli(Z a) { return {-a.y, a.x}; }  // i * a
-
-constexpr double kCos4 = 0.70710678118654752440;   // cos(pi / 4)
-constexpr double kCos8 = 0.92387953251128675613;   // cos(pi / 8)
-constexpr double kSin8 = 0.38268343236508977173;   // sin(pi / 8)
-
-// Inverse DFTs in place, natural order: X_k = sum_j x_j exp(+2 pi i j k / R).
-__device__ inline void idft4(Z& a0, Z& a1, Z& a2, Z& a3) {
-    const Z t0 = zadd(a0, a2), t1 = zsub(a0, a2), t2 = zadd(a1, a3), t3 = zmuli(zsub(a1, a3));
-    a0 = zadd(t0, t2);
-    a2 = zsub(t0, t2);
-    a1 = zadd(t1, t3);
-    a3 = zsub(t1, t3);
+void launch_rotate_nufft_f64(hipStream_t s, const nufft::NufftLaunch& l) {
+    const NufftPlan& plan = *l.plan;
+#define LEAP_NUFFT(N_, W_)                                                                                   \
+    if (plan.N == N_ && plan.W == W_) {                                                                      \
+        nufft::launch_nw<N_, W_, double>(s, l.tile, l.nbl_grid, l.src, l.src_pols, l.scale, l.uvw, l.dd, l.D, \
+                                         l.T, l.nbl, l.C, l.P_out, l.args, l.avg, l.accumulate, l.tail,      \
+                                         l.tail_doubles);                                                    \
+        return;                                                                                              \
+    }
+    LEAP_NUFFT(256, 10)
+    LEAP_NUFFT(256, 12)
+    LEAP_NUFFT(512, 10)
+    LEAP_NUFFT(512, 12)
+    LEAP_NUFFT(1024, 10)
+    LEAP_NUFFT(1024, 12)
+#undef LEAP_NUFFT
+    throw Error(LEAP_ERR_INTERNAL, "no FP64 NUFFT kernel for grid " + std::to_string(plan.N) + ", width " +
+                                       std::to_string(plan.W));
 }
-
-// idft4 of (a0, a1, 0, 0).
-__device__ inline void idft4_half(Z& a0, Z& a1, Z& a2, Z& a3) {
-    const Z x0 = a0, t1 = zmuli(a1);
-    a0 = zadd(x0, a1);
-    a2 = zsub(x0, a1);
-    a3 = zsub(x0, t1);
-    a1 = zadd(x0, t1);
-}
-
-template <bool HALF>
-__device__ inline void idft4_any(Z& a0, Z& a1, Z& a2, Z& a3) {
-    if constexpr (HALF)
-        idft4_half(a0, a1, a2, a3);
-    else
-        idft4(a0, a1, a2, a3);
-}
-
-// HALF: the upper half of the inputs is zero (the first pass's padding).
-template <int R, bool HALF>
-__device__ inline void idft(Z* x);
-
-template <>
-__device__ inline void idft<4, false>(Z* x) {
-    idft4(x[0], x[1], x[2], x[3]);
-}
-
-template <>
-__device__ inline void idft<4, true>(Z* x) {
-    idft4_half(x[0], x[1], x[2], x[3]);
-}
-
-template <bool HALF>
-__device__ inline void idft8(Z* x) {
-    // even / odd halves, then X_k = E_k + w8^k O_k, X_{k+4} = E_k - w8^k O_k
-    Z e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
-    idft4_any<HALF>(e[0], e[1], e[2], e[3]);
-    idft4_any<HALF>(o[0], o[1], o[2], o[3]);
-    o[1] = cmul(o[1], Z{kCos4, kCos4});
-    o[2] = zmuli(o[2]);
-    o[3] = cmul(o[3], Z{-kCos4, kCos4});
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        x[k] = zadd(e[k], o[k]);
-        x[k + 4] = zsub(e[k], o[k]);
-    }
-}
-
-template <>
-__device__ inline void idft<8, false>(Z* x) {
-    idft8<false>(x);
-}
-
-template <>
-__device__ inline void idft<8, true>(Z* x) {
-    idft8<true>(x);
-}
-
-template <bool HALF>
-__device__ inline void idft16(Z* x) {
-    // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1, twiddle w16^(n2 k1), DFT4 over n2
-    Z y[4][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2) {
-        y[n2][0] = x[n2];
-        y[n2][1] = x[4 + n2];
-        y[n2][2] = x[8 + n2];
-        y[n2][3] = x[12 + n2];
-        idft4_any<HALF>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
-    }
-    const Z w1{kCos8, kSin8}, w2{kCos4, kCos4}, w3{kSin8, kCos8}, w6{-kCos4, kCos4}, w9{-kCos8, -kSin8};
-    y[1][1] = cmul(y[1][1], w1);
-    y[1][2] = cmul(y[1][2], w2);
-    y[1][3] = cmul(y[1][3], w3);
-    y[2][1] = cmul(y[2][1], w2);
-    y[2][2] = zmuli(y[2][2]);
-    y[2][3] = cmul(y[2][3], w6);
-    y[3][1] = cmul(y[3][1], w3);
-    y[3][2] = cmul(y[3][2], w6);
-    y[3][3] = cmul(y[3][3], w9);
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        Z z0 = y[0][k1], z1 = y[1][k1], z2 = y[2][k1], z3 = y[3][k1];
-        idft4(z0, z1, z2, z3);
-        x[k1] = z0;
-        x[k1 + 4] = z1;
-        x[k1 + 8] = z2;
-        x[k1 + 12] = z3;
-    }
-}
-
-template <>
-__device__ inline void idft<16, false>(Z* x) {
-    idft16<false>(x);
-}
-
-template <>
-__device__ inline void idft<16, true>(Z* x) {
-    idft16<true>(x);
-}
-
-// Orders the wave's LDS accesses across lanes (the workgroup is one wave).
-__device__ inline void wave_fence() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ inline uint32_t padded(uint32_t e) { return e + (e >> 4); }
-
-// Stockham passes of an N-point transform on one wave.  Register slot m of a
-// lane holds element lane + 64 m on entry to every pass.  A pass of radix R
-// and span NS has B = N / (64 R) butterflies per lane: butterfly j = lane + 64 q
-// takes elements j + r N / R (slots q + B r), multiplies element r by
-// w^r, w = exp(2 pi i (j mod NS) / (NS R)), transforms, and its output r goes
-// to (j / NS) NS R + j mod NS + r NS.  HALF: the upper half of the inputs is
-// zero (first pass).  PHI: every output of the lane is also multiplied by phi
-// (folded into the input twiddles: element r by phi w^r).
-template <int N, int R, int NS, bool HALF, bool PHI>
-__device__ inline void pass_compute(Z* v, const Z* base, Z phi) {
-    constexpr int B = N / (kWave * R);
-#pragma unroll
-    for (int q = 0; q < B; ++q) {
-        Z x[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) x[r] = v[q + B * r];
-        if constexpr (PHI) {
-            Z t = phi;
-            x[0] = cmul(x[0], t);
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                t = cmul(t, base[q]);
-                x[r] = cmul(x[r], t);
-            }
-        } else if constexpr (NS > 1) {
-            // Odd and even powers as two interleaved chains (step w^2): half
-            // the dependent depth of one chain, three values live.
-            const Z w1 = base[q];
-            const Z w2 = cmul(w1, w1);
-            Z po = w1, pe = w2;
-            x[1] = cmul(x[1], po);
-            x[2] = cmul(x[2], pe);
-#pragma unroll
-            for (int r = 3; r + 1 < R; r += 2) {
-                po = cmul(po, w2);
-                pe = cmul(pe, w2);
-                x[r] = cmul(x[r], po);
-                x[r + 1] = cmul(x[r + 1], pe);
-            }
-            if constexpr (R % 2 == 0) {
-                po = cmul(po, w2);
-                x[R - 1] = cmul(x[R - 1], po);
-            }
-        }
-        idft<R, HALF>(x);
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[q + B * r] = x[r];
-    }
-}
-
-// For every pass here (o0 mod 16) + (r NS mod 16) < 16, so
-// padded(o0 + r NS) = padded(o0) + r NS + (r NS) / 16: one address per lane
-// plus compile-time offsets.
-template <int N, int R, int NS>
-__device__ inline void pass_store_padded(Z* lds, const Z* v, uint32_t lane) {
-    constexpr int B = N / (kWave * R);
-    static_assert(NS >= 16 || (NS * R <= 16 && 16 % (NS * R) == 0) || (NS == 8 && R == 8) || (NS == 4 && R == 4),
-                  "padded store offsets");
-#pragma unroll
-    for (int q = 0; q < B; ++q) {
-        const uint32_t j = lane + kWave * q;
-        const uint32_t o0 = (j / NS) * NS * R + j % NS;
-        Z* base = lds + padded(o0);
-#pragma unroll
-        for (int r = 0; r < R; ++r) base[r * NS + (r * NS) / 16] = v[q + B * r];
-    }
-}
-
-// Element lane + 64 m sits at padded(lane) + 68 m.
-template <int N>
-__device__ inline void load_padded(const Z* lds, Z* v, uint32_t lane) {
-    const Z* base = lds + padded(lane);
-#pragma unroll
-    for (int m = 0; m < N / kWave; ++m) v[m] = base[(kWave + kWave / 16) * m];
-}
-
-// Last pass (NS R = N): output r of butterfly q is element lane + 64 (q + B r),
-// i.e. slot m goes to lane + 64 m; the first W - 1 elements are repeated after N.
-template <int N, int W>
-__device__ inline void store_grid(Z* lds, const Z* v, uint32_t lane) {
-    Z* base = lds + lane;
-#pragma unroll
-    for (int m = 0; m < N / kWave; ++m) base[kWave * m] = v[m];
-    if (lane < uint32_t(W - 1)) base[N] = v[0];
-}
-
-// Hides a value from loop-invariant code motion, so powers derived from it are
-// recomputed where they are used instead of being kept in registers across
-// the row loop.
-__device__ inline void opaque(Z& z) {
-    asm volatile("" : "+v"(z.x), "+v"(z.y));
-}
-
-// Per-lane base twiddles of the passes after the first (row-independent; the
-// powers each pass needs are recomputed per row from these, see opaque()).
-template <int N>
-struct Twiddles;
-
-// phi = exp(-2 pi i H (lane mod V) / V), V = N / 64: the frequency offset
-// h = 64 H applied to the lane's outputs (see the file comment).
-template <int N>
-__device__ inline Z offset_phase(uint32_t lane, int32_t H) {
-    constexpr int V = N / kWave;
-    const Phasor64 a = phasor64(-kTwoPi * double((uint32_t(H) * (lane % V)) % V) / double(V));
-    return {a.c, a.s};
-}
-
-template <>
-struct Twiddles<1024> {  // passes (16, 1), (16, 16), (4, 256)
-    Z p2[1], p3[4];
-    __device__ void hide() {
-        opaque(p2[0]);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) opaque(p3[q]);
-    }
-    __device__ void init(uint32_t lane) {
-        const Phasor64 a = phasor64(kTwoPi * double(lane % 16) / 256.0);
-        p2[0] = {a.c, a.s};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const Phasor64 b = phasor64(kTwoPi * double(lane + kWave * q) / 1024.0);
-            p3[q] = {b.c, b.s};
-        }
-    }
-};
-
-template <>
-struct Twiddles<512> {  // passes (8, 1), (8, 8), (8, 64)
-    Z p2[1], p3[1];
-    __device__ void hide() {
-        opaque(p2[0]);
-        opaque(p3[0]);
-    }
-    __device__ void init(uint32_t lane) {
-        const Phasor64 a = phasor64(kTwoPi * double(lane % 8) / 64.0);
-        const Phasor64 b = phasor64(kTwoPi * double(lane) / 512.0);
-        p2[0] = {a.c, a.s};
-        p3[0] = {b.c, b.s};
-    }
-};
-
-template <>
-struct Twiddles<256> {  // passes (4, 1), (4, 4), (4, 16), (4, 64)
-    Z p2[1], p3[1], p4[1];
-    __device__ void hide() {
-        opaque(p2[0]);
-        opaque(p3[0]);
-        opaque(p4[0]);
-    }
-    __device__ void init(uint32_t lane) {
-        const Phasor64 a = phasor64(kTwoPi * double(lane % 4) / 16.0);
-        const Phasor64 b = phasor64(kTwoPi * double(lane % 16) / 64.0);
-        const Phasor64 c = phasor64(kTwoPi * double(lane) / 256.0);
-        p2[0] = {a.c, a.s};
-        p3[0] = {b.c, b.s};
-        p4[0] = {c.c, c.s};
-    }
-};
-
-// g_m = sum_k f~_k exp(2 pi i k m / N) into lds[0, N + W - 1), from the lane's
-// slots (slot i < N / 128: channel lane + 64 i; the rest zero) and the lane's
-// offset phase phi.
-template <int N, int W>
-__device__ inline void row_fft(Z* v, Z* lds, const Twiddles<N>& tw, Z phi, uint32_t lane) {
-    if constexpr (N == 1024) {
-        pass_compute<N, 16, 1, true, false>(v, nullptr, phi);
-        wave_fence();  // the previous row's interpolation has read the grid
-        pass_store_padded<N, 16, 1>(lds, v, lane);
-        wave_fence();
-        load_padded<N>(lds, v, lane);
-        pass_compute<N, 16, 16, false, false>(v, tw.p2, phi);
-        wave_fence();
-        pass_store_padded<N, 16, 16>(lds, v, lane);
-        wave_fence();
-        load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 256, false, true>(v, tw.p3, phi);
-        wave_fence();
-        store_grid<N, W>(lds, v, lane);
-        wave_fence();
-    } else if constexpr (N == 512) {
-        pass_compute<N, 8, 1, true, false>(v, nullptr, phi);
-        wave_fence();
-        pass_store_padded<N, 8, 1>(lds, v, lane);
-        wave_fence();
-        load_padded<N>(lds, v, lane);
-        pass_compute<N, 8, 8, false, false>(v, tw.p2, phi);
-        wave_fence();
-        pass_store_padded<N, 8, 8>(lds, v, lane);
-        wave_fence();
-        load_padded<N>(lds, v, lane);
-        pass_compute<N, 8, 64, false, true>(v, tw.p3, phi);
-        wave_fence();
-        store_grid<N, W>(lds, v, lane);
-        wave_fence();
-    } else {
-        static_assert(N == 256, "grid sizes 256, 512, 1024");
-        pass_compute<N, 4, 1, true, false>(v, nullptr, phi);
-        wave_fence();
-        pass_store_padded<N, 4, 1>(lds, v, lane);
-        wave_fence();
-        load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 4, false, false>(v, tw.p2, phi);
-        wave_fence();
-        pass_store_padded<N, 4, 4>(lds, v, lane);
-        wave_fence();
-        load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 16, false, false>(v, tw.p3, phi);
-        wave_fence();
-        pass_store_padded<N, 4, 16>(lds, v, lane);
-        wave_fence();
-        load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 64, false, true>(v, tw.p4, phi);
-        wave_fence();
-        store_grid<N, W>(lds, v, lane);
-        wave_fence();
-    }
-}
-
-// Row sources.  The lane owns channels c = lane + 64 i, i < N / 128 (C <= N / 2);
-// load() fetches them for one row (channels past C read channel C - 1 and are
-// zeroed by freq()), freq() returns the precorrected pol-0 values f~.
-//   P == 0: packed rows of FP64 complex, already precorrected (pack kernel).
-//   P >= 1: the complex64 cube [row][channel][pol]: a channel whose 2P floats
-//           are not all finite is zero (CpuLeapCalibrator.cc:201-214), pol 0 is
-//           widened and multiplied by the channel's precorrection factor.
-template <int N, int P>
-struct Source {
-    static constexpr int CPL = N / (2 * kWave);
-    float raw[CPL][2 * P];
-    double sc[CPL];    // precorrection, 0 past C
-    uint32_t off[CPL];  // float offset of the channel (clamped) in a row
-    __device__ void init(const LEAP_GLOBAL double* scale, uint32_t lane, uint32_t C) {
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            const uint32_t c = lane + kWave * i;
-            sc[i] = c < C ? scale[c] : 0.0;
-            off[i] = min(c, C - 1) * (2 * P);
-        }
-    }
-    // base advanced by `rows` rows of C channels
-    __device__ static const void* advance(const void* base, size_t rows, uint32_t C) {
-        return (const float*)base + rows * C * (2 * P);
-    }
-    __device__ void load(const void* base, size_t row, uint32_t, uint32_t C) {
-        const LEAP_GLOBAL float* r = (const LEAP_GLOBAL float*)base + row * C * (2 * P);
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            const LEAP_GLOBAL float* q = r + off[i];
-            if constexpr (P == 4 || P == 2) {
-#pragma unroll
-                for (int h = 0; h < P / 2; ++h) {
-                    const f32x4 f = ((const LEAP_GLOBAL f32x4*)q)[h];
-                    raw[i][4 * h] = f.x;
-                    raw[i][4 * h + 1] = f.y;
-                    raw[i][4 * h + 2] = f.z;
-                    raw[i][4 * h + 3] = f.w;
-                }
-            } else {
-                const f32x2 f = *(const LEAP_GLOBAL f32x2*)q;
-                raw[i][0] = f.x;
-                raw[i][1] = f.y;
-            }
-        }
-    }
-    __device__ void freq(Z* x, uint32_t, uint32_t) const {
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            bool finite = true;
-#pragma unroll
-            for (int k = 0; k < 2 * P; ++k) finite = finite && __builtin_isfinite(raw[i][k]);
-            const float re = finite ? raw[i][0] : 0.0f, im = finite ? raw[i][1] : 0.0f;
-            x[i] = Z{double(re) * sc[i], double(im) * sc[i]};
-        }
-    }
-};
-
-template <int N>
-struct Source<N, 0> {
-    static constexpr int CPL = N / (2 * kWave);
-    Z raw[CPL];
-    __device__ void init(const LEAP_GLOBAL double*, uint32_t, uint32_t) {}
-    __device__ static const void* advance(const void* base, size_t rows, uint32_t C) {
-        return (const Z*)base + rows * C;
-    }
-    __device__ void load(const void* base, size_t row, uint32_t lane, uint32_t C) {
-        const LEAP_GLOBAL Z* r = (const LEAP_GLOBAL Z*)base + row * C;
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) {
-            const uint32_t c = min(lane + kWave * i, C - 1);
-            raw[i].x = r[c].x;
-            raw[i].y = r[c].y;
-        }
-    }
-    __device__ void freq(Z* x, uint32_t lane, uint32_t C) const {
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) x[i] = lane + kWave * i < C ? raw[i] : Z{0.0, 0.0};
-    }
-};
-
-// SP: the row source (0 packed rows, 1 / 2 / 4 a complex64 cube with that many pols).
-template <int N, int W, int NDL, int SP>
-__global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __restrict__ w,
-                                                              const LEAP_GLOBAL double* __restrict__ scale,
-                                                              const LEAP_GLOBAL double* __restrict__ uvw,
-                                                              const LEAP_GLOBAL double* __restrict__ dd, uint32_t D,
-                                                              uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
-                                                              NufftArgs args, LEAP_GLOBAL double* __restrict__ avg,
-                                                              int32_t mode, NufftSpan span) {
-#pragma clang fp contract(off)
-    constexpr int V = N / kWave;
-    static_assert(W % 2 == 0 && W <= kNufftMaxWidth && N / 16 >= W - 1, "kernel width");
-    __shared__ Z s_g[N + N / 16];
-    const uint32_t lane = threadIdx.x;
-    // The span is resolved here into the row source, uvw and output pointers and
-    // a row count, so the row loop holds no more values than an unchunked one.
-    const uint32_t item = blockIdx.x;
-    const uint32_t chunk = span.chunks > 1 ? item % span.chunks : 0;
-    const uint32_t bl = span.bl_base + (span.chunks > 1 ? item / span.chunks : item);
-    const uint32_t t_begin = uint32_t(uint64_t(chunk) * T / max(span.chunks, 1u));
-    const uint32_t t_count = uint32_t(uint64_t(chunk + 1) * T / max(span.chunks, 1u)) - t_begin;
-    const void* wr = Source<N, SP>::advance(w, size_t(t_begin) * nbl, C);
-    const LEAP_GLOBAL double* uvr = uvw + size_t(t_begin) * nbl * 3;
-    LEAP_GLOBAL double* out = mode == kNufftPartial ? avg + size_t(item) * D * 2 : avg + size_t(bl) * Pout * 2;
-    const uint32_t dstride = mode == kNufftPartial ? 2u : nbl * Pout * 2;  // doubles between directions
-    const uint32_t d0 = blockIdx.y * (kWave * NDL);
-    double dz[NDL][3];
-#pragma unroll
-    for (int j = 0; j < NDL; ++j) {
-        const uint32_t d = d0 + lane + kWave * j;
-        const uint32_t dc = d < D ? d : D - 1;
-        dz[j][0] = dd[9 * size_t(dc) + 6];
-        dz[j][1] = dd[9 * size_t(dc) + 7];
-        dz[j][2] = dd[9 * size_t(dc) + 8];
-    }
-    Twiddles<N> tw;
-    tw.init(lane);
-    Z sum[NDL];
-#pragma unroll
-    for (int j = 0; j < NDL; ++j) sum[j] = {0.0, 0.0};
-    const Z phi = offset_phase<N>(lane, args.h / kWave);
-    // The next row's loads are issued right after the current row is converted
-    // and stay in flight while it is transformed and interpolated.  The load is
-    // unconditional (the last row reloads itself), so the row buffer keeps its
-    // registers from one iteration to the next.
-    Source<N, SP> src;
-    src.init(scale, lane, C);
-    if (t_count > 0) src.load(wr, bl, lane, C);
-    for (uint32_t t = 0; t < t_count; ++t) {
-        Z v[V];
-        src.freq(v, lane, C);
-#pragma unroll
-        for (int m = V / 2; m < V; ++m) v[m] = Z{0.0, 0.0};  // padding (not read by the first pass)
-        __builtin_amdgcn_sched_barrier(0);
-        const size_t row = size_t(t) * nbl + bl;
-        src.load(wr, size_t(t + 1 < t_count ? t + 1 : t) * nbl + bl, lane, C);
-        const double u = uvr[3 * row], vv = uvr[3 * row + 1], ww = uvr[3 * row + 2];
-        tw.hide();
-        row_fft<N, W>(v, s_g, tw, phi, lane);
-        // Interpolation.  Per direction: b = sf dkappa, grid coordinate
-        // ug = N frac(b / 2 pi), first tap m0 = ceil(ug - W/2), offset
-        // s = ug - m0 - (W - 1)/2 in (-1/2, 1/2]; tap pairs (jp, W - 1 - jp) take
-        // weights P_jp(s), P_jp(-s), P_jp(s) = E(s^2) + s O(s^2).  The pairs
-        // are the outer loop so that at most two pairs' coefficients are live.
-        // The NDL directions' steps are interleaved (independent chains side by
-        // side) with each direction's operations unchanged.
-        double sfv[NDL], sv[NDL], s2[NDL], x[NDL], m0f[NDL];
-        const Z* gp[NDL];
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) sfv[j] = shift_factor(dz[j], u, vv, ww);
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) x[j] = sfv[j] * args.dkappa * kInvTwoPi;  // turns of b
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) x[j] -= floor(x[j]);  // [0, 1]
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) {
-            x[j] = x[j] * double(N);  // grid coordinate ug in [0, N]
-            m0f[j] = ceil(x[j] - 0.5 * W);
-        }
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) {
-            sv[j] = (x[j] - m0f[j]) - 0.5 * (W - 1);
-            s2[j] = sv[j] * sv[j];
-            int32_t m0 = int32_t(m0f[j]);
-            if (m0 < 0) m0 += N;  // taps m0 .. m0 + W - 1 <= N + W - 2 (wrapped copy)
-            m0 = min(max(m0, 0), N - 1);  // a non-finite uvw stays inside the grid
-            gp[j] = s_g + m0;
-        }
-        Z acc[NDL];
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) acc[j] = {0.0, 0.0};
-        // A pair's coefficients are read with scalar loads one pair ahead and
-        // not kept across the loop (all W/2 pairs do not fit in SGPRs).  Scalar
-        // and LDS loads share one wait counter and scalar loads return out of
-        // order, so every wait is for all of them: the pair's 2 NDL grid reads
-        // are issued together with the next pair's coefficients, ahead of the
-        // weight polynomials that hide their latency, and waited for once.
-        double cc[W];
-        {
-            const double* cp = args.coef;
-            asm volatile("" : "+s"(cp));
-            const AS4double* cf = (const AS4double*)cp;
-#pragma unroll
-            for (int k = 0; k < W; ++k) cc[k] = cf[k];
-        }
-#pragma unroll
-        for (int jp = 0; jp < W / 2; ++jp) {
-            Z ga[NDL], gb[NDL];
-#pragma unroll
-            for (int j = 0; j < NDL; ++j) {
-                ga[j] = gp[j][jp];
-                gb[j] = gp[j][W - 1 - jp];
-            }
-            double cn[W];
-            if (jp + 1 < W / 2) {
-                const double* cp = args.coef + (jp + 1) * kNufftMaxWidth;
-                asm volatile("" : "+s"(cp));
-                const AS4double* cf = (const AS4double*)cp;
-#pragma unroll
-                for (int k = 0; k < W; ++k) cn[k] = cf[k];
-            }
-            __builtin_amdgcn_sched_barrier(0);
-            double wl[NDL], wh[NDL], e[NDL], o[NDL];
-#pragma unroll
-            for (int j = 0; j < NDL; ++j) {
-                e[j] = cc[W - 2];
-                o[j] = cc[W - 1];
-            }
-#pragma unroll
-            for (int k = W - 4; k >= 0; k -= 2) {
-#pragma unroll
-                for (int j = 0; j < NDL; ++j) {
-                    e[j] = fma(e[j], s2[j], cc[k]);
-                    o[j] = fma(o[j], s2[j], cc[k + 1]);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < NDL; ++j) {
-                wl[j] = fma(sv[j], o[j], e[j]);
-                wh[j] = fma(-sv[j], o[j], e[j]);
-            }
-#pragma unroll
-            for (int j = 0; j < NDL; ++j) {
-                acc[j].x = fma(ga[j].x, wl[j], acc[j].x);
-                acc[j].y = fma(ga[j].y, wl[j], acc[j].y);
-                acc[j].x = fma(gb[j].x, wh[j], acc[j].x);
-                acc[j].y = fma(gb[j].y, wh[j], acc[j].y);
-            }
-            if (jp + 1 < W / 2) {
-#pragma unroll
-                for (int k = 0; k < W; ++k) cc[k] = cn[k];
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        double th[NDL];
-        Phasor64 ph[NDL];
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) th[j] = sfv[j] * args.kappa_mid;
-        phasor64_n<NDL>(th, ph);
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) {
-            const Z r = cmul(Z{ph[j].c, ph[j].s}, acc[j]);
-            sum[j].x += r.x;
-            sum[j].y += r.y;
-        }
-    }
-    // The output indices are recomputed here from an opaque copy of the lane,
-    // so they are not kept in registers across the row loop.
-    uint32_t ln = lane;
-    asm volatile("" : "+v"(ln));
-#pragma unroll
-    for (int j = 0; j < NDL; ++j) {
-        const uint32_t d = d0 + ln + kWave * j;
-        if (d >= D) continue;
-        LEAP_GLOBAL double* o = out + size_t(d) * dstride;
-        if (mode == kNufftPartial) {
-            o[0] = sum[j].x;
-            o[1] = sum[j].y;
-        } else if (mode == kNufftAccumulate) {
-            o[0] += sum[j].x;
-            o[1] += sum[j].y;
-        } else {
-            o[0] = sum[j].x;
-            o[1] = sum[j].y;
-            for (uint32_t q = 2; q < 2 * Pout; ++q) o[q] = 0.0;  // pols not accumulated
-        }
-    }
-}
-
-// avgData of the chunked baselines: the chunks' partial sums added in chunk
-// order (each chunk summed its rows in row order).
-__global__ void nufft_tail_sum(const double* __restrict__ partial, uint32_t rem, uint32_t chunks, uint32_t D,
-                               uint32_t nbl, uint32_t bl_base, uint32_t Pout, double* __restrict__ avg) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= size_t(rem) * D) return;
-    const uint32_t r = uint32_t(i / D), d = uint32_t(i % D);
-    double x = 0.0, y = 0.0;
-    for (uint32_t c = 0; c < chunks; ++c) {
-        const double* p = partial + ((size_t(r) * chunks + c) * D + d) * 2;
-        x += p[0];
-        y += p[1];
-    }
-    double* o = avg + (size_t(d) * nbl + bl_base + r) * Pout * 2;
-    o[0] = x;
-    o[1] = y;
-    for (uint32_t q = 2; q < 2 * Pout; ++q) o[q] = 0.0;
-}
-
-// Workgroups of one NUFFT kernel resident on the whole device at once.
-template <class K>
-uint32_t resident_workgroups(K kernel) {
-    int per_cu = 0, dev = 0, cus = 0;
-    LEAP_HIP(hipGetDevice(&dev));
-    LEAP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    LEAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0));
-    return uint32_t(std::max(1, per_cu) * std::max(1, cus));
-}
-
-// One launch of rotate_average_nufft<N, W, NDL, SP> over nbl_grid baselines.
-// When a single direction block covers D and the baselines do not fill whole
-// rounds of resident workgroups, the whole rounds run as usual and the
-// remaining baselines' rows are split into chunks so that the last round is
-// full (and short): their partial sums go to the tail scratch and are added by
-// nufft_tail_sum.  Not for accumulating launches.
-template <int N, int W, int NDL, int SP>
-void launch_nufft_tile(hipStream_t s, uint32_t nbl_grid, const void* w, const LEAP_GLOBAL double* gs,
-                       const LEAP_GLOBAL double* gu, const LEAP_GLOBAL double* gd, uint32_t D, uint32_t T, uint32_t nbl,
-                       uint32_t C, uint32_t Pout, const NufftArgs& a, LEAP_GLOBAL double* ga, int32_t accumulate,
-                       double* tail, size_t tail_doubles) {
-    auto* kernel = rotate_average_nufft<N, W, NDL, SP>;
-    const uint32_t ny = (D + kWave * NDL - 1) / (kWave * NDL);
-    if (tail && !accumulate && ny == 1 && T >= 2) {
-        static const uint32_t slots = resident_workgroups(kernel);
-        const uint32_t full = nbl_grid / slots * slots, rem = nbl_grid - full;
-        const uint32_t chunks = rem ? std::min(T, (slots + rem - 1) / rem) : 0;
-        if (rem && chunks >= 2 && size_t(rem) * chunks * D * 2 <= tail_doubles) {
-            // Each launch is checked on its own: a later successful launch may
-            // reset the error state of an earlier failed one.
-            if (full) {
-                kernel<<<dim3(full, 1), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, kNufftSet,
-                                                       NufftSpan{0, 1});
-                LEAP_HIP(hipGetLastError());
-            }
-            kernel<<<dim3(rem * chunks, 1), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a,
-                                                           (LEAP_GLOBAL double*)tail, kNufftPartial,
-                                                           NufftSpan{full, chunks});
-            LEAP_HIP(hipGetLastError());
-            const size_t n = size_t(rem) * D;
-            nufft_tail_sum<<<dim3(uint32_t((n + 255) / 256)), 256, 0, s>>>(tail, rem, chunks, D, nbl, full, Pout,
-                                                                        (double*)ga);
-            return;
-        }
-    }
-    kernel<<<dim3(nbl_grid, ny), kWave, 0, s>>>(w, gs, gu, gd, D, T, nbl, C, Pout, a, ga,
-                                               accumulate ? kNufftAccumulate : kNufftSet, NufftSpan{0, 1});
-}
-
-template <int N, int W, int SP>
-void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
-                const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout, const NufftArgs& a,
-                double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
-    const auto* gs = (const LEAP_GLOBAL double*)scale;
-    const auto* gu = (const LEAP_GLOBAL double*)uvw;
-    const auto* gd = (const LEAP_GLOBAL double*)dd;
-    auto* ga = (LEAP_GLOBAL double*)avg;
-    switch (tile) {
-        case 641:
-            launch_nufft_tile<N, W, 1, SP>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
-                                           tail_doubles);
-            break;
-        case 1282:
-            launch_nufft_tile<N, W, 2, SP>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
-                                           tail_doubles);
-            break;
-        case 2564:
-            launch_nufft_tile<N, W, 4, SP>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
-                                           tail_doubles);
-            break;
-        default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported NUFFT tile " + std::to_string(tile));
-    }
-}
-
-template <int N, int W>
-void launch_nw(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, int32_t src_pols, const double* scale,
-               const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
-               const NufftArgs& a, double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
-    switch (src_pols) {
-        case 0: launch_nws<N, W, 0>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        case 1: launch_nws<N, W, 1>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        case 2: launch_nws<N, W, 2>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        case 4: launch_nws<N, W, 4>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        default: throw Error(LEAP_ERR_INTERNAL, "unsupported NUFFT row source");
-    }
-}
-
-}  // namespace
 
 size_t nufft_tail_bytes(uint32_t D) {
     int dev = 0, cus = 0;
@@ -818,29 +44,34 @@ void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, con
     require(P_out >= 1, LEAP_ERR_INTERNAL, "NUFFT output needs at least pol 0");
     require(plan.N != 0 && plan.scale.size() == C, LEAP_ERR_INTERNAL, "NUFFT plan does not match the channel count");
     require(!cube || (P == 1 || P == 2 || P == 4), LEAP_ERR_INTERNAL, "NUFFT cube source needs 1, 2 or 4 pols");
-    NufftArgs a{};
-    a.coef = tables;
-    a.kappa_mid = (f0 + double(plan.h) * df) / kSpeedOfLight;
-    a.dkappa = df / kSpeedOfLight;
-    a.h = int32_t(plan.h);
-    const double* scale = tables + kNufftCoefDoubles;
-    const int32_t sp = cube ? int32_t(P) : 0;
-#define LEAP_NUFFT(N_, W_)                                                                                   \
-    if (plan.N == N_ && plan.W == W_) {                                                                      \
-        launch_nw<N_, W_>(s, tile, nbl_grid, src, sp, scale, uvw, dd, D, T, nbl, C, P_out, a, avg,           \
-                          accumulate, tail, tail_bytes / sizeof(double));                                    \
-        LEAP_HIP(hipGetLastError());                                                                         \
-        return;                                                                                              \
-    }
-    LEAP_NUFFT(256, 10)
-    LEAP_NUFFT(256, 12)
-    LEAP_NUFFT(512, 10)
-    LEAP_NUFFT(512, 12)
-    LEAP_NUFFT(1024, 10)
-    LEAP_NUFFT(1024, 12)
-#undef LEAP_NUFFT
-    throw Error(LEAP_ERR_INTERNAL, "no NUFFT kernel for grid " + std::to_string(plan.N) + ", width " +
-                                       std::to_string(plan.W));
+    nufft::NufftLaunch l{};
+    l.tile = tile;
+    l.plan = &plan;
+    l.src = src;
+    l.src_pols = cube ? int32_t(P) : 0;
+    l.scale = tables + kNufftCoefDoubles;
+    l.uvw = uvw;
+    l.dd = dd;
+    l.D = D;
+    l.T = T;
+    l.nbl = nbl;
+    l.nbl_grid = nbl_grid;
+    l.C = C;
+    l.P_out = P_out;
+    l.args.coef = tables;
+    l.args.coef32 = reinterpret_cast<const float*>(tables + nufft_coef32_offset(C));
+    l.args.kappa_mid = (f0 + double(plan.h) * df) / kSpeedOfLight;
+    l.args.dkappa = df / kSpeedOfLight;
+    l.args.h = int32_t(plan.h);
+    l.avg = avg;
+    l.accumulate = accumulate;
+    l.tail = tail;
+    l.tail_doubles = tail_bytes / sizeof(double);
+    if (plan.fp32)
+        launch_rotate_nufft_f32(s, l);
+    else
+        launch_rotate_nufft_f64(s, l);
+    LEAP_HIP(hipGetLastError());
 }
 
 }  // namespace leap

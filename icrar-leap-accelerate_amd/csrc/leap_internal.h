@@ -78,6 +78,10 @@ PhaseMatrix phase_matrix(const int32_t* a1, const int32_t* a2, const uint8_t* fg
 // for an incidence-plus-constraint matrix A.
 std::vector<double> gram_pinv(const PhaseMatrix& pm);
 
+// Dimension of the null space of A (components of the baseline graph without
+// the constraint antenna); the reference warns "Ad is degenerate" when > 0.
+int32_t null_components(const PhaseMatrix& pm);
+
 // Filtered baselines (flags of the first epoch OR short baselines).
 std::vector<uint8_t> filtered_baselines(const leap_ms_view& ms, double threshold);
 
@@ -115,8 +119,9 @@ constexpr int kNufftCoefDoubles = (kNufftMaxWidth / 2) * kNufftMaxWidth;
 constexpr uint32_t kNufftMinDirections = 16;
 constexpr uint32_t kNufftMinChannels = 96;
 struct NufftPlan {
+    bool fp32 = false;  // FP32 transform, grid and interpolation (W 8 or 10) instead of FP64 (W 10 or 12)
     uint32_t N = 0;  // grid size (power of two, >= 2C, >= 256)
-    uint32_t W = 0;  // kernel width in grid points (10 or 12)
+    uint32_t W = 0;  // kernel width in grid points
     uint32_t h = 0;  // 64 * round(C / 128): frequency k = c - h
     double beta = 0.0;
     // coef[j][m]: coefficient of s^m in tap j's weight polynomial, j < W / 2
@@ -124,7 +129,15 @@ struct NufftPlan {
     std::vector<double> scale;  // C precorrection factors (2 pi / N) / psi^(c - h)
 };
 bool nufft_supported(uint32_t C);
-NufftPlan make_nufft_plan(uint32_t C);
+inline bool is_nufft(int32_t channel_sum) {
+    return channel_sum == LEAP_CHANNEL_SUM_NUFFT || channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32;
+}
+NufftPlan make_nufft_plan(uint32_t C, bool fp32);
+// The plan's device tables: kNufftCoefDoubles coefficients (coef[j][m]), the C
+// precorrection factors, then the coefficients rounded to FP32 (from double
+// nufft_coef32_offset(C) on).
+std::vector<double> nufft_tables(const NufftPlan& plan);
+inline size_t nufft_coef32_offset(uint32_t C) { return size_t(kNufftCoefDoubles) + C; }
 
 }  // namespace leap
 
@@ -136,24 +149,40 @@ struct leap_metadata {
     double freq_start_hz = 0, freq_inc_hz = 0;
     double phase_centre_ra = 0, phase_centre_dec = 0;
     leap::PhaseMatrix A, A1;
-    std::vector<double> Gp, G1p;  // N x N pseudo-inverses of A^T A and A1^T A1
+    // The reference's degeneracy warnings (MetaData.cc:129-136): Ad A != I, Ad1 A1 != I.
+    bool ad_degenerate = false, ad1_degenerate = false;
     // device
     leap::DeviceBuffer d_AdT;      // K x N row-major == Ad column-major (N x K) transposed
     leap::DeviceBuffer d_Ad1T;     // K1 x N
     leap::DeviceBuffer d_I1;       // K1 - 1 ints (baseline index of phase angle k)
     leap::DeviceBuffer d_pairs;    // K - 1 int2: (antenna1, antenna2) of baseline I(k)
+    // the inversion's adjoint: A's baseline indices I, A1's antenna pairs, and
+    // each antenna's rows of A (CSR: offsets N + 1, entries 2 row + (antenna2 ? 1 : 0))
+    leap::DeviceBuffer d_I, d_pairs1, d_csr_off, d_csr_ent;
     leap::DeviceBuffer d_lambda;   // channels doubles: wavelength of each channel
-    leap::DeviceBuffer d_Gp;       // scratch for Ad assembly
+    leap::DeviceBuffer d_Gp;       // G+ = pinv(A^T A) then G1+ = pinv(A1^T A1), N x N each
     leap::DeviceBuffer d_AdT_planes;  // split-3 x BF16 planes of AdT in MFMA operand layout
-    leap::NufftPlan nufft;       // N == 0 when the channel count is not supported
-    leap::DeviceBuffer d_nufft;  // nufft.coef (kNufftCoefDoubles) then nufft.scale (channels) on the device
-    const double* nufft_scale() const { return d_nufft.as<double>() + leap::kNufftCoefDoubles; }
+    // NUFFT plans (N == 0 when the channel count is not supported) and their
+    // device tables (leap::nufft_tables): FP64 transform (LEAP_CHANNEL_SUM_NUFFT)
+    // and FP32 transform (LEAP_CHANNEL_SUM_NUFFT_FP32).
+    leap::NufftPlan nufft32, nufft64;
+    leap::DeviceBuffer d_nufft32, d_nufft64;
+    const leap::NufftPlan& nufft_plan(int32_t channel_sum) const {
+        return channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? nufft32 : nufft64;
+    }
+    const double* nufft_tables(int32_t channel_sum) const {
+        return (channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? d_nufft32 : d_nufft64).as<double>();
+    }
+    const double* nufft_scale(int32_t channel_sum) const {
+        return nufft_tables(channel_sum) + leap::kNufftCoefDoubles;
+    }
 };
 
 // TuneKey::precision holds the batched tuner's mode code; these bits mark the
 // packed path and its NUFFT channel sums.
 constexpr int32_t kTunePackedBit = 1024;
 constexpr int32_t kTuneNufftBit = 2048;
+constexpr int32_t kTuneNufft32Bit = 4096;
 
 struct TuneKey {
     int32_t precision, vis_dtype, all_pols;
@@ -191,6 +220,8 @@ struct CapturedSequence {
 
 struct leap_ctx {
     int device = 0;
+    leap_log_cb log = nullptr;  // diagnostics the reference logs (NULL: dropped)
+    void* log_user = nullptr;
     hipStream_t stream = nullptr;
     hipStream_t pack_stream = nullptr;  // pol-0 packs overlapping the Horner launches
     // host -> device staging of host-resident visibilities: pinned double buffer, copy stream
@@ -222,7 +253,10 @@ struct leap_ctx {
     // uvw, avgData adjoint, uvw / dz gradients, the antennas' baseline lists and their gradient
     leap::DeviceBuffer d_adj_M, d_adj_ants, d_adj_dp, d_adj_uvw, d_adj_abar, d_adj_guvw, d_adj_gdzp, d_adj_gdz,
         d_adj_off, d_adj_bls, d_adj_sgn, d_adj_gant;
-    leap::PinnedBuffer h_adj_avg, h_adj_abar;  // pol 0 of avgData with its arguments; the avgData adjoint
+    // pol 0 of avgData with its arguments; the inversion adjoint's intermediates; the
+    // per-direction losses and offset gradients copied back (pinned)
+    leap::DeviceBuffer d_adj_avg0, d_adj_inv;
+    leap::PinnedBuffer h_adj_out;
     std::vector<double> h_avg, h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;
@@ -303,6 +337,30 @@ void launch_dd(hipStream_t s, double ra0, double dec0, const double* d_dirs, uin
 void launch_rotate_f64(hipStream_t s, const void* vis, int32_t vis_dtype, const double* uvw,
                        const double* dd, uint32_t row_start, uint32_t T, uint32_t nbl, uint32_t C,
                        uint32_t P, const double* lambda, double* avg);
+// G+ = pinv(A^T A) on the device (kernels_f64.hip); all pointers device memory.
+struct GramPinv {
+    int32_t N, constraint;
+    const int32_t* csr_off;   // N + 1: antenna a's rows of A
+    const int32_t* csr_ent;   // 2 k + (a is antenna2 of row k)
+    const int2* pairs;        // antenna pairs of A's rows
+    int32_t n_components;     // components to invert (unused antennas left out)
+    const int32_t* comp_off;  // n_components + 1 offsets into members
+    const int32_t* members;   // antennas of each component, ascending
+    const uint8_t* has_c;     // 1 where the component holds the constraint antenna
+    const int64_t* ws_off;    // offset (doubles) of each component's 2 n^2 workspace
+    double* ws;
+    double* G;                // N x N scratch (the Gram matrix)
+    double* Gp;               // N x N output
+    int32_t* status;          // set to 1 when a block is not positive definite
+};
+void launch_gram_pinv(hipStream_t s, const GramPinv& g);
+// Components of the baseline graph of A (plus the constraint) whose block of G+
+// is computed: every component but unused antennas (union-find, host).
+struct GramComponents {
+    std::vector<int32_t> off, members;
+    std::vector<uint8_t> has_c;
+};
+GramComponents gram_components(const PhaseMatrix& pm);
 void launch_assemble_ad(hipStream_t s, const double* Gp, int32_t N, const int2* pairs,
                         int32_t n_pairs, int32_t constraint, double* AdT);
 void launch_inversion_f64(hipStream_t s, const leap_metadata& md, const double* avg, uint32_t D,
@@ -339,8 +397,40 @@ void launch_perturb_uvw(hipStream_t s, const double* uvw0, const double* M, cons
 void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, const double* uvw, const double* dd,
                            const double* abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
                            double df, double* g_uvw, double* g_dz_partial);
-void launch_reduce_gdz(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double* out);
+// g_dz[d][k] (3 D doubles at out) reduced over baselines, then dL/d(ra, dec) of each
+// direction (2 D doubles at out + 3 D); dirs = the directions (ra, dec) on the device.
+void launch_reduce_gdz_offsets(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double ra0, double dec0,
+                               const double* dirs, double* out);
 void launch_gather_pol0(hipStream_t s, const double* avg, uint32_t n, uint32_t P, double* out);
+// The residual loss of D calibrations and the adjoint of the inversion
+// (kernels_adjoint.hip); every array [d][...] on the device.
+struct InversionAdjoint {
+    uint32_t D, nbl;
+    int32_t K, N, K1;
+    const double* cal;       // D x N calibrations (forward inversion)
+    const double* avg0;      // D x nbl complex pol 0 of avgData
+    const double* ph;        // D x nbl arguments of avg0
+    const int2* pairs;       // K - 1 antenna pairs of A's rows
+    const int32_t* I;        // K - 1 baselines of A's rows
+    const int2* pairs1;      // K1 - 1 antenna pairs of A1's rows
+    const int32_t* I1;       // K1 - 1 baselines of A1's rows
+    const int32_t* csr_off;  // N + 1: antenna a's rows are csr_ent[csr_off[a] .. csr_off[a + 1])
+    const int32_t* csr_ent;  // 2 (K - 1): 2 row + 1 where a is the row's antenna2 (sign -1)
+    const double* Gp;        // N x N pseudo-inverse of A^T A
+    const double* G1p;       // N x N pseudo-inverse of A1^T A1
+    double* res;             // D x (K - 1) residuals
+    double* part;            // D x ceil((K - 1) / 256) partial loss sums
+    double* loss;            // D per-direction losses
+    // backward (skipped when abar is null)
+    double* g_arg;           // D x nbl
+    double* g_cal;           // D x N
+    double* y;               // D x N
+    double* gacal1;          // D x (K - 1)
+    double* g_cal1;          // D x N
+    double* y1;              // D x N
+    double* abar;            // D x nbl complex: dL/dRe avg + i dL/dIm avg
+};
+void launch_inversion_adjoint(hipStream_t s, const InversionAdjoint& a);
 void launch_antenna_grad(hipStream_t s, const double* g_uvw, const double* M, const int32_t* csr_off,
                          const int32_t* csr_bl, const int32_t* csr_sign, int32_t n_ant, uint32_t nbl, uint32_t T,
                          double* out);

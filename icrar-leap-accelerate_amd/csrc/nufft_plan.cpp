@@ -16,9 +16,13 @@
 // of 64, so channel c's grid index is congruent to c modulo the wavefront
 // width (each lane's channels land in its own FFT slots); N is the power of two
 // >= 2C (at least 256), which keeps |k| <= N / 4; with sigma = N / (2 max|k|)
-// >= 2, W = 10 for sigma >= 2.5, else 12, and beta = 0.97 pi W (1 - 1 / (2 sigma)).
-// The relative error of F against sum |v_c| stays below 1e-10 (about 1e-11
-// with W = 10, 3e-12 with W = 12) for every C up to 512.
+// >= 2 and beta = 0.97 pi W (1 - 1 / (2 sigma)):
+//   FP64 transform: W = 10 for sigma >= 2.5, else 12.  The relative error of F
+//     against sum |v_c| stays below 1e-10 (about 1e-11 with W = 10, 3e-12 with
+//     W = 12) for every C up to 512.
+//   FP32 transform: W = 8 for sigma >= 2.5, else 10 (the kernel's own error,
+//     about 3e-9 and 2e-10, then stays below the FP32 transform's rounding,
+//     about 2e-8 of sum |v_c|).
 //
 // Step 3's weights come from polynomials in s = u - m0 - (W - 1) / 2 in
 // (-1/2, 1/2]: tap j's weight is phi((s + (W-1)/2 - j) / (W/2)) = P_j(s), and
@@ -77,7 +81,17 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
 
 bool nufft_supported(uint32_t C) { return C >= 1 && C <= kNufftMaxChannels; }
 
-NufftPlan make_nufft_plan(uint32_t C) {
+std::vector<double> nufft_tables(const NufftPlan& p) {
+    const size_t C = p.scale.size();
+    std::vector<double> t(&p.coef[0][0], &p.coef[0][0] + kNufftCoefDoubles);
+    t.insert(t.end(), p.scale.begin(), p.scale.end());
+    t.resize(nufft_coef32_offset(uint32_t(C)) + (kNufftCoefDoubles + 1) / 2, 0.0);
+    float* f = reinterpret_cast<float*>(t.data() + nufft_coef32_offset(uint32_t(C)));
+    for (int i = 0; i < kNufftCoefDoubles; ++i) f[i] = float((&p.coef[0][0])[i]);
+    return t;
+}
+
+NufftPlan make_nufft_plan(uint32_t C, bool fp32) {
     require(nufft_supported(C), LEAP_ERR_INVALID_ARGUMENT,
             "the NUFFT channel sums support 1.." + std::to_string(kNufftMaxChannels) + " channels, not " +
                 std::to_string(C));
@@ -88,7 +102,11 @@ NufftPlan make_nufft_plan(uint32_t C) {
     p.h = 64 * ((C + 64) / 128);  // 64 * round(C / 128)
     const uint32_t kmax = std::max<uint32_t>(1, std::max(p.h, C - 1 - std::min(C - 1, p.h)));
     const double sigma = double(N) / (2.0 * kmax);
-    p.W = sigma >= 2.5 ? 10 : 12;
+    p.fp32 = fp32;
+    if (fp32)
+        p.W = sigma >= 2.5 ? 8 : 10;
+    else
+        p.W = sigma >= 2.5 ? 10 : 12;
     p.beta = 0.97 * M_PI * p.W * (1.0 - 1.0 / (2.0 * sigma));
     // psi^(k) = (W pi / N) int_{-1}^{1} phi(z) cos(k z W pi / N) dz
     std::vector<double> gx, gw;

@@ -22,6 +22,7 @@ LEAP_PRECISION_MIXED = 1
 LEAP_CHANNEL_SUM_AUTO = 0
 LEAP_CHANNEL_SUM_HORNER = 1
 LEAP_CHANNEL_SUM_NUFFT = 2
+LEAP_CHANNEL_SUM_NUFFT_FP32 = 3
 LEAP_STRATEGY_PER_DIRECTION = 0
 LEAP_STRATEGY_BATCHED = 1
 
@@ -119,6 +120,7 @@ class Solution(ctypes.Structure):
 
 
 SOLUTION_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Solution))
+LOG_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p)
 
 
 class BatchSet(ctypes.Structure):
@@ -167,6 +169,7 @@ EXPORTS = [
     "leap_hip_create", "leap_hip_destroy", "leap_hip_last_error", "leap_hip_version",
     "leap_hip_calibrate", "leap_hip_calibrate_md", "leap_hip_calibrate_md_noalloc",
     "leap_hip_metadata_create", "leap_hip_metadata_destroy", "leap_hip_metadata_dims", "leap_hip_metadata_copy",
+    "leap_hip_metadata_diagnostics", "leap_hip_set_log_callback",
     "leap_hip_rotate_visibilities", "leap_hip_apply_inversion", "leap_hip_dd_matrix",
     "leap_synth_create", "leap_synth_destroy", "leap_synth_view", "leap_synth_sky", "leap_synth_fill_vis_host",
     "leap_hip_synth_fill_vis_device",
@@ -176,7 +179,7 @@ EXPORTS = [
     "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes",
     "leap_hip_tuning_save", "leap_hip_tuning_load", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
-    "leap_multi_item_cost", "leap_phasor64", "leap_nufft_plan",
+    "leap_multi_item_cost", "leap_phasor64", "leap_nufft_plan", "leap_nufft_plan32",
 ]
 
 
@@ -220,6 +223,7 @@ def lib():
     L.leap_phasor64.argtypes = [_dp, ctypes.c_uint32, _dp, _dp]
     _u32p = ctypes.POINTER(ctypes.c_uint32)
     L.leap_nufft_plan.argtypes = [ctypes.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
+    L.leap_nufft_plan32.argtypes = [ctypes.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
     L.leap_multi_item_cost.restype = ctypes.c_double
     L.leap_hip_metadata_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), ctypes.c_int32,
                                            ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
@@ -228,6 +232,8 @@ def lib():
     L.leap_hip_metadata_dims.argtypes = [ctypes.c_void_p, _i32p]
     L.leap_hip_metadata_dims.restype = None
     L.leap_hip_metadata_copy.argtypes = [ctypes.c_void_p, _dp, _i32p, _dp, _dp, _i32p, _dp]
+    L.leap_hip_metadata_diagnostics.argtypes = [ctypes.c_void_p, _i32p, _i32p]
+    L.leap_hip_set_log_callback.argtypes = [ctypes.c_void_p, LOG_CB, ctypes.c_void_p]
     L.leap_hip_rotate_visibilities.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MsView),
                                                ctypes.c_uint32, ctypes.c_uint32, _dp, ctypes.c_uint32,
                                                ctypes.POINTER(Options), _dp]
@@ -474,15 +480,16 @@ def multi_item_cost(T, C, tile):
     return lib().leap_multi_item_cost(T, C, tile)
 
 
-def nufft_plan(C):
-    """The NUFFT plan the mixed path uses for C channels: dict of N, W, h, beta,
-    coef (W/2 x W weight-polynomial coefficients in s) and scale (C precorrection factors)."""
+def nufft_plan(C, fp32=False):
+    """The NUFFT plan the mixed path uses for C channels (fp32: the FP32 transform of
+    LEAP_CHANNEL_SUM_NUFFT_FP32, else the FP64 one of LEAP_CHANNEL_SUM_NUFFT): dict of N, W,
+    h, beta, coef (W/2 x W weight-polynomial coefficients in s) and scale (C precorrection factors)."""
     N, W, h = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
     beta = ctypes.c_double()
     coef = np.zeros(6 * 12)
     scale = np.zeros(max(int(C), 1))
-    _check(lib().leap_nufft_plan(C, ctypes.byref(N), ctypes.byref(W), ctypes.byref(h), ctypes.byref(beta),
-                                 _dptr(coef), _dptr(scale)))
+    fn = lib().leap_nufft_plan32 if fp32 else lib().leap_nufft_plan
+    _check(fn(C, ctypes.byref(N), ctypes.byref(W), ctypes.byref(h), ctypes.byref(beta), _dptr(coef), _dptr(scale)))
     w = W.value
     return {"N": N.value, "W": w, "h": h.value, "beta": beta.value,
             "coef": coef.reshape(6, 12)[: w // 2, :w].copy(), "scale": scale[: int(C)].copy()}
@@ -530,6 +537,11 @@ class Context:
     def synth_fill_device(self, synth, vis_dtype, vis_ptr, uvw_ptr):
         self._chk(lib().leap_hip_synth_fill_vis_device(self.h, synth.h, vis_dtype, vis_ptr,
                                                        ctypes.cast(uvw_ptr, _dp) if uvw_ptr else None))
+
+    def set_log(self, fn):
+        """fn(level, message) receives the context's diagnostics (None drops them)."""
+        self._log_cb = LOG_CB(lambda _u, level, msg: fn(level, msg.decode())) if fn else LOG_CB()
+        self._chk(lib().leap_hip_set_log_callback(self.h, self._log_cb, None))
 
     # metadata / kernels ---------------------------------------------------
     def metadata(self, view, ref_ant=None, threshold=0.0, use_cache=False):
@@ -677,6 +689,12 @@ class Metadata:
                                             I1.ctypes.data_as(_i32p), _dptr(Ad1)))
         return {"A": A.T.copy(), "I": I[: self.K - 1].copy(), "Ad": Ad.T.copy(), "A1": A1.T.copy(),
                 "I1": I1[: self.K1 - 1].copy(), "Ad1": Ad1.T.copy()}
+
+    def diagnostics(self):
+        """(Ad degenerate, Ad1 degenerate): the reference's warnings of MetaData.cc:129-136."""
+        a, a1 = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().leap_hip_metadata_diagnostics(self.h, ctypes.byref(a), ctypes.byref(a1)))
+        return bool(a.value), bool(a1.value)
 
     def rotate(self, view, directions, row_start=0, n_rows=None, opts=None):
         dirs = np.ascontiguousarray(directions, dtype=np.float64).reshape(-1, 2)

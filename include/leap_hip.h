@@ -124,7 +124,15 @@ typedef enum leap_channel_sum {
      * (host-resident or complex128 cubes, 3 pols) are first packed to pol-0 rows.  Error of a
      * row's channel sum: about 1e-11 of sum |v| (the tests bound avgData by 5e-10 of the sum
      * of |v| over the interval's rows and channels). */
-    LEAP_CHANNEL_SUM_NUFFT = 2
+    LEAP_CHANNEL_SUM_NUFFT = 2,
+    /* The same NUFFT with the transform, the grid and the interpolation in FP32 (W = 8 or
+     * 10; the shift factor, grid coordinate, phasor and the sums over rows stay FP64):
+     * about a third less time at 384 channels, 2e-8 to 7e-8 of sum |v| per row (tests:
+     * avgData within 2e-7 of the sum of |v|).  Outside the 1e-5 rad budget of the mixed
+     * precision: a baseline whose sum nearly cancels takes a phase error of that order,
+     * which the inversion spreads over the calibration (measured 4e-5 rad on 48 stations,
+     * 2.5e-2 rad on the 256-station, 128-timestep set). */
+    LEAP_CHANNEL_SUM_NUFFT_FP32 = 3
 } leap_channel_sum;
 
 typedef struct leap_options {
@@ -171,6 +179,10 @@ typedef int (*leap_solution_cb)(void* user, const leap_solution* solution);
 
 typedef struct leap_ctx leap_ctx;
 typedef struct leap_metadata leap_metadata;
+
+/* Diagnostics the reference writes to its log (Boost LOG(warning)). */
+typedef enum leap_log_level { LEAP_LOG_INFO = 0, LEAP_LOG_WARNING = 1 } leap_log_level;
+typedef void (*leap_log_cb)(void* user, int32_t level, const char* message);
 
 /* ---- context -------------------------------------------------------------- */
 int leap_hip_create(leap_ctx** out, int32_t device);
@@ -272,6 +284,13 @@ int leap_hip_metadata_create(leap_ctx* ctx, const leap_ms_view* ms, int32_t refe
                              double min_baseline_threshold, int32_t use_cache,
                              leap_metadata** out);
 void leap_hip_metadata_destroy(leap_metadata* md);
+/* Receives the context's diagnostics (NULL drops them): the warnings "Ad is
+ * degenerate" / "Ad1 is degenerate" of MetaData::MetaData (model/cpu/MetaData.cc:129-136),
+ * raised when (Ad A).isApprox(I, 0.001) fails, i.e. when the baseline graph has
+ * a component without the constraint antenna (an unused antenna column included). */
+int leap_hip_set_log_callback(leap_ctx* ctx, leap_log_cb callback, void* user);
+/* Those two conditions of a built metadata (1 = degenerate). */
+int leap_hip_metadata_diagnostics(const leap_metadata* md, int32_t* ad_degenerate, int32_t* ad1_degenerate);
 /* dims = {K = A.rows, N = A.cols, K1 = A1.rows, nbaselines, referenceAntenna} */
 void leap_hip_metadata_dims(const leap_metadata* md, int32_t* dims);
 /* Host copies, column-major like Eigen; any pointer may be NULL.
@@ -344,6 +363,9 @@ int leap_phasor64(const double* theta, uint32_t n, double* cos_out, double* sin_
  * channels directly, CpuLeapCalibrator.cc:181-217); exported so the plan can be
  * checked without a GPU. */
 int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef, double* scale);
+/* The same for the FP32 transform of LEAP_CHANNEL_SUM_NUFFT_FP32 (narrower kernel). */
+int leap_nufft_plan32(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
+                      double* scale);
 
 /* ---- synthetic measurement sets ------------------------------------------- */
 typedef struct leap_synth_config {

@@ -546,6 +546,31 @@ void oracle_metadata_dims(const oracle_metadata* md, int32_t* dims) {
     dims[4] = int32_t(md->reference_antenna);
 }
 
+// (Ad A).isApprox(I, 0.001) of MetaData.cc:129-136 with Eigen's definition of
+// isApprox: ||X - I||_F^2 <= p^2 min(||X||_F^2, ||I||_F^2).  Returns 1 when the
+// reference would log "Ad is degenerate".
+namespace {
+int32_t degenerate(const Mat& Ad, const Mat& A) {
+    const int32_t n = A.cols, m = A.rows;
+    double diff2 = 0.0, x2 = 0.0;
+    for (int32_t j = 0; j < n; ++j)
+        for (int32_t i = 0; i < n; ++i) {
+            double x = 0.0;
+            for (int32_t k = 0; k < m; ++k) x += Ad(i, k) * A(k, j);
+            const double d = x - (i == j ? 1.0 : 0.0);
+            diff2 += d * d;
+            x2 += x * x;
+        }
+    const double p = 0.001;
+    return diff2 <= p * p * std::min(x2, double(n)) ? 0 : 1;
+}
+}  // namespace
+
+void oracle_metadata_degenerate(const oracle_metadata* md, int32_t* ad, int32_t* ad1) {
+    if (ad) *ad = degenerate(md->Ad, md->A);
+    if (ad1) *ad1 = degenerate(md->Ad1, md->A1);
+}
+
 void oracle_metadata_copy(const oracle_metadata* md, double* A, int32_t* I, double* Ad,
                           double* A1, int32_t* I1, double* Ad1) {
     if (A) std::copy(md->A.v.begin(), md->A.v.end(), A);

@@ -88,6 +88,9 @@ int oracle_metadata_create(const oracle_ms_view* ms, int32_t ref_ant,
 void oracle_metadata_destroy(oracle_metadata* md);
 /* dims: [K, N, K1, nbl, reference_antenna] */
 void oracle_metadata_dims(const oracle_metadata* md, int32_t* dims);
+/* The degeneracy tests of MetaData.cc:129-136 ((Ad A).isApprox(I, 0.001) and
+ * the same for A1) evaluated on the dense matrices: 1 = the reference warns. */
+void oracle_metadata_degenerate(const oracle_metadata* md, int32_t* ad, int32_t* ad1);
 /* Any pointer may be NULL.  Sizes follow the dims. */
 void oracle_metadata_copy(const oracle_metadata* md, double* A, int32_t* I, double* Ad,
                           double* A1, int32_t* I1, double* Ad1);

@@ -101,6 +101,8 @@ class Oracle:
         lib.oracle_metadata_dims.restype = None
         lib.oracle_metadata_copy.argtypes = [ctypes.c_void_p, _c_dp, _c_i32p, _c_dp, _c_dp, _c_i32p, _c_dp]
         lib.oracle_metadata_copy.restype = None
+        lib.oracle_metadata_degenerate.argtypes = [ctypes.c_void_p, _c_i32p, _c_i32p]
+        lib.oracle_metadata_degenerate.restype = None
         lib.oracle_rotate_visibilities.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), ctypes.c_uint32,
                                                    ctypes.c_uint32, _c_dp, _c_dp]
         lib.oracle_apply_inversion.argtypes = [ctypes.c_void_p, _c_dp, _c_dp]
@@ -245,6 +247,12 @@ class OracleMetaData:
                                                             _ptr(d, ctypes.c_double),
                                                             out.ctypes.data_as(_c_dp)))
         return out
+
+    def degenerate(self):
+        """(Ad degenerate, Ad1 degenerate) by the reference's isApprox test on the dense matrices."""
+        a, a1 = ctypes.c_int32(), ctypes.c_int32()
+        self.o.lib.oracle_metadata_degenerate(self.h, ctypes.byref(a), ctypes.byref(a1))
+        return bool(a.value), bool(a1.value)
 
     def apply_inversion(self, avg):
         avg = np.ascontiguousarray(avg, dtype=np.complex128)

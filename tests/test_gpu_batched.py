@@ -91,7 +91,8 @@ def test_batched_fp64_calibrate_matches_oracle(leap, ctx, oracle, small, slice_)
     assert wrapped(got, want).max() < FP64_TOL
 
 
-ROUTES = {"nufft": dict(channel_sum=2), "horner": dict(channel_sum=1), "fp32-blocks": dict(fp32_blocks=1)}
+ROUTES = {"nufft": dict(channel_sum=2), "nufft-fp32": dict(channel_sum=3), "horner": dict(channel_sum=1),
+          "fp32-blocks": dict(fp32_blocks=1)}
 
 
 @pytest.mark.parametrize("route", list(ROUTES))
@@ -115,8 +116,9 @@ def test_mixed_precision_calibrate_error_budget(leap, ctx, oracle, dtype, route)
     per_dir_rms = np.sqrt((err ** 2).mean(axis=(0, 2)))
     print(f"mixed {np.dtype(dtype).name} {route}: wrapped error max {per_dir_max.max():.2e} rad, "
           f"rms {per_dir_rms.max():.2e} rad (worst direction)")
-    # The FP32 blocked variant is kept for comparison; it does not meet the 1e-5 rad budget.
-    assert per_dir_max.max() < (MIXED_TOL if not fp32_blocks else 1e-2)
+    # The FP32 blocked Horner and the FP32 NUFFT transform are kept for comparison (faster); they
+    # do not meet the 1e-5 rad budget.
+    assert per_dir_max.max() < {"fp32-blocks": 1e-2, "nufft-fp32": 1e-4}.get(route, MIXED_TOL)
     if dtype == np.complex64:
         # the cube itself was rounded to complex64: report the FP64-cube distance too
         ref64, _ = oracle.calibrate(ms64, dirs)

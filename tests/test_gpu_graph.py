@@ -57,12 +57,12 @@ class DeviceSet:
 
 
 CASES = [  # precision, strategy, direction block, channel sums
-    (0, 0, 0, 0), (0, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 2), (0, 1, 64, 0),
+    (0, 0, 0, 0), (0, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 2), (1, 1, 0, 3), (0, 1, 64, 0),
 ]
 
 
 @pytest.mark.parametrize("precision,strategy,db,cs", CASES,
-                         ids=["fp64-per-dir", "fp64-batched", "mixed", "mixed-nufft", "fp64-db64"])
+                         ids=["fp64-per-dir", "fp64-batched", "mixed", "mixed-nufft", "mixed-nufft-fp32", "fp64-db64"])
 def test_graph_replays_equal_eager(leap, ctx, precision, strategy, db, cs):
     a = DeviceSet(leap, ctx, np.complex64 if precision else np.complex128, n_antennas=24, n_timesteps=9,
                   n_channels=50, seed=81, nan_fraction=0.002)
@@ -98,7 +98,7 @@ def test_graph_replays_equal_eager(leap, ctx, precision, strategy, db, cs):
 
 
 @pytest.mark.parametrize("precision,strategy,db,cs", CASES,
-                         ids=["fp64-per-dir", "fp64-batched", "mixed", "mixed-nufft", "fp64-db64"])
+                         ids=["fp64-per-dir", "fp64-batched", "mixed", "mixed-nufft", "mixed-nufft-fp32", "fp64-db64"])
 def test_graph_capture_first_on_a_fresh_context(leap, precision, strategy, db, cs):
     """A captured calibration as the first call of a new context: every buffer the captured
     sequence uses (the NUFFT's chunked-round scratch included) is reserved before the capture,

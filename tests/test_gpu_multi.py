@@ -139,6 +139,14 @@ def test_uneven_batch_full_size(leap, ctx, oracle, precision):
                 assert np.array_equal(got, one), "multi-set and single-set results differ"
             print(f"uneven batch {'mixed' if precision else 'fp64'} tile {db}x{nd}: one launch {multi_ms:.3f} ms, "
                   f"three single-set launches {single_ms:.3f} ms")
+        # the same batch (tile 64 x 1) captured as one HIP graph and replayed: bit-equal to the eager launch
+        gopts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=64,
+                             lane_directions=1, graph=1)
+        for k in range(3):
+            resg = ctx.calibrate_batch([(md, v, d, (0, -1, -1)) for (_, _, _, v, md, d) in sets], gopts)
+            assert ctx.last_graph() == (1 if k == 0 else 2)
+            for (got, ge), (want, we) in zip(resg, res):
+                assert np.array_equal(got, want) and np.array_equal(ge, we), k
         for (s, ms, host, view, md, dirs), (got, _) in zip(sets[:2], res[:2]):
             pick = [0, 33, 63]
             want, _ = oracle.calibrate(dict(ms, vis=host), dirs[pick])

@@ -205,3 +205,68 @@ def test_errors_are_reported_not_fatal(leap, ctx):
     # the context still works afterwards
     cal, _ = ctx.calibrate(view, [(0.5, -0.47)])
     assert np.isfinite(cal).all()
+
+
+def test_degeneracy_diagnostics_match_reference_rule(leap, ctx, oracle):
+    """"Ad is degenerate" / "Ad1 is degenerate" (MetaData.cc:129-136): the product's
+    combinatorial test against the oracle's (Ad A).isApprox(I, 0.001) on the dense
+    matrices, and the messages through the log callback."""
+    import planted as pl
+
+    base = pl.inversion_ms(6, autos=True)
+    a1, a2 = base["antenna1"], base["antenna2"]
+    touches3 = np.nonzero(((a1 == 3) | (a2 == 3)) & (a1 != a2))[0]
+    b03 = int(np.nonzero((a1 == 0) & (a2 == 3))[0][0])
+    cases = {
+        "clean": (pl.inversion_ms(6, autos=True), (False, False)),
+        "antenna 3 isolated": (pl.inversion_ms(6, autos=True, flagged=touches3), (True, True)),
+        "no (0, 3) baseline": (pl.inversion_ms(6, autos=True, flagged=(b03,)), (False, True)),
+    }
+    messages = []
+    ctx.set_log(lambda level, msg: messages.append((level, msg)))
+    try:
+        for name, (ms, want) in cases.items():
+            messages.clear()
+            md = ctx.metadata(leap.view_from_arrays(ms), ref_ant=0)
+            assert oracle.metadata(ms, ref_ant=0).degenerate() == want, name
+            assert md.diagnostics() == want, name
+            expect = [(1, m) for m, on in zip(("Ad is degenerate", "Ad1 is degenerate"), want) if on]
+            assert messages == expect, name
+    finally:
+        ctx.set_log(None)
+
+
+@pytest.mark.parametrize("case", ["connected", "isolated", "components"])
+def test_device_pseudo_inverse_matches_host_gram_routine(leap, ctx, case):
+    """The metadata's Ad and Ad1, computed from G+ = pinv(A^T A) on the device,
+    against the host routine leap_phase_matrix_pinv (same Gram construction on
+    the CPU) and LAPACK, including components without the constraint antenna
+    and an unused antenna."""
+    from test_capi import _host_phase_matrix, _incidence
+
+    rng = np.random.default_rng(7)
+    if case == "connected":
+        a1, a2, fg = _incidence(30, True, rng, 0.1)
+    elif case == "isolated":
+        a1, a2, fg = _incidence(24, True, rng, 0.05, isolate=(5, 17))
+    else:  # {0..3} with the constraint, {4..7} without it, antenna 8 unused
+        a1 = np.array([0, 0, 1, 2, 4, 4, 5, 6, 0], dtype=np.int32)
+        a2 = np.array([1, 2, 3, 3, 5, 6, 7, 7, 8], dtype=np.int32)
+        fg = np.array([0, 0, 0, 0, 0, 0, 0, 0, 1], dtype=np.uint8)
+    nbl = len(a1)
+    n_ant = int(max(a1.max(), a2.max())) + 1
+    ms = {"num_stations": n_ant, "total_antennas": n_ant, "num_baselines": nbl, "num_rows": nbl,
+          "num_channels": 1, "num_pols": 1, "freq_start_hz": 1e8, "freq_inc_hz": 1e6, "phase_centre": (0.1, -0.5),
+          "antenna1": a1, "antenna2": a2, "time": np.zeros(nbl), "flag": fg.astype(np.uint8),
+          "uvw": np.zeros((nbl, 3)), "vis": np.zeros((nbl, 1, 1), dtype=np.complex128)}
+    ref = 0
+    md = ctx.metadata(leap.view_from_arrays(ms), ref_ant=ref)
+    got = md.copy()
+    A, _, Ad_host = _host_phase_matrix(leap, a1, a2, fg.astype(np.uint8), None)
+    A1, _, Ad1_host = _host_phase_matrix(leap, a1, a2, fg.astype(np.uint8), ref)
+    assert np.array_equal(got["A"], A) and np.array_equal(got["A1"], A1)
+    err = max(np.abs(got["Ad"] - Ad_host).max(), np.abs(got["Ad1"] - Ad1_host).max())
+    print(f"device vs host Gram pseudo-inverse ({case}): {err:.2e}")
+    assert err < 1e-13
+    assert np.abs(got["Ad"] - np.linalg.pinv(A)).max() < 1e-11
+    assert np.abs(got["Ad1"] - np.linalg.pinv(A1)).max() < 1e-11

@@ -191,3 +191,19 @@ def test_short_baselines_feed_the_phase_matrix(oracle):
     md = oracle.metadata(ms, ref_ant=2, threshold=60.0)  # baseline 0 (0,1) is short
     assert md.I.tolist() == [1, 2]
     assert md.A.tolist() == [[1, 0, -1], [0, 1, -1], [1, 0, 0]]
+
+
+def test_degeneracy_rule(oracle):
+    """(Ad A).isApprox(I, 0.001) (MetaData.cc:129-136) fails exactly when the
+    baseline graph has a component without the constraint antenna: Ad A is then
+    the projection I - sum_S 1_S 1_S^T / |S|, at Frobenius distance sqrt(#S) from I."""
+    base = pl.inversion_ms(6, autos=True)
+    a1, a2 = base["antenna1"], base["antenna2"]
+    touches3 = np.nonzero(((a1 == 3) | (a2 == 3)) & (a1 != a2))[0]
+    b03 = int(np.nonzero((a1 == 0) & (a2 == 3))[0][0])
+    assert oracle.metadata(base, ref_ant=0).degenerate() == (False, False)
+    iso = oracle.metadata(pl.inversion_ms(6, autos=True, flagged=touches3), ref_ant=0)
+    assert iso.degenerate() == (True, True)
+    P = iso.Ad @ iso.A
+    assert abs(np.linalg.norm(P - np.eye(6)) - 1.0) < 1e-12  # one null component: antenna 3
+    assert oracle.metadata(pl.inversion_ms(6, autos=True, flagged=(b03,)), ref_ant=0).degenerate() == (False, True)
