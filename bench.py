@@ -509,9 +509,13 @@ def run_adjoint(args):
         md.loss_grad(view, dirs, M, off, dp)
         log(f"[adjoint] warmup step {i + 1}/{args.warmup} done")
     steps = args.steps
+    fwd_k, adj_k = [], []
     t = time.perf_counter()
     for _ in range(steps):
         md.loss_grad(view, dirs, M, off, dp)  # returns host values: every step ends synchronised
+        f_ms, a_ms = ctx.last_loss_grad_times()
+        fwd_k.append(f_ms)
+        adj_k.append(a_ms)
     elapsed = time.perf_counter() - t
     t = time.perf_counter()
     for _ in range(steps):
@@ -542,6 +546,20 @@ def run_adjoint(args):
     ctx.free(vis_ptr)
     ctx.free(uvw_ptr)
     ctx.close()
+    # Roofline of the adjoint kernel (HBM): it reads the cube, uvw and abar and writes the uvw
+    # gradient and the per-baseline dz partials; the forward kernel reads the cube and uvw and
+    # writes avgData.  Kernel times from HIP events on the library's stream.
+    a_ms, f_ms = float(np.median(adj_k)), float(np.median(fwd_k))
+    adj_bytes = R * C * P * 8 + R * 24 + D * nbl * 16 + R * 24 + nbl * D * 24
+    fwd_bytes = R * C * P * 8 + R * 24 + D * nbl * P * 16
+    roofline = {"kernel": "rotate_adjoint_kernel", "bound": "hbm",
+                "achieved": round(adj_bytes / (a_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(adj_bytes / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel_ms": round(a_ms, 4), "algorithmic_bytes_per_launch": adj_bytes,
+                "forward": {"kernel": "rotate_average_horner64 (16-direction tile)", "kernel_ms": round(f_ms, 4),
+                            "algorithmic_bytes_per_launch": fwd_bytes,
+                            "achieved": round(fwd_bytes / (f_ms * 1e-3) / 1e9, 1),
+                            "frac": round(fwd_bytes / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     return {
         "metric": "calibration-residual loss + gradient evaluations/s (128 st, 8128 bl, 64 t, 192 ch, 16 dirs)",
         "value": round(steps / elapsed, 3),
@@ -560,7 +578,7 @@ def run_adjoint(args):
                                "resident in HBM",
                    "stations": 128, "baselines": nbl, "timesteps": 64, "channels": C, "pols": P,
                    "directions": D, "forward_only_ms": round(fwd / steps * 1e3, 3)},
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
 

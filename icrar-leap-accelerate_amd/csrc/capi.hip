@@ -1226,6 +1226,7 @@ int leap_hip_create(leap_ctx** out, int32_t device) {
         LEAP_HIP(hipEventCreate(&ctx->ev_start));
         LEAP_HIP(hipEventCreate(&ctx->ev_stop));
         LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        for (auto& e : ctx->ev_grad) LEAP_HIP(hipEventCreate(&e));
         for (int b = 0; b < 2; ++b) {
             LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_packed[b], hipEventDisableTiming));
             LEAP_HIP(hipEventCreateWithFlags(&ctx->ev_consumed[b], hipEventDisableTiming));
@@ -1262,6 +1263,8 @@ void leap_hip_destroy(leap_ctx* ctx) {
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (auto& e : ctx->ev_grad)
+        if (e) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; ++b) {
         if (ctx->ev_packed[b]) (void)hipEventDestroy(ctx->ev_packed[b]);
         if (ctx->ev_consumed[b]) (void)hipEventDestroy(ctx->ev_consumed[b]);
@@ -1621,6 +1624,12 @@ extern "C" int leap_hip_last_route(const leap_ctx* ctx, int32_t* channel_sum, in
     if (direct_cube) *direct_cube = ctx->last_direct_cube;
     return LEAP_OK;
 }
+extern "C" int leap_hip_last_loss_grad_times(const leap_ctx* ctx, double* forward_ms, double* adjoint_ms) {
+    if (!ctx) return LEAP_ERR_INVALID_ARGUMENT;
+    if (forward_ms) *forward_ms = ctx->last_grad_forward_ms;
+    if (adjoint_ms) *adjoint_ms = ctx->last_grad_adjoint_ms;
+    return LEAP_OK;
+}
 extern "C" int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx) { return ctx ? ctx->last_workgroups_per_cu : 0; }
 extern "C" int leap_hip_last_graph(const leap_ctx* ctx) { return ctx ? ctx->last_graph : 0; }
 extern "C" uint64_t leap_hip_last_staged_bytes(const leap_ctx* ctx) { return ctx ? ctx->last_stage_bytes : 0; }
@@ -1692,9 +1701,12 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         const size_t per_dir = size_t(nbl) * md->pols * 2;
         ctx->d_avg.reserve(per_dir * D * sizeof(double));
         const int32_t tile = D <= 16 ? 162 : D <= 32 ? 321 : D <= 64 ? 641 : D <= 128 ? 1282 : 2562;
+        LEAP_HIP(hipEventRecord(ctx->ev_grad[0], s));
         launch_rotate_batched(s, LEAP_PRECISION_FP64, tile, false, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(),
                               D, T, nbl, nbl, md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
                               ctx->d_avg.as<double>(), 0);
+        LEAP_HIP(hipEventRecord(ctx->ev_grad[1], s));
+        ctx->last_grad_adjoint_ms = 0.0;
         // the inversion (CpuLeapCalibrator.cc:159-178) by the calibration's own kernels
         leap_options inv{};
         invert_directions(ctx, *md, D, inv);
@@ -1747,6 +1759,9 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         double* h_out = ctx->h_adj_out.as<double>();
         LEAP_HIP(hipMemcpyAsync(h_out, ia.loss, size_t(D) * sizeof(double), hipMemcpyDeviceToHost, s));
         LEAP_HIP(hipStreamSynchronize(s));
+        float fwd_ms = 0.f;
+        LEAP_HIP(hipEventElapsedTime(&fwd_ms, ctx->ev_grad[0], ctx->ev_grad[1]));
+        ctx->last_grad_forward_ms = fwd_ms;
         double loss = 0.0;
         for (uint32_t d = 0; d < D; ++d) loss += h_out[d];  // directions in order
         *loss_out = loss;
@@ -1757,9 +1772,11 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         d_guvw.reserve(size_t(R) * 3 * sizeof(double));
         d_gdzp.reserve(size_t(nbl) * D * 3 * sizeof(double));
         d_gdz.reserve(size_t(D) * 5 * sizeof(double));  // g_dz (3 D), then dL/d(ra, dec) (2 D)
+        LEAP_HIP(hipEventRecord(ctx->ev_grad[2], s));
         launch_rotate_adjoint(s, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(), d_abar.as<double>(), D, T, nbl,
                               md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz, d_guvw.as<double>(),
                               d_gdzp.as<double>());
+        LEAP_HIP(hipEventRecord(ctx->ev_grad[3], s));
         // dL/d(dz row of dd) reduced over baselines and mapped to dL/d(ra, dec) (MetaData.cc:166-203
         // differentiated) on the device
         launch_reduce_gdz_offsets(s, d_gdzp.as<double>(), nbl, D, md->phase_centre_ra, md->phase_centre_dec,
@@ -1800,6 +1817,9 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
             LEAP_HIP(hipStreamSynchronize(s));
         }
         LEAP_HIP(hipStreamSynchronize(s));
+        float adj_ms = 0.f;
+        LEAP_HIP(hipEventElapsedTime(&adj_ms, ctx->ev_grad[2], ctx->ev_grad[3]));
+        ctx->last_grad_adjoint_ms = adj_ms;
         if (grad_offsets) std::copy(h_out + D, h_out + 3 * size_t(D), grad_offsets);
     });
 }

@@ -257,6 +257,8 @@ struct leap_ctx {
     // per-direction losses and offset gradients copied back (pinned)
     leap::DeviceBuffer d_adj_avg0, d_adj_inv;
     leap::PinnedBuffer h_adj_out;
+    hipEvent_t ev_grad[4] = {nullptr, nullptr, nullptr, nullptr};  // around the forward and adjoint kernels
+    double last_grad_forward_ms = 0.0, last_grad_adjoint_ms = 0.0;
     std::vector<double> h_avg, h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;

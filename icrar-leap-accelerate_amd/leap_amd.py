@@ -176,7 +176,7 @@ EXPORTS = [
     "leap_hip_malloc", "leap_hip_free", "leap_hip_memcpy_d2h", "leap_hip_memcpy_h2d",
     "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_route",
     "leap_hip_last_workgroups_per_cu",
-    "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes",
+    "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes", "leap_hip_last_loss_grad_times",
     "leap_hip_tuning_save", "leap_hip_tuning_load", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
     "leap_multi_item_cost", "leap_phasor64", "leap_nufft_plan", "leap_nufft_plan32",
@@ -259,6 +259,7 @@ def lib():
     L.leap_hip_last_direction_block.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_route.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.leap_hip_last_workgroups_per_cu.argtypes = [ctypes.c_void_p]
+    L.leap_hip_last_loss_grad_times.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.leap_hip_last_graph.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_pack_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int32)]
     L.leap_hip_last_staged_bytes.argtypes = [ctypes.c_void_p]
@@ -634,6 +635,12 @@ class Context:
 
     def last_workgroups_per_cu(self):
         return int(lib().leap_hip_last_workgroups_per_cu(self.h))
+
+    def last_loss_grad_times(self):
+        """(forward kernel ms, adjoint kernel ms) of the last loss_grad call."""
+        f, a = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().leap_hip_last_loss_grad_times(self.h, ctypes.byref(f), ctypes.byref(a)))
+        return f.value, a.value
 
     def last_graph(self):
         return int(lib().leap_hip_last_graph(self.h))

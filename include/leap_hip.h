@@ -434,6 +434,10 @@ int leap_hip_last_route(const leap_ctx* ctx, int32_t* channel_sum, int32_t* dire
 /* Workgroups per CU of the last multi-set launch (-1: one work item per
  * workgroup; 0 if none). */
 int leap_hip_last_workgroups_per_cu(const leap_ctx* ctx);
+/* Device times (HIP events on the context's stream) of the last
+ * leap_hip_residual_loss_grad's forward rotate/average kernel and adjoint
+ * kernel (0 when no gradient was asked for). */
+int leap_hip_last_loss_grad_times(const leap_ctx* ctx, double* forward_ms, double* adjoint_ms);
 /* Graph use of the last calibrate call: 0 none, 1 captured a new graph,
  * 2 replayed the context's graph. */
 int leap_hip_last_graph(const leap_ctx* ctx);

@@ -116,11 +116,13 @@ def test_full_size_batched_fp64_graph(leap, ctx):
     """Batching in FP64: 64 directions (offsets up to 0.1 rad) on the complex128
     cube in one pass, captured as a HIP graph and replayed; every replay
     bit-equal to the eager call, and within 5e-9 rad of the plain per-direction
-    path.  The batched route sums the channels by FP64 Horner, whose error grows
-    as C eps (about 1e-13 of sum |v| at 384 channels); a baseline whose sum
-    nearly cancels turns that into a phase error the inversion spreads over the
-    calibration (2.8e-9 rad measured here, below 1e-9 with offsets up to
-    0.02 rad in test_full_size_fp64)."""
+    path.  The batched route takes the phase of channel c as sf (kappa0 + c dkappa)
+    (a Horner recurrence in exp(i sf dkappa)), the plain path as sf / lambda_c with
+    the rounded wavelength, as the reference does (CpuLeapCalibrator.cc:195): for
+    phases of 1e3 to 1e4 rad the two FP64 roundings differ by about 1e-12 rad per
+    term, growing with the offset, and a baseline whose sum nearly cancels turns
+    that into a phase error the inversion spreads over the calibration (2.8e-9 rad
+    measured here, below 1e-9 with offsets up to 0.02 rad in test_full_size_fp64)."""
     s = leap.SynthMS(**bench.c1_config())
     vis_p, uvw_p, view = device_cube(leap, ctx, s, leap.LEAP_VIS_C128, 16)
     try:
