@@ -121,20 +121,42 @@ __device__ inline void idft4_half(C2<F>& a0, C2<F>& a1, C2<F>& a2, C2<F>& a3) {
     a1 = zadd(x0, t1);
 }
 
-template <bool HALF, typename F>
-__device__ inline void idft4_any(C2<F>& a0, C2<F>& a1, C2<F>& a2, C2<F>& a3) {
-    if constexpr (HALF)
-        idft4_half(a0, a1, a2, a3);
-    else
+// idft4 of (a0, .., a_{NZ-1}, 0, ..): NZ inputs that are not known to be zero.
+template <int NZ, typename F>
+__device__ inline void idft4_nz(C2<F>& a0, C2<F>& a1, C2<F>& a2, C2<F>& a3) {
+    static_assert(NZ >= 1 && NZ <= 4, "1 to 4 inputs");
+    if constexpr (NZ == 4) {
         idft4(a0, a1, a2, a3);
+    } else if constexpr (NZ == 3) {
+        const C2<F> t0 = zadd(a0, a2), t1 = zsub(a0, a2), t3 = zmuli(a1);
+        a0 = zadd(t0, a1);
+        a2 = zsub(t0, a1);
+        a1 = zadd(t1, t3);
+        a3 = zsub(t1, t3);
+    } else if constexpr (NZ == 2) {
+        idft4_half(a0, a1, a2, a3);
+    } else {
+        a1 = a0;
+        a2 = a0;
+        a3 = a0;
+    }
 }
 
-template <bool HALF, typename F>
+// Number of the inputs x[first + stride k], k < 4, below NZ (at least 1).
+constexpr int sub_nz(int NZ, int first, int stride) {
+    int n = 0;
+    for (int k = 0; k < 4; ++k) n += first + stride * k < NZ ? 1 : 0;
+    return n < 1 ? 1 : n;
+}
+
+// NZ: the inputs from x[NZ] on are zero (the first pass's padding).
+template <int NZ, typename F>
 __device__ inline void idft8(C2<F>* x) {
+    static_assert(NZ >= 2 && NZ <= 8, "2 to 8 inputs");
     // even / odd halves, then X_k = E_k + w8^k O_k, X_{k+4} = E_k - w8^k O_k
     C2<F> e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
-    idft4_any<HALF>(e[0], e[1], e[2], e[3]);
-    idft4_any<HALF>(o[0], o[1], o[2], o[3]);
+    idft4_nz<sub_nz(NZ, 0, 2)>(e[0], e[1], e[2], e[3]);
+    idft4_nz<sub_nz(NZ, 1, 2)>(o[0], o[1], o[2], o[3]);
     o[1] = cmul(o[1], C2<F>{F(kCos4), F(kCos4)});
     o[2] = zmuli(o[2]);
     o[3] = cmul(o[3], C2<F>{F(-kCos4), F(kCos4)});
@@ -145,8 +167,9 @@ __device__ inline void idft8(C2<F>* x) {
     }
 }
 
-template <bool HALF, typename F>
+template <int NZ, typename F>
 __device__ inline void idft16(C2<F>* x) {
+    static_assert(NZ >= 4 && NZ <= 16, "4 to 16 inputs");
     // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1, twiddle w16^(n2 k1), DFT4 over n2
     C2<F> y[4][4];
 #pragma unroll
@@ -155,8 +178,11 @@ __device__ inline void idft16(C2<F>* x) {
         y[n2][1] = x[4 + n2];
         y[n2][2] = x[8 + n2];
         y[n2][3] = x[12 + n2];
-        idft4_any<HALF>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
     }
+    idft4_nz<sub_nz(NZ, 0, 4)>(y[0][0], y[0][1], y[0][2], y[0][3]);
+    idft4_nz<sub_nz(NZ, 1, 4)>(y[1][0], y[1][1], y[1][2], y[1][3]);
+    idft4_nz<sub_nz(NZ, 2, 4)>(y[2][0], y[2][1], y[2][2], y[2][3]);
+    idft4_nz<sub_nz(NZ, 3, 4)>(y[3][0], y[3][1], y[3][2], y[3][3]);
     const C2<F> w1{F(kCos8), F(kSin8)}, w2{F(kCos4), F(kCos4)}, w3{F(kSin8), F(kCos8)}, w6{F(-kCos4), F(kCos4)},
         w9{F(-kCos8), F(-kSin8)};
     y[1][1] = cmul(y[1][1], w1);
@@ -179,19 +205,17 @@ __device__ inline void idft16(C2<F>* x) {
     }
 }
 
-// HALF: the upper half of the inputs is zero (the first pass's padding).
-template <int R, bool HALF, typename F>
+// NZ: the inputs from x[NZ] on are zero (the first pass's padding); NZ = R for
+// the other passes.
+template <int R, int NZ, typename F>
 __device__ inline void idft(C2<F>* x) {
     static_assert(R == 4 || R == 8 || R == 16, "radices 4, 8, 16");
     if constexpr (R == 4) {
-        if constexpr (HALF)
-            idft4_half(x[0], x[1], x[2], x[3]);
-        else
-            idft4(x[0], x[1], x[2], x[3]);
+        idft4_nz<NZ>(x[0], x[1], x[2], x[3]);
     } else if constexpr (R == 8) {
-        idft8<HALF>(x);
+        idft8<NZ>(x);
     } else {
-        idft16<HALF>(x);
+        idft16<NZ>(x);
     }
 }
 
@@ -209,10 +233,11 @@ __device__ inline uint32_t padded(uint32_t e) { return e + (e >> 4); }
 // and span NS has B = N / (64 R) butterflies per lane: butterfly j = lane + 64 q
 // takes elements j + r N / R (slots q + B r), multiplies element r by
 // w^r, w = exp(2 pi i (j mod NS) / (NS R)), transforms, and its output r goes
-// to (j / NS) NS R + j mod NS + r NS.  HALF: the upper half of the inputs is
-// zero (first pass).  PHI: every output of the lane is also multiplied by phi
-// (folded into the input twiddles: element r by phi w^r).
-template <int N, int R, int NS, bool HALF, bool PHI, typename F>
+// to (j / NS) NS R + j mod NS + r NS.  NZ: inputs r >= NZ are zero (first pass:
+// the lane's channel slots end at NZ; NZ = R otherwise).  PHI: every output of
+// the lane is also multiplied by phi (folded into the input twiddles: element
+// r by phi w^r).
+template <int N, int R, int NS, int NZ, bool PHI, typename F>
 __device__ inline void pass_compute(C2<F>* v, const C2<F>* base, C2<F> phi) {
     constexpr int B = N / (kWave * R);
 #pragma unroll
@@ -248,7 +273,7 @@ __device__ inline void pass_compute(C2<F>* v, const C2<F>* base, C2<F> phi) {
                 x[R - 1] = cmul(x[R - 1], po);
             }
         }
-        idft<R, HALF>(x);
+        idft<R, NZ>(x);
 #pragma unroll
         for (int r = 0; r < R; ++r) v[q + B * r] = x[r];
     }
@@ -411,55 +436,56 @@ __device__ inline void transpose_rows_1024(C2<F>* v) {
 }
 
 // g_m = sum_k f~_k exp(2 pi i k m / N) into lds[0, N + W - 1), from the lane's
-// slots (slot i < N / 128: channel lane + 64 i; the rest zero) and the lane's
-// offset phase phi.
-template <int N, int W, typename F>
+// slots (slot i < CPL <= N / 128: channel lane + 64 i; the rest zero) and the
+// lane's offset phase phi.
+template <int N, int W, int CPL, typename F>
 __device__ inline void row_fft(C2<F>* v, C2<F>* lds, const Twiddles<N, F>& tw, C2<F> phi, uint32_t lane) {
+    static_assert(CPL >= 1 && 2 * kWave * CPL <= N, "channel slots within the lower half");
     if constexpr (N == 1024) {
-        pass_compute<N, 16, 1, true, false>(v, (const C2<F>*)nullptr, phi);
+        pass_compute<N, 16, 1, CPL, false>(v, (const C2<F>*)nullptr, phi);
         wave_fence();  // the previous row's interpolation has read the grid
         pass_store_padded<N, 16, 1>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 16, 16, false, false>(v, tw.p2, phi);
+        pass_compute<N, 16, 16, 16, false>(v, tw.p2, phi);
         transpose_rows_1024(v);
-        pass_compute<N, 4, 256, false, true>(v, tw.p3, phi);
+        pass_compute<N, 4, 256, 4, true>(v, tw.p3, phi);
         wave_fence();
         store_grid<N, W>(lds, v, lane);
         wave_fence();
     } else if constexpr (N == 512) {
-        pass_compute<N, 8, 1, true, false>(v, (const C2<F>*)nullptr, phi);
+        pass_compute<N, 8, 1, CPL, false>(v, (const C2<F>*)nullptr, phi);
         wave_fence();
         pass_store_padded<N, 8, 1>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 8, 8, false, false>(v, tw.p2, phi);
+        pass_compute<N, 8, 8, 8, false>(v, tw.p2, phi);
         wave_fence();
         pass_store_padded<N, 8, 8>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 8, 64, false, true>(v, tw.p3, phi);
+        pass_compute<N, 8, 64, 8, true>(v, tw.p3, phi);
         wave_fence();
         store_grid<N, W>(lds, v, lane);
         wave_fence();
     } else {
         static_assert(N == 256, "grid sizes 256, 512, 1024");
-        pass_compute<N, 4, 1, true, false>(v, (const C2<F>*)nullptr, phi);
+        pass_compute<N, 4, 1, CPL, false>(v, (const C2<F>*)nullptr, phi);
         wave_fence();
         pass_store_padded<N, 4, 1>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 4, false, false>(v, tw.p2, phi);
+        pass_compute<N, 4, 4, 4, false>(v, tw.p2, phi);
         wave_fence();
         pass_store_padded<N, 4, 4>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 16, false, false>(v, tw.p3, phi);
+        pass_compute<N, 4, 16, 4, false>(v, tw.p3, phi);
         wave_fence();
         pass_store_padded<N, 4, 16>(lds, v, lane);
         wave_fence();
         load_padded<N>(lds, v, lane);
-        pass_compute<N, 4, 64, false, true>(v, tw.p4, phi);
+        pass_compute<N, 4, 64, 4, true>(v, tw.p4, phi);
         wave_fence();
         store_grid<N, W>(lds, v, lane);
         wave_fence();
@@ -477,16 +503,81 @@ __device__ inline C2<F> to_transform(double re, double im) {
         return {float(re), float(im)};
 }
 
-// Row sources.  The lane owns channels c = lane + 64 i, i < N / 128 (C <= N / 2);
-// load() fetches them for one row (channels past C read channel C - 1 and are
-// zeroed by freq()), freq() returns the precorrected pol-0 values f~.
+// (dd . uvw).z - w in metres, the path difference whose -2 pi multiple is the
+// reference's shift factor (CpuLeapCalibrator.cc:190).
+__device__ inline double path_difference(const double dz[3], double u, double v, double w) {
+    double t = dz[0] * u;
+    t = fma(dz[1], v, t);
+    t = fma(dz[2], w, t);
+    return t - w;
+}
+
+// exp(2 pi i tr) for a phase tr in turns: k = rint(64 tr), r = 2 pi (tr - k / 64)
+// with |r| <= pi / 64 (tr - k / 64 is exact for |tr| < 2^46), the table entry
+// exp(2 pi i (k mod 64) / 64) times the series of exp(i r): cos to r^8 and sin
+// to r^7, whose first omitted terms are below 2.5e-20 and 4.6e-18.
+constexpr int kTurnTable = 64;
+
+// One entry per lane; a wave fence orders it before the first read.
+__device__ inline void turn_table_init(Z* table, uint32_t lane) {
+    if (lane < uint32_t(kTurnTable)) {
+        const Phasor64 p = phasor64(kTwoPi * double(lane) / double(kTurnTable));
+        table[lane] = {p.c, p.s};
+    }
+}
+
+// exp(2 pi i dl[j] scale) for the NDL directions of a lane, their steps
+// interleaved; the table reads are issued before the series that hides them.
+template <int NDL>
+__device__ inline void turn_phasor_n(const Z* table, const double* dl, double scale, Z* out) {
+    double tr[NDL], kf[NDL], r[NDL], r2[NDL], c[NDL], sn[NDL];
+    Z e[NDL];
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) {
+        tr[j] = dl[j] * scale;
+        kf[j] = rint(tr[j] * double(kTurnTable));
+    }
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) e[j] = table[int32_t(kf[j]) & (kTurnTable - 1)];
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) {
+        r[j] = fma(kf[j], -1.0 / kTurnTable, tr[j]) * kTwoPi;
+        r2[j] = r[j] * r[j];
+    }
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) {
+        c[j] = fma(1.0 / 40320.0, r2[j], -1.0 / 720.0);
+        sn[j] = fma(-1.0 / 5040.0, r2[j], 1.0 / 120.0);
+    }
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) {
+        c[j] = fma(c[j], r2[j], 1.0 / 24.0);
+        sn[j] = fma(sn[j], r2[j], -1.0 / 6.0);
+    }
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) {
+        c[j] = fma(c[j], r2[j], -0.5);
+        sn[j] = fma(r[j] * r2[j], sn[j], r[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) c[j] = fma(c[j], r2[j], 1.0);
+#pragma unroll
+    for (int j = 0; j < NDL; ++j) {
+        out[j].x = fma(e[j].x, c[j], -(e[j].y * sn[j]));
+        out[j].y = fma(e[j].x, sn[j], e[j].y * c[j]);
+    }
+}
+
+// Row sources.  The lane owns channels c = lane + 64 i, i < CPL (C <= 64 CPL
+// <= N / 2); load() fetches them for one row (channels past C read channel
+// C - 1 and are zeroed by freq()), freq() returns the precorrected pol-0
+// values f~.
 //   P == 0: packed rows of FP64 complex, already precorrected (pack kernel).
 //   P >= 1: the complex64 cube [row][channel][pol]: a channel whose 2P floats
 //           are not all finite is zero (CpuLeapCalibrator.cc:201-214), pol 0 is
 //           widened and multiplied by the channel's precorrection factor.
-template <int N, int P>
+template <int N, int P, int CPL>
 struct Source {
-    static constexpr int CPL = N / (2 * kWave);
     float raw[CPL][2 * P];
     double sc[CPL];    // precorrection, 0 past C
     uint32_t off[CPL];  // float offset of the channel (clamped) in a row
@@ -536,9 +627,8 @@ struct Source {
     }
 };
 
-template <int N>
-struct Source<N, 0> {
-    static constexpr int CPL = N / (2 * kWave);
+template <int N, int CPL>
+struct Source<N, 0, CPL> {
     Z raw[CPL];
     __device__ void init(const LEAP_GLOBAL double*, uint32_t, uint32_t) {}
     __device__ static const void* advance(const void* base, size_t rows, uint32_t C) {
@@ -562,8 +652,9 @@ struct Source<N, 0> {
 };
 
 // SP: the row source (0 packed rows, 1 / 2 / 4 a complex64 cube with that many pols).
+// CPL: channel slots per lane (C <= 64 CPL <= N / 2).
 // F: the type of the transform, the grid and the interpolation (double or float).
-template <int N, int W, int NDL, int SP, typename F>
+template <int N, int W, int NDL, int SP, int CPL, typename F>
 __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __restrict__ w,
                                                               const LEAP_GLOBAL double* __restrict__ scale,
                                                               const LEAP_GLOBAL double* __restrict__ uvw,
@@ -576,7 +667,10 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     constexpr int V = N / kWave;
     static_assert(W % 2 == 0 && W <= kNufftMaxWidth && N / 16 >= W - 1, "kernel width");
     __shared__ ZF s_g[N + N / 16];
+    __shared__ Z s_turn[kTurnTable];
     const uint32_t lane = threadIdx.x;
+    turn_table_init(s_turn, lane);
+    wave_fence();
     // The span is resolved here into the row source, uvw and output pointers and
     // a row count, so the row loop holds no more values than an unchunked one.
     const uint32_t item = blockIdx.x;
@@ -584,7 +678,7 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     const uint32_t bl = span.bl_base + (span.chunks > 1 ? item / span.chunks : item);
     const uint32_t t_begin = uint32_t(uint64_t(chunk) * T / max(span.chunks, 1u));
     const uint32_t t_count = uint32_t(uint64_t(chunk + 1) * T / max(span.chunks, 1u)) - t_begin;
-    const void* wr = Source<N, SP>::advance(w, size_t(t_begin) * nbl, C);
+    const void* wr = Source<N, SP, CPL>::advance(w, size_t(t_begin) * nbl, C);
     const LEAP_GLOBAL double* uvr = uvw + size_t(t_begin) * nbl * 3;
     LEAP_GLOBAL double* out = mode == kNufftPartial ? avg + size_t(item) * D * 2 : avg + size_t(bl) * Pout * 2;
     const uint32_t dstride = mode == kNufftPartial ? 2u : nbl * Pout * 2;  // doubles between directions
@@ -604,6 +698,7 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
 #pragma unroll
     for (int j = 0; j < NDL; ++j) sum[j] = {0.0, 0.0};
     const ZF phi = offset_phase<N, F>(lane, args.h / kWave);
+    const double neg_dkappa = -args.dkappa, neg_kappa_mid = -args.kappa_mid;
     const F* coef;
     if constexpr (std::is_same_v<F, double>)
         coef = args.coef;
@@ -613,40 +708,40 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     // and stay in flight while it is transformed and interpolated.  The load is
     // unconditional (the last row reloads itself), so the row buffer keeps its
     // registers from one iteration to the next.
-    Source<N, SP> src;
+    Source<N, SP, CPL> src;
     src.init(scale, lane, C);
     if (t_count > 0) src.load(wr, bl, lane, C);
     for (uint32_t t = 0; t < t_count; ++t) {
         ZF v[V];
         src.template freq<F>(v, lane, C);
 #pragma unroll
-        for (int m = V / 2; m < V; ++m) v[m] = ZF{F(0), F(0)};  // padding (not read by the first pass)
+        for (int m = CPL; m < V; ++m) v[m] = ZF{F(0), F(0)};  // padding (not read by the first pass)
         __builtin_amdgcn_sched_barrier(0);
         const size_t row = size_t(t) * nbl + bl;
         src.load(wr, size_t(t + 1 < t_count ? t + 1 : t) * nbl + bl, lane, C);
         const double u = uvr[3 * row], vv = uvr[3 * row + 1], ww = uvr[3 * row + 2];
         tw.hide();
-        row_fft<N, W>(v, s_g, tw, phi, lane);
-        // Interpolation.  Per direction: b = sf dkappa, grid coordinate
-        // ug = N frac(b / 2 pi), first tap m0 = ceil(ug - W/2), offset
-        // s = ug - m0 - (W - 1)/2 in (-1/2, 1/2]; tap pairs (jp, W - 1 - jp) take
-        // weights P_jp(s), P_jp(-s), P_jp(s) = E(s^2) + s O(s^2).  The pairs
-        // are the outer loop so that at most two pairs' coefficients are live.
-        // The NDL directions' steps are interleaved (independent chains side by
-        // side) with each direction's operations unchanged.  The grid
-        // coordinate is formed in FP64; s, the weights and the tap sums in F.
-        double sfv[NDL], sv[NDL], x[NDL], m0f[NDL];
+        row_fft<N, W, CPL>(v, s_g, tw, phi, lane);
+        // Interpolation.  Per direction: the path difference
+        // dl = dd_z . uvw - w (the shift factor is sf = -2 pi dl), so b / 2 pi =
+        // -dl dkappa turns; grid coordinate ug = N frac(b / 2 pi), first tap
+        // m0 = ceil(ug - W/2), offset s = ug - m0 - (W - 1)/2 in (-1/2, 1/2]; tap
+        // pairs (jp, W - 1 - jp) take weights P_jp(s), P_jp(-s),
+        // P_jp(s) = E(s^2) + s O(s^2).  The pairs are the outer loop so that at
+        // most two pairs' coefficients are live.  The NDL directions' steps are
+        // interleaved (independent chains side by side) with each direction's
+        // operations unchanged.  The grid coordinate is formed in FP64; s, the
+        // weights and the tap sums in F.
+        double dl[NDL], sv[NDL], x[NDL], m0f[NDL];
         F svf[NDL], s2[NDL];
         const ZF* gp[NDL];
 #pragma unroll
-        for (int j = 0; j < NDL; ++j) sfv[j] = shift_factor(dz[j], u, vv, ww);
+        for (int j = 0; j < NDL; ++j) dl[j] = path_difference(dz[j], u, vv, ww);
 #pragma unroll
-        for (int j = 0; j < NDL; ++j) x[j] = sfv[j] * args.dkappa * kInvTwoPi;  // turns of b
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) x[j] -= floor(x[j]);  // [0, 1]
+        for (int j = 0; j < NDL; ++j) x[j] = dl[j] * neg_dkappa;  // turns of b
 #pragma unroll
         for (int j = 0; j < NDL; ++j) {
-            x[j] = x[j] * double(N);  // grid coordinate ug in [0, N]
+            x[j] = __builtin_amdgcn_fract(x[j]) * double(N);  // grid coordinate ug in [0, N)
             m0f[j] = ceil(x[j] - 0.5 * W);
         }
 #pragma unroll
@@ -654,9 +749,10 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
             sv[j] = (x[j] - m0f[j]) - 0.5 * (W - 1);
             svf[j] = F(sv[j]);
             s2[j] = svf[j] * svf[j];
-            int32_t m0 = int32_t(m0f[j]);
-            if (m0 < 0) m0 += N;  // taps m0 .. m0 + W - 1 <= N + W - 2 (wrapped copy)
-            m0 = min(max(m0, 0), N - 1);  // a non-finite uvw stays inside the grid
+            // m0 in [-W/2, N - W/2]; a negative m0 wraps to m0 + N, whose taps
+            // end at N + W/2 - 1 <= N + W - 2 (the wrapped copy).  The mask also
+            // keeps the taps of a non-finite uvw inside the grid.
+            const int32_t m0 = int32_t(m0f[j]) & (N - 1);
             gp[j] = s_g + m0;
         }
         ZF acc[NDL];
@@ -725,16 +821,17 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-        double th[NDL];
-        Phasor64 ph[NDL];
-#pragma unroll
-        for (int j = 0; j < NDL; ++j) th[j] = sfv[j] * args.kappa_mid;
-        phasor64_n<NDL>(th, ph);
+        // exp(i sf kappa_mid) = exp(2 pi i tr), tr = -dl kappa_mid turns, from
+        // the turn table and a short series; added to the row sums with FMAs.
+        Z ph[NDL];
+        turn_phasor_n<NDL>(s_turn, dl, neg_kappa_mid, ph);
 #pragma unroll
         for (int j = 0; j < NDL; ++j) {
-            const Z r = cmul(Z{ph[j].c, ph[j].s}, Z{double(acc[j].x), double(acc[j].y)});
-            sum[j].x += r.x;
-            sum[j].y += r.y;
+            const double ax = double(acc[j].x), ay = double(acc[j].y);
+            sum[j].x = fma(ph[j].x, ax, sum[j].x);
+            sum[j].x = fma(-ph[j].y, ay, sum[j].x);
+            sum[j].y = fma(ph[j].x, ay, sum[j].y);
+            sum[j].y = fma(ph[j].y, ax, sum[j].y);
         }
     }
     // The output indices are recomputed here from an opaque copy of the lane,
@@ -789,18 +886,18 @@ uint32_t resident_workgroups(K kernel) {
     return uint32_t(std::max(1, per_cu) * std::max(1, cus));
 }
 
-// One launch of rotate_average_nufft<N, W, NDL, SP, F> over nbl_grid baselines.
+// One launch of rotate_average_nufft<N, W, NDL, SP, CPL, F> over nbl_grid baselines.
 // When a single direction block covers D and the baselines do not fill whole
 // rounds of resident workgroups, the whole rounds run as usual and the
 // remaining baselines' rows are split into chunks so that the last round is
 // full (and short): their partial sums go to the tail scratch and are added by
 // nufft_tail_sum.  Not for accumulating launches.
-template <int N, int W, int NDL, int SP, typename F>
+template <int N, int W, int NDL, int SP, int CPL, typename F>
 void launch_nufft_tile(hipStream_t s, uint32_t nbl_grid, const void* w, const LEAP_GLOBAL double* gs,
                        const LEAP_GLOBAL double* gu, const LEAP_GLOBAL double* gd, uint32_t D, uint32_t T, uint32_t nbl,
                        uint32_t C, uint32_t Pout, const NufftArgs& a, LEAP_GLOBAL double* ga, int32_t accumulate,
                        double* tail, size_t tail_doubles) {
-    auto* kernel = rotate_average_nufft<N, W, NDL, SP, F>;
+    auto* kernel = rotate_average_nufft<N, W, NDL, SP, CPL, F>;
     const uint32_t ny = (D + kWave * NDL - 1) / (kWave * NDL);
     if (tail && !accumulate && ny == 1 && T >= 2) {
         static const uint32_t slots = resident_workgroups(kernel);
@@ -828,8 +925,8 @@ void launch_nufft_tile(hipStream_t s, uint32_t nbl_grid, const void* w, const LE
                                                accumulate ? kNufftAccumulate : kNufftSet, NufftSpan{0, 1});
 }
 
-template <int N, int W, int SP, typename F>
-void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
+template <int N, int W, int SP, int CPL, typename F>
+void launch_nwsc(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
                 const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout, const NufftArgs& a,
                 double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
     const auto* gs = (const LEAP_GLOBAL double*)scale;
@@ -838,19 +935,38 @@ void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, c
     auto* ga = (LEAP_GLOBAL double*)avg;
     switch (tile) {
         case 641:
-            launch_nufft_tile<N, W, 1, SP, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
+            launch_nufft_tile<N, W, 1, SP, CPL, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
                                               tail_doubles);
             break;
         case 1282:
-            launch_nufft_tile<N, W, 2, SP, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
+            launch_nufft_tile<N, W, 2, SP, CPL, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
                                               tail_doubles);
             break;
         case 2564:
-            launch_nufft_tile<N, W, 4, SP, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
+            launch_nufft_tile<N, W, 4, SP, CPL, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
                                               tail_doubles);
             break;
         default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported NUFFT tile " + std::to_string(tile));
     }
+}
+
+// Channel slots per lane: N / 128 in general; for the 1024-point grid with
+// kernel width 10 and an FP64 transform (257..384 channels: 5 or 6 slots), a
+// kernel with 6 slots, which skips the loads, checks and first-pass butterflies
+// of the two slots that are padding.
+template <int N, int W, int SP, typename F>
+void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
+                const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout, const NufftArgs& a,
+                double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
+    if constexpr (N == 1024 && W == 10 && std::is_same_v<F, double>) {
+        if (C <= 6 * kWave) {
+            launch_nwsc<N, W, SP, 6, F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate,
+                                        tail, tail_doubles);
+            return;
+        }
+    }
+    launch_nwsc<N, W, SP, N / (2 * kWave), F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg,
+                                              accumulate, tail, tail_doubles);
 }
 
 template <int N, int W, typename F>
