@@ -888,6 +888,14 @@ struct MultiTables {
     size_t n_items = 0;
 };
 
+// Appends n bytes at the next 256-byte boundary of the block; returns their offset.
+size_t put_bytes(std::vector<uint8_t>& bytes, const void* src, size_t n) {
+    const size_t at = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(at + std::max<size_t>(n, 1), 0);
+    if (n) std::memcpy(bytes.data() + at, src, n);
+    return at;
+}
+
 // Plans one launch (host LPT schedule) and appends its tables to bytes.
 MultiTables append_tables(leap_ctx* ctx, std::vector<uint8_t>& bytes, const std::vector<MultiSetDesc>& descs,
                           const std::vector<MultiShape>& shapes, int32_t tile, int32_t per_cu) {
@@ -896,17 +904,108 @@ MultiTables append_tables(leap_ctx* ctx, std::vector<uint8_t>& bytes, const std:
     for (const MultiShape& m : shapes) t.n_items += size_t(m.nbl) * ((m.D + db - 1) / db);
     t.blocks = multi_grid(ctx, tile, per_cu, t.n_items);
     const MultiPlan plan = plan_multi(shapes, tile, t.blocks);
-    auto put = [&](const void* src, size_t n) {
-        const size_t at = (bytes.size() + 255) & ~size_t(255);
-        bytes.resize(at + std::max<size_t>(n, 1), 0);
-        if (n) std::memcpy(bytes.data() + at, src, n);
-        return at;
-    };
-    t.descs = put(descs.data(), descs.size() * sizeof(MultiSetDesc));
-    t.items = put(plan.items.data(), plan.items.size() * sizeof(MultiItem));
-    t.offsets = put(plan.off.data(), plan.off.size() * sizeof(uint32_t));
+    t.descs = put_bytes(bytes, descs.data(), descs.size() * sizeof(MultiSetDesc));
+    t.items = put_bytes(bytes, plan.items.data(), plan.items.size() * sizeof(MultiItem));
+    t.offsets = put_bytes(bytes, plan.off.data(), plan.off.size() * sizeof(uint32_t));
     return t;
 }
+
+// Sets of a multi-set call that take the NUFFT channel sums and share one
+// launch (equal nufft_sets_key): their descriptors and the work items
+// (set | direction block << 16, baseline), in decreasing rows per item (the
+// dispatcher then starts the longest items first), set and block order within.
+struct NufftGroup {
+    const NufftPlan* plan = nullptr;
+    uint64_t key = 0;
+    uint32_t C = 0, P = 0;
+    int32_t tile = 0;
+    std::vector<NufftSetDesc> sets;
+    std::vector<uint32_t> items;
+};
+
+// Whether a set of a multi-set call takes the NUFFT channel sums (FP64
+// transform): mixed precision, a complex64 cube with 1, 2 or 4 pols, and the
+// single-set automatic rule (96..512 channels, 16 or more directions) unless
+// the NUFFT is asked for.
+bool set_takes_nufft(const leap_options& opts, const leap_metadata& md, const leap_ms_view& ms, uint32_t D) {
+    if (opts.precision != LEAP_PRECISION_MIXED || D == 0) return false;
+    const bool cube_ok = nufft_cube_supported(ms.vis_dtype, ms.num_pols) && md.nufft64.N != 0;
+    if (opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+        require(cube_ok, LEAP_ERR_INVALID_ARGUMENT,
+                "the multi-set NUFFT route needs 1.." + std::to_string(kNufftMaxChannels) +
+                    " channels and a complex64 cube with 1, 2 or 4 pols");
+        return true;
+    }
+    return opts.channel_sum == LEAP_CHANNEL_SUM_AUTO && cube_ok && md.channels >= kNufftMinChannels &&
+           D >= kNufftMinDirections;
+}
+
+// NUFFT tile of a multi-set launch: the caller's when it is a NUFFT tile,
+// otherwise 64, 128 or 256 directions per workgroup, the fewest that hold the
+// largest set's directions.
+int32_t nufft_group_tile(const leap_options& opts, uint32_t max_d) {
+    if (opts.direction_block != 0) {
+        const int32_t t = opts.direction_block * 10 + (opts.lane_directions ? opts.lane_directions : 1);
+        if (t == 641 || t == 1282 || t == 2564) return t;
+    }
+    return max_d <= 64 ? 641 : max_d <= 128 ? 1282 : 2564;
+}
+
+void plan_nufft_group(NufftGroup& g, const leap_options& opts) {
+    uint32_t max_d = 0;
+    for (const NufftSetDesc& d : g.sets) max_d = std::max(max_d, d.D);
+    g.tile = nufft_group_tile(opts, max_d);
+    const uint32_t per_block = 64u * uint32_t(g.tile % 10);
+    require(g.sets.size() <= 0xffff, LEAP_ERR_RANGE, "too many sets for one NUFFT launch");
+    std::vector<uint32_t> order(g.sets.size());
+    for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return g.sets[a].T > g.sets[b].T; });
+    g.items.clear();
+    for (uint32_t s : order) {
+        const NufftSetDesc& d = g.sets[s];
+        const uint32_t blocks = (d.D + per_block - 1) / per_block;
+        require(blocks <= 0xffff, LEAP_ERR_RANGE, "too many direction blocks for one NUFFT launch");
+        for (uint32_t b = 0; b < blocks; ++b)
+            for (uint32_t bl = 0; bl < d.nbl; ++bl) {
+                g.items.push_back(s | (b << 16));
+                g.items.push_back(bl);
+            }
+    }
+    require(g.items.size() / 2 < (size_t(1) << 31), LEAP_ERR_RANGE, "too many work items for one launch");
+}
+
+// A round of a multi-set call: the Horner launch's tables and each NUFFT
+// group's descriptors and items, at byte offsets of one uploaded block.
+struct RoundTables {
+    MultiTables horner;
+    struct Nufft {
+        size_t descs = 0, items = 0;
+        uint32_t n_items = 0, C = 0, P = 0;
+        int32_t tile = 0;
+        const NufftPlan* plan = nullptr;
+    };
+    std::vector<Nufft> nufft;
+};
+
+RoundTables append_round(leap_ctx* ctx, std::vector<uint8_t>& bytes, const std::vector<MultiSetDesc>& descs,
+                         const std::vector<MultiShape>& shapes, int32_t tile, int32_t per_cu,
+                         const std::vector<NufftGroup>& groups) {
+    RoundTables t;
+    if (!descs.empty()) t.horner = append_tables(ctx, bytes, descs, shapes, tile, per_cu);
+    for (const NufftGroup& g : groups) {
+        RoundTables::Nufft n;
+        n.descs = put_bytes(bytes, g.sets.data(), g.sets.size() * sizeof(NufftSetDesc));
+        n.items = put_bytes(bytes, g.items.data(), g.items.size() * sizeof(uint32_t));
+        n.n_items = uint32_t(g.items.size() / 2);
+        n.C = g.C;
+        n.P = g.P;
+        n.tile = g.tile;
+        n.plan = g.plan;
+        t.nufft.push_back(n);
+    }
+    return t;
+}
+
 
 void launch_tables(leap_ctx* ctx, const void* block, const MultiTables& t, int32_t tile) {
     if (t.n_items == 0) return;
@@ -914,6 +1013,15 @@ void launch_tables(leap_ctx* ctx, const void* block, const MultiTables& t, int32
     launch_rotate_multi(ctx->stream, tile, reinterpret_cast<const MultiSetDesc*>(b + t.descs),
                         reinterpret_cast<const uint2*>(b + t.items), reinterpret_cast<const uint32_t*>(b + t.offsets),
                         t.blocks);
+}
+
+void launch_round(leap_ctx* ctx, const void* block, const RoundTables& t, int32_t tile) {
+    const char* b = static_cast<const char*>(block);
+    launch_tables(ctx, block, t.horner, tile);
+    for (const RoundTables::Nufft& n : t.nufft)
+        launch_rotate_nufft_sets(ctx->stream, n.tile, *n.plan, n.C, n.P,
+                                 reinterpret_cast<const NufftSetDesc*>(b + n.descs),
+                                 reinterpret_cast<const uint32_t*>(b + n.items), n.n_items);
 }
 
 // Plans, uploads and launches one multi-set rotate/average.  The host copy of
@@ -931,6 +1039,21 @@ void run_multi(leap_ctx* ctx, const std::vector<MultiSetDesc>& descs, const std:
     if (timed) record_rotate_begin(ctx);
     launch_tables(ctx, ctx->d_tables.ptr, t, tile);
     if (timed) record_rotate_end(ctx);
+}
+
+// Plans, uploads and launches one round of a multi-set call (the Horner
+// tables and the NUFFT groups in one block), timed as one rotate.
+void run_round(leap_ctx* ctx, const std::vector<MultiSetDesc>& descs, const std::vector<MultiShape>& shapes,
+               int32_t tile, int32_t per_cu, const std::vector<NufftGroup>& groups) {
+    LEAP_HIP(hipStreamSynchronize(ctx->stream));  // the previous upload has left h_tables
+    ctx->h_tables.clear();
+    const RoundTables t = append_round(ctx, ctx->h_tables, descs, shapes, tile, per_cu, groups);
+    ctx->d_tables.reserve(ctx->h_tables.size());
+    LEAP_HIP(hipMemcpyAsync(ctx->d_tables.ptr, ctx->h_tables.data(), ctx->h_tables.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
+    record_rotate_begin(ctx);
+    launch_round(ctx, ctx->d_tables.ptr, t, tile);
+    record_rotate_end(ctx);
 }
 
 // Tile and workgroups per CU of the multi-set kernel: the caller's, or the
@@ -1014,6 +1137,7 @@ struct BatchState {
     int32_t solutions;
     uint32_t interval;
     std::vector<double> h_cal;
+    bool nufft = false;  // NUFFT channel sums (avgData holds pol 0 alone)
 };
 
 void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets, const leap_options& opts,
@@ -1022,7 +1146,9 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
     require(opts.strategy == LEAP_STRATEGY_BATCHED, LEAP_ERR_INVALID_ARGUMENT,
             "several measurement sets in one launch need the batched strategy");
     require(!opts.all_pols && !opts.fp32_blocks, LEAP_ERR_NOT_IMPLEMENTED,
-            "the multi-set launch accumulates polarisation 0 with the FP64 Horner kernel only");
+            "the multi-set launch accumulates polarisation 0 with the FP64 Horner or NUFFT channel sums only");
+    require(opts.channel_sum != LEAP_CHANNEL_SUM_NUFFT_FP32, LEAP_ERR_NOT_IMPLEMENTED,
+            "the multi-set launch takes the NUFFT with its FP64 transform (channel_sum 2)");
     std::vector<BatchState> st(n_sets);
     int32_t rounds = 0;
     for (uint32_t i = 0; i < n_sets; ++i) {
@@ -1038,10 +1164,15 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
         int32_t range[3];
         evaluate_slice(in.slice_start, in.slice_interval, in.slice_end, T, range);
         st[i] = BatchState{in.metadata, in.ms, in.directions, in.n_directions,
-                           (range[2] - range[0]) / range[1], uint32_t(range[1]), {}};
+                           (range[2] - range[0]) / range[1], uint32_t(range[1]), {},
+                           set_takes_nufft(opts, *in.metadata, *in.ms, in.n_directions)};
         rounds = std::max(rounds, st[i].solutions);
     }
     while (ctx->batch_sets.size() < n_sets) ctx->batch_sets.push_back(std::make_unique<SetBuffers>());
+    ctx->last_channel_sum = std::any_of(st.begin(), st.end(), [](const BatchState& b) { return b.nufft; })
+                                ? LEAP_CHANNEL_SUM_NUFFT
+                                : LEAP_CHANNEL_SUM_HORNER;
+    ctx->last_direct_cube = 1;
     begin_timing(ctx);
     for (uint32_t i = 0; i < n_sets; ++i) {
         BatchState& b = st[i];
@@ -1058,9 +1189,10 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
         launch_dd(ctx->stream, b.md->phase_centre_ra, b.md->phase_centre_dec, buf.d_dirs.as<double>(), b.D,
                   buf.d_dd.as<double>());
     }
-    // Descriptors of round r: solution interval r of every set that has one.
+    // Descriptors of round r: solution interval r of every set that has one,
+    // Horner sets into descs / shapes, NUFFT sets into groups of one launch.
     auto round_descs = [&](int32_t r, std::vector<MultiSetDesc>& descs, std::vector<MultiShape>& shapes,
-                           std::vector<uint32_t>& active) {
+                           std::vector<NufftGroup>& groups, std::vector<uint32_t>& active) {
         for (uint32_t i = 0; i < n_sets; ++i) {
             BatchState& b = st[i];
             if (r >= b.solutions || b.D == 0) continue;
@@ -1069,14 +1201,38 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             // CpuLeapCalibrator.cc:113-119: rows [r * interval * nbl, +interval * nbl).
             const DeviceRows rows = stage_rows_into(ctx, *b.ms, uint32_t(size_t(r) * b.interval * nbl),
                                                     uint32_t(size_t(r + 1) * b.interval * nbl), buf.d_vis, buf.d_uvw);
+            active.push_back(i);
+            if (b.nufft) {
+                const NufftPlan& plan = b.md->nufft64;
+                const uint32_t C = b.md->channels;
+                const uint64_t key = nufft_sets_key(plan, C, b.md->pols);
+                auto g = std::find_if(groups.begin(), groups.end(), [&](const NufftGroup& x) { return x.key == key; });
+                if (g == groups.end()) {
+                    groups.emplace_back();
+                    g = groups.end() - 1;
+                    g->plan = &plan;
+                    g->key = key;
+                    g->C = C;
+                    g->P = b.md->pols;
+                }
+                const double* tables = b.md->nufft_tables(LEAP_CHANNEL_SUM_NUFFT);
+                const double df = b.md->freq_inc_hz;
+                g->sets.push_back(NufftSetDesc{rows.vis, rows.uvw, buf.d_dd.as<double>(), buf.d_avg.as<double>(),
+                                               tables + kNufftCoefDoubles, tables,
+                                               reinterpret_cast<const float*>(tables + nufft_coef32_offset(C)),
+                                               (b.md->freq_start_hz + double(plan.h) * df) / kSpeedOfLight,
+                                               df / kSpeedOfLight, int32_t(plan.h), nbl, b.interval, C, b.D});
+                continue;
+            }
             descs.push_back(MultiSetDesc{rows.vis, rows.uvw, buf.d_dd.as<double>(), buf.d_avg.as<double>(), nbl,
                                          b.interval, b.md->channels, b.md->pols, b.D,
                                          b.ms->vis_dtype == LEAP_VIS_C64 ? 1 : 0,
                                          b.md->freq_start_hz / kSpeedOfLight, b.md->freq_inc_hz / kSpeedOfLight});
             shapes.push_back(MultiShape{nbl, b.interval, b.md->channels, b.D});
-            active.push_back(i);
         }
+        for (NufftGroup& g : groups) plan_nufft_group(g, opts);
     };
+    auto avg_pols = [&](uint32_t i) { return st[i].nufft ? 1u : st[i].md->pols; };
     auto deliver = [&](int32_t r, auto&& cal_of) {
         for (uint32_t i = 0; i < n_sets; ++i) {
             const BatchState& b = st[i];
@@ -1099,6 +1255,7 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
         const size_t n_rounds = static_cast<size_t>(rounds);
         std::vector<std::vector<MultiSetDesc>> rdescs(n_rounds);
         std::vector<std::vector<MultiShape>> rshapes(n_rounds);
+        std::vector<std::vector<NufftGroup>> rgroups(n_rounds);
         std::vector<std::vector<uint32_t>> ractive(n_rounds);
         std::vector<size_t> cal_at(n_sets, 0);
         size_t cal_total = 0;
@@ -1112,7 +1269,7 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             reserve_solution(ctx, *b.md, b.D, b.interval, ro);
             cal_total += size_t(b.md->A.cols) * b.D * size_t(b.solutions);
         }
-        for (int32_t r = 0; r < rounds; ++r) round_descs(r, rdescs[r], rshapes[r], ractive[r]);
+        for (int32_t r = 0; r < rounds; ++r) round_descs(r, rdescs[r], rshapes[r], rgroups[r], ractive[r]);
         for (int32_t r = 0; r < rounds && launch.first == 0; ++r)
             if (!rdescs[r].empty()) launch = multi_tile(ctx, rdescs[r], rshapes[r], opts);
         ctx->last_direction_block = launch.first;
@@ -1130,10 +1287,9 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             seq = ctx->seq_batch.get();
             seq->key = key;
             std::vector<uint8_t> bytes;
-            std::vector<MultiTables> tables(n_rounds);
+            std::vector<RoundTables> tables(n_rounds);
             for (int32_t r = 0; r < rounds; ++r)
-                if (!rdescs[r].empty())
-                    tables[r] = append_tables(ctx, bytes, rdescs[r], rshapes[r], launch.first, launch.second);
+                tables[r] = append_round(ctx, bytes, rdescs[r], rshapes[r], launch.first, launch.second, rgroups[r]);
             seq->d_tables.reserve(bytes.size());
             LEAP_HIP(hipMemcpyAsync(seq->d_tables.ptr, bytes.data(), bytes.size(), hipMemcpyHostToDevice,
                                     ctx->stream));
@@ -1141,12 +1297,12 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             seq->d_cal.reserve(cal_total * sizeof(double));
             capture(ctx, *seq, [&] {
                 for (int32_t r = 0; r < rounds; ++r) {
-                    launch_tables(ctx, seq->d_tables.ptr, tables[r], launch.first);
+                    launch_round(ctx, seq->d_tables.ptr, tables[r], launch.first);
                     for (uint32_t i : ractive[r]) {
                         const BatchState& b = st[i];
                         const size_t per_sol = size_t(b.md->A.cols) * b.D;
                         invert_directions(ctx, *b.md, b.D, opts, ctx->batch_sets[i]->d_avg.as<double>(),
-                                          seq->d_cal.as<double>() + cal_at[i] + per_sol * size_t(r));
+                                          seq->d_cal.as<double>() + cal_at[i] + per_sol * size_t(r), avg_pols(i));
                     }
                 }
             });
@@ -1174,17 +1330,18 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
     for (int32_t r = 0; r < rounds; ++r) {
         std::vector<MultiSetDesc> descs;
         std::vector<MultiShape> shapes;
+        std::vector<NufftGroup> groups;
         std::vector<uint32_t> active;
-        round_descs(r, descs, shapes, active);
-        if (!descs.empty()) {
-            if (launch.first == 0) launch = multi_tile(ctx, descs, shapes, opts);
+        round_descs(r, descs, shapes, groups, active);
+        if (!active.empty()) {
+            if (launch.first == 0 && !descs.empty()) launch = multi_tile(ctx, descs, shapes, opts);
             ctx->last_direction_block = launch.first;
             ctx->last_workgroups_per_cu = launch.second;
-            run_multi(ctx, descs, shapes, launch.first, launch.second, nullptr, true);
+            run_round(ctx, descs, shapes, launch.first, launch.second, groups);
             for (uint32_t i : active) {
                 BatchState& b = st[i];
                 SetBuffers& buf = *ctx->batch_sets[i];
-                invert_directions(ctx, *b.md, b.D, opts, buf.d_avg.as<double>(), buf.d_cal.as<double>());
+                invert_directions(ctx, *b.md, b.D, opts, buf.d_avg.as<double>(), buf.d_cal.as<double>(), avg_pols(i));
                 LEAP_HIP(hipMemcpyAsync(b.h_cal.data(), buf.d_cal.ptr, b.h_cal.size() * sizeof(double),
                                         hipMemcpyDeviceToHost, ctx->stream));
             }

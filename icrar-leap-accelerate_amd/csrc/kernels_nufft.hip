@@ -24,6 +24,35 @@ void launch_rotate_nufft_f64(hipStream_t s, const nufft::NufftLaunch& l) {
                                        std::to_string(plan.W));
 }
 
+uint64_t nufft_sets_key(const NufftPlan& plan, uint32_t C, uint32_t P) {
+    uint32_t slots = plan.N / 128;
+    if (!plan.fp32 && plan.N == 1024 && plan.W == 10) slots = uint32_t(nufft::nufft_slots<1024, 10, double>(C));
+    return (uint64_t(plan.N) << 32) | (uint64_t(plan.W) << 24) | (uint64_t(plan.fp32 ? 1 : 0) << 16) |
+           (uint64_t(slots) << 8) | uint64_t(P);
+}
+
+void launch_rotate_nufft_sets(hipStream_t s, int32_t tile, const NufftPlan& plan, uint32_t C, uint32_t P,
+                              const NufftSetDesc* sets, const uint32_t* items, uint32_t n_items) {
+    if (n_items == 0) return;
+    require(!plan.fp32, LEAP_ERR_NOT_IMPLEMENTED, "the multi-set NUFFT launch uses the FP64 transform");
+    require(P == 1 || P == 2 || P == 4, LEAP_ERR_INTERNAL, "the multi-set NUFFT launch reads complex64 cubes");
+#define LEAP_NUFFT_SETS(N_, W_)                                                                  \
+    if (plan.N == N_ && plan.W == W_) {                                                          \
+        nufft::launch_nw_sets<N_, W_, double>(s, tile, int32_t(P), C, sets, items, n_items);     \
+        LEAP_HIP(hipGetLastError());                                                             \
+        return;                                                                                  \
+    }
+    LEAP_NUFFT_SETS(256, 10)
+    LEAP_NUFFT_SETS(256, 12)
+    LEAP_NUFFT_SETS(512, 10)
+    LEAP_NUFFT_SETS(512, 12)
+    LEAP_NUFFT_SETS(1024, 10)
+    LEAP_NUFFT_SETS(1024, 12)
+#undef LEAP_NUFFT_SETS
+    throw Error(LEAP_ERR_INTERNAL, "no FP64 NUFFT kernel for grid " + std::to_string(plan.N) + ", width " +
+                                       std::to_string(plan.W));
+}
+
 size_t nufft_tail_bytes(uint32_t D) {
     int dev = 0, cus = 0;
     LEAP_HIP(hipGetDevice(&dev));

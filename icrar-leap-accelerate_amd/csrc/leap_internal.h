@@ -328,6 +328,27 @@ void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, con
 // Scratch for launch_rotate_nufft's chunked last round (D directions, a single
 // direction block): at most twice the resident workgroups x D complex sums.
 size_t nufft_tail_bytes(uint32_t D);
+// One set of a multi-set NUFFT launch (FP64 transform, complex64 cube).
+struct NufftSetDesc {
+    const void* vis;      // [T nbl][C][P] complex64
+    const double* uvw;    // [T nbl][3]
+    const double* dd;     // [D][3][3]
+    double* avg;          // [D][nbl] complex128: pol 0 only
+    const double* scale;  // C precorrection factors
+    const double* coef;   // weight polynomials (the plan's device tables)
+    const float* coef32;
+    double kappa_mid, dkappa;  // (f0 + h df) / c_light, df / c_light
+    int32_t h;
+    uint32_t nbl, T, C, D;
+};
+// Sets whose keys are equal share one multi-set NUFFT launch (same grid, kernel
+// width, channel slots per lane and cube pols).
+uint64_t nufft_sets_key(const NufftPlan& plan, uint32_t C, uint32_t P);
+// NUFFT channel sums of several sets in one launch: workgroup x takes
+// items[x] = (set | direction block << 16, baseline); tile 641 / 1282 / 2564.
+// plan, C and P: those of any set of the launch (equal keys).
+void launch_rotate_nufft_sets(hipStream_t s, int32_t tile, const NufftPlan& plan, uint32_t C, uint32_t P,
+                              const NufftSetDesc* sets, const uint32_t* items, uint32_t n_items);
 bool packed_tile_supported(int32_t tile);
 void launch_rotate_packed(hipStream_t s, int32_t tile, const double* w, const double* uvw, const double* dd,
                           uint32_t D, uint32_t T, uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double f0,

@@ -651,38 +651,25 @@ struct Source<N, 0, CPL> {
     }
 };
 
+// The rows [0, t_count) of baseline bl (rows t nbl + bl of the row source wr
+// and of uvr) for the directions d0 + lane + 64 j, j < NDL, of D; the sums go
+// to out + d dstride (mode kNufftSet: pol 0, pols 1 .. Pout - 1 zeroed;
+// kNufftAccumulate: added; kNufftPartial: written).  s_g is the wave's grid,
+// s_turn its phasor table (filled, and fenced, before the call).
 // SP: the row source (0 packed rows, 1 / 2 / 4 a complex64 cube with that many pols).
 // CPL: channel slots per lane (C <= 64 CPL <= N / 2).
 // F: the type of the transform, the grid and the interpolation (double or float).
 template <int N, int W, int NDL, int SP, int CPL, typename F>
-__global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __restrict__ w,
-                                                              const LEAP_GLOBAL double* __restrict__ scale,
-                                                              const LEAP_GLOBAL double* __restrict__ uvw,
-                                                              const LEAP_GLOBAL double* __restrict__ dd, uint32_t D,
-                                                              uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
-                                                              NufftArgs args, LEAP_GLOBAL double* __restrict__ avg,
-                                                              int32_t mode, NufftSpan span) {
+__device__ __forceinline__ void nufft_rows(const void* wr, const LEAP_GLOBAL double* __restrict__ scale,
+                                           const LEAP_GLOBAL double* __restrict__ uvr,
+                                           const LEAP_GLOBAL double* __restrict__ dd, uint32_t D, uint32_t t_count,
+                                           uint32_t nbl, uint32_t bl, uint32_t C, const NufftArgs& args, uint32_t d0,
+                                           LEAP_GLOBAL double* __restrict__ out, uint32_t dstride, int32_t mode,
+                                           uint32_t Pout, C2<F>* s_g, const Z* s_turn, uint32_t lane) {
 #pragma clang fp contract(off)
     using ZF = C2<F>;
     constexpr int V = N / kWave;
     static_assert(W % 2 == 0 && W <= kNufftMaxWidth && N / 16 >= W - 1, "kernel width");
-    __shared__ ZF s_g[N + N / 16];
-    __shared__ Z s_turn[kTurnTable];
-    const uint32_t lane = threadIdx.x;
-    turn_table_init(s_turn, lane);
-    wave_fence();
-    // The span is resolved here into the row source, uvw and output pointers and
-    // a row count, so the row loop holds no more values than an unchunked one.
-    const uint32_t item = blockIdx.x;
-    const uint32_t chunk = span.chunks > 1 ? item % span.chunks : 0;
-    const uint32_t bl = span.bl_base + (span.chunks > 1 ? item / span.chunks : item);
-    const uint32_t t_begin = uint32_t(uint64_t(chunk) * T / max(span.chunks, 1u));
-    const uint32_t t_count = uint32_t(uint64_t(chunk + 1) * T / max(span.chunks, 1u)) - t_begin;
-    const void* wr = Source<N, SP, CPL>::advance(w, size_t(t_begin) * nbl, C);
-    const LEAP_GLOBAL double* uvr = uvw + size_t(t_begin) * nbl * 3;
-    LEAP_GLOBAL double* out = mode == kNufftPartial ? avg + size_t(item) * D * 2 : avg + size_t(bl) * Pout * 2;
-    const uint32_t dstride = mode == kNufftPartial ? 2u : nbl * Pout * 2;  // doubles between directions
-    const uint32_t d0 = blockIdx.y * (kWave * NDL);
     double dz[NDL][3];
 #pragma unroll
     for (int j = 0; j < NDL; ++j) {
@@ -857,6 +844,36 @@ __global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __rest
     }
 }
 
+// One launch over a set's baselines (blockIdx.x, resolved through the span)
+// and direction blocks of 64 NDL (blockIdx.y).
+template <int N, int W, int NDL, int SP, int CPL, typename F>
+__global__ __launch_bounds__(kWave) void rotate_average_nufft(const void* __restrict__ w,
+                                                              const LEAP_GLOBAL double* __restrict__ scale,
+                                                              const LEAP_GLOBAL double* __restrict__ uvw,
+                                                              const LEAP_GLOBAL double* __restrict__ dd, uint32_t D,
+                                                              uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
+                                                              NufftArgs args, LEAP_GLOBAL double* __restrict__ avg,
+                                                              int32_t mode, NufftSpan span) {
+    __shared__ C2<F> s_g[N + N / 16];
+    __shared__ Z s_turn[kTurnTable];
+    const uint32_t lane = threadIdx.x;
+    turn_table_init(s_turn, lane);
+    wave_fence();
+    // The span is resolved here into the row source, uvw and output pointers and
+    // a row count, so the row loop holds no more values than an unchunked one.
+    const uint32_t item = blockIdx.x;
+    const uint32_t chunk = span.chunks > 1 ? item % span.chunks : 0;
+    const uint32_t bl = span.bl_base + (span.chunks > 1 ? item / span.chunks : item);
+    const uint32_t t_begin = uint32_t(uint64_t(chunk) * T / max(span.chunks, 1u));
+    const uint32_t t_count = uint32_t(uint64_t(chunk + 1) * T / max(span.chunks, 1u)) - t_begin;
+    const void* wr = Source<N, SP, CPL>::advance(w, size_t(t_begin) * nbl, C);
+    const LEAP_GLOBAL double* uvr = uvw + size_t(t_begin) * nbl * 3;
+    LEAP_GLOBAL double* out = mode == kNufftPartial ? avg + size_t(item) * D * 2 : avg + size_t(bl) * Pout * 2;
+    const uint32_t dstride = mode == kNufftPartial ? 2u : nbl * Pout * 2;  // doubles between directions
+    nufft_rows<N, W, NDL, SP, CPL, F>(wr, scale, uvr, dd, D, t_count, nbl, bl, C, args, blockIdx.y * (kWave * NDL),
+                                      out, dstride, mode, Pout, s_g, s_turn, lane);
+}
+
 // avgData of the chunked baselines: the chunks' partial sums added in chunk
 // order (each chunk summed its rows in row order).
 static __global__ void nufft_tail_sum(const double* __restrict__ partial, uint32_t rem, uint32_t chunks, uint32_t D,
@@ -925,65 +942,98 @@ void launch_nufft_tile(hipStream_t s, uint32_t nbl_grid, const void* w, const LE
                                                accumulate ? kNufftAccumulate : kNufftSet, NufftSpan{0, 1});
 }
 
-template <int N, int W, int SP, int CPL, typename F>
-void launch_nwsc(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
-                const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout, const NufftArgs& a,
-                double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
-    const auto* gs = (const LEAP_GLOBAL double*)scale;
-    const auto* gu = (const LEAP_GLOBAL double*)uvw;
-    const auto* gd = (const LEAP_GLOBAL double*)dd;
-    auto* ga = (LEAP_GLOBAL double*)avg;
-    switch (tile) {
-        case 641:
-            launch_nufft_tile<N, W, 1, SP, CPL, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
-                                              tail_doubles);
-            break;
-        case 1282:
-            launch_nufft_tile<N, W, 2, SP, CPL, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
-                                              tail_doubles);
-            break;
-        case 2564:
-            launch_nufft_tile<N, W, 4, SP, CPL, F>(s, nbl_grid, w, gs, gu, gd, D, T, nbl, C, Pout, a, ga, accumulate, tail,
-                                              tail_doubles);
-            break;
-        default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported NUFFT tile " + std::to_string(tile));
-    }
-}
+template <int V>
+using IC = std::integral_constant<int, V>;
 
 // Channel slots per lane: N / 128 in general; for the 1024-point grid with
 // kernel width 10 and an FP64 transform (257..384 channels: 5 or 6 slots), a
 // kernel with 6 slots, which skips the loads, checks and first-pass butterflies
 // of the two slots that are padding.
-template <int N, int W, int SP, typename F>
-void launch_nws(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, const double* scale, const double* uvw,
-                const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout, const NufftArgs& a,
-                double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
-    if constexpr (N == 1024 && W == 10 && std::is_same_v<F, double>) {
-        if (C <= 6 * kWave) {
-            launch_nwsc<N, W, SP, 6, F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate,
-                                        tail, tail_doubles);
-            return;
+template <int N, int W, typename F>
+constexpr int nufft_slots(uint32_t C) {
+    if constexpr (N == 1024 && W == 10 && std::is_same_v<F, double>) return C <= 6 * kWave ? 6 : 8;
+    return N / (2 * kWave);
+}
+
+// Calls fn(IC<NDL>, IC<SP>, IC<CPL>) for a runtime tile (64 NDL directions per
+// workgroup: 641, 1282, 2564), row source (src_pols) and channel count.
+template <int N, int W, typename F, class Fn>
+void dispatch_nufft(int32_t tile, int32_t src_pols, uint32_t C, Fn&& fn) {
+    auto with_ndl = [&](auto sp, auto cpl) {
+        switch (tile) {
+            case 641: fn(IC<1>{}, sp, cpl); break;
+            case 1282: fn(IC<2>{}, sp, cpl); break;
+            case 2564: fn(IC<4>{}, sp, cpl); break;
+            default: throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported NUFFT tile " + std::to_string(tile));
         }
+    };
+    auto with_cpl = [&](auto sp) {
+        if constexpr (N == 1024 && W == 10 && std::is_same_v<F, double>) {
+            if (nufft_slots<N, W, F>(C) == 6) {
+                with_ndl(sp, IC<6>{});
+                return;
+            }
+        }
+        with_ndl(sp, IC<N / (2 * kWave)>{});
+    };
+    switch (src_pols) {
+        case 0: with_cpl(IC<0>{}); break;
+        case 1: with_cpl(IC<1>{}); break;
+        case 2: with_cpl(IC<2>{}); break;
+        case 4: with_cpl(IC<4>{}); break;
+        default: throw Error(LEAP_ERR_INTERNAL, "unsupported NUFFT row source");
     }
-    launch_nwsc<N, W, SP, N / (2 * kWave), F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg,
-                                              accumulate, tail, tail_doubles);
 }
 
 template <int N, int W, typename F>
 void launch_nw(hipStream_t s, int32_t tile, uint32_t nbl_grid, const void* w, int32_t src_pols, const double* scale,
                const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t Pout,
                const NufftArgs& a, double* avg, int32_t accumulate, double* tail, size_t tail_doubles) {
-    switch (src_pols) {
-        case 0: launch_nws<N, W, 0, F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        case 1: launch_nws<N, W, 1, F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        case 2: launch_nws<N, W, 2, F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        case 4: launch_nws<N, W, 4, F>(s, tile, nbl_grid, w, scale, uvw, dd, D, T, nbl, C, Pout, a, avg, accumulate, tail, tail_doubles);
-                break;
-        default: throw Error(LEAP_ERR_INTERNAL, "unsupported NUFFT row source");
-    }
+    dispatch_nufft<N, W, F>(tile, src_pols, C, [&](auto ndl, auto sp, auto cpl) {
+        launch_nufft_tile<N, W, decltype(ndl)::value, decltype(sp)::value, decltype(cpl)::value, F>(
+            s, nbl_grid, w, (const LEAP_GLOBAL double*)scale, (const LEAP_GLOBAL double*)uvw,
+            (const LEAP_GLOBAL double*)dd, D, T, nbl, C, Pout, a, (LEAP_GLOBAL double*)avg, accumulate, tail,
+            tail_doubles);
+    });
+}
+
+// Several sets in one launch: workgroup x takes item x = (set | direction block
+// << 16, baseline) and sums all T rows of that baseline for the block's
+// directions into pol 0 of the set's avgData (the items come in decreasing
+// cost, so the dispatcher fills the GPU largest first).  Every set of a launch
+// has the same grid, kernel width, channel slots and complex64 cube pols; each
+// (baseline, directions) sum is the code and the order of the single-set
+// kernel.
+template <int N, int W, int NDL, int SP, int CPL, typename F>
+__global__ __launch_bounds__(kWave) void rotate_average_nufft_sets(const LEAP_GLOBAL NufftSetDesc* __restrict__ sets,
+                                                                   const LEAP_GLOBAL uint32_t* __restrict__ items) {
+    __shared__ C2<F> s_g[N + N / 16];
+    __shared__ Z s_turn[kTurnTable];
+    const uint32_t lane = threadIdx.x;
+    turn_table_init(s_turn, lane);
+    wave_fence();
+    const uint32_t key = items[2 * blockIdx.x], bl = items[2 * blockIdx.x + 1];
+    const LEAP_GLOBAL NufftSetDesc& sd = sets[key & 0xffffu];
+    const uint32_t nbl = sd.nbl;
+    const NufftArgs a{sd.coef, sd.coef32, sd.kappa_mid, sd.dkappa, sd.h};
+    nufft_rows<N, W, NDL, SP, CPL, F>(sd.vis, (const LEAP_GLOBAL double*)sd.scale, (const LEAP_GLOBAL double*)sd.uvw,
+                                      (const LEAP_GLOBAL double*)sd.dd, sd.D, sd.T, nbl, bl, sd.C, a,
+                                      (key >> 16) * (kWave * NDL), (LEAP_GLOBAL double*)sd.avg + size_t(bl) * 2,
+                                      nbl * 2, kNufftSet, 1, s_g, s_turn, lane);
+}
+
+template <int N, int W, typename F>
+void launch_nw_sets(hipStream_t s, int32_t tile, int32_t src_pols, uint32_t C, const NufftSetDesc* sets,
+                    const uint32_t* items, uint32_t n_items) {
+    dispatch_nufft<N, W, F>(tile, src_pols, C, [&](auto ndl, auto sp, auto cpl) {
+        if constexpr (decltype(sp)::value == 0) {
+            throw Error(LEAP_ERR_INTERNAL, "the multi-set NUFFT kernel reads complex64 cubes");
+        } else {
+            rotate_average_nufft_sets<N, W, decltype(ndl)::value, decltype(sp)::value, decltype(cpl)::value, F>
+                <<<dim3(n_items), kWave, 0, s>>>((const LEAP_GLOBAL NufftSetDesc*)sets,
+                                                 (const LEAP_GLOBAL uint32_t*)items);
+        }
+    });
 }
 
 // Launch arguments shared by both transform precisions.

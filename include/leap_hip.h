@@ -257,8 +257,15 @@ typedef int (*leap_batch_cb)(void* user, uint32_t set, const leap_solution* solu
  * directions, is rotated and averaged in ONE persistent launch: a device
  * descriptor per set and (set, baseline, direction block) work items
  * statically load-balanced over the grid on the host (longest processing time
- * first), so the result does not depend on scheduling.  Needs the batched
- * strategy, all_pols = 0 and fp32_blocks = 0.  No reference counterpart: the
+ * first), so the result does not depend on scheduling.  In mixed precision the
+ * sets that qualify for the NUFFT channel sums (channel_sum automatic: 96..512
+ * channels, 16 or more directions; or channel_sum = LEAP_CHANNEL_SUM_NUFFT)
+ * and hold complex64 cubes with 1, 2 or 4 pols take them instead, one launch
+ * per group of sets that share a grid and kernel width, with their
+ * (set, baseline, direction block) items in decreasing rows per item; each
+ * item's sums are those of the single-set NUFFT kernel.  Needs the batched
+ * strategy, all_pols = 0, fp32_blocks = 0 and channel_sum other than
+ * LEAP_CHANNEL_SUM_NUFFT_FP32.  No reference counterpart: the
  * reference calibrates one measurement set per ILeapCalibrator::Calibrate call
  * (algorithm/ILeapCalibrator.h:62-68). */
 int leap_hip_calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,

@@ -1,11 +1,17 @@
 """Several measurement sets of uneven shape in one persistent launch.
 
 Small sets run against the oracle (FP64 <= 1e-9 rad wrapped, mixed <= 1e-5
-rad); the three sets of the uneven benchmark batch run at full size against
-the single-set batched kernel that reads the cube directly with the same tile,
-which must agree bit for bit (the same FP64 Horner tile code runs per work
-item), and against the oracle
-for a few directions of the two smaller sets.
+rad), on the FP64 Horner tile and, in mixed precision, on the NUFFT channel
+sums (sets of 96..512 channels with 16 or more directions and complex64 cubes,
+one launch per group of sets that share a grid and kernel width).  Every
+(baseline, direction block) of the NUFFT launch runs the single-set kernel's
+code in its order, so it equals the single-set NUFFT bit for bit wherever the
+single-set launch does not split a baseline's rows into chunks (more than one
+direction block).  The three sets of the uneven benchmark batch run at full
+size against the single-set batched kernel that reads the cube directly with
+the same tile, which must agree bit for bit on the Horner route (the same FP64
+Horner tile code runs per work item), within rounding on the NUFFT route, and
+against the oracle for a few directions of the two smaller sets.
 """
 import ctypes
 
@@ -82,6 +88,76 @@ def test_multi_mixed_matches_oracle(leap, ctx, oracle, small_sets):
     assert worst < 1e-5
 
 
+# complex64 sets for the NUFFT route: synth kwargs, directions, slice; the last
+# one (48 channels) stays on the Horner tile under the automatic rule.
+NUFFT_SETS = [
+    (dict(n_antennas=16, n_timesteps=4, n_channels=200, seed=81, nan_fraction=0.002), 40, (0, 2, -1)),
+    (dict(n_antennas=22, autocorrelations=0, n_timesteps=3, n_channels=130, seed=82, n_pols=2), 70, (0, -1, -1)),
+    (dict(n_antennas=12, n_timesteps=5, n_channels=300, seed=83, n_pols=1), 20, (0, 1, -1)),
+    (dict(n_antennas=14, n_timesteps=4, n_channels=384, seed=84), 150, (0, -1, -1)),
+    (dict(n_antennas=10, n_timesteps=3, n_channels=48, seed=85), 30, (0, -1, -1)),
+]
+
+
+@pytest.fixture(scope="module")
+def nufft_sets(leap):
+    out = []
+    for kw, nd, slice_ in NUFFT_SETS:
+        s = leap.SynthMS(**kw)
+        ms = s.host_dict(s.fill_host(np.complex64))
+        out.append((ms, dirs_near(ms["phase_centre"], nd, nd + 7, spread=0.05), slice_))
+    return out
+
+
+@pytest.mark.parametrize("channel_sum", [0, 2], ids=["auto", "nufft"])
+def test_multi_nufft_matches_oracle(leap, ctx, oracle, nufft_sets, channel_sum):
+    views = [leap.view_from_arrays(ms) for ms, _, _ in nufft_sets]
+    mds = [ctx.metadata(v) for v in views]
+    opts = leap.options(precision=leap.LEAP_PRECISION_MIXED, strategy=leap.LEAP_STRATEGY_BATCHED,
+                        channel_sum=channel_sum)
+    res = ctx.calibrate_batch([(md, v, d, sl) for md, v, (_, d, sl) in zip(mds, views, nufft_sets)], opts)
+    assert ctx.last_route()[0] == 2
+    worst = 0.0
+    for (ms, dirs, slice_), (got, ge) in zip(nufft_sets, res):
+        want, we = oracle.calibrate(ms, dirs, slice_)
+        assert np.array_equal(ge, we)
+        worst = max(worst, wrapped(got, want).max())
+    print(f"multi-set NUFFT ({'auto' if channel_sum == 0 else 'forced'}): max wrapped error {worst:.2e} rad")
+    assert worst < 1e-5
+
+
+@pytest.mark.parametrize("db,nd", [(64, 1), (128, 2)])
+def test_multi_nufft_equals_single_set(leap, ctx, nufft_sets, db, nd):
+    """Sets with more directions than one block: the single-set launch runs every
+    (baseline, direction block) whole, as the multi-set launch does."""
+    pick = [(ms, d, sl) for ms, d, sl in nufft_sets[:4] if len(d) > 64 * nd]
+    assert pick
+    views = [leap.view_from_arrays(ms) for ms, _, _ in pick]
+    mds = [ctx.metadata(v) for v in views]
+    opts = leap.options(precision=leap.LEAP_PRECISION_MIXED, strategy=leap.LEAP_STRATEGY_BATCHED,
+                        direction_block=db, lane_directions=nd)
+    res = ctx.calibrate_batch([(md, v, d, sl) for md, v, (_, d, sl) in zip(mds, views, pick)], opts)
+    assert ctx.last_route()[0] == 2
+    for md, v, (ms, d, sl), (got, ge) in zip(mds, views, pick, res):
+        one, oe = ctx.calibrate(v, d, sl, opts=opts, md=md)
+        assert ctx.last_route()[0] == 2
+        assert np.array_equal(ge, oe)
+        assert np.array_equal(got, one), "multi-set and single-set NUFFT results differ"
+
+
+def test_multi_nufft_rejects_unsupported(leap, ctx, small_sets, nufft_sets):
+    ms, dirs, slice_ = small_sets[0][1], small_sets[0][2], small_sets[0][3]
+    v = leap.view_from_arrays(ms)  # complex128
+    md = ctx.metadata(v)
+    with pytest.raises(leap.LeapError, match="complex64 cube"):
+        ctx.calibrate_batch([(md, v, dirs, slice_)], leap.options(precision=1, strategy=1, channel_sum=2))
+    ms, dirs, slice_ = nufft_sets[0]
+    v = leap.view_from_arrays(ms)
+    md = ctx.metadata(v)
+    with pytest.raises(leap.LeapError, match="FP64 transform"):
+        ctx.calibrate_batch([(md, v, dirs, slice_)], leap.options(precision=1, strategy=1, channel_sum=3))
+
+
 def test_multi_rejects_unsupported(leap, ctx, small_sets):
     s, ms, dirs, slice_, ref = small_sets[0]
     v = leap.view_from_arrays(ms)
@@ -124,21 +200,40 @@ def test_uneven_batch_full_size(leap, ctx, oracle, precision):
             view = leap.view_from_arrays(ms, vis_device=vis_p, vis_dtype=vd, uvw_device=uvw_p)
             dirs = dirs_near(ms["phase_centre"], 64, 70 + i, spread=0.1)
             sets.append((s, ms, host, view, ctx.metadata(view), dirs))
+        batch = [(md, v, d, (0, -1, -1)) for (_, _, _, v, md, d) in sets]
         for db, nd in ((128, 2), (64, 1)):
-            opts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
-                                lane_directions=nd)
-            res = ctx.calibrate_batch([(md, v, d, (0, -1, -1)) for (_, _, _, v, md, d) in sets], opts)
-            multi_ms = ctx.last_kernel_times()[0]
+            # the Horner tile for every set (mixed: asked for), bit-equal to the single-set kernel
+            # that shares the multi-set tile code and reads the cube directly
+            hopts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
+                                 lane_directions=nd, channel_sum=1 if precision else 0)
+            hres = ctx.calibrate_batch(batch, hopts)
+            assert ctx.last_route()[0] == 1
+            horner_ms = ctx.last_kernel_times()[0]
             single_ms = 0.0
-            # the single-set kernel that shares the multi-set tile code: straight from the cube
             one_opts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
-                                    lane_directions=nd, direct_cube=1)
-            for (s, ms, host, view, md, dirs), (got, _) in zip(sets, res):
+                                    lane_directions=nd, direct_cube=1, channel_sum=1 if precision else 0)
+            for (s, ms, host, view, md, dirs), (got, _) in zip(sets, hres):
                 one, _ = ctx.calibrate(view, dirs, opts=one_opts, md=md)
                 single_ms += ctx.last_kernel_times()[0]
                 assert np.array_equal(got, one), "multi-set and single-set results differ"
-            print(f"uneven batch {'mixed' if precision else 'fp64'} tile {db}x{nd}: one launch {multi_ms:.3f} ms, "
-                  f"three single-set launches {single_ms:.3f} ms")
+            res = hres
+            line = (f"uneven batch {'mixed' if precision else 'fp64'} tile {db}x{nd}: Horner, one launch "
+                    f"{horner_ms:.3f} ms, three single-set launches {single_ms:.3f} ms")
+            if precision:
+                # the automatic route: the 300- and 96-channel sets on the NUFFT, the 48-channel set on Horner
+                opts = leap.options(precision=1, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
+                                    lane_directions=nd)
+                res = ctx.calibrate_batch(batch, opts)
+                assert ctx.last_route()[0] == 2
+                nufft_ms = ctx.last_kernel_times()[0]
+                assert np.array_equal(res[1][0], hres[1][0])  # the Horner set: the same tile code
+                for (s, ms, host, view, md, dirs), (got, _) in zip(sets, res):
+                    one, _ = ctx.calibrate(view, dirs, opts=opts, md=md)
+                    # the single-set NUFFT launch splits its last round's rows into chunks
+                    err = wrapped(got, one).max()
+                    assert err < 1e-10, err
+                line += f"; automatic route (NUFFT for 2 of 3 sets), one round {nufft_ms:.3f} ms"
+            print(line)
         # the same batch (tile 64 x 1) captured as one HIP graph and replayed: bit-equal to the eager launch
         gopts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=64,
                              lane_directions=1, graph=1)
