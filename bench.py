@@ -546,20 +546,28 @@ def run_adjoint(args):
     ctx.free(vis_ptr)
     ctx.free(uvw_ptr)
     ctx.close()
-    # Roofline of the adjoint kernel (HBM): it reads the cube, uvw and abar and writes the uvw
-    # gradient and the per-baseline dz partials; the forward kernel reads the cube and uvw and
-    # writes avgData.  Kernel times from HIP events on the library's stream.
+    # Roofline of the dominant kernel, the forward pass with row sums (HBM): it reads the cube and
+    # uvw, writes avgData and each row's sums (e0 P, e0 z P', 32 bytes per row and direction); it
+    # also runs 16 flop per (row, channel, direction) (the P and P' Horner chains), reported
+    # against the FP64 vector peak.  The adjoint kernel reads the row sums, uvw and abar and writes
+    # the uvw gradient and the per-baseline dz partials.  Kernel times from HIP events on the
+    # library's stream.
     a_ms, f_ms = float(np.median(adj_k)), float(np.median(fwd_k))
-    adj_bytes = R * C * P * 8 + R * 24 + D * nbl * 16 + R * 24 + nbl * D * 24
-    fwd_bytes = R * C * P * 8 + R * 24 + D * nbl * P * 16
-    roofline = {"kernel": "rotate_adjoint_kernel", "bound": "hbm",
-                "achieved": round(adj_bytes / (a_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(adj_bytes / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel_ms": round(a_ms, 4), "algorithmic_bytes_per_launch": adj_bytes,
-                "forward": {"kernel": "rotate_average_horner64 (16-direction tile)", "kernel_ms": round(f_ms, 4),
-                            "algorithmic_bytes_per_launch": fwd_bytes,
-                            "achieved": round(fwd_bytes / (f_ms * 1e-3) / 1e9, 1),
-                            "frac": round(fwd_bytes / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    q_bytes = R * D * 32
+    fwd_bytes = R * C * P * 8 + R * 24 + D * nbl * P * 16 + q_bytes
+    fwd_flops = 16 * R * C * D
+    adj_bytes = q_bytes + R * 24 + D * nbl * 16 + R * 24 + nbl * D * 24
+    roofline = {"kernel": "rotate_average_horner64_q (16-direction tile, row sums kept)", "bound": "hbm",
+                "achieved": round(fwd_bytes / (f_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(fwd_bytes / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel_ms": round(f_ms, 4), "algorithmic_bytes_per_launch": fwd_bytes,
+                "valu": {"achieved_tflops": round(fwd_flops / (f_ms * 1e-3) / 1e12, 2), "peak_tflops": 78.6,
+                         "frac": round(fwd_flops / (f_ms * 1e-3) / 1e12 / 78.6, 4),
+                         "algorithmic_flops_per_launch": fwd_flops},
+                "adjoint": {"kernel": "adjoint_q_kernel (from the row sums)", "kernel_ms": round(a_ms, 4),
+                            "algorithmic_bytes_per_launch": adj_bytes,
+                            "achieved": round(adj_bytes / (a_ms * 1e-3) / 1e9, 1),
+                            "frac": round(adj_bytes / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     return {
         "metric": "calibration-residual loss + gradient evaluations/s (128 st, 8128 bl, 64 t, 192 ch, 16 dirs)",
         "value": round(steps / elapsed, 3),

@@ -1815,6 +1815,58 @@ extern "C" int leap_hip_last_pack_times(const leap_ctx* ctx, double* pack_ms_avg
     return LEAP_OK;
 }
 
+namespace {
+
+// Device tables of a set's baselines for the residual-loss gradient, rebuilt
+// only when the antennas change: each baseline's antennas, and each antenna's
+// baselines (+1 where it is antenna2, -1 where antenna1; autocorrelations,
+// whose uvw is 0, left out) as offsets, baselines and signs.
+void adjoint_antenna_tables(leap_ctx* ctx, const leap_ms_view& ms, int32_t N) {
+    const uint32_t nbl = ms.num_baselines;
+    uint64_t key = 1469598103934665603ull;  // FNV-1a over the antenna columns
+    auto mix = [&](uint64_t v) {
+        for (int k = 0; k < 8; ++k) {
+            key ^= (v >> (8 * k)) & 0xffu;
+            key *= 1099511628211ull;
+        }
+    };
+    mix(nbl);
+    mix(uint64_t(uint32_t(N)));
+    for (uint32_t b = 0; b < nbl; ++b) mix((uint64_t(uint32_t(ms.antenna1[b])) << 32) | uint32_t(ms.antenna2[b]));
+    if (key == ctx->adj_ants_key && ctx->d_adj_ants.ptr && ctx->d_adj_off.ptr) return;
+    hipStream_t s = ctx->stream;
+    std::vector<int2> ants(nbl);
+    std::vector<int32_t> off(size_t(N) + 1, 0), bls, sgn;
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> lists(static_cast<size_t>(N));
+    for (uint32_t b = 0; b < nbl; ++b) {
+        ants[b] = make_int2(ms.antenna1[b], ms.antenna2[b]);
+        if (ms.antenna1[b] == ms.antenna2[b]) continue;
+        lists[ms.antenna2[b]].emplace_back(int32_t(b), 1);
+        lists[ms.antenna1[b]].emplace_back(int32_t(b), -1);
+    }
+    for (int32_t a = 0; a < N; ++a) {
+        off[a + 1] = off[a] + int32_t(lists[a].size());
+        for (auto& e : lists[a]) {
+            bls.push_back(e.first);
+            sgn.push_back(e.second);
+        }
+    }
+    ctx->d_adj_ants.reserve(nbl * sizeof(int2));
+    ctx->d_adj_off.reserve(off.size() * sizeof(int32_t));
+    ctx->d_adj_bls.reserve(std::max<size_t>(1, bls.size()) * sizeof(int32_t));
+    ctx->d_adj_sgn.reserve(std::max<size_t>(1, sgn.size()) * sizeof(int32_t));
+    LEAP_HIP(hipMemcpyAsync(ctx->d_adj_ants.ptr, ants.data(), nbl * sizeof(int2), hipMemcpyHostToDevice, s));
+    LEAP_HIP(hipMemcpyAsync(ctx->d_adj_off.ptr, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (!bls.empty()) {
+        LEAP_HIP(hipMemcpyAsync(ctx->d_adj_bls.ptr, bls.data(), bls.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        LEAP_HIP(hipMemcpyAsync(ctx->d_adj_sgn.ptr, sgn.data(), sgn.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
+    LEAP_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    ctx->adj_ants_key = key;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Calibration-residual loss and its gradient: forward rotate/average and
 // inversion, the loss, the inversion's adjoint and the adjoint of
@@ -1833,6 +1885,11 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         hipStream_t s = ctx->stream;
         const uint32_t nbl = ms->num_baselines, R = ms->num_rows, T = R / nbl;
         const int32_t N = md->A.cols;
+        const bool want_grad = grad_offsets || grad_antenna;
+        // With a gradient the forward pass keeps each row's sums for the adjoint
+        // (32 bytes per row and direction) unless they outweigh the cube itself.
+        const bool row_sums = want_grad && size_t(32) * D <= vis_elem_bytes(*ms) * md->channels;
+        if (antenna_delta || grad_antenna) adjoint_antenna_tables(ctx, *ms, N);
         std::vector<double> dirs(size_t(D) * 2);
         for (uint32_t d = 0; d < 2 * D; ++d) dirs[d] = directions[d] + (offsets ? offsets[d] : 0.0);
         upload_directions(ctx, *md, dirs.data(), D);
@@ -1843,12 +1900,8 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         LEAP_HIP(hipMemcpyAsync(d_M.ptr, enu_to_uvw, size_t(T) * 9 * sizeof(double), hipMemcpyHostToDevice, s));
         const double* uvw = rows.uvw;
         if (antenna_delta) {
-            std::vector<int2> ants(nbl);
-            for (uint32_t b = 0; b < nbl; ++b) ants[b] = make_int2(ms->antenna1[b], ms->antenna2[b]);
-            d_ants.reserve(nbl * sizeof(int2));
             d_dp.reserve(size_t(N) * 3 * sizeof(double));
             d_uvwp.reserve(size_t(R) * 3 * sizeof(double));
-            LEAP_HIP(hipMemcpyAsync(d_ants.ptr, ants.data(), nbl * sizeof(int2), hipMemcpyHostToDevice, s));
             LEAP_HIP(hipMemcpyAsync(d_dp.ptr, antenna_delta, size_t(N) * 3 * sizeof(double), hipMemcpyHostToDevice, s));
             launch_perturb_uvw(s, rows.uvw, d_M.as<double>(), d_ants.as<int2>(), d_dp.as<double>(), nbl, R,
                                d_uvwp.as<double>());
@@ -1858,10 +1911,16 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         const size_t per_dir = size_t(nbl) * md->pols * 2;
         ctx->d_avg.reserve(per_dir * D * sizeof(double));
         const int32_t tile = D <= 16 ? 162 : D <= 32 ? 321 : D <= 64 ? 641 : D <= 128 ? 1282 : 2562;
+        if (row_sums) ctx->d_adj_q.reserve(size_t(R) * D * 4 * sizeof(double));
         LEAP_HIP(hipEventRecord(ctx->ev_grad[0], s));
-        launch_rotate_batched(s, LEAP_PRECISION_FP64, tile, false, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(),
-                              D, T, nbl, nbl, md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
-                              ctx->d_avg.as<double>(), 0);
+        if (row_sums)
+            launch_rotate_forward_q(s, tile, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(), D, T, nbl,
+                                    md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
+                                    ctx->d_avg.as<double>(), ctx->d_adj_q.as<double>());
+        else
+            launch_rotate_batched(s, LEAP_PRECISION_FP64, tile, false, rows.vis, ms->vis_dtype, uvw,
+                                  ctx->d_dd.as<double>(), D, T, nbl, nbl, md->channels, md->pols, md->freq_start_hz,
+                                  md->freq_inc_hz, ctx->d_avg.as<double>(), 0);
         LEAP_HIP(hipEventRecord(ctx->ev_grad[1], s));
         ctx->last_grad_adjoint_ms = 0.0;
         // the inversion (CpuLeapCalibrator.cc:159-178) by the calibration's own kernels
@@ -1873,7 +1932,6 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         launch_gather_pol0(s, ctx->d_avg.as<double>(), D * nbl, md->pols, d_avg0.as<double>());
         // loss and the adjoint of the inversion, on the device
         const int32_t K = md->A.rows, K1 = md->A1.rows, nI = K - 1;
-        const bool want_grad = grad_offsets || grad_antenna;
         const size_t rb = size_t(std::max(1, (nI + 255) / 256));
         const size_t n_inv = size_t(D) * (size_t(std::max(nI, 1)) * 2 + rb + 1 + size_t(nbl) + 4 * size_t(N));
         ctx->d_adj_inv.reserve(n_inv * sizeof(double));
@@ -1912,17 +1970,24 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         ia.y1 = take(size_t(D) * N);
         ia.abar = want_grad ? ctx->d_adj_abar.as<double>() : nullptr;
         launch_inversion_adjoint(s, ia);
-        ctx->h_adj_out.reserve(size_t(D) * 3 * sizeof(double));
+        ctx->h_adj_out.reserve((size_t(D) * 3 + size_t(N) * 3) * sizeof(double));
         double* h_out = ctx->h_adj_out.as<double>();
         LEAP_HIP(hipMemcpyAsync(h_out, ia.loss, size_t(D) * sizeof(double), hipMemcpyDeviceToHost, s));
-        LEAP_HIP(hipStreamSynchronize(s));
-        float fwd_ms = 0.f;
-        LEAP_HIP(hipEventElapsedTime(&fwd_ms, ctx->ev_grad[0], ctx->ev_grad[1]));
-        ctx->last_grad_forward_ms = fwd_ms;
-        double loss = 0.0;
-        for (uint32_t d = 0; d < D; ++d) loss += h_out[d];  // directions in order
-        *loss_out = loss;
-        if (!want_grad) return;
+        // The losses are read after the last synchronisation (the gradient's
+        // work is queued behind their copy).
+        auto finish_loss = [&] {
+            float fwd_ms = 0.f;
+            LEAP_HIP(hipEventElapsedTime(&fwd_ms, ctx->ev_grad[0], ctx->ev_grad[1]));
+            ctx->last_grad_forward_ms = fwd_ms;
+            double loss = 0.0;
+            for (uint32_t d = 0; d < D; ++d) loss += h_out[d];  // directions in order
+            *loss_out = loss;
+        };
+        if (!want_grad) {
+            LEAP_HIP(hipStreamSynchronize(s));
+            finish_loss();
+            return;
+        }
         // adjoint of rotate -> average on the device
         DeviceBuffer &d_abar = ctx->d_adj_abar, &d_guvw = ctx->d_adj_guvw, &d_gdzp = ctx->d_adj_gdzp,
                      &d_gdz = ctx->d_adj_gdz;
@@ -1930,9 +1995,13 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         d_gdzp.reserve(size_t(nbl) * D * 3 * sizeof(double));
         d_gdz.reserve(size_t(D) * 5 * sizeof(double));  // g_dz (3 D), then dL/d(ra, dec) (2 D)
         LEAP_HIP(hipEventRecord(ctx->ev_grad[2], s));
-        launch_rotate_adjoint(s, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(), d_abar.as<double>(), D, T, nbl,
-                              md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz, d_guvw.as<double>(),
-                              d_gdzp.as<double>());
+        if (row_sums)
+            launch_adjoint_q(s, ctx->d_adj_q.as<double>(), uvw, ctx->d_dd.as<double>(), d_abar.as<double>(), D, T, nbl,
+                             md->freq_start_hz, md->freq_inc_hz, d_guvw.as<double>(), d_gdzp.as<double>());
+        else
+            launch_rotate_adjoint(s, rows.vis, ms->vis_dtype, uvw, ctx->d_dd.as<double>(), d_abar.as<double>(), D, T,
+                                  nbl, md->channels, md->pols, md->freq_start_hz, md->freq_inc_hz,
+                                  d_guvw.as<double>(), d_gdzp.as<double>());
         LEAP_HIP(hipEventRecord(ctx->ev_grad[3], s));
         // dL/d(dz row of dd) reduced over baselines and mapped to dL/d(ra, dec) (MetaData.cc:166-203
         // differentiated) on the device
@@ -1942,38 +2011,17 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
             LEAP_HIP(hipMemcpyAsync(h_out + D, d_gdz.as<double>() + 3 * size_t(D), size_t(D) * 2 * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
         if (grad_antenna) {
-            // baselines of each antenna: +1 where it is antenna2, -1 where antenna1
-            std::vector<int32_t> off(size_t(N) + 1, 0), bls, sgn;
-            std::vector<std::vector<std::pair<int32_t, int32_t>>> lists(static_cast<size_t>(N));
-            for (uint32_t b = 0; b < nbl; ++b) {
-                if (ms->antenna1[b] == ms->antenna2[b]) continue;  // uvw of an autocorrelation is 0
-                lists[ms->antenna2[b]].emplace_back(int32_t(b), 1);
-                lists[ms->antenna1[b]].emplace_back(int32_t(b), -1);
-            }
-            for (int32_t a = 0; a < N; ++a) {
-                off[a + 1] = off[a] + int32_t(lists[a].size());
-                for (auto& e : lists[a]) {
-                    bls.push_back(e.first);
-                    sgn.push_back(e.second);
-                }
-            }
-            DeviceBuffer &d_off = ctx->d_adj_off, &d_bls = ctx->d_adj_bls, &d_sgn = ctx->d_adj_sgn,
-                         &d_gant = ctx->d_adj_gant;
-            d_off.reserve(off.size() * sizeof(int32_t));
-            d_bls.reserve(std::max<size_t>(1, bls.size()) * sizeof(int32_t));
-            d_sgn.reserve(std::max<size_t>(1, sgn.size()) * sizeof(int32_t));
+            DeviceBuffer& d_gant = ctx->d_adj_gant;
             d_gant.reserve(size_t(N) * 3 * sizeof(double));
-            LEAP_HIP(hipMemcpyAsync(d_off.ptr, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-            if (!bls.empty()) {
-                LEAP_HIP(hipMemcpyAsync(d_bls.ptr, bls.data(), bls.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-                LEAP_HIP(hipMemcpyAsync(d_sgn.ptr, sgn.data(), sgn.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-            }
-            launch_antenna_grad(s, d_guvw.as<double>(), d_M.as<double>(), d_off.as<int32_t>(), d_bls.as<int32_t>(),
-                                d_sgn.as<int32_t>(), N, nbl, T, d_gant.as<double>());
-            LEAP_HIP(hipMemcpyAsync(grad_antenna, d_gant.ptr, size_t(N) * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-            LEAP_HIP(hipStreamSynchronize(s));
+            launch_antenna_grad(s, d_guvw.as<double>(), d_M.as<double>(), ctx->d_adj_off.as<int32_t>(),
+                                ctx->d_adj_bls.as<int32_t>(), ctx->d_adj_sgn.as<int32_t>(), N, nbl, T,
+                                d_gant.as<double>());
+            LEAP_HIP(hipMemcpyAsync(h_out + 3 * size_t(D), d_gant.ptr, size_t(N) * 3 * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
         }
         LEAP_HIP(hipStreamSynchronize(s));
+        finish_loss();
+        if (grad_antenna) std::copy(h_out + 3 * size_t(D), h_out + 3 * size_t(D) + size_t(N) * 3, grad_antenna);
         float adj_ms = 0.f;
         LEAP_HIP(hipEventElapsedTime(&adj_ms, ctx->ev_grad[2], ctx->ev_grad[3]));
         ctx->last_grad_adjoint_ms = adj_ms;

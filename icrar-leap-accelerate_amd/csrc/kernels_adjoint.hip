@@ -6,9 +6,12 @@
 // Given abar(b, d) = dL/dRe avg + i dL/dIm avg (so dL = Re(conj(abar) d avg)):
 //   dL/dsf(r, d) = Re(conj(abar) * i * sum_c kappa_c v_c exp(i sf kappa_c))
 //                = Re(conj(abar) * i * e0 * (kappa0 P(z) + dkappa z P'(z)))
-// with e0 = exp(i sf kappa0), z = exp(i sf dkappa), P(z) = sum_c v_c z^c.  P and P'
-// are evaluated together by Horner's rule (p' <- p' z + p; p <- p z + v), so the
-// phasors are recomputed rather than stored.  Then
+// with e0 = exp(i sf kappa0), z = exp(i sf dkappa), P(z) = sum_c v_c z^c.  When
+// the forward pass has kept each row's sums e0 P(z) and e0 z P'(z) (32 bytes per
+// row and direction, rotate_average_horner64_q), adjoint_q_kernel reads them
+// and the cube is not read again; otherwise rotate_adjoint_kernel evaluates P
+// and P' together by Horner's rule (p' <- p' z + p; p <- p z + v) from the
+// cube.  Then
 //   dL/d dz_d   += -2 pi dL/dsf(r, d) uvw(r)            (per baseline, reduced later)
 //   dL/d uvw(r)  = sum_d -2 pi dL/dsf(r, d) (dz_d - e_z)
 // and the antenna gradient is dL/d dp_a = sum_t M_t^T (sum_{b: a2 = a} g_uvw - sum_{b: a1 = a} g_uvw).
@@ -226,6 +229,90 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
     }
 }
 
+// The same outputs from the forward pass's per-row sums q[row][d] = (e0 P(z),
+// e0 z P'(z)) (rotate_average_horner64 with row sums), so the cube is not read
+// again: s1 = kappa0 q0 + dkappa q1 is the sum above.  Lanes, rows and the
+// reductions as in rotate_adjoint_kernel.
+template <int DL, int ND>
+__global__ __launch_bounds__(kThreads) void adjoint_q_kernel(const double* __restrict__ q,
+                                                             const double* __restrict__ uvw,
+                                                             const double* __restrict__ dd,
+                                                             const double* __restrict__ abar, uint32_t D, uint32_t T,
+                                                             uint32_t nbl, double kappa0, double dkappa,
+                                                             double* __restrict__ g_uvw,
+                                                             double* __restrict__ g_dz_partial) {
+    constexpr int RP = kThreads / DL;
+    constexpr int DW = DL * ND;
+    __shared__ double s_g[RP][DW][3];
+    const uint32_t bl = blockIdx.x;
+    const uint32_t lane_d = threadIdx.x % DL;
+    const uint32_t rsub = threadIdx.x / DL;
+    bool dir_ok[ND];
+    double dz[ND][3], gdz[ND][3];
+    Cx ab[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+        const uint32_t dj = lane_d + DL * j;
+        dir_ok[j] = dj < D;
+        const uint32_t dc = dir_ok[j] ? dj : 0;
+        dz[j][0] = dd[9 * dc + 6];
+        dz[j][1] = dd[9 * dc + 7];
+        dz[j][2] = dd[9 * dc + 8];
+        ab[j] = Cx{0.0, 0.0};
+        if (dir_ok[j]) ab[j] = {abar[2 * (size_t(dc) * nbl + bl)], abar[2 * (size_t(dc) * nbl + bl) + 1]};
+        gdz[j][0] = gdz[j][1] = gdz[j][2] = 0.0;
+    }
+    for (uint32_t tb = 0; tb < T; tb += RP) {
+        const uint32_t t = tb + rsub;
+        const bool row_ok = t < T;
+        const size_t row = size_t(t) * nbl + bl;
+        double u = 0.0, v = 0.0, w = 0.0;
+        if (row_ok) {
+            u = uvw[3 * row];
+            v = uvw[3 * row + 1];
+            w = uvw[3 * row + 2];
+        }
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            double gsf = 0.0;
+            if (row_ok && dir_ok[j]) {
+                const double* qp = q + (row * D + lane_d + DL * j) * 4;
+                const Cx s1 = {kappa0 * qp[0] + dkappa * qp[2], kappa0 * qp[1] + dkappa * qp[3]};
+                gsf = -ab[j].x * s1.y + ab[j].y * s1.x;
+                gdz[j][0] += -kTwoPi * gsf * u;
+                gdz[j][1] += -kTwoPi * gsf * v;
+                gdz[j][2] += -kTwoPi * gsf * w;
+            }
+            s_g[rsub][lane_d + DL * j][0] = -kTwoPi * gsf * dz[j][0];
+            s_g[rsub][lane_d + DL * j][1] = -kTwoPi * gsf * dz[j][1];
+            s_g[rsub][lane_d + DL * j][2] = -kTwoPi * gsf * (dz[j][2] - 1.0);
+        }
+        __syncthreads();
+        if (lane_d < 3 && row_ok) {
+            double s = 0.0;
+            for (int j = 0; j < DW; ++j) s += s_g[rsub][j][lane_d];  // direction order
+            g_uvw[3 * row + lane_d] = s;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+        for (int k = 0; k < 3; ++k) s_g[rsub][lane_d + DL * j][k] = dir_ok[j] ? gdz[j][k] : 0.0;
+    __syncthreads();
+    if (rsub == 0) {
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            if (!dir_ok[j]) continue;
+            const uint32_t dj = lane_d + DL * j;
+            for (int k = 0; k < 3; ++k) {
+                double s = 0.0;
+                for (int r = 0; r < RP; ++r) s += s_g[r][dj][k];
+                g_dz_partial[(size_t(bl) * D + dj) * 3 + k] = s;
+            }
+        }
+    }
+}
+
 // g_dz[d][k] = sum over baselines of g_dz_partial[b][d][k]: one workgroup per output
 // i = 3 d + k, each lane a strided slice of the baselines, then a fixed-order tree in LDS.
 __global__ __launch_bounds__(256) void reduce_gdz_kernel(const double* __restrict__ part, uint32_t nbl, uint32_t D,
@@ -377,18 +464,30 @@ __global__ void inv_garg_res_kernel(const double* __restrict__ res, const int32_
     g -= 2.0 * res[size_t(d) * nI + n];
 }
 
-// out[d][a] = (base ? base[d][a] : 0) + sum over antenna a's rows e (row order) of
-// sign_e * scale * src[d][row_e]; csr_ent[e] = 2 row + (sign < 0).
-__global__ void inv_antenna_sum_kernel(const double* __restrict__ src, int32_t nI, double scale,
-                                       const int32_t* __restrict__ csr_off, const int32_t* __restrict__ csr_ent,
-                                       int32_t N, const double* __restrict__ base, double* __restrict__ out) {
+// Sum of the 64 lanes' values of a wave by a butterfly of xor shuffles: a fixed
+// order, so every lane gets the same bits on every run.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// out[d][a] = (base ? base[d][a] : 0) + sum over antenna a's rows e of
+// sign_e * scale * src[d][row_e]; csr_ent[e] = 2 row + (sign < 0).  One wave
+// per (a, d) (four per workgroup): lane l sums entries l, l + 64, ... in order,
+// then the lanes are summed by wave_sum.
+__global__ __launch_bounds__(256) void inv_antenna_sum_kernel(const double* __restrict__ src, int32_t nI,
+                                                              double scale, const int32_t* __restrict__ csr_off,
+                                                              const int32_t* __restrict__ csr_ent, int32_t N,
+                                                              const double* __restrict__ base,
+                                                              double* __restrict__ out) {
 #pragma clang fp contract(off)
-    const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t d = blockIdx.y;
-    if (a >= N) return;
-    double acc = base ? base[size_t(d) * N + a] : 0.0;
+    const int32_t a = int32_t(blockIdx.x * 4 + threadIdx.x / 64);
+    const uint32_t lane = threadIdx.x % 64, d = blockIdx.y;
+    if (a >= N) return;  // whole waves
     const double* x = src + size_t(d) * nI;
-    for (int32_t e = csr_off[a]; e < csr_off[a + 1]; ++e) {
+    double acc = 0.0;
+    for (int32_t e = csr_off[a] + int32_t(lane); e < csr_off[a + 1]; e += 64) {
         const int32_t v = csr_ent[e];
         const double t = scale * x[v >> 1];
         if (v & 1)
@@ -396,21 +495,24 @@ __global__ void inv_antenna_sum_kernel(const double* __restrict__ src, int32_t n
         else
             acc += t;
     }
-    out[size_t(d) * N + a] = acc;
+    acc = wave_sum(acc);
+    if (lane == 0) out[size_t(d) * N + a] = (base ? base[size_t(d) * N + a] : 0.0) + acc;
 }
 
-// y[d][a] = sum_i Gp[a N + i] g[d][i] (i in order; Gp is symmetric, so column a is row a).
-__global__ void inv_gram_gemv_kernel(const double* __restrict__ Gp, int32_t N, const double* __restrict__ g,
-                                     double* __restrict__ y) {
+// y[d][a] = sum_i Gp[a N + i] g[d][i] (Gp is symmetric, so column a is row a):
+// one wave per (a, d), lane l takes i = l, l + 64, ..., then wave_sum.
+__global__ __launch_bounds__(256) void inv_gram_gemv_kernel(const double* __restrict__ Gp, int32_t N,
+                                                            const double* __restrict__ g, double* __restrict__ y) {
 #pragma clang fp contract(off)
-    const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t d = blockIdx.y;
-    if (a >= N) return;
+    const int32_t a = int32_t(blockIdx.x * 4 + threadIdx.x / 64);
+    const uint32_t lane = threadIdx.x % 64, d = blockIdx.y;
+    if (a >= N) return;  // whole waves
     const double* col = Gp + size_t(a) * N;
     const double* x = g + size_t(d) * N;
     double t = 0.0;
-    for (int32_t i = 0; i < N; ++i) t += col[i] * x[i];
-    y[size_t(d) * N + a] = t;
+    for (int32_t i = int32_t(lane); i < N; i += 64) t += col[i] * x[i];
+    t = wave_sum(t);
+    if (lane == 0) y[size_t(d) * N + a] = t;
 }
 
 // g = y[a1(n)] - y[a2(n)]: g_arg[d][n] += g (row n of avgData), gacal1[d][n] = -2 pi g.
@@ -465,22 +567,22 @@ void launch_inversion_adjoint(hipStream_t s, const InversionAdjoint& a) {
     if (!a.abar) return;
     inv_garg_init_kernel<<<dim3((nbl + 255) / 256, D), 256, 0, s>>>(a.g_arg, nbl);
     LEAP_HIP(hipGetLastError());
-    const dim3 gn((std::max(1, nI) + 255) / 256, D), ga((a.N + 63) / 64, D);
+    const dim3 gn((std::max(1, nI) + 255) / 256, D), ga((a.N + 3) / 4, D);
     if (nI > 0) {
         inv_garg_res_kernel<<<gn, 256, 0, s>>>(a.res, a.I, nI, nbl, a.g_arg);
         LEAP_HIP(hipGetLastError());
     }
-    inv_antenna_sum_kernel<<<ga, 64, 0, s>>>(a.res, nI, 2.0, a.csr_off, a.csr_ent, a.N, nullptr, a.g_cal);
+    inv_antenna_sum_kernel<<<ga, 256, 0, s>>>(a.res, nI, 2.0, a.csr_off, a.csr_ent, a.N, nullptr, a.g_cal);
     LEAP_HIP(hipGetLastError());
-    inv_gram_gemv_kernel<<<ga, 64, 0, s>>>(a.Gp, a.N, a.g_cal, a.y);
+    inv_gram_gemv_kernel<<<ga, 256, 0, s>>>(a.Gp, a.N, a.g_cal, a.y);
     LEAP_HIP(hipGetLastError());
     if (nI > 0) {
         inv_rows_kernel<<<gn, 256, 0, s>>>(a.y, a.N, a.pairs, nI, nbl, a.g_arg, a.gacal1);
         LEAP_HIP(hipGetLastError());
     }
-    inv_antenna_sum_kernel<<<ga, 64, 0, s>>>(a.gacal1, nI, 1.0, a.csr_off, a.csr_ent, a.N, a.g_cal, a.g_cal1);
+    inv_antenna_sum_kernel<<<ga, 256, 0, s>>>(a.gacal1, nI, 1.0, a.csr_off, a.csr_ent, a.N, a.g_cal, a.g_cal1);
     LEAP_HIP(hipGetLastError());
-    inv_gram_gemv_kernel<<<ga, 64, 0, s>>>(a.G1p, a.N, a.g_cal1, a.y1);
+    inv_gram_gemv_kernel<<<ga, 256, 0, s>>>(a.G1p, a.N, a.g_cal1, a.y1);
     LEAP_HIP(hipGetLastError());
     if (nI1 > 0) {
         inv_rows1_kernel<<<dim3((nI1 + 255) / 256, D), 256, 0, s>>>(a.y1, a.N, a.pairs1, a.I1, nI1, nbl, a.g_arg);
@@ -509,6 +611,29 @@ void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, co
                                                                dkappa, g_uvw, g_dz_partial)
     if (D <= 16)
         LEAP_ADJ(8, 2);
+    else if (D <= 32)
+        LEAP_ADJ(32, 1);
+    else if (D <= 64)
+        LEAP_ADJ(64, 1);
+    else if (D <= 128)
+        LEAP_ADJ(128, 1);
+    else
+        LEAP_ADJ(256, 1);
+#undef LEAP_ADJ
+    LEAP_HIP(hipGetLastError());
+}
+
+void launch_adjoint_q(hipStream_t s, const double* q, const double* uvw, const double* dd, const double* abar,
+                      uint32_t D, uint32_t T, uint32_t nbl, double f0, double df, double* g_uvw,
+                      double* g_dz_partial) {
+    const double kappa0 = f0 / kSpeedOfLight, dkappa = df / kSpeedOfLight;
+    require(D >= 1 && D <= 256, LEAP_ERR_NOT_IMPLEMENTED, "the adjoint handles 1 to 256 directions per call");
+    const dim3 grid(nbl);
+#define LEAP_ADJ(DL_, ND_)                                                                                         \
+    adjoint_q_kernel<DL_, ND_><<<grid, kThreads, 0, s>>>(q, uvw, dd, abar, D, T, nbl, kappa0, dkappa, g_uvw, \
+                                                         g_dz_partial)
+    if (D <= 16)
+        LEAP_ADJ(16, 1);
     else if (D <= 32)
         LEAP_ADJ(32, 1);
     else if (D <= 64)

@@ -216,12 +216,19 @@ struct Horner64Smem {
 
 
 // One (baseline, direction block) tile; every thread of the workgroup calls it.
-template <typename VT, int DB, int ND, int NP, int CC>
+// QOUT (pol 0 only): every row's sums are also written to
+// qsum[row][d] = (e0 P(z), e0 z P'(z)), the second from derivative chains run
+// beside the Horner chains (E' <- E' y + E before E <- E y + v, y = z^2):
+// z P'(z) = 2 y E'(y) + z (O(y) + 2 y O'(y)).  The residual-loss adjoint reads
+// them instead of the cube.
+template <typename VT, int DB, int ND, int NP, int CC, bool QOUT = false>
 __device__ __attribute__((always_inline)) inline void horner64_tile(
     Horner64Smem<DB, ND, NP, CC>& sm, const LEAP_GLOBAL VT* __restrict__ vis, const LEAP_GLOBAL double* __restrict__ uvw,
     const LEAP_GLOBAL double* __restrict__ dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
-    double kappa0, double dkappa, LEAP_GLOBAL double* __restrict__ avg, uint32_t bl, uint32_t dblock) {
+    double kappa0, double dkappa, LEAP_GLOBAL double* __restrict__ avg, uint32_t bl, uint32_t dblock,
+    LEAP_GLOBAL double* __restrict__ qsum = nullptr) {
 #pragma clang fp contract(off)
+    static_assert(!QOUT || NP == 1, "row sums of pol 0");
     constexpr int L = DB / ND;
     constexpr int RP = kThreads / L;
     auto& s_v = sm.v;
@@ -264,6 +271,7 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
         double sf[ND];
         C2<double> z[ND], z2[ND];
         C2<double> he[ND][NP], ho[ND][NP];
+        C2<double> de[QOUT ? ND : 1], dox[QOUT ? ND : 1];  // derivative chains (QOUT)
 #pragma unroll
         for (int j = 0; j < ND; ++j) {
             sf[j] = shift_factor(dz[j], u, v, w);
@@ -273,6 +281,10 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
             for (int q = 0; q < NP; ++q) {
                 he[j][q] = {0.0, 0.0};
                 ho[j][q] = {0.0, 0.0};
+            }
+            if constexpr (QOUT) {
+                de[j] = {0.0, 0.0};
+                dox[j] = {0.0, 0.0};
             }
         }
         for (int ch = int(n_chunks) - 1; ch >= 0; --ch) {
@@ -351,6 +363,10 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
                         const C2<double> ve = s_v[rsub][q][2 * k], vo = s_v[rsub][q][2 * k + 1];
 #pragma unroll
                         for (int j = 0; j < ND; ++j) {
+                            if constexpr (QOUT) {
+                                de[j] = horner_step(de[j], z2[j], he[j][q]);
+                                dox[j] = horner_step(dox[j], z2[j], ho[j][q]);
+                            }
                             he[j][q] = horner_step(he[j][q], z2[j], ve);
                             ho[j][q] = horner_step(ho[j][q], z2[j], vo);
                         }
@@ -360,6 +376,10 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
                         const C2<double> ve = s_v[rsub][q][2 * k], vo = s_v[rsub][q][2 * k + 1];
 #pragma unroll
                         for (int j = 0; j < ND; ++j) {
+                            if constexpr (QOUT) {
+                                de[j] = horner_step(de[j], z2[j], he[j][q]);
+                                dox[j] = horner_step(dox[j], z2[j], ho[j][q]);
+                            }
                             he[j][q] = horner_step(he[j][q], z2[j], ve);
                             ho[j][q] = horner_step(ho[j][q], z2[j], vo);
                         }
@@ -378,6 +398,18 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
                     const C2<double> r = cmul(e0, h);
                     acc[j][q].x += r.x;
                     acc[j][q].y += r.y;
+                    if constexpr (QOUT) {
+                        if (dir_ok[j]) {
+                            const C2<double> ye = cmul(z2[j], de[j]), yo = cmul(z2[j], dox[j]);
+                            const C2<double> zi = cmul(z[j], C2<double>{ho[j][q].x + 2.0 * yo.x, ho[j][q].y + 2.0 * yo.y});
+                            const C2<double> r2 = cmul(e0, C2<double>{2.0 * ye.x + zi.x, 2.0 * ye.y + zi.y});
+                            LEAP_GLOBAL double* qo = qsum + ((size_t(t) * nbl + bl) * D + dir[j]) * 4;
+                            qo[0] = r.x;
+                            qo[1] = r.y;
+                            qo[2] = r2.x;
+                            qo[3] = r2.y;
+                        }
+                    }
                 }
             }
         }
@@ -433,6 +465,18 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64(
     horner64_tile<VT, DB, ND, NP, CC>(sm, (const LEAP_GLOBAL VT*)vis, (const LEAP_GLOBAL double*)uvw,
                                       (const LEAP_GLOBAL double*)dd, D, T, nbl, C, P, kappa0, dkappa,
                                       (LEAP_GLOBAL double*)avg, blockIdx.x, blockIdx.y);
+}
+
+// The residual-loss forward pass: avgData and the per-row sums q (QOUT above).
+template <typename VT, int DB, int ND, int CC>
+__global__ __launch_bounds__(kThreads) void rotate_average_horner64_q(
+    const VT* __restrict__ vis, const double* __restrict__ uvw, const double* __restrict__ dd, uint32_t D, uint32_t T,
+    uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa, double* __restrict__ avg,
+    double* __restrict__ q) {
+    __shared__ Horner64Smem<DB, ND, 1, CC> sm;
+    horner64_tile<VT, DB, ND, 1, CC, true>(sm, (const LEAP_GLOBAL VT*)vis, (const LEAP_GLOBAL double*)uvw,
+                                           (const LEAP_GLOBAL double*)dd, D, T, nbl, C, P, kappa0, dkappa,
+                                           (LEAP_GLOBAL double*)avg, blockIdx.x, blockIdx.y, (LEAP_GLOBAL double*)q);
 }
 
 // Several measurement sets in one persistent launch: workgroup g runs the
@@ -882,6 +926,39 @@ void launch_rotate_batched(hipStream_t s, int32_t precision, int32_t direction_b
         else
             launch_np<double, double>(s, direction_block, all_pols, vis, uvw, dd, D, T, nbl, nbl_grid, C, P, kappa0, dkappa, avg);
     }
+}
+
+void launch_rotate_forward_q(hipStream_t s, int32_t tile, const void* vis, int32_t vis_dtype, const double* uvw,
+                             const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
+                             double df, double* avg, double* q) {
+    if (nbl == 0 || D == 0) return;
+    const double kappa0 = f0 / kSpeedOfLight;
+    const double dkappa = df / kSpeedOfLight;
+    const bool c64 = vis_dtype == LEAP_VIS_C64;
+#define LEAP_FQ(DB_, ND_)                                                                                         \
+    case DB_ * 10 + ND_: {                                                                                        \
+        constexpr int RP = kThreads / (DB_ / ND_);                                                                \
+        constexpr int CC = 2048 / RP < 256 ? 2048 / RP : 256;                                                     \
+        const dim3 grid(nbl, (D + DB_ - 1) / DB_);                                                                \
+        if (c64)                                                                                                  \
+            rotate_average_horner64_q<float, DB_, ND_, CC><<<grid, kThreads, 0, s>>>(                             \
+                static_cast<const float*>(vis), uvw, dd, D, T, nbl, C, P, kappa0, dkappa, avg, q);                \
+        else                                                                                                      \
+            rotate_average_horner64_q<double, DB_, ND_, CC><<<grid, kThreads, 0, s>>>(                            \
+                static_cast<const double*>(vis), uvw, dd, D, T, nbl, C, P, kappa0, dkappa, avg, q);               \
+        LEAP_HIP(hipGetLastError());                                                                              \
+        return;                                                                                                   \
+    }
+    switch (tile) {
+        LEAP_FQ(16, 2)
+        LEAP_FQ(32, 1)
+        LEAP_FQ(64, 1)
+        LEAP_FQ(128, 2)
+        LEAP_FQ(256, 2)
+        default: break;
+    }
+#undef LEAP_FQ
+    throw Error(LEAP_ERR_INVALID_ARGUMENT, "unsupported forward tile " + std::to_string(tile));
 }
 
 }  // namespace leap

@@ -256,6 +256,10 @@ struct leap_ctx {
     // pol 0 of avgData with its arguments; the inversion adjoint's intermediates; the
     // per-direction losses and offset gradients copied back (pinned)
     leap::DeviceBuffer d_adj_avg0, d_adj_inv;
+    // the forward pass's per-row sums (e0 P, e0 z P') for the adjoint, and the key
+    // (antennas of every baseline, antenna count) the baseline tables were built for
+    leap::DeviceBuffer d_adj_q;
+    uint64_t adj_ants_key = 0;
     leap::PinnedBuffer h_adj_out;
     hipEvent_t ev_grad[4] = {nullptr, nullptr, nullptr, nullptr};  // around the forward and adjoint kernels
     double last_grad_forward_ms = 0.0, last_grad_adjoint_ms = 0.0;
@@ -420,6 +424,16 @@ void launch_perturb_uvw(hipStream_t s, const double* uvw0, const double* M, cons
 void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, const double* uvw, const double* dd,
                            const double* abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
                            double df, double* g_uvw, double* g_dz_partial);
+// The residual-loss forward pass that also keeps each row's sums
+// q[row][d] = (e0 P(z), e0 z P'(z)) (4 doubles; kernels_batched.hip), pol 0 of
+// avgData as launch_rotate_batched in FP64; tiles 162, 321, 641, 1282, 2562.
+void launch_rotate_forward_q(hipStream_t s, int32_t tile, const void* vis, int32_t vis_dtype, const double* uvw,
+                             const double* dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
+                             double df, double* avg, double* q);
+// launch_rotate_adjoint's outputs from those row sums (no pass over the cube).
+void launch_adjoint_q(hipStream_t s, const double* q, const double* uvw, const double* dd, const double* abar,
+                      uint32_t D, uint32_t T, uint32_t nbl, double f0, double df, double* g_uvw,
+                      double* g_dz_partial);
 // g_dz[d][k] (3 D doubles at out) reduced over baselines, then dL/d(ra, dec) of each
 // direction (2 D doubles at out + 3 D); dirs = the directions (ra, dec) on the device.
 void launch_reduce_gdz_offsets(hipStream_t s, const double* part, uint32_t nbl, uint32_t D, double ra0, double dec0,
