@@ -488,7 +488,9 @@ def run_adjoint(args):
     autocorrelations (8128 baselines), 64 timesteps of 2 s, 192 channels, complex64 cube generated
     on the device, 16 directions, direction offsets N(0, 1e-4 rad), antenna position corrections
     N(0, 0.01 m).  A step is one forward (rotate, average, inversion, loss) and backward (adjoint
-    of the inversion, then of rotate/average with the phasors recomputed) pass."""
+    of the inversion, then of rotate/average) pass, in the library's automatic adjoint mode (here
+    the forward pass keeps each row's channel sums and the adjoint reads them); the same steps with
+    the adjoint recomputing the phasors from the cube are timed beside it (config.recompute)."""
     cfg = dict(n_antennas=128, autocorrelations=0, n_timesteps=64, n_channels=192, seed=4)
     D = args.directions or 16
     synth = la.SynthMS(**cfg)
@@ -521,7 +523,18 @@ def run_adjoint(args):
     for _ in range(steps):
         md.loss_grad(view, dirs, M, off, dp, want_grad=False)
     fwd = time.perf_counter() - t
-    log(f"[adjoint] {steps} timed steps in {elapsed:.3f} s (forward only {fwd:.3f} s)")
+    ctx.set_adjoint_mode(la.LEAP_ADJOINT_RECOMPUTE)
+    rf_k, ra_k = [], []
+    t = time.perf_counter()
+    for _ in range(steps):
+        md.loss_grad(view, dirs, M, off, dp)
+        f_ms, a_ms = ctx.last_loss_grad_times()
+        rf_k.append(f_ms)
+        ra_k.append(a_ms)
+    rec = time.perf_counter() - t
+    ctx.set_adjoint_mode(la.LEAP_ADJOINT_AUTO)
+    log(f"[adjoint] {steps} timed steps in {elapsed:.3f} s (forward only {fwd:.3f} s, recomputing adjoint "
+        f"{rec:.3f} s)")
     cpu = None
     if args.cpu_timesteps > 0:
         sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -585,7 +598,12 @@ def run_adjoint(args):
                                "128 stations / 8128 baselines (no autos) / 64 t / 192 ch / 4 pol, complex64 cube "
                                "resident in HBM",
                    "stations": 128, "baselines": nbl, "timesteps": 64, "channels": C, "pols": P,
-                   "directions": D, "forward_only_ms": round(fwd / steps * 1e3, 3)},
+                   "directions": D, "forward_only_ms": round(fwd / steps * 1e3, 3),
+                   "adjoint_mode": "automatic: row sums kept by the forward pass",
+                   "recompute": {"ms_per_step": round(rec / steps * 1e3, 3),
+                                 "forward_kernel_ms": round(float(np.median(rf_k)), 4),
+                                 "adjoint_kernel_ms": round(float(np.median(ra_k)), 4),
+                                 "note": "the adjoint reads the cube again and recomputes the phasors, P and P'"}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

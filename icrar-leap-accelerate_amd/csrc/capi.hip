@@ -1872,6 +1872,15 @@ void adjoint_antenna_tables(leap_ctx* ctx, const leap_ms_view& ms, int32_t N) {
 // inversion, the loss, the inversion's adjoint and the adjoint of
 // rotate/average, all on the device; the host sums the per-direction losses.
 // ---------------------------------------------------------------------------
+extern "C" int leap_hip_set_adjoint_mode(leap_ctx* ctx, int32_t mode) {
+    return guarded(ctx, [&] {
+        require(ctx != nullptr, LEAP_ERR_INVALID_ARGUMENT, "null context");
+        require(mode == LEAP_ADJOINT_AUTO || mode == LEAP_ADJOINT_RECOMPUTE || mode == LEAP_ADJOINT_ROW_SUMS,
+                LEAP_ERR_INVALID_ARGUMENT, "adjoint mode must be 0 (automatic), 1 (recompute) or 2 (row sums)");
+        ctx->adjoint_mode = mode;
+    });
+}
+
 extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
                                            const double* directions, const double* offsets, uint32_t D,
                                            const double* antenna_delta, const double* enu_to_uvw, double* loss_out,
@@ -1887,8 +1896,11 @@ extern "C" int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* m
         const int32_t N = md->A.cols;
         const bool want_grad = grad_offsets || grad_antenna;
         // With a gradient the forward pass keeps each row's sums for the adjoint
-        // (32 bytes per row and direction) unless they outweigh the cube itself.
-        const bool row_sums = want_grad && size_t(32) * D <= vis_elem_bytes(*ms) * md->channels;
+        // (32 bytes per row and direction) unless they outweigh the cube itself
+        // or the context asks for the recomputing adjoint.
+        const bool row_sums = want_grad && ctx->adjoint_mode != LEAP_ADJOINT_RECOMPUTE &&
+                              (ctx->adjoint_mode == LEAP_ADJOINT_ROW_SUMS ||
+                               size_t(32) * D <= vis_elem_bytes(*ms) * md->channels);
         if (antenna_delta || grad_antenna) adjoint_antenna_tables(ctx, *ms, N);
         std::vector<double> dirs(size_t(D) * 2);
         for (uint32_t d = 0; d < 2 * D; ++d) dirs[d] = directions[d] + (offsets ? offsets[d] : 0.0);

@@ -260,6 +260,7 @@ struct leap_ctx {
     // (antennas of every baseline, antenna count) the baseline tables were built for
     leap::DeviceBuffer d_adj_q;
     uint64_t adj_ants_key = 0;
+    int32_t adjoint_mode = LEAP_ADJOINT_AUTO;
     leap::PinnedBuffer h_adj_out;
     hipEvent_t ev_grad[4] = {nullptr, nullptr, nullptr, nullptr};  // around the forward and adjoint kernels
     double last_grad_forward_ms = 0.0, last_grad_adjoint_ms = 0.0;

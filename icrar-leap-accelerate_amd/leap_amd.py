@@ -25,6 +25,9 @@ LEAP_CHANNEL_SUM_NUFFT = 2
 LEAP_CHANNEL_SUM_NUFFT_FP32 = 3
 LEAP_STRATEGY_PER_DIRECTION = 0
 LEAP_STRATEGY_BATCHED = 1
+LEAP_ADJOINT_AUTO = 0
+LEAP_ADJOINT_RECOMPUTE = 1
+LEAP_ADJOINT_ROW_SUMS = 2
 
 _STATUS = {
     -1: "invalid argument",
@@ -177,6 +180,7 @@ EXPORTS = [
     "leap_hip_last_kernel_times", "leap_hip_last_direction_block", "leap_hip_last_route",
     "leap_hip_last_workgroups_per_cu",
     "leap_hip_last_graph", "leap_hip_last_pack_times", "leap_hip_last_staged_bytes", "leap_hip_last_loss_grad_times",
+    "leap_hip_set_adjoint_mode",
     "leap_hip_tuning_save", "leap_hip_tuning_load", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
     "leap_multi_item_cost", "leap_phasor64", "leap_nufft_plan", "leap_nufft_plan32",
@@ -260,6 +264,7 @@ def lib():
     L.leap_hip_last_route.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.leap_hip_last_workgroups_per_cu.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_loss_grad_times.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.leap_hip_set_adjoint_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.leap_hip_last_graph.argtypes = [ctypes.c_void_p]
     L.leap_hip_last_pack_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int32)]
     L.leap_hip_last_staged_bytes.argtypes = [ctypes.c_void_p]
@@ -635,6 +640,10 @@ class Context:
 
     def last_workgroups_per_cu(self):
         return int(lib().leap_hip_last_workgroups_per_cu(self.h))
+
+    def set_adjoint_mode(self, mode):
+        """LEAP_ADJOINT_AUTO / _RECOMPUTE / _ROW_SUMS for later loss_grad calls."""
+        self._chk(lib().leap_hip_set_adjoint_mode(self.h, int(mode)))
 
     def last_loss_grad_times(self):
         """(forward kernel ms, adjoint kernel ms) of the last loss_grad call."""

@@ -333,11 +333,25 @@ int leap_hip_dd_matrix(leap_ctx* ctx, double phase_centre_ra, double phase_centr
  *   direction = directions[d] + offsets[d]                      (ra, dec radians)
  *   loss      = sum_d sum_{n < K-1} wrap((A cal_d)(n) - arg avgData_d(I(n)))^2
  * with avgData and cal computed in FP64 as leap_hip_calibrate does (pol 0).
- * The gradient runs a device adjoint of rotate/average that recomputes the
- * phasors (Horner's rule for the sum and its derivative) and the adjoint of
- * arg and the pseudo-inverse products on the host.  offsets (D x 2),
+ * Everything runs on the device: the forward rotate/average and inversion,
+ * the loss, the adjoint of arg and of the pseudo-inverse products, and the
+ * adjoint of rotate/average as leap_hip_set_adjoint_mode selects; one
+ * synchronisation per call.  offsets (D x 2),
  * antenna_delta (N x 3, metres east/north/up) may be NULL (zero); grad_offsets
  * (D x 2) and grad_antenna (N x 3) may be NULL.  1 <= D <= 256. */
+/* How leap_hip_residual_loss_grad forms the adjoint of rotate/average:
+ *   LEAP_ADJOINT_AUTO (the default): as ROW_SUMS when the row sums take no more
+ *     memory than the visibility cube (32 D <= channels x pols x element bytes),
+ *     else as RECOMPUTE;
+ *   LEAP_ADJOINT_RECOMPUTE: the forward pass keeps nothing; the adjoint reads
+ *     the cube again and recomputes the phasors, the channel sum P(z) and its
+ *     derivative by Horner's rule;
+ *   LEAP_ADJOINT_ROW_SUMS: the forward pass also keeps each row's channel sums
+ *     e0 P(z) and e0 z P'(z) (32 bytes per row and direction, no phasors) and
+ *     the adjoint reads them instead of the cube.
+ * The result is the same to rounding.  Applies to later calls on ctx. */
+enum { LEAP_ADJOINT_AUTO = 0, LEAP_ADJOINT_RECOMPUTE = 1, LEAP_ADJOINT_ROW_SUMS = 2 };
+int leap_hip_set_adjoint_mode(leap_ctx* ctx, int32_t mode);
 int leap_hip_residual_loss_grad(leap_ctx* ctx, const leap_metadata* md, const leap_ms_view* ms,
                                 const double* directions, const double* offsets, uint32_t n_directions,
                                 const double* antenna_delta, const double* enu_to_uvw, double* loss,
