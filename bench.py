@@ -138,6 +138,84 @@ def run_c0(args):
     return out
 
 
+# The uneven batch of three sets: antennas, autocorrelations, timesteps, channels.
+UNEVEN = [(36, 0, 32, 300), (128, 1, 14, 48), (256, 1, 64, 96)]
+
+
+def run_c2(args):
+    """Batching in FP64 and the uneven batch: 64 directions (offsets up to 0.1 rad) on the 256-station
+    complex128 cube captured as one HIP graph and replayed each step; then the three uneven sets in one
+    call per step, in FP64 (Horner tile, captured) and in mixed precision on their complex64 cubes
+    (the automatic route: the NUFFT for the 300- and 96-channel sets, Horner for the 48-channel one)."""
+    steps = args.steps
+    out = {}
+    synth = la.SynthMS(**c1_config())
+    R, C, P = synth.num_rows, synth.num_channels, synth.num_pols
+    ctx = la.Context(0)
+    vis_ptr = ctx.malloc(R * C * P * 16)
+    uvw_ptr = ctx.malloc(R * 24)
+    ctx.synth_fill_device(synth, la.LEAP_VIS_C128, vis_ptr, uvw_ptr)
+    view = la.view_from_arrays(synth.host_dict(), vis_device=vis_ptr, vis_dtype=la.LEAP_VIS_C128, uvw_device=uvw_ptr)
+    md = ctx.metadata(view)
+    dirs = directions((view.phase_centre_ra_rad, view.phase_centre_dec_rad), 64, seed=13, spread=0.1)
+    opts = la.options(strategy=la.LEAP_STRATEGY_BATCHED, graph=1)
+    for _ in range(max(1, args.warmup)):
+        ctx.calibrate(view, dirs, md=md, opts=opts)
+    step_s = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        ctx.calibrate(view, dirs, md=md, opts=opts)
+        step_s.append(time.perf_counter() - t)
+    replayed = ctx.last_graph() == 2
+    out["single_set"] = {"workload": "256 stations / 32896 baselines / 128 t / 384 ch / 4 pol, complex128 cube "
+                                     "103.5 GB in HBM, 64 directions per step, FP64, one HIP graph replayed",
+                         "graph_replayed": replayed, "tile": ctx.last_direction_block(),
+                         "ms_per_step_median": round(float(np.median(step_s)) * 1e3, 3),
+                         "directions_per_s": round(64 * steps / sum(step_s), 1)}
+    log(f"[c2] 64 FP64 directions, graph: {out['single_set']['ms_per_step_median']} ms per step")
+    ctx.free(vis_ptr)
+    ctx.free(uvw_ptr)
+    # the uneven batch, complex128 for FP64 and complex64 for mixed precision
+    ptrs = []
+    for name, prec, vd, elem in (("fp64", 0, la.LEAP_VIS_C128, 16), ("mixed", 1, la.LEAP_VIS_C64, 8)):
+        batch = []
+        for i, (n_ant, autos, T, Cs) in enumerate(UNEVEN):
+            s = la.SynthMS(n_antennas=n_ant, autocorrelations=autos, n_timesteps=T, n_channels=Cs,
+                           freq_start_hz=50e6, freq_inc_hz=781.25e3 * 384 / Cs, seed=60 + i)
+            vp = ctx.malloc(s.num_rows * s.num_channels * s.num_pols * elem)
+            up = ctx.malloc(s.num_rows * 24)
+            ptrs += [vp, up]
+            ctx.synth_fill_device(s, vd, vp, up)
+            v = la.view_from_arrays(s.host_dict(), vis_device=vp, vis_dtype=vd, uvw_device=up)
+            batch.append((ctx.metadata(v), v, directions((v.phase_centre_ra_rad, v.phase_centre_dec_rad), 64,
+                                                          seed=70 + i, spread=0.1), (0, -1, -1)))
+        bopts = la.options(precision=prec, strategy=la.LEAP_STRATEGY_BATCHED, graph=1 if prec == 0 else 0)
+        for _ in range(max(1, args.warmup)):
+            ctx.calibrate_batch(batch, bopts)
+        step_s, k_ms = [], []
+        for _ in range(steps):
+            t = time.perf_counter()
+            ctx.calibrate_batch(batch, bopts)
+            step_s.append(time.perf_counter() - t)
+            k_ms.append(ctx.last_kernel_times()[0])
+        route = ctx.last_route()[0]
+        out[f"uneven_batch_{name}"] = {
+            "workload": "36 antennas / 630 baselines / 300 ch / 32 t; 128 / 8256 / 48 / 14; 256 / 32896 / 96 / 64; "
+                        f"64 directions each, {'complex128' if elem == 16 else 'complex64'} cubes in HBM, one call "
+                        "per step" + (" captured as a HIP graph" if prec == 0 else ""),
+            "ms_per_step_median": round(float(np.median(step_s)) * 1e3, 3),
+            "rotate_kernel_ms": None if prec == 0 else round(float(np.median(k_ms)), 4),
+            "directions_per_s": round(3 * 64 * steps / sum(step_s), 1),
+            "channel_sums": "FP64 Horner" if prec == 0 else
+                            ("NUFFT (300- and 96-channel sets) + Horner (48 channels)" if route == 2 else "Horner")}
+        log(f"[c2] uneven batch {name}: {out[f'uneven_batch_{name}']['ms_per_step_median']} ms per step")
+        for p_ in ptrs:
+            ctx.free(p_)
+        ptrs = []
+    ctx.close()
+    return out
+
+
 def cpu_baseline(ctx, synth, vis_ptr, vis_dtype, timesteps):
     """The oracle (single thread) on the first `timesteps` timesteps of the same cube, scaled in T."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -234,6 +312,8 @@ def main():
                     help="batched-mixed: skip the per-direction error against the plain FP64 path")
     ap.add_argument("--no-c0", dest="c0", action="store_false",
                     help="headline: skip the 128-antenna set timed in full on the oracle and the GPU")
+    ap.add_argument("--no-c2", dest="c2", action="store_false",
+                    help="headline: skip 64 FP64 directions under graph capture and the uneven three-set batch")
     ap.add_argument("--cpu-timesteps", type=int, default=2, help="timesteps in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--copy-probe-gb", type=float, default=0.0,
                     help="only run device-to-device copies of this size (calibrates rocprofv3 FETCH_SIZE / "
@@ -261,6 +341,8 @@ def main():
         out["config"]["speedup_vs_plain_fp64"] = round(mixed["value"] / plain["value"], 2)
         if args.c0:
             out["config"]["c0"] = run_c0(args)
+        if args.c2:
+            out["config"]["c2"] = run_c2(args)
         out["config"]["workload"] = ("batched mixed-precision calibration (complex64 cube 51.7 GB resident in "
                                      f"HBM, {out['config']['directions_per_step']} directions per step, one pass "
                                      "over the cube) vs plain FP64 per-direction calibration (complex128 cube "
