@@ -33,7 +33,9 @@
 namespace leap {
 namespace {
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 using namespace dev;
 
@@ -190,6 +192,206 @@ __global__ __launch_bounds__(kThreads) void rotate_average_batched(
     }
 }
 
+// One (row, channel) element of a cube with PP pols, read as vector units:
+// a 16-byte unit per complex128 pol, pairs of complex64 pols as 16-byte units
+// (8-byte units for an odd pol count).
+template <typename VT, int PP>
+struct StageUnit {
+    typedef std::conditional_t<sizeof(VT) == 8, f64x2, std::conditional_t<PP % 2 == 0, f32x4, f32x2>> T;
+    static constexpr int K = sizeof(T) / sizeof(VT);  // scalars per unit
+    static constexpr int N = 2 * PP / K;               // units per element
+};
+
+// Bytes of raw element data a thread holds in flight while staging a chunk.
+constexpr int kStageBytes = 64;
+
+// Stages the pol-0 values of a chunk's RP x CC (row, channel) elements into
+// LDS as FP64 complex, 0 where the element lies past the rows (tb + r >= T)
+// or channels (c >= cn) or any of its pols is not finite.  Thread i takes
+// elements i, i + 256, ...; they are taken EB at a time, and every pol of the
+// EB elements is loaded before any is checked, so all their loads are in
+// flight together (EB * 2 * PP * sizeof(VT) <= kStageBytes where the chunk
+// allows).  Slots past the rows or channels read a valid element (row 0 of the
+// baseline, channel 0) and store zero.
+template <typename VT, int PP, int RP, int CC, typename SV>
+__device__ __attribute__((always_inline)) inline void stage_pol0(SV& s_v, const LEAP_GLOBAL VT* __restrict__ vis,
+                                                                 uint32_t tb, uint32_t T, uint32_t nbl, uint32_t bl,
+                                                                 uint32_t C, uint32_t c0, uint32_t cn) {
+    using SU = StageUnit<VT, PP>;
+    using U = typename SU::T;
+    constexpr int E = RP * CC / kThreads;
+    static_assert(RP * CC % kThreads == 0, "whole elements per thread");
+    // EB: the largest power of two <= E whose batch fits kStageBytes (E is a power of two)
+    constexpr int fit = kStageBytes / int(2 * PP * sizeof(VT));
+    constexpr int EB0 = fit >= 8 ? 8 : fit >= 4 ? 4 : fit >= 2 ? 2 : 1;
+    constexpr int EB = EB0 > E ? E : EB0;
+    static_assert((E & (E - 1)) == 0 && E % EB == 0, "whole batches");
+    // One batch at a time: a rolled loop keeps the next batch's loads from being
+    // hoisted over this one (registers for one batch only).
+#pragma unroll 1
+    for (int e0 = 0; e0 < E; e0 += EB) {
+        U raw[EB][SU::N];
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+            const uint32_t i = threadIdx.x + kThreads * (e0 + e);
+            const uint32_t r = i / CC, c = i - r * CC;
+            const bool ok = tb + r < T && c < cn;
+            const size_t el = ok ? (size_t(tb + r) * nbl + bl) * C + c0 + c : size_t(bl) * C;
+            const LEAP_GLOBAL U* p = reinterpret_cast<const LEAP_GLOBAL U*>(vis + el * (2 * PP));
+#pragma unroll
+            for (int k = 0; k < SU::N; ++k) raw[e][k] = p[k];
+        }
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+            const uint32_t i = threadIdx.x + kThreads * (e0 + e);
+            const uint32_t r = i / CC, c = i - r * CC;
+            const bool ok = tb + r < T && c < cn;
+            bool finite = true;
+#pragma unroll
+            for (int k = 0; k < SU::N; ++k)
+#pragma unroll
+                for (int m = 0; m < SU::K; ++m) finite = finite && __builtin_isfinite(raw[e][k][m]);
+            s_v[r][0][c] = (ok && finite) ? C2<double>{double(raw[e][0][0]), double(raw[e][0][1])}
+                                          : C2<double>{0.0, 0.0};
+        }
+    }
+}
+
+// s_waitcnt vmcnt(0) with the export and LDS/scalar counters left free
+// (gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15).
+constexpr int kWaitVectorMem = 0x0F70;
+
+#define LEAP_LDS __attribute__((address_space(3)))
+
+// Channels per LDS chunk of the FP64 Horner tiles with RP rows in flight:
+// pol-0 tiles take 256 (row, channel) elements per chunk (at least 16
+// channels), so the raw copy of a chunk (all pols, up to 64 bytes an element)
+// is at most 16 KB, with kDmaSlots of them in flight; tiles keeping every pol
+// take 64 channels.  The recurrences run over the chunks in order, so the
+// chunk size does not change the result.
+constexpr int kDmaElems = 256;  // (row, channel) elements per pol-0 chunk
+constexpr int kDmaSlots = 2;    // chunks copied ahead
+constexpr int horner64_chunk(int RP, int NP) {
+    return NP == 1 ? (kDmaElems / RP > 16 ? kDmaElems / RP : 16) : 64;
+}
+
+// Raw copy of one chunk, as the cube holds it, in 16-byte units.
+template <int BYTES>
+struct Horner64Raw {
+    f32x4 u[BYTES / 16];
+};
+
+// Bytes of the raw copy of a pol-0 tile's chunk for components of ES bytes
+// (up to 4 pols); tiles keeping every pol stage through registers.
+constexpr int horner64_raw_bytes(int RP, int CC, int NP, int ES) { return NP == 1 ? kDmaSlots * RP * CC * 8 * ES : 16; }
+
+// s_waitcnt vmcnt(n) (the export and LDS/scalar counters left free) for the
+// copies one chunk issues per wave (a larger n waits as for 8, which is safe).
+__device__ __attribute__((always_inline)) inline void wait_vmem_but(uint32_t n) {
+    switch (n) {
+        case 0: __builtin_amdgcn_s_waitcnt(kWaitVectorMem); break;
+        case 1: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 1); break;
+        case 2: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 2); break;
+        case 3: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 3); break;
+        case 4: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 4); break;
+        case 5: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 5); break;
+        case 6: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 6); break;
+        case 7: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 7); break;
+        default: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 8); break;
+    }
+}
+
+// The per-tile part of a chunk copy.  Thread tid copies units tid + 256 k
+// (k < K) of the chunk; with urow = CC EB / 16 units per row a power of two,
+// unit g lies in row g >> lu at unit g & (urow - 1), so its cube address is
+// the chunk's first byte + (tid >> lu) rowbytes + 16 (tid & (urow - 1)) (the
+// lane's part, fixed for the tile) + the uniform part of 256 k.
+struct DmaLane {
+    uint64_t rowbytes;  // bytes between timesteps of a baseline: nbl C EB
+    uint64_t lane_off;  // the lane's part (pow2 only)
+    uint32_t urow;      // units per row
+    int32_t lu;         // log2(urow), or -1 when urow is not a power of two
+};
+
+__device__ __attribute__((always_inline)) inline DmaLane dma_lane(uint32_t CCEB16, uint32_t nbl, uint32_t C,
+                                                                  uint32_t EB) {
+    DmaLane d;
+    d.urow = CCEB16;
+    d.rowbytes = uint64_t(nbl) * C * EB;
+    d.lu = (d.urow & (d.urow - 1)) == 0 ? int32_t(__builtin_ctz(d.urow)) : -1;
+    const uint32_t tid = threadIdx.x;
+    d.lane_off = d.lu >= 0 ? uint64_t(tid >> d.lu) * d.rowbytes + 16u * (tid & (d.urow - 1)) : 0;
+    return d;
+}
+
+// Starts the copy of rows [tb, tb + RP) x channels [c0, c0 + CC) of the cube
+// into raw with global_load_lds_dwordx4 (lane i of a wave writes the 16 bytes
+// at the wave's LDS address + 16 i), row r at byte r * CC * EB.  EB, the bytes
+// of an element, is a multiple of 16 and the cube 16-byte aligned.  Rows past
+// T read row tb, units past the chunk's cn channels the row's first unit: the
+// conversion zeroes those slots.  Whole chunks of a power-of-two row take one
+// address add per copy; the rest divide per unit.  The copy has landed once
+// every thread has waited for vmcnt(0) and passed a barrier.
+template <int RP, int CC>
+__device__ __attribute__((always_inline)) inline void dma_chunk(LEAP_LDS f32x4* raw,
+                                                                const LEAP_GLOBAL unsigned char* vis, uint32_t EB,
+                                                                const DmaLane& dl, uint32_t tb, uint32_t T,
+                                                                uint32_t nbl, uint32_t bl, uint32_t C, uint32_t c0,
+                                                                uint32_t cn) {
+    static_assert(RP * CC % 64 == 0, "whole waves of units");
+    const uint32_t urow = dl.urow;
+    const uint32_t total = RP * urow;  // a multiple of 256
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const LEAP_GLOBAL unsigned char* first = vis + ((size_t(tb) * nbl + bl) * C + c0) * EB;
+    if (dl.lu >= 0 && tb + RP <= T && cn == uint32_t(CC)) {
+        const LEAP_GLOBAL unsigned char* lane_first = first + dl.lane_off;
+        for (uint32_t q = 0; q < total; q += kThreads) {
+            const uint64_t step = uint64_t(q >> dl.lu) * dl.rowbytes + 16u * (q & (urow - 1));  // uniform
+            __builtin_amdgcn_global_load_lds((const LEAP_GLOBAL void*)(lane_first + step),
+                                             (LEAP_LDS void*)(raw + q + 64 * wave), 16, 0, 0);
+        }
+    } else {
+        const uint32_t uvalid = cn * (EB / 16);
+        const uint32_t lane = threadIdx.x % 64;
+        for (uint32_t g0 = wave * 64; g0 < total; g0 += kThreads) {
+            const uint32_t g = g0 + lane;
+            const uint32_t r = g / urow, u = g - r * urow;
+            const size_t row = size_t(tb + r < T ? tb + r : tb) * nbl + bl;
+            const LEAP_GLOBAL unsigned char* src = vis + (row * C + c0) * EB + size_t(u < uvalid ? u : 0) * 16;
+            __builtin_amdgcn_global_load_lds((const LEAP_GLOBAL void*)src, (LEAP_LDS void*)(raw + g0), 16, 0, 0);
+        }
+    }
+    // keep the copies issued here, ahead of the channel loop that follows
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Converts a chunk copied by dma_chunk to pol-0 FP64 values in s_v: element
+// (r, c) is unit range [(r CC + c) NU, + NU) of raw; 0 past the rows or
+// channels or where any of its pols is not finite.
+template <typename VT, int PP, int RP, int CC, typename SV>
+__device__ __attribute__((always_inline)) inline void convert_chunk(SV& s_v, const LEAP_LDS f32x4* raw, uint32_t tb,
+                                                                    uint32_t T, uint32_t cn) {
+    using SU = StageUnit<VT, PP>;
+    using U = typename SU::T;
+    static_assert(sizeof(U) == 16 && RP * CC % kThreads == 0, "16-byte units, whole elements per thread");
+    const LEAP_LDS U* x = reinterpret_cast<const LEAP_LDS U*>(raw);
+#pragma unroll
+    for (int e = 0; e < RP * CC / kThreads; ++e) {
+        const uint32_t i = threadIdx.x + kThreads * e;
+        const uint32_t r = i / CC, c = i - r * CC;
+        U v[SU::N];
+#pragma unroll
+        for (int k = 0; k < SU::N; ++k) v[k] = x[i * SU::N + k];
+        bool finite = true;
+#pragma unroll
+        for (int k = 0; k < SU::N; ++k)
+#pragma unroll
+            for (int m = 0; m < SU::K; ++m) finite = finite && __builtin_isfinite(v[k][m]);
+        const bool ok = tb + r < T && c < cn;
+        s_v[r][0][c] = (ok && finite) ? C2<double>{double(v[0][0]), double(v[0][1])} : C2<double>{0.0, 0.0};
+    }
+}
+
 // Full-row FP64 Horner: per (row, direction) one exact phasor exp(i sf kappa0)
 // and the step z = exp(i sf dkappa) are computed with FP64 sincos, and the
 // whole channel axis is evaluated as P(z) = P_even(z^2) + z P_odd(z^2) by two
@@ -210,8 +412,10 @@ struct Horner64Smem {
     static_assert(DB % ND == 0 && kThreads % (DB / ND) == 0, "lane layout");
     static constexpr int L = DB / ND;
     static constexpr int RP = kThreads / L;
+    typedef double Red[(RP > 1) ? RP : 1][L][ND][NP][2];  // per-row sums of each direction
     C2<double> v[RP][NP][CC];
-    double red[(RP > 1) ? RP : 1][L][ND][NP][2];
+    // pol-0 tiles combine their row sums in the raw chunk buffer, idle once the last chunk is in
+    double red[NP == 1 ? 1 : sizeof(Red) / sizeof(double)];
 };
 
 
@@ -221,9 +425,15 @@ struct Horner64Smem {
 // beside the Horner chains (E' <- E' y + E before E <- E y + v, y = z^2):
 // z P'(z) = 2 y E'(y) + z (O(y) + 2 y O'(y)).  The residual-loss adjoint reads
 // them instead of the cube.
-template <typename VT, int DB, int ND, int NP, int CC, bool QOUT = false>
+// Pol-0 tiles on a cube of 16-byte-multiple elements (complex128, or complex64
+// with 2 or 4 pols) at a 16-byte-aligned address copy chunks into a ring of
+// kDmaSlots raw buffers with dma_chunk, kDmaSlots chunks ahead of the chains,
+// so the cube streams in behind the arithmetic and takes no registers; other
+// cubes are staged through registers (stage_pol0).  The pol-0 tiles combine
+// their row sums in the raw buffer, which is idle once the last chunk is in.
+template <typename VT, int DB, int ND, int NP, int CC, bool QOUT = false, int RB = 16>
 __device__ __attribute__((always_inline)) inline void horner64_tile(
-    Horner64Smem<DB, ND, NP, CC>& sm, const LEAP_GLOBAL VT* __restrict__ vis, const LEAP_GLOBAL double* __restrict__ uvw,
+    Horner64Smem<DB, ND, NP, CC>& sm, Horner64Raw<RB>& sraw, const LEAP_GLOBAL VT* __restrict__ vis, const LEAP_GLOBAL double* __restrict__ uvw,
     const LEAP_GLOBAL double* __restrict__ dd, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P,
     double kappa0, double dkappa, LEAP_GLOBAL double* __restrict__ avg, uint32_t bl, uint32_t dblock,
     LEAP_GLOBAL double* __restrict__ qsum = nullptr) {
@@ -232,7 +442,9 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
     constexpr int L = DB / ND;
     constexpr int RP = kThreads / L;
     auto& s_v = sm.v;
-    auto& s_red = sm.red;
+    using Red = typename Horner64Smem<DB, ND, NP, CC>::Red;
+    static_assert(NP > 1 || sizeof(Red) <= RB, "row sums fit the raw buffer");
+    Red& s_red = *reinterpret_cast<Red*>(NP == 1 ? static_cast<void*>(&sraw.u[0]) : static_cast<void*>(&sm.red[0]));
 
     const uint32_t lane_d = threadIdx.x % L;
     const uint32_t rsub = threadIdx.x / L;
@@ -251,6 +463,42 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
         dz[j][2] = dd[9 * dc + 8];
     }
     const uint32_t n_chunks = (C + CC - 1) / CC;
+
+    // DMA staging: chunks are copied kDmaSlots ahead into a ring of raw slots
+    // (each sized for 4 pols), consumed in order.
+    const uint32_t EB = 2 * P * uint32_t(sizeof(VT));
+    constexpr uint32_t kSlotUnits = uint32_t(RP) * CC * 8 * sizeof(VT) / 16;
+    bool dma = false;
+    if constexpr (NP == 1) {
+        dma = EB % 16 == 0 && P <= 4 && (reinterpret_cast<uintptr_t>(vis) & 15) == 0 &&
+              size_t(kDmaSlots) * kSlotUnits * 16 <= sizeof(sraw.u);
+    }
+    // address-space cast of the shared buffer: the copies take an LDS pointer
+    LEAP_LDS f32x4* raw = (LEAP_LDS f32x4*)(&sraw.u[0]);
+    const LEAP_GLOBAL unsigned char* visb = reinterpret_cast<const LEAP_GLOBAL unsigned char*>(vis);
+    // copies per wave and chunk (RP * CC * EB / 16 units in waves of 64, over 4 waves)
+    const uint32_t dma_per_wave = uint32_t(RP) * CC * (EB / 16) / kThreads;
+    const DmaLane dl = dma_lane(uint32_t(CC) * (EB / 16), nbl, C, EB);
+    // the next chunk to copy: row block dtb, chunk dch (chunks run from the last down)
+    uint32_t dtb = 0, dch = n_chunks - 1, dslot = 0, cslot = 0;
+    uint32_t n_ahead = 0;  // chunks copied or in flight and not yet converted
+    auto dma_next = [&]() {
+        if (dtb >= T) return;
+        ++n_ahead;
+        const uint32_t dc0 = dch * CC;
+        dma_chunk<RP, CC>(raw + dslot * kSlotUnits, visb, EB, dl, dtb, T, nbl, bl, C, dc0, min(uint32_t(CC), C - dc0));
+        dslot = dslot + 1 == kDmaSlots ? 0 : dslot + 1;
+        if (dch == 0) {
+            dch = n_chunks - 1;
+            dtb += RP;
+        } else {
+            --dch;
+        }
+    };
+    if (dma) {
+#pragma unroll
+        for (int k = 0; k < kDmaSlots; ++k) dma_next();
+    }
 
     C2<double> acc[ND][NP];
 #pragma unroll
@@ -290,99 +538,91 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
         for (int ch = int(n_chunks) - 1; ch >= 0; --ch) {
             const uint32_t c0 = uint32_t(ch) * CC;
             const uint32_t cn = min(uint32_t(CC), C - c0);
-            __syncthreads();
-            // 16-direction tiles on a complex64 4-pol cube (the residual-loss forward pass): every
-            // element of the thread is loaded (two 16-byte loads each) before any is checked, so
-            // all are in flight together; slots past the rows or channels read a valid element
-            // (row 0 of the baseline, channel 0) and store zero.  Other tiles stage element by element.
-            bool staged = false;
-            if constexpr (DB == 16 && NP == 1 && sizeof(VT) == 4) {
-                if (P == 4) {
-                    constexpr int E = RP * CC / kThreads;
-                    f32x4 raw[E][2];
-#pragma unroll
-                    for (int e = 0; e < E; ++e) {
-                        const uint32_t i = threadIdx.x + kThreads * e;
-                        const uint32_t r = i / CC, c = i - r * CC;
-                        const bool ok = tb + r < T && c < cn;
-                        const size_t el = ok ? (size_t(tb + r) * nbl + bl) * C + c0 + c : size_t(bl) * C;
-                        const LEAP_GLOBAL f32x4* q = reinterpret_cast<const LEAP_GLOBAL f32x4*>(vis) + 2 * el;
-                        raw[e][0] = q[0];
-                        raw[e][1] = q[1];
+            if (dma) {
+                // This chunk's copy has landed (later copies may stay in flight) and the
+                // previous chunk's chains are done.
+                wait_vmem_but((n_ahead - 1) * dma_per_wave);
+                --n_ahead;
+                __syncthreads();
+                const LEAP_LDS f32x4* slot = raw + cslot * kSlotUnits;
+                cslot = cslot + 1 == kDmaSlots ? 0 : cslot + 1;
+                if constexpr (NP == 1) {
+                    if constexpr (sizeof(VT) == 8) {
+                        switch (P) {
+                            case 1: convert_chunk<VT, 1, RP, CC>(s_v, slot, tb, T, cn); break;
+                            case 2: convert_chunk<VT, 2, RP, CC>(s_v, slot, tb, T, cn); break;
+                            case 3: convert_chunk<VT, 3, RP, CC>(s_v, slot, tb, T, cn); break;
+                            default: convert_chunk<VT, 4, RP, CC>(s_v, slot, tb, T, cn); break;
+                        }
+                    } else {
+                        if (P == 2)
+                            convert_chunk<VT, 2, RP, CC>(s_v, slot, tb, T, cn);
+                        else
+                            convert_chunk<VT, 4, RP, CC>(s_v, slot, tb, T, cn);
                     }
-#pragma unroll
-                    for (int e = 0; e < E; ++e) {
-                        const uint32_t i = threadIdx.x + kThreads * e;
-                        const uint32_t r = i / CC, c = i - r * CC;
-                        const bool ok = tb + r < T && c < cn;
-                        bool finite = true;
-#pragma unroll
-                        for (int h = 0; h < 2; ++h)
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) finite = finite && __builtin_isfinite(raw[e][h][k]);
-                        s_v[r][0][c] = (ok && finite) ? C2<double>{double(raw[e][0].x), double(raw[e][0].y)}
-                                                      : C2<double>{0.0, 0.0};
-                    }
+                }
+                __syncthreads();
+                dma_next();  // into the slot just converted
+            } else {
+                __syncthreads();
+                // Pol 0 of every element of the chunk, gathered a batch at a time (stage_pol0);
+                // tiles that keep several pols stage element by element.
+                bool staged = false;
+                if constexpr (NP == 1 && RP * CC % kThreads == 0) {
                     staged = true;
-                }
-            }
-            if (!staged)
-            for (uint32_t i = threadIdx.x; i < RP * uint32_t(CC); i += kThreads) {
-                const uint32_t r = i / CC, c = i - r * CC;
-                const uint32_t tt = tb + r;
-                C2<double> val[NP];
-#pragma unroll
-                for (int q = 0; q < NP; ++q) val[q] = {0.0, 0.0};
-                if (tt < T && c < cn) {
-                    const size_t row = size_t(tt) * nbl + bl;
-                    const LEAP_GLOBAL VT* p = vis + ((row * C + c0 + c) * P) * 2;
-                    bool finite = true;
-                    VT x[8];
-                    for (uint32_t q = 0; q < P; ++q) {
-                        x[2 * q] = p[2 * q];
-                        x[2 * q + 1] = p[2 * q + 1];
-                        finite = finite && __builtin_isfinite(x[2 * q]) && __builtin_isfinite(x[2 * q + 1]);
-                    }
-                    if (finite) {
-#pragma unroll
-                        for (int q = 0; q < NP; ++q) val[q] = {double(x[2 * q]), double(x[2 * q + 1])};
+                    switch (P) {
+                        case 1: stage_pol0<VT, 1, RP, CC>(s_v, vis, tb, T, nbl, bl, C, c0, cn); break;
+                        case 2: stage_pol0<VT, 2, RP, CC>(s_v, vis, tb, T, nbl, bl, C, c0, cn); break;
+                        case 3: stage_pol0<VT, 3, RP, CC>(s_v, vis, tb, T, nbl, bl, C, c0, cn); break;
+                        case 4: stage_pol0<VT, 4, RP, CC>(s_v, vis, tb, T, nbl, bl, C, c0, cn); break;
+                        default: staged = false; break;
                     }
                 }
+                if (!staged) {
+                    for (uint32_t i = threadIdx.x; i < RP * uint32_t(CC); i += kThreads) {
+                        const uint32_t r = i / CC, c = i - r * CC;
+                        const uint32_t tt = tb + r;
+                        C2<double> val[NP];
 #pragma unroll
-                for (int q = 0; q < NP; ++q) s_v[r][q][c] = val[q];
+                        for (int q = 0; q < NP; ++q) val[q] = {0.0, 0.0};
+                        if (tt < T && c < cn) {
+                            const size_t row = size_t(tt) * nbl + bl;
+                            const LEAP_GLOBAL VT* p = vis + ((row * C + c0 + c) * P) * 2;
+                            bool finite = true;
+                            VT x[8];
+                            for (uint32_t q = 0; q < P; ++q) {
+                                x[2 * q] = p[2 * q];
+                                x[2 * q + 1] = p[2 * q + 1];
+                                finite = finite && __builtin_isfinite(x[2 * q]) && __builtin_isfinite(x[2 * q + 1]);
+                            }
+                            if (finite) {
+#pragma unroll
+                                for (int q = 0; q < NP; ++q) val[q] = {double(x[2 * q]), double(x[2 * q + 1])};
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) s_v[r][q][c] = val[q];
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
             if (!row_ok || !any_dir) continue;
-            // Entries past cn are zero in LDS, so every chunk runs whole pairs.
-            const int npairs = int((cn + 1) / 2);
+            // Every chunk runs all CC / 2 pairs.  Only a row's first chunk (the last channels)
+            // can be partial; its entries past cn are zero in LDS and its chains start at +0,
+            // which zero entries leave at +0, so the padding changes no bit of the result.
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                if (npairs == CC / 2) {
 #pragma unroll 4
-                    for (int k = CC / 2 - 1; k >= 0; --k) {
-                        const C2<double> ve = s_v[rsub][q][2 * k], vo = s_v[rsub][q][2 * k + 1];
+                for (int k = CC / 2 - 1; k >= 0; --k) {
+                    const C2<double> ve = s_v[rsub][q][2 * k], vo = s_v[rsub][q][2 * k + 1];
 #pragma unroll
-                        for (int j = 0; j < ND; ++j) {
-                            if constexpr (QOUT) {
-                                de[j] = horner_step(de[j], z2[j], he[j][q]);
-                                dox[j] = horner_step(dox[j], z2[j], ho[j][q]);
-                            }
-                            he[j][q] = horner_step(he[j][q], z2[j], ve);
-                            ho[j][q] = horner_step(ho[j][q], z2[j], vo);
+                    for (int j = 0; j < ND; ++j) {
+                        if constexpr (QOUT) {
+                            de[j] = horner_step(de[j], z2[j], he[j][q]);
+                            dox[j] = horner_step(dox[j], z2[j], ho[j][q]);
                         }
-                    }
-                } else {
-                    for (int k = npairs - 1; k >= 0; --k) {
-                        const C2<double> ve = s_v[rsub][q][2 * k], vo = s_v[rsub][q][2 * k + 1];
-#pragma unroll
-                        for (int j = 0; j < ND; ++j) {
-                            if constexpr (QOUT) {
-                                de[j] = horner_step(de[j], z2[j], he[j][q]);
-                                dox[j] = horner_step(dox[j], z2[j], ho[j][q]);
-                            }
-                            he[j][q] = horner_step(he[j][q], z2[j], ve);
-                            ho[j][q] = horner_step(ho[j][q], z2[j], vo);
-                        }
+                        he[j][q] = horner_step(he[j][q], z2[j], ve);
+                        ho[j][q] = horner_step(ho[j][q], z2[j], vo);
                     }
                 }
             }
@@ -457,12 +697,14 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
 }
 
 template <typename VT, int DB, int ND, int NP, int CC>
-__global__ __launch_bounds__(kThreads) void rotate_average_horner64(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NP == 1 && ND <= 2 && DB / ND >= 16 ? 4 : 1, 8))) void rotate_average_horner64(
     const VT* __restrict__ vis, const double* __restrict__ uvw, const double* __restrict__ dd,
     uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa,
     double* __restrict__ avg) {
     __shared__ Horner64Smem<DB, ND, NP, CC> sm;
-    horner64_tile<VT, DB, ND, NP, CC>(sm, (const LEAP_GLOBAL VT*)vis, (const LEAP_GLOBAL double*)uvw,
+    constexpr int RB = horner64_raw_bytes(kThreads / (DB / ND), CC, NP, sizeof(VT));
+    __shared__ Horner64Raw<RB> sraw;
+    horner64_tile<VT, DB, ND, NP, CC, false, RB>(sm, sraw, (const LEAP_GLOBAL VT*)vis, (const LEAP_GLOBAL double*)uvw,
                                       (const LEAP_GLOBAL double*)dd, D, T, nbl, C, P, kappa0, dkappa,
                                       (LEAP_GLOBAL double*)avg, blockIdx.x, blockIdx.y);
 }
@@ -474,7 +716,9 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64_q(
     uint32_t nbl, uint32_t C, uint32_t P, double kappa0, double dkappa, double* __restrict__ avg,
     double* __restrict__ q) {
     __shared__ Horner64Smem<DB, ND, 1, CC> sm;
-    horner64_tile<VT, DB, ND, 1, CC, true>(sm, (const LEAP_GLOBAL VT*)vis, (const LEAP_GLOBAL double*)uvw,
+    constexpr int RB = horner64_raw_bytes(kThreads / (DB / ND), CC, 1, sizeof(VT));
+    __shared__ Horner64Raw<RB> sraw;
+    horner64_tile<VT, DB, ND, 1, CC, true, RB>(sm, sraw, (const LEAP_GLOBAL VT*)vis, (const LEAP_GLOBAL double*)uvw,
                                            (const LEAP_GLOBAL double*)dd, D, T, nbl, C, P, kappa0, dkappa,
                                            (LEAP_GLOBAL double*)avg, blockIdx.x, blockIdx.y, (LEAP_GLOBAL double*)q);
 }
@@ -483,10 +727,14 @@ __global__ __launch_bounds__(kThreads) void rotate_average_horner64_q(
 // (set, baseline, direction block) items items[off[g] .. off[g + 1]) planned on
 // the host, each against its set's descriptor.
 template <int DB, int ND>
-__global__ __launch_bounds__(kThreads) void rotate_average_multi(const MultiSetDesc* __restrict__ descs,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(ND <= 2 ? 4 : 1, 8))) void rotate_average_multi(const MultiSetDesc* __restrict__ descs,
                                                                  const uint2* __restrict__ items,
                                                                  const uint32_t* __restrict__ off) {
-    __shared__ Horner64Smem<DB, ND, 1, 256> sm;
+    constexpr int RP = kThreads / (DB / ND);
+    constexpr int CC = horner64_chunk(RP, 1);
+    constexpr int RB = horner64_raw_bytes(RP, CC, 1, sizeof(double));
+    __shared__ Horner64Smem<DB, ND, 1, CC> sm;
+    __shared__ Horner64Raw<RB> sraw;
     const uint32_t i1 = off[blockIdx.x + 1];
     for (uint32_t i = off[blockIdx.x]; i < i1; ++i) {
         const uint2 it = items[i];
@@ -496,11 +744,12 @@ __global__ __launch_bounds__(kThreads) void rotate_average_multi(const MultiSetD
         const LEAP_GLOBAL double* dd = (const LEAP_GLOBAL double*)m.dd;
         LEAP_GLOBAL double* avg = (LEAP_GLOBAL double*)m.avg;
         if (m.c64)
-            horner64_tile<float, DB, ND, 1, 256>(sm, (const LEAP_GLOBAL float*)m.vis, uvw, dd, m.D, m.T, m.nbl, m.C,
-                                                 m.P, m.kappa0, m.dkappa, avg, bl, db);
+            horner64_tile<float, DB, ND, 1, CC, false, RB>(sm, sraw, (const LEAP_GLOBAL float*)m.vis, uvw, dd, m.D,
+                                                           m.T, m.nbl, m.C, m.P, m.kappa0, m.dkappa, avg, bl, db);
         else
-            horner64_tile<double, DB, ND, 1, 256>(sm, (const LEAP_GLOBAL double*)m.vis, uvw, dd, m.D, m.T, m.nbl,
-                                                  m.C, m.P, m.kappa0, m.dkappa, avg, bl, db);
+            horner64_tile<double, DB, ND, 1, CC, false, RB>(sm, sraw, (const LEAP_GLOBAL double*)m.vis, uvw, dd,
+                                                            m.D, m.T, m.nbl, m.C, m.P, m.kappa0, m.dkappa, avg, bl,
+                                                            db);
         __syncthreads();
     }
 }
@@ -739,11 +988,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(ND == 
 template <typename VT, typename A, int DB, int ND, int NP>
 void launch_one(hipStream_t s, const void* vis, const double* uvw, const double* dd, uint32_t D, uint32_t T,
                 uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, double kappa0, double dkappa, double* avg) {
-    // Channels per LDS chunk: at most 32 KB of pol-0 visibilities for the tile's RP rows (256
-    // channels from 8 rows up).  The recurrences run over the chunks in order, so the chunk
-    // size does not change the result.
     constexpr int RP = kThreads / (DB / ND);
-    constexpr int CC = (NP == 1) ? (2048 / RP < 256 ? 2048 / RP : 256) : 64;
+    constexpr int CC = sizeof(A) == 8 ? horner64_chunk(RP, NP) : (NP == 1) ? (2048 / RP < 256 ? 2048 / RP : 256) : 64;
     const dim3 grid(nbl_grid, (D + DB - 1) / DB);
     if constexpr (sizeof(A) == 8) {
         rotate_average_horner64<VT, DB, ND, NP, CC><<<grid, kThreads, 0, s>>>(
@@ -938,7 +1184,7 @@ void launch_rotate_forward_q(hipStream_t s, int32_t tile, const void* vis, int32
 #define LEAP_FQ(DB_, ND_)                                                                                         \
     case DB_ * 10 + ND_: {                                                                                        \
         constexpr int RP = kThreads / (DB_ / ND_);                                                                \
-        constexpr int CC = 2048 / RP < 256 ? 2048 / RP : 256;                                                     \
+        constexpr int CC = horner64_chunk(RP, 1);                                                                 \
         const dim3 grid(nbl, (D + DB_ - 1) / DB_);                                                                \
         if (c64)                                                                                                  \
             rotate_average_horner64_q<float, DB_, ND_, CC><<<grid, kThreads, 0, s>>>(                             \

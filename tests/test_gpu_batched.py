@@ -63,6 +63,38 @@ def test_batched_lane_directions(leap, ctx, oracle, small, db, nd):
         assert np.abs(got[d, :, 0] - want[:, 0]).max() <= 1e-12 * np.abs(want[:, 0]).max()
 
 
+@pytest.mark.parametrize("tile", [(0, 0), (64, 2), (256, 1)])
+@pytest.mark.parametrize("n_pols", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_batched_device_cube_alignment(leap, ctx, oracle, dtype, n_pols, tile):
+    """A device cube at a 16-byte-aligned address and the same cube 8 bytes past one give the
+    same bits (the pol-0 tiles copy 16-byte-multiple elements of an aligned cube straight into
+    LDS and stage the rest through registers), both within the FP64 bar of the oracle."""
+    s = leap.SynthMS(n_antennas=12, n_timesteps=9, n_channels=83, n_pols=n_pols, nan_fraction=0.004, seed=41)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(dtype))
+    omd = oracle.metadata(ms)
+    md = ctx.metadata(leap.view_from_arrays(ms))
+    dirs = dirs_near(ms["phase_centre"], 45, 7)
+    db, nd = tile
+    opts = leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db, lane_directions=nd)
+    raw = np.ascontiguousarray(ms["vis"]).view(np.uint8)
+    base = ctx.malloc(raw.nbytes + 64)
+    try:
+        got = []
+        for off in (0, 8):
+            ctx.h2d(base + off, raw)
+            view = leap.view_from_arrays(ms, vis_device=base + off, vis_dtype=leap.LEAP_VIS_C64 if dtype == np.complex64
+                                         else leap.LEAP_VIS_C128)
+            got.append(md.rotate(view, dirs, opts=opts))
+    finally:
+        ctx.free(base)
+    assert np.array_equal(got[0], got[1])
+    for d in (0, 13, 44):
+        want = omd.rotate(dirs[d])
+        assert np.abs(got[0][d, :, 0] - want[:, 0]).max() <= 1e-12 * np.abs(want[:, 0]).max()
+
+
 @pytest.mark.parametrize("nd", [1, 2])
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_batched_all_pols_matches_oracle(leap, ctx, oracle, dtype, nd):
