@@ -473,7 +473,9 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         elif args.fp32_blocks:
             kernel, prof = "rotate_average_batched", None
         else:
-            kernel, prof = "rotate_average_horner64", "rotate_average_horner64_c1_traffic.json"
+            # counters of the complex64 (256 directions) and complex128 (64 directions) cubes
+            kernel = "rotate_average_horner64"
+            prof = "rotate_average_horner64_c2_traffic.json" if elem == 16 else "rotate_average_horner64_c1_traffic.json"
         roofline = {"kernel": kernel,
                     "bound": "valu-fp64", "achieved": round(achieved, 2), "peak": VALU_FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / VALU_FP64_PEAK_TFLOPS, 4),

@@ -190,10 +190,12 @@ bool use_packed(const leap_options& opts) {
 // baselines, 128 t, 384 ch): pack 5.2 TB/s; packed Horner 32 / 50 / 55 TFLOP/s
 // with 1 / 2 / 4 directions per lane (up to 64, 128, more directions); the
 // Horner kernel straight from the cube reads at most 6.4 TB/s and computes
-// 30 / 41 TFLOP/s (up to 128 / more directions).  E.g. 256 directions: packed
-// for complex64 (72 vs 80 ms), from the cube for complex128 (80 vs 82 ms);
-// 64 directions on complex128: from the cube (31 vs 52 ms).  Host-resident
-// cubes always take the packed route (pol 0 only crosses the bus).
+// 30 TFLOP/s up to 128 directions, 51 (complex64) or 45 (complex128) above.
+// E.g. 256 directions: from the cube for complex64 (65 vs 72 ms) and
+// complex128 (74 vs 82 ms); 64 directions on complex128: from the cube (27 vs
+// 50 ms); the pack pays from about 800 (complex64) or 480 (complex128)
+// directions.  Host-resident cubes always take the packed route (pol 0 only
+// crosses the bus).
 leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, uint32_t D) {
     leap_options r = o;
     if (r.channel_sum == LEAP_CHANNEL_SUM_AUTO)
@@ -218,7 +220,8 @@ leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, u
     const double s = ms.vis_dtype == LEAP_VIS_C64 ? 8.0 : 16.0;
     const double flops = 8.0 * rc * D;
     const double packed = rc * (ms.num_pols * s + 16.0) / 5.2e12 + flops / (D > 128 ? 55e12 : D > 64 ? 50e12 : 32e12);
-    const double direct = std::max(rc * ms.num_pols * s / 6.4e12, flops / (D > 128 ? 41e12 : 30e12));
+    const double direct_rate = D > 128 ? (ms.vis_dtype == LEAP_VIS_C64 ? 51e12 : 45e12) : 30e12;
+    const double direct = std::max(rc * ms.num_pols * s / 6.4e12, flops / direct_rate);
     if (packed > direct) r.direct_cube = 1;
     return r;
 }
