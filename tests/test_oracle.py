@@ -212,6 +212,43 @@ def test_rotate_nan_skips_whole_channel(oracle, leap):
     assert np.abs(avg - expect).max() < 1e-9 * np.abs(expect).max()
 
 
+def test_rotate_infinities_follow_the_complex_product(oracle, leap):
+    """An infinite pol is skipped exactly when its rotated product has a NaN part
+    (CpuLeapCalibrator.cc:196-206 with std::complex multiplication): (inf + 0.25i) times
+    exp(i 0) = 1 + 0i has the part inf * 0 = NaN, so at the phase centre it is skipped like a NaN;
+    off the centre both parts are infinite and the baseline's sum turns infinite; inf + inf i
+    always has one NaN part (inf - inf) and is skipped."""
+    s = leap.SynthMS(n_antennas=8, n_timesteps=3, n_channels=10, seed=14)
+    s.fill_host()
+    nbl = s.host_dict()["num_baselines"]
+    rows = {"inf": (1 * nbl + 2, 4, 0, complex(np.inf, 0.25)), "infinf": (2 * nbl + 5, 6, 1, complex(np.inf, np.inf))}
+
+    def cube(kind, nan=False):
+        v = s.vis.copy()
+        r, c, p, x = rows[kind]
+        v[r, c, p] = complex(np.nan, 0.0) if nan else x
+        return s.host_dict(v)
+
+    centre = s.host_dict()["phase_centre"]
+    off = (centre[0] + 0.01, centre[1] - 0.015)
+    ms_inf, ms_nan = cube("inf"), cube("inf", nan=True)
+    # at the phase centre: skipped, the same sums as a NaN in its place
+    a = oracle.metadata(ms_inf).rotate(centre)
+    b = oracle.metadata(ms_nan).rotate(centre)
+    assert np.array_equal(a, b) and np.isfinite(a).all()
+    # off the centre: summed, baseline 2's pol 0 infinite in both parts, the rest as with a NaN
+    a = oracle.metadata(ms_inf).rotate(off)
+    b = oracle.metadata(ms_nan).rotate(off)
+    assert np.isinf(a[2, 0].real) and np.isinf(a[2, 0].imag)
+    mask = np.ones(a.shape, dtype=bool)
+    mask[2, :] = False  # the other pols of that (row, channel) are summed too
+    assert np.array_equal(a[mask], b[mask])
+    # inf + inf i: skipped off the centre as well
+    a = oracle.metadata(cube("infinf")).rotate(off)
+    b = oracle.metadata(cube("infinf", nan=True)).rotate(off)
+    assert np.array_equal(a, b) and np.isfinite(a).all()
+
+
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_rotate_matches_numpy_restatement(oracle, small_ms, dtype):
     ms = small_ms.host_dict(small_ms.vis.astype(dtype))
