@@ -152,9 +152,12 @@ typedef struct leap_options {
                                  options and device addresses; results reach the callback
                                  after the whole graph.  Needs vis_on_device and uvw_device. */
     int32_t direct_cube;      /* batched pol 0: 1 = run the Horner kernel straight from the cube
-                                 through LDS; 0 = first pack pol 0 (FP64, non-finite (row,
-                                 channel)s zeroed; 16 B per row and channel of device memory) and
-                                 feed the Horner kernel through scalar loads (faster) */
+                                 through LDS; 0 = automatic: a tuned call on a device cube packs
+                                 pol 0 first (FP64, non-finite (row, channel)s zeroed; 16 B per
+                                 row and channel of device memory; the Horner kernel then reads
+                                 it with scalar loads) only where a cost model of measured rates
+                                 expects the pack to pay (from about 800 directions on complex64,
+                                 480 on complex128); host cubes and packed tiles always pack */
     int32_t pack_timesteps;   /* packed path: timesteps per pack chunk; chunks are packed on a
                                  second stream while the Horner kernel runs the previous one.
                                  0 = automatic; >= the interval's timesteps = one chunk */
