@@ -95,6 +95,44 @@ def test_batched_device_cube_alignment(leap, ctx, oracle, dtype, n_pols, tile):
         assert np.abs(got[0][d, :, 0] - want[:, 0]).max() <= 1e-12 * np.abs(want[:, 0]).max()
 
 
+EDGE_SHAPES = [  # (antennas, timesteps, channels, pols)
+    (4, 1, 1, 1),    # one row block shorter than the rows in flight, a single channel
+    (5, 2, 5, 3),    # fewer channels than a chunk; 48-byte complex128 elements (rows not a power of two)
+    (6, 3, 33, 4),   # one channel past a 32-channel chunk
+    (3, 11, 64, 2),  # whole chunks, a partial last row block
+    (2, 4, 200, 4),  # three baselines (two antennas and their autocorrelations)
+]
+
+
+@pytest.mark.parametrize("tile", [(64, 2), (32, 1), (256, 2)])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("shape", EDGE_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_batched_fp64_edge_shapes(leap, ctx, oracle, shape, dtype, tile):
+    """The FP64 tiles straight from a device cube at the shapes the chunking and the cube copies
+    handle specially, against the oracle."""
+    n_ant, n_t, n_ch, n_pols = shape
+    s = leap.SynthMS(n_antennas=n_ant, n_timesteps=n_t, n_channels=n_ch, n_pols=n_pols, nan_fraction=0.01, seed=43)
+    s.fill_host(np.complex128)
+    ms = s.host_dict(s.vis.astype(dtype))
+    omd = oracle.metadata(ms)
+    md = ctx.metadata(leap.view_from_arrays(ms))
+    dirs = dirs_near(ms["phase_centre"], 9, 5)
+    raw = np.ascontiguousarray(ms["vis"]).view(np.uint8)
+    p = ctx.malloc(max(raw.nbytes, 16))
+    try:
+        ctx.h2d(p, raw)
+        view = leap.view_from_arrays(ms, vis_device=p,
+                                     vis_dtype=leap.LEAP_VIS_C64 if dtype == np.complex64 else leap.LEAP_VIS_C128)
+        got = md.rotate(view, dirs, opts=leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=tile[0],
+                                                      lane_directions=tile[1]))
+    finally:
+        ctx.free(p)
+    for d in range(len(dirs)):
+        want = omd.rotate(dirs[d])
+        scale = max(np.abs(want[:, 0]).max(), 1e-300)
+        assert np.abs(got[d, :, 0] - want[:, 0]).max() <= 1e-12 * scale, d
+
+
 @pytest.mark.parametrize("nd", [1, 2])
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_batched_all_pols_matches_oracle(leap, ctx, oracle, dtype, nd):
