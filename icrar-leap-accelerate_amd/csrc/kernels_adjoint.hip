@@ -17,6 +17,7 @@
 // and the antenna gradient is dL/d dp_a = sum_t M_t^T (sum_{b: a2 = a} g_uvw - sum_{b: a1 = a} g_uvw).
 #include <algorithm>
 
+#include "cube_ring.h"
 #include "leap_internal.h"
 
 namespace leap {
@@ -27,8 +28,6 @@ constexpr int kThreads = 256;
 struct Cx {
     double x, y;
 };
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline Cx cmulx(Cx a, Cx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ inline Cx hstep(Cx a, Cx z, Cx v) {
@@ -58,18 +57,22 @@ __global__ void perturb_uvw_kernel(const double* __restrict__ uvw0, const double
 }
 
 // One workgroup per baseline; DL lanes per row with ND directions each (direction lane + DL j,
-// D <= DL ND), 256/DL rows side by side, channels staged through LDS kCC at a time (at most 32 KB
-// of visibilities per workgroup).  The Horner recurrences run over the chunks in order, so the
-// chunk size does not change the result.
-template <int DL, int ND>
+// D <= DL ND), RP = 256/DL rows side by side, channels in chunks of CC = 256/RP (at least 16).
+// A cube the ring accepts (cube_ring.h: complex128, or complex64 with 2 or 4 pols, 16-byte
+// aligned) streams into LDS with global_load_lds, two chunks ahead for complex64 and one for
+// complex128; others are staged element by element.  The Horner recurrences run over the chunks
+// in order, so the chunk size does not change the result.
+template <typename VT, int DL, int ND>
 __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
-    const void* __restrict__ vis, int32_t c64, const double* __restrict__ uvw, const double* __restrict__ dd,
+    const VT* __restrict__ vis, const double* __restrict__ uvw, const double* __restrict__ dd,
     const double* __restrict__ abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double kappa0,
     double dkappa, double* __restrict__ g_uvw, double* __restrict__ g_dz_partial) {
     constexpr int RP = kThreads / DL;
-    constexpr int kCC = 2048 / RP < 256 ? 2048 / RP : 256;  // channels per LDS chunk
-    constexpr int DW = DL * ND;                               // directions of the workgroup
+    constexpr int kCC = 256 / RP > 16 ? 256 / RP : 16;  // channels per chunk
+    constexpr int DW = DL * ND;                          // directions of the workgroup
+    using Ring = dev::CubeRing<VT, RP, kCC, sizeof(VT) == 4 ? 2 : 1>;
     __shared__ Cx s_v[RP][kCC];
+    __shared__ dev::f32x4 s_raw[Ring::kBytes / 16];
     __shared__ double s_g[RP][DW][3];
     const uint32_t bl = blockIdx.x;
     const uint32_t lane_d = threadIdx.x % DL;
@@ -92,6 +95,10 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
         gdz[j][0] = gdz[j][1] = gdz[j][2] = 0.0;
     }
     const uint32_t n_chunks = (C + kCC - 1) / kCC;
+    Ring ring;
+    const bool streamed = Ring::accepts((const void*)vis, P);
+    // address-space casts: the copies take global and LDS pointers
+    if (streamed) ring.start((LEAP_LDS dev::f32x4*)(&s_raw[0]), (const LEAP_GLOBAL VT*)vis, P, T, nbl, bl, C);
 
     for (uint32_t tb = 0; tb < T; tb += RP) {
         const uint32_t t = tb + rsub;
@@ -115,67 +122,41 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
         for (int ch = int(n_chunks) - 1; ch >= 0; --ch) {
             const uint32_t c0 = uint32_t(ch) * kCC;
             const uint32_t cn = min(uint32_t(kCC), C - c0);
-            __syncthreads();
-            if (c64 && P == 4) {
-                // complex64, 4 pols: every element of the thread is loaded (two 16-byte loads
-                // each) before any is checked, so all of them are in flight together.  Slots past
-                // the rows or channels read a valid element (row 0 of the baseline, channel 0) and
-                // store zero.
-                constexpr int E = RP * kCC / kThreads;
-                f32x4 raw[E][2];
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const uint32_t i = threadIdx.x + kThreads * e;
+            if (streamed) {
+                // this chunk's copy has landed and the previous chunk's recurrences are done
+                const LEAP_LDS dev::f32x4* slot = ring.next_slot();
+                __syncthreads();
+                ring.convert(slot, tb, cn, [&](uint32_t r, uint32_t c, double re, double im) { s_v[r][c] = Cx{re, im}; });
+                __syncthreads();
+                ring.issue();  // into the slot just converted
+            } else {
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < RP * uint32_t(kCC); i += kThreads) {
                     const uint32_t r = i / kCC, c = i - r * kCC;
-                    const bool ok = tb + r < T && c < cn;
-                    const size_t el = ok ? (size_t(tb + r) * nbl + bl) * C + c0 + c : size_t(bl) * C;
-                    const f32x4* q = static_cast<const f32x4*>(vis) + 2 * el;
-                    raw[e][0] = q[0];
-                    raw[e][1] = q[1];
-                }
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const uint32_t i = threadIdx.x + kThreads * e;
-                    const uint32_t r = i / kCC, c = i - r * kCC;
-                    const bool ok = tb + r < T && c < cn;
-                    bool finite = true;
-#pragma unroll
-                    for (int h = 0; h < 2; ++h)
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) finite = finite && __builtin_isfinite(raw[e][h][k]);
-                    s_v[r][c] = (ok && finite) ? Cx{double(raw[e][0].x), double(raw[e][0].y)} : Cx{0.0, 0.0};
-                }
-            } else
-            for (uint32_t i = threadIdx.x; i < RP * uint32_t(kCC); i += kThreads) {
-                const uint32_t r = i / kCC, c = i - r * kCC;
-                const uint32_t tt = tb + r;
-                Cx val = {0.0, 0.0};
-                if (tt < T && c < cn) {
-                    const size_t e = ((size_t(tt) * nbl + bl) * C + c0 + c) * P;
-                    bool finite = true;
-                    double x0 = 0.0, y0 = 0.0;
-                    for (uint32_t q = 0; q < P; ++q) {
-                        double xr, xi;
-                        if (c64) {
-                            xr = static_cast<const float*>(vis)[2 * (e + q)];
-                            xi = static_cast<const float*>(vis)[2 * (e + q) + 1];
-                        } else {
-                            xr = static_cast<const double*>(vis)[2 * (e + q)];
-                            xi = static_cast<const double*>(vis)[2 * (e + q) + 1];
+                    const uint32_t tt = tb + r;
+                    Cx val = {0.0, 0.0};
+                    if (tt < T && c < cn) {
+                        const size_t e = ((size_t(tt) * nbl + bl) * C + c0 + c) * P;
+                        bool finite = true;
+                        double x0 = 0.0, y0 = 0.0;
+                        for (uint32_t q = 0; q < P; ++q) {
+                            const double xr = vis[2 * (e + q)], xi = vis[2 * (e + q) + 1];
+                            finite = finite && __builtin_isfinite(xr) && __builtin_isfinite(xi);
+                            if (q == 0) {
+                                x0 = xr;
+                                y0 = xi;
+                            }
                         }
-                        finite = finite && __builtin_isfinite(xr) && __builtin_isfinite(xi);
-                        if (q == 0) {
-                            x0 = xr;
-                            y0 = xi;
-                        }
+                        if (finite) val = {x0, y0};
                     }
-                    if (finite) val = {x0, y0};
+                    s_v[r][c] = val;
                 }
-                s_v[r][c] = val;
+                __syncthreads();
             }
-            __syncthreads();
             if (!row_ok || !any_dir) continue;
-            for (int k = int(cn) - 1; k >= 0; --k) {
+            // Whole chunks: only a row's first chunk can be partial; its entries past cn are
+            // zero and its recurrences start at +0, which zero entries leave at +0.
+            for (int k = kCC - 1; k >= 0; --k) {
                 const Cx x = s_v[rsub][k];
 #pragma unroll
                 for (int j = 0; j < ND; ++j) {
@@ -603,12 +584,18 @@ void launch_rotate_adjoint(hipStream_t s, const void* vis, int32_t vis_dtype, co
                            const double* abar, uint32_t D, uint32_t T, uint32_t nbl, uint32_t C, uint32_t P, double f0,
                            double df, double* g_uvw, double* g_dz_partial) {
     const double kappa0 = f0 / kSpeedOfLight, dkappa = df / kSpeedOfLight;
-    const int32_t c64 = vis_dtype == LEAP_VIS_C64 ? 1 : 0;
+    const bool c64 = vis_dtype == LEAP_VIS_C64;
     require(D >= 1 && D <= 256, LEAP_ERR_NOT_IMPLEMENTED, "the adjoint handles 1 to 256 directions per call");
     const dim3 grid(nbl);
-#define LEAP_ADJ(DL_, ND_)                                                                                  \
-    rotate_adjoint_kernel<DL_, ND_><<<grid, kThreads, 0, s>>>(vis, c64, uvw, dd, abar, D, T, nbl, C, P, kappa0, \
-                                                               dkappa, g_uvw, g_dz_partial)
+#define LEAP_ADJ(DL_, ND_)                                                                                     \
+    if (c64)                                                                                                   \
+        rotate_adjoint_kernel<float, DL_, ND_><<<grid, kThreads, 0, s>>>(static_cast<const float*>(vis), uvw, dd, \
+                                                                         abar, D, T, nbl, C, P, kappa0, dkappa,   \
+                                                                         g_uvw, g_dz_partial);                    \
+    else                                                                                                       \
+        rotate_adjoint_kernel<double, DL_, ND_><<<grid, kThreads, 0, s>>>(static_cast<const double*>(vis), uvw,   \
+                                                                          dd, abar, D, T, nbl, C, P, kappa0,       \
+                                                                          dkappa, g_uvw, g_dz_partial)
     if (D <= 16)
         LEAP_ADJ(8, 2);
     else if (D <= 32)

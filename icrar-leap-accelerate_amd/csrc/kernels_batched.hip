@@ -28,14 +28,10 @@
 // for finite and NaN inputs the two rules agree, and for an infinite input the
 // reference's result depends on the direction (inf * 0 = NaN), where here the
 // channel is always skipped.
-#include "batched_common.h"
+#include "cube_ring.h"
 
 namespace leap {
 namespace {
-
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 using namespace dev;
 
@@ -192,16 +188,6 @@ __global__ __launch_bounds__(kThreads) void rotate_average_batched(
     }
 }
 
-// One (row, channel) element of a cube with PP pols, read as vector units:
-// a 16-byte unit per complex128 pol, pairs of complex64 pols as 16-byte units
-// (8-byte units for an odd pol count).
-template <typename VT, int PP>
-struct StageUnit {
-    typedef std::conditional_t<sizeof(VT) == 8, f64x2, std::conditional_t<PP % 2 == 0, f32x4, f32x2>> T;
-    static constexpr int K = sizeof(T) / sizeof(VT);  // scalars per unit
-    static constexpr int N = 2 * PP / K;               // units per element
-};
-
 // Bytes of raw element data a thread holds in flight while staging a chunk.
 constexpr int kStageBytes = 64;
 
@@ -257,12 +243,6 @@ __device__ __attribute__((always_inline)) inline void stage_pol0(SV& s_v, const 
     }
 }
 
-// s_waitcnt vmcnt(0) with the export and LDS/scalar counters left free
-// (gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15).
-constexpr int kWaitVectorMem = 0x0F70;
-
-#define LEAP_LDS __attribute__((address_space(3)))
-
 // Channels per LDS chunk of the FP64 Horner tiles with RP rows in flight:
 // pol-0 tiles take 256 (row, channel) elements per chunk (at least 16
 // channels), so the raw copy of a chunk (all pols, up to 64 bytes an element)
@@ -284,113 +264,6 @@ struct Horner64Raw {
 // Bytes of the raw copy of a pol-0 tile's chunk for components of ES bytes
 // (up to 4 pols); tiles keeping every pol stage through registers.
 constexpr int horner64_raw_bytes(int RP, int CC, int NP, int ES) { return NP == 1 ? kDmaSlots * RP * CC * 8 * ES : 16; }
-
-// s_waitcnt vmcnt(n) (the export and LDS/scalar counters left free) for the
-// copies one chunk issues per wave (a larger n waits as for 8, which is safe).
-__device__ __attribute__((always_inline)) inline void wait_vmem_but(uint32_t n) {
-    switch (n) {
-        case 0: __builtin_amdgcn_s_waitcnt(kWaitVectorMem); break;
-        case 1: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 1); break;
-        case 2: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 2); break;
-        case 3: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 3); break;
-        case 4: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 4); break;
-        case 5: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 5); break;
-        case 6: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 6); break;
-        case 7: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 7); break;
-        default: __builtin_amdgcn_s_waitcnt(kWaitVectorMem | 8); break;
-    }
-}
-
-// The per-tile part of a chunk copy.  Thread tid copies units tid + 256 k
-// (k < K) of the chunk; with urow = CC EB / 16 units per row a power of two,
-// unit g lies in row g >> lu at unit g & (urow - 1), so its cube address is
-// the chunk's first byte + (tid >> lu) rowbytes + 16 (tid & (urow - 1)) (the
-// lane's part, fixed for the tile) + the uniform part of 256 k.
-struct DmaLane {
-    uint64_t rowbytes;  // bytes between timesteps of a baseline: nbl C EB
-    uint64_t lane_off;  // the lane's part (pow2 only)
-    uint32_t urow;      // units per row
-    int32_t lu;         // log2(urow), or -1 when urow is not a power of two
-};
-
-__device__ __attribute__((always_inline)) inline DmaLane dma_lane(uint32_t CCEB16, uint32_t nbl, uint32_t C,
-                                                                  uint32_t EB) {
-    DmaLane d;
-    d.urow = CCEB16;
-    d.rowbytes = uint64_t(nbl) * C * EB;
-    d.lu = (d.urow & (d.urow - 1)) == 0 ? int32_t(__builtin_ctz(d.urow)) : -1;
-    const uint32_t tid = threadIdx.x;
-    d.lane_off = d.lu >= 0 ? uint64_t(tid >> d.lu) * d.rowbytes + 16u * (tid & (d.urow - 1)) : 0;
-    return d;
-}
-
-// Starts the copy of rows [tb, tb + RP) x channels [c0, c0 + CC) of the cube
-// into raw with global_load_lds_dwordx4 (lane i of a wave writes the 16 bytes
-// at the wave's LDS address + 16 i), row r at byte r * CC * EB.  EB, the bytes
-// of an element, is a multiple of 16 and the cube 16-byte aligned.  Rows past
-// T read row tb, units past the chunk's cn channels the row's first unit: the
-// conversion zeroes those slots.  Whole chunks of a power-of-two row take one
-// address add per copy; the rest divide per unit.  The copy has landed once
-// every thread has waited for vmcnt(0) and passed a barrier.
-template <int RP, int CC>
-__device__ __attribute__((always_inline)) inline void dma_chunk(LEAP_LDS f32x4* raw,
-                                                                const LEAP_GLOBAL unsigned char* vis, uint32_t EB,
-                                                                const DmaLane& dl, uint32_t tb, uint32_t T,
-                                                                uint32_t nbl, uint32_t bl, uint32_t C, uint32_t c0,
-                                                                uint32_t cn) {
-    static_assert(RP * CC % 64 == 0, "whole waves of units");
-    const uint32_t urow = dl.urow;
-    const uint32_t total = RP * urow;  // a multiple of 256
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    const LEAP_GLOBAL unsigned char* first = vis + ((size_t(tb) * nbl + bl) * C + c0) * EB;
-    if (dl.lu >= 0 && tb + RP <= T && cn == uint32_t(CC)) {
-        const LEAP_GLOBAL unsigned char* lane_first = first + dl.lane_off;
-        for (uint32_t q = 0; q < total; q += kThreads) {
-            const uint64_t step = uint64_t(q >> dl.lu) * dl.rowbytes + 16u * (q & (urow - 1));  // uniform
-            __builtin_amdgcn_global_load_lds((const LEAP_GLOBAL void*)(lane_first + step),
-                                             (LEAP_LDS void*)(raw + q + 64 * wave), 16, 0, 0);
-        }
-    } else {
-        const uint32_t uvalid = cn * (EB / 16);
-        const uint32_t lane = threadIdx.x % 64;
-        for (uint32_t g0 = wave * 64; g0 < total; g0 += kThreads) {
-            const uint32_t g = g0 + lane;
-            const uint32_t r = g / urow, u = g - r * urow;
-            const size_t row = size_t(tb + r < T ? tb + r : tb) * nbl + bl;
-            const LEAP_GLOBAL unsigned char* src = vis + (row * C + c0) * EB + size_t(u < uvalid ? u : 0) * 16;
-            __builtin_amdgcn_global_load_lds((const LEAP_GLOBAL void*)src, (LEAP_LDS void*)(raw + g0), 16, 0, 0);
-        }
-    }
-    // keep the copies issued here, ahead of the channel loop that follows
-    __builtin_amdgcn_sched_barrier(0);
-}
-
-// Converts a chunk copied by dma_chunk to pol-0 FP64 values in s_v: element
-// (r, c) is unit range [(r CC + c) NU, + NU) of raw; 0 past the rows or
-// channels or where any of its pols is not finite.
-template <typename VT, int PP, int RP, int CC, typename SV>
-__device__ __attribute__((always_inline)) inline void convert_chunk(SV& s_v, const LEAP_LDS f32x4* raw, uint32_t tb,
-                                                                    uint32_t T, uint32_t cn) {
-    using SU = StageUnit<VT, PP>;
-    using U = typename SU::T;
-    static_assert(sizeof(U) == 16 && RP * CC % kThreads == 0, "16-byte units, whole elements per thread");
-    const LEAP_LDS U* x = reinterpret_cast<const LEAP_LDS U*>(raw);
-#pragma unroll
-    for (int e = 0; e < RP * CC / kThreads; ++e) {
-        const uint32_t i = threadIdx.x + kThreads * e;
-        const uint32_t r = i / CC, c = i - r * CC;
-        U v[SU::N];
-#pragma unroll
-        for (int k = 0; k < SU::N; ++k) v[k] = x[i * SU::N + k];
-        bool finite = true;
-#pragma unroll
-        for (int k = 0; k < SU::N; ++k)
-#pragma unroll
-            for (int m = 0; m < SU::K; ++m) finite = finite && __builtin_isfinite(v[k][m]);
-        const bool ok = tb + r < T && c < cn;
-        s_v[r][0][c] = (ok && finite) ? C2<double>{double(v[0][0]), double(v[0][1])} : C2<double>{0.0, 0.0};
-    }
-}
 
 // Full-row FP64 Horner: per (row, direction) one exact phasor exp(i sf kappa0)
 // and the step z = exp(i sf dkappa) are computed with FP64 sincos, and the
@@ -464,41 +337,15 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
     }
     const uint32_t n_chunks = (C + CC - 1) / CC;
 
-    // DMA staging: chunks are copied kDmaSlots ahead into a ring of raw slots
-    // (each sized for 4 pols), consumed in order.
-    const uint32_t EB = 2 * P * uint32_t(sizeof(VT));
-    constexpr uint32_t kSlotUnits = uint32_t(RP) * CC * 8 * sizeof(VT) / 16;
+    // Pol-0 tiles stream accepted cubes through the ring (cube_ring.h), kDmaSlots chunks ahead.
+    using Ring = CubeRing<VT, RP, CC, kDmaSlots>;
+    static_assert(NP > 1 || Ring::kBytes <= RB, "the ring fits the raw buffer");
+    Ring ring;
     bool dma = false;
-    if constexpr (NP == 1) {
-        dma = EB % 16 == 0 && P <= 4 && (reinterpret_cast<uintptr_t>(vis) & 15) == 0 &&
-              size_t(kDmaSlots) * kSlotUnits * 16 <= sizeof(sraw.u);
-    }
+    if constexpr (NP == 1) dma = Ring::accepts((const void*)vis, P);
     // address-space cast of the shared buffer: the copies take an LDS pointer
-    LEAP_LDS f32x4* raw = (LEAP_LDS f32x4*)(&sraw.u[0]);
-    const LEAP_GLOBAL unsigned char* visb = reinterpret_cast<const LEAP_GLOBAL unsigned char*>(vis);
-    // copies per wave and chunk (RP * CC * EB / 16 units in waves of 64, over 4 waves)
-    const uint32_t dma_per_wave = uint32_t(RP) * CC * (EB / 16) / kThreads;
-    const DmaLane dl = dma_lane(uint32_t(CC) * (EB / 16), nbl, C, EB);
-    // the next chunk to copy: row block dtb, chunk dch (chunks run from the last down)
-    uint32_t dtb = 0, dch = n_chunks - 1, dslot = 0, cslot = 0;
-    uint32_t n_ahead = 0;  // chunks copied or in flight and not yet converted
-    auto dma_next = [&]() {
-        if (dtb >= T) return;
-        ++n_ahead;
-        const uint32_t dc0 = dch * CC;
-        dma_chunk<RP, CC>(raw + dslot * kSlotUnits, visb, EB, dl, dtb, T, nbl, bl, C, dc0, min(uint32_t(CC), C - dc0));
-        dslot = dslot + 1 == kDmaSlots ? 0 : dslot + 1;
-        if (dch == 0) {
-            dch = n_chunks - 1;
-            dtb += RP;
-        } else {
-            --dch;
-        }
-    };
-    if (dma) {
-#pragma unroll
-        for (int k = 0; k < kDmaSlots; ++k) dma_next();
-    }
+    if constexpr (NP == 1)
+        if (dma) ring.start((LEAP_LDS f32x4*)(&sraw.u[0]), vis, P, T, nbl, bl, C);
 
     C2<double> acc[ND][NP];
 #pragma unroll
@@ -538,32 +385,22 @@ __device__ __attribute__((always_inline)) inline void horner64_tile(
         for (int ch = int(n_chunks) - 1; ch >= 0; --ch) {
             const uint32_t c0 = uint32_t(ch) * CC;
             const uint32_t cn = min(uint32_t(CC), C - c0);
-            if (dma) {
-                // This chunk's copy has landed (later copies may stay in flight) and the
-                // previous chunk's chains are done.
-                wait_vmem_but((n_ahead - 1) * dma_per_wave);
-                --n_ahead;
-                __syncthreads();
-                const LEAP_LDS f32x4* slot = raw + cslot * kSlotUnits;
-                cslot = cslot + 1 == kDmaSlots ? 0 : cslot + 1;
-                if constexpr (NP == 1) {
-                    if constexpr (sizeof(VT) == 8) {
-                        switch (P) {
-                            case 1: convert_chunk<VT, 1, RP, CC>(s_v, slot, tb, T, cn); break;
-                            case 2: convert_chunk<VT, 2, RP, CC>(s_v, slot, tb, T, cn); break;
-                            case 3: convert_chunk<VT, 3, RP, CC>(s_v, slot, tb, T, cn); break;
-                            default: convert_chunk<VT, 4, RP, CC>(s_v, slot, tb, T, cn); break;
-                        }
-                    } else {
-                        if (P == 2)
-                            convert_chunk<VT, 2, RP, CC>(s_v, slot, tb, T, cn);
-                        else
-                            convert_chunk<VT, 4, RP, CC>(s_v, slot, tb, T, cn);
-                    }
+            bool streamed = false;
+            if constexpr (NP == 1) {
+                if (dma) {
+                    // This chunk's copy has landed (later copies may stay in flight) and the
+                    // previous chunk's chains are done.
+                    const LEAP_LDS f32x4* slot = ring.next_slot();
+                    __syncthreads();
+                    ring.convert(slot, tb, cn, [&](uint32_t r, uint32_t c, double re, double im) {
+                        s_v[r][0][c] = C2<double>{re, im};
+                    });
+                    __syncthreads();
+                    ring.issue();  // into the slot just converted
+                    streamed = true;
                 }
-                __syncthreads();
-                dma_next();  // into the slot just converted
-            } else {
+            }
+            if (!streamed) {
                 __syncthreads();
                 // Pol 0 of every element of the chunk, gathered a batch at a time (stage_pol0);
                 // tiles that keep several pols stage element by element.
