@@ -843,7 +843,8 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
     ctx->last_graph = 0;
     begin_timing(ctx);
     if (D > 0) upload_directions(ctx, md, dirs, D);
-    ctx->h_cal.resize(size_t(N) * D);
+    ctx->h_cal.reserve(std::max<size_t>(1, size_t(N) * D) * sizeof(double));
+    double* h_cal = ctx->h_cal.as<double>();
     for (int32_t s = 0; s < solutions; ++s) {  // :104
         // :113-119 — rows [s * interval * nbl, +interval * nbl); the slice start is not added.
         const uint32_t row_begin = uint32_t(size_t(s) * interval * nbl);
@@ -855,12 +856,12 @@ void calibrate_loop(leap_ctx* ctx, const leap_metadata& md, const leap_ms_view& 
         sol.n_directions = D;
         sol.directions = dirs;
         sol.n_antennas = uint32_t(N);
-        sol.calibrations = ctx->h_cal.data();
+        sol.calibrations = h_cal;
         if (D > 0) {
             const DeviceRows rows = stage_rows(ctx, ms, row_begin, row_end, !packed_route(opts, 0));
             const uint32_t ap = rotate_directions(ctx, md, ms, rows, interval, D, opts, 0, true);
             invert_directions(ctx, md, D, opts, nullptr, nullptr, ap);
-            LEAP_HIP(hipMemcpyAsync(ctx->h_cal.data(), ctx->d_cal.ptr, size_t(N) * D * sizeof(double),
+            LEAP_HIP(hipMemcpyAsync(h_cal, ctx->d_cal.ptr, size_t(N) * D * sizeof(double),
                                     hipMemcpyDeviceToHost, ctx->stream));
             LEAP_HIP(hipStreamSynchronize(ctx->stream));
         }

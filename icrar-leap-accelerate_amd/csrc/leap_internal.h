@@ -264,7 +264,9 @@ struct leap_ctx {
     leap::PinnedBuffer h_adj_out;
     hipEvent_t ev_grad[4] = {nullptr, nullptr, nullptr, nullptr};  // around the forward and adjoint kernels
     double last_grad_forward_ms = 0.0, last_grad_adjoint_ms = 0.0;
-    std::vector<double> h_avg, h_cal;
+    std::vector<double> h_avg;
+    // calibrations of the last solution interval copied back (pinned, so the copy is a direct DMA)
+    leap::PinnedBuffer h_cal;
     // pseudo-inverse cache keyed by a hash of A (use_cache)
     std::map<uint64_t, std::vector<double>> gram_cache;
     // autotuned direction blocks of the batched kernel
