@@ -222,7 +222,7 @@ constexpr int kFullTM = 2, kFullWaves = 8, kFullThreads = 64 * kFullWaves;
 constexpr int kFullMaxTiles = 16;                         // per dimension: 256 rows or columns
 constexpr int kFullBChunks = kFullMaxTiles * 3 * 64;      // 16-byte chunks of one K-step's B
 constexpr int kFullBPerThread = kFullBChunks / kFullThreads;
-constexpr int kFullSplits = 160;                          // K-splits of the whole-output kernel
+constexpr int kFullSplits = 256;                          // K-splits of the whole-output kernel (one per CU)
 static_assert(kFullBChunks % kFullThreads == 0 && kFullWaves * kFullTM == kFullMaxTiles, "tiling");
 
 __global__ __launch_bounds__(kFullThreads) void gemm_bf16x3_full(const bf16x8* __restrict__ Ap,
