@@ -14,6 +14,7 @@
 // 2^-24 relative.  The split planes are stored in the operand layout of
 // v_mfma_f32_16x16x32_bf16 (8 consecutive k of one row or column per lane), so
 // each lane loads 16 contiguous bytes per plane and K-step.
+#include "cube_ring.h"
 #include "inversion_common.h"
 #include "leap_internal.h"
 
@@ -213,15 +214,16 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_mfma(const bf16x8* __restrict
 // Whole-output variant for M, N <= 256 (Mt, Nt <= 16 tiles): a workgroup of 8
 // waves takes one K-split for all output tiles, so every operand byte comes
 // from HBM once per split.  Wave w owns m-tiles 2w, 2w + 1 and all n-tiles (32
-// tiles, 128 FP32 accumulators per lane).  Per K-step the workgroup stages the
-// B operand (all n-tiles, 3 planes, 48 KB) in LDS, double-buffered, while each
-// wave holds its A tiles in registers; the next K-step's A and B loads are in
-// flight during the current K-step's products.  Per tile and K-step the six
-// products go in the order of gemm_bf16x3_mfma.
+// tiles, 128 FP32 accumulators per lane).  The B operand of a K-step (all
+// n-tiles, 3 planes, 48 KB) is copied into LDS with global_load_lds, two
+// K-steps ahead, through three stages (144 KB, so one workgroup per CU); each
+// wave holds its A tiles in registers, loaded one K-step ahead.  Per tile and
+// K-step the six products go in the order of gemm_bf16x3_mfma.
 constexpr int kFullTM = 2, kFullWaves = 8, kFullThreads = 64 * kFullWaves;
 constexpr int kFullMaxTiles = 16;                         // per dimension: 256 rows or columns
 constexpr int kFullBChunks = kFullMaxTiles * 3 * 64;      // 16-byte chunks of one K-step's B
-constexpr int kFullBPerThread = kFullBChunks / kFullThreads;
+constexpr int kFullBCopies = kFullBChunks / kFullThreads;  // 1 KB copies per wave and K-step
+constexpr int kFullStages = 3;                            // the K-step in use and two copied ahead
 constexpr int kFullSplits = 256;                          // K-splits of the whole-output kernel (one per CU)
 static_assert(kFullBChunks % kFullThreads == 0 && kFullWaves * kFullTM == kFullMaxTiles, "tiling");
 
@@ -229,11 +231,12 @@ __global__ __launch_bounds__(kFullThreads) void gemm_bf16x3_full(const bf16x8* _
                                                                  const bf16x8* __restrict__ Bp, uint32_t Mt,
                                                                  int32_t Nt, int32_t Ks, int32_t ks_per_split,
                                                                  float* __restrict__ partial) {
-    __shared__ bf16x8 s_b[2][kFullBChunks];  // [buffer][(n-tile * 3 + plane) * 64 + lane]
+    __shared__ bf16x8 s_b[kFullStages][kFullBChunks];  // [stage][(n-tile * 3 + plane) * 64 + lane]
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid % 64;
-    const uint32_t m0 = (tid / 64) * kFullTM;
-    const bool rows = m0 < Mt;  // wave-uniform; a wave without rows still stages B and meets the barriers
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const uint32_t m0 = wave * kFullTM;
+    const bool rows = m0 < Mt;  // wave-uniform; a wave without rows still copies B and meets the barriers
     const int32_t split = blockIdx.x;
     const size_t a_total = size_t(Mt) * Ks * 64;
     const size_t b_total = size_t(Nt) * Ks * 64;
@@ -244,53 +247,57 @@ __global__ __launch_bounds__(kFullThreads) void gemm_bf16x3_full(const bf16x8* _
     for (int i = 0; i < kFullTM; ++i)
 #pragma unroll
         for (int j = 0; j < kFullMaxTiles; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // This thread's B chunks c = tid + kFullThreads r: n-tile c / 192, plane (c / 64) % 3, lane c % 64.
-    auto load_b = [&](int32_t ks, bf16x8* b) {
+    // B of K-step ks into stage st: wave w copies the 64-chunk groups g = w + 8 r (n-tile g / 3,
+    // plane g % 3), 1 KB each.  Groups of absent n-tiles, and K-steps past the split (whose stage is
+    // not read again), copy valid data (n-tile Nt - 1, K-step ks1 - 1) so that every wave issues
+    // exactly kFullBCopies copies per K-step and the waits below count them exactly.
+    auto copy_b = [&](int32_t ks, int st) {
+        const int32_t kc = ks < ks1 ? ks : ks1 - 1;
 #pragma unroll
-        for (int r = 0; r < kFullBPerThread; ++r) {
-            const uint32_t c = tid + kFullThreads * r;
-            const uint32_t nt = c / 192, p = (c / 64) % 3, l = c % 64;
-            if (int32_t(nt) < Nt) b[r] = Bp[p * b_total + (size_t(nt) * Ks + ks) * 64 + l];
+        for (int r = 0; r < kFullBCopies; ++r) {
+            const uint32_t g = wave + kFullWaves * r;
+            const uint32_t nt = g / 3, p = g % 3;
+            const uint32_t ntc = nt < uint32_t(Nt) ? nt : uint32_t(Nt) - 1;
+            const bf16x8* src = Bp + p * b_total + (size_t(ntc) * Ks + kc) * 64 + lane;
+            __builtin_amdgcn_global_load_lds((const LEAP_GLOBAL void*)src, (LEAP_LDS void*)(&s_b[st][g * 64]), 16, 0,
+                                             0);
         }
+        __builtin_amdgcn_sched_barrier(0);
     };
-    auto store_b = [&](int buf, const bf16x8* b) {
-#pragma unroll
-        for (int r = 0; r < kFullBPerThread; ++r) {
-            const uint32_t c = tid + kFullThreads * r;
-            if (int32_t(c / 192) < Nt) s_b[buf][c] = b[r];
-        }
-    };
+    // A of K-step ks (clamped to the split, as above) into a; a wave without rows loads m-tiles
+    // 0 and 1 and does not use them, so every wave issues the same loads
     auto load_a = [&](int32_t ks, bf16x8 (*a)[3]) {
-        if (!rows) return;
+        const int32_t kc = ks < ks1 ? ks : ks1 - 1;
+        const uint32_t ma = rows ? m0 : 0;
 #pragma unroll
         for (int i = 0; i < kFullTM; ++i) {
-            const bf16x8* t = Ap + (size_t(m0 + i) * Ks + ks) * 64;  // m0 + kFullTM <= Mt (a multiple of 4)
+            const bf16x8* t = Ap + (size_t(ma + i) * Ks + kc) * 64;  // ma + kFullTM <= Mt (a multiple of 4)
             a[i][0] = t[lane];
             a[i][1] = t[a_total + lane];
             a[i][2] = t[2 * a_total + lane];
         }
     };
-    bf16x8 a[kFullTM][3], an[kFullTM][3], b[kFullBPerThread];
-    if (ks0 < ks1) {
-        load_b(ks0, b);
+    bf16x8 a[kFullTM][3], an[kFullTM][3];
+    if (ks0 < ks1) {  // uniform
+        copy_b(ks0, 0);
+        copy_b(ks0 + 1, 1);
         load_a(ks0, a);
-        store_b(0, b);
+        __builtin_amdgcn_s_waitcnt(dev::kWaitVectorMem);
     }
-    __syncthreads();
     for (int32_t ks = ks0; ks < ks1; ++ks) {
-        const int buf = (ks - ks0) & 1;
-        const bool more = ks + 1 < ks1;  // uniform over the workgroup
-        if (more) {
-            load_b(ks + 1, b);
-            load_a(ks + 1, an);
-        }
+        const int st = (ks - ks0) % kFullStages;
+        // Past the barrier: this K-step's B is in stage st (every wave waited for its copies), and
+        // every wave is done with the previous K-step, whose stage the copy below refills.
+        __syncthreads();
+        load_a(ks + 1, an);
+        copy_b(ks + 2, (ks - ks0 + 2) % kFullStages);
         if (rows) {
 #pragma unroll
             for (int j = 0; j < kFullMaxTiles; ++j) {
                 if (j < Nt) {  // uniform
-                    const bf16x8 bh = s_b[buf][(j * 3) * 64 + lane];
-                    const bf16x8 bm = s_b[buf][(j * 3 + 1) * 64 + lane];
-                    const bf16x8 bl = s_b[buf][(j * 3 + 2) * 64 + lane];
+                    const bf16x8 bh = s_b[st][(j * 3) * 64 + lane];
+                    const bf16x8 bm = s_b[st][(j * 3 + 1) * 64 + lane];
+                    const bf16x8 bl = s_b[st][(j * 3 + 2) * 64 + lane];
                     // the kFullTM tiles' accumulation chains interleaved
 #pragma unroll
                     for (int i = 0; i < kFullTM; ++i)
@@ -313,14 +320,12 @@ __global__ __launch_bounds__(kFullThreads) void gemm_bf16x3_full(const bf16x8* _
                 }
             }
         }
-        if (more) {
-            store_b(buf ^ 1, b);  // that buffer was last read before the previous barrier
 #pragma unroll
-            for (int i = 0; i < kFullTM; ++i)
+        for (int i = 0; i < kFullTM; ++i)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) a[i][q] = an[i][q];
-        }
-        __syncthreads();
+            for (int q = 0; q < 3; ++q) a[i][q] = an[i][q];
+        // the next K-step's A and B (issued before the copies of the K-step after it) have landed
+        dev::wait_vmem_but(kFullBCopies);
     }
     if (!rows) return;
     const uint32_t M16 = Mt * 16;
