@@ -303,7 +303,7 @@ def main():
     ap.add_argument("--direct-cube", action="store_true",
                     help="batched: run the Horner kernel straight from the cube (no pol-0 pack)")
     ap.add_argument("--pack-timesteps", type=int, default=0, help="timesteps per pack chunk (0 = automatic)")
-    ap.add_argument("--channel-sum", default="auto", choices=["auto", "horner", "nufft", "nufft-fp32"],
+    ap.add_argument("--channel-sum", default="auto", choices=["auto", "horner", "nufft", "nufft-fp32", "nufft-w16"],
                     help="mixed precision: channel sums by FP64 Horner or the NUFFT (auto: NUFFT where supported)")
     ap.add_argument("--host-cube", action="store_true",
                     help="batched modes: the cube stays in (pageable) host memory and every step stages it "
@@ -366,7 +366,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     opts.lane_directions = args.lane_directions
     opts.direct_cube = 1 if args.direct_cube else 0
     opts.pack_timesteps = args.pack_timesteps
-    opts.channel_sum = {"auto": 0, "horner": 1, "nufft": 2, "nufft-fp32": 3}[args.channel_sum]
+    opts.channel_sum = {"auto": 0, "horner": 1, "nufft": 2, "nufft-fp32": 3, "nufft-w16": 4}[args.channel_sum]
 
     cfg = c1_config()
     log(f"[{mode}] building synthetic measurement set (host columns)")
@@ -417,8 +417,10 @@ def run_mode(args, mode, D, with_cpu, steps=None):
     k_ms = float(np.mean(rot_ms))
     tile = ctx.last_direction_block()
     route_cs, route_dc = ctx.last_route() if mode != "plain" else (0, 0)
-    nufft = mode == "batched-mixed" and route_cs in (la.LEAP_CHANNEL_SUM_NUFFT, la.LEAP_CHANNEL_SUM_NUFFT_FP32)
+    nufft = mode != "plain" and route_cs in (la.LEAP_CHANNEL_SUM_NUFFT, la.LEAP_CHANNEL_SUM_NUFFT_FP32,
+                                             la.LEAP_CHANNEL_SUM_NUFFT_W16)
     nufft32 = nufft and route_cs == la.LEAP_CHANNEL_SUM_NUFFT_FP32
+    nufft16 = nufft and route_cs == la.LEAP_CHANNEL_SUM_NUFFT_W16
     packed = (mode != "plain" and not args.fp32_blocks and not route_dc and tile in (641, 1282, 2564))
     if nufft:
         # NUFFT channel sums: per row one N-point complex FFT (5 N log2 N flops,
@@ -428,7 +430,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         # accumulation (FMA = 2 flops).  The roofline's bound is HBM: the
         # algorithmic bytes (the cube once, uvw, pol 0 of avgData) over the
         # kernel time against 8 TB/s; the vector-ALU rate is reported beside it.
-        plan = la.nufft_plan(C, fp32=nufft32)
+        plan = la.nufft_plan(C, fp32=nufft32, w16=nufft16)
         n_grid, width = plan["N"], plan["W"]
         per_row = 5 * n_grid * np.log2(n_grid)
         per_dir = 2 * (width * width // 2 + 2 * width) + NUFFT_DIR_FLOPS
@@ -438,8 +440,10 @@ def run_mode(args, mode, D, with_cpu, steps=None):
         # calibrations on this route keep pol 0 of avgData only (16 B per direction and baseline)
         alg_bytes = src_bytes + R * 24 + D * nbl * 16
         gbs = alg_bytes / (k_ms * 1e-3) / 1e9
-        kname = "rotate_average_nufft<float>" if nufft32 else "rotate_average_nufft<double>"
-        prof = "rotate_average_nufft32_c1" if nufft32 else "rotate_average_nufft_c1"
+        kname = ("rotate_average_nufft<float>" if nufft32 else
+                 "rotate_average_nufft<double>, width 16" if nufft16 else "rotate_average_nufft<double>")
+        prof = ("rotate_average_nufft32_c1" if nufft32 else
+                "rotate_average_nufft_w16_c2" if nufft16 else "rotate_average_nufft_c1")
         roofline = {"kernel": kname, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profile(prof + "_traffic.json"),
@@ -451,7 +455,8 @@ def run_mode(args, mode, D, with_cpu, steps=None):
                              "algorithmic_flops_per_launch": alg_flops},
                     "nufft": {"transform": "fp32" if nufft32 else "fp64", "grid": n_grid, "width": width,
                               "fft_flops_per_row": int(per_row), "flops_per_row_direction": per_dir,
-                              "source": "complex64 cube" if route_dc else "packed pol-0 rows"},
+                              "source": (("complex64" if elem == 8 else "complex128") + " cube") if route_dc
+                              else "packed pol-0 rows"},
                     # the direct sum's 8 flops per (row, channel, direction) over the kernel time
                     "direct_sum_equivalent_tflops": round(8 * R * C * D / (k_ms * 1e-3) / 1e12, 1)}
     elif mode == "plain":

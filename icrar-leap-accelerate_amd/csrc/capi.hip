@@ -41,12 +41,14 @@ leap_options resolve(const leap_options* o) {
     require(r.pack_timesteps >= 0, LEAP_ERR_INVALID_ARGUMENT, "pack_timesteps must be >= 0");
     require(r.workgroups_per_cu >= -1 && r.workgroups_per_cu <= 64, LEAP_ERR_INVALID_ARGUMENT,
             "workgroups per CU must be -1 (one work item each), 0 (tuned) or 1 to 64");
-    require(r.channel_sum >= LEAP_CHANNEL_SUM_AUTO && r.channel_sum <= LEAP_CHANNEL_SUM_NUFFT_FP32,
+    require(r.channel_sum >= LEAP_CHANNEL_SUM_AUTO && r.channel_sum <= LEAP_CHANNEL_SUM_NUFFT_W16,
             LEAP_ERR_INVALID_ARGUMENT,
-            "channel_sum must be 0 (automatic), 1 (Horner), 2 (NUFFT) or 3 (NUFFT, FP32 transform)");
+            "channel_sum must be 0 (automatic), 1 (Horner), 2 (NUFFT), 3 (NUFFT, FP32 transform) or 4 (NUFFT, "
+            "width 16)");
     if (is_nufft(r.channel_sum)) {
-        require(r.precision == LEAP_PRECISION_MIXED, LEAP_ERR_INVALID_ARGUMENT,
-                "the NUFFT channel sums are a mixed-precision method");
+        require(r.precision == LEAP_PRECISION_MIXED || r.channel_sum == LEAP_CHANNEL_SUM_NUFFT_W16,
+                LEAP_ERR_INVALID_ARGUMENT,
+                "the width-10 NUFFT channel sums are a mixed-precision method (FP64: channel_sum 4, width 16)");
         require(!r.all_pols && !r.fp32_blocks, LEAP_ERR_INVALID_ARGUMENT,
                 "the NUFFT channel sums run on pol 0 (no all_pols or fp32_blocks)");
     }
@@ -178,6 +180,16 @@ void upload_directions(leap_ctx* ctx, const leap_metadata& md, const double* dir
               ctx->d_dd.as<double>());
 }
 
+// Whether the NUFFT kernel can read the cube itself: a supported type and pol
+// count at an address its loads allow (16 bytes for complex128 and for
+// complex64 with 2 or 4 pols, whose channels it loads 16 bytes at a time; 8
+// for one complex64 pol).  Row offsets keep that alignment.
+bool nufft_cube_ok(const leap_ms_view& ms, bool w16) {
+    const uintptr_t align = ms.vis_dtype == LEAP_VIS_C128 || ms.num_pols >= 2 ? 16 : 8;
+    return nufft_cube_supported(ms.vis_dtype, ms.num_pols, w16) &&
+           (!ms.vis_on_device || reinterpret_cast<uintptr_t>(ms.vis) % align == 0);
+}
+
 // The batched pol-0 path through the pol-0 pack and the scalar-load Horner kernel.
 bool use_packed(const leap_options& opts) {
     return opts.strategy == LEAP_STRATEGY_BATCHED && !opts.all_pols &&
@@ -198,18 +210,23 @@ bool use_packed(const leap_options& opts) {
 // crosses the bus).
 leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, uint32_t D) {
     leap_options r = o;
-    if (r.channel_sum == LEAP_CHANNEL_SUM_AUTO)
-        r.channel_sum = use_packed(r) && r.precision == LEAP_PRECISION_MIXED && nufft_supported(ms.num_channels) &&
-                                ms.num_channels >= kNufftMinChannels && D >= kNufftMinDirections
-                            ? LEAP_CHANNEL_SUM_NUFFT
-                            : LEAP_CHANNEL_SUM_HORNER;
+    if (r.channel_sum == LEAP_CHANNEL_SUM_AUTO) {
+        const bool fp64 = r.precision == LEAP_PRECISION_FP64;
+        const bool nufft = use_packed(r) && nufft_supported(ms.num_channels) &&
+                           ms.num_channels >= kNufftMinChannels &&
+                           D >= (fp64 ? kNufftW16MinDirections : kNufftMinDirections);
+        r.channel_sum = !nufft ? LEAP_CHANNEL_SUM_HORNER : fp64 ? LEAP_CHANNEL_SUM_NUFFT_W16 : LEAP_CHANNEL_SUM_NUFFT;
+    }
     if (is_nufft(r.channel_sum)) {
         require(nufft_supported(ms.num_channels), LEAP_ERR_INVALID_ARGUMENT,
                 "the NUFFT channel sums support 1.." + std::to_string(kNufftMaxChannels) + " channels");
-        const bool cube_ok = nufft_cube_supported(ms.vis_dtype, ms.num_pols);
+        const bool w16 = r.channel_sum == LEAP_CHANNEL_SUM_NUFFT_W16;
+        const bool cube_ok = nufft_cube_ok(ms, w16);
         require(!r.direct_cube || cube_ok, LEAP_ERR_INVALID_ARGUMENT,
-                "the NUFFT kernel reads the cube directly only for complex64 with 1, 2 or 4 pols");
-        // A device-resident complex64 cube is read by the NUFFT kernel itself
+                w16 ? "the NUFFT kernel reads the cube directly only with 1, 2 or 4 pols at an aligned address"
+                    : "the NUFFT kernel reads the cube directly only for complex64 with 1, 2 or 4 pols at an "
+                      "aligned address");
+        // A device-resident cube of a supported type is read by the NUFFT kernel itself
         // (no pack pass) unless pack chunks were asked for; host cubes are packed
         // while they are staged (pol 0 only crosses the bus).
         if (cube_ok && ms.vis_on_device && r.pack_timesteps == 0) r.direct_cube = 1;
@@ -248,7 +265,8 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
     const bool nufft = is_nufft(opts.channel_sum);
     const TuneKey key{opts.precision * 4 + (opts.fp32_blocks ? 2 : 0) + (opts.lane_directions ? 1 : 0) +
                           16 * opts.lane_directions + (packed ? kTunePackedBit : 0) + (nufft ? kTuneNufftBit : 0) +
-                          (opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? kTuneNufft32Bit : 0),
+                          (opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? kTuneNufft32Bit : 0) +
+                          (opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT_W16 ? kTuneNufftW16Bit : 0),
                       ms.vis_dtype, opts.all_pols, D, T, md.channels, md.pols};
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) return it->second;
@@ -286,7 +304,8 @@ int32_t tuned_direction_block(leap_ctx* ctx, const leap_metadata& md, const leap
             if (nufft)
                 launch_rotate_nufft(ctx->stream, tile, md.nufft_plan(opts.channel_sum),
                                     md.nufft_tables(opts.channel_sum),
-                                    packed ? ctx->d_packed.ptr : rows.vis, !packed, rows.uvw, ctx->d_dd.as<double>(),
+                                    packed ? ctx->d_packed.ptr : rows.vis, !packed, ms.vis_dtype, rows.uvw,
+                                    ctx->d_dd.as<double>(),
                                     D, T, md.nbl, probe_bl, md.channels, md.pols, md.pols, md.freq_start_hz,
                                     md.freq_inc_hz, ctx->d_avg.as<double>(), 0);
             else if (packed)
@@ -366,7 +385,7 @@ void rotate_packed_pipeline(leap_ctx* ctx, const leap_metadata& md, const leap_m
         record_rotate_begin(ctx);
         if (nufft)
             launch_rotate_nufft(ctx->stream, tile, md.nufft_plan(opts.channel_sum),
-                                md.nufft_tables(opts.channel_sum), buf[k % 2], false,
+                                md.nufft_tables(opts.channel_sum), buf[k % 2], false, ms.vis_dtype,
                                 rows.uvw + size_t(t0) * md.nbl * 3, ctx->d_dd.as<double>(), D, nt, md.nbl, md.nbl,
                                 md.channels, md.pols, avg_pols, md.freq_start_hz, md.freq_inc_hz,
                                 ctx->d_avg.as<double>(), k > 0 ? 1 : 0, ctx->d_nufft_tail.as<double>(),
@@ -443,7 +462,7 @@ uint32_t rotate_directions(leap_ctx* ctx, const leap_metadata& md, const leap_ms
         record_rotate_begin(ctx);
         ctx->d_nufft_tail.reserve(nufft_tail_bytes(D));
         launch_rotate_nufft(ctx->stream, db, md.nufft_plan(opts.channel_sum), md.nufft_tables(opts.channel_sum),
-                            rows.vis, true, rows.uvw,
+                            rows.vis, true, ms.vis_dtype, rows.uvw,
                             ctx->d_dd.as<double>(), D, T, md.nbl, md.nbl, md.channels, md.pols, avg_pols,
                             md.freq_start_hz, md.freq_inc_hz, ctx->d_avg.as<double>(), 0,
                             ctx->d_nufft_tail.as<double>(), nufft_tail_bytes(D));
@@ -658,17 +677,22 @@ leap_metadata* create_metadata(leap_ctx* ctx, const leap_ms_view& ms, int32_t re
     LEAP_HIP(hipStreamSynchronize(s));  // the host vectors above end with this function
     md->d_AdT_planes.reserve(bf16_planes_bytes(uint32_t(N), K));
     launch_split_b(s, md->d_AdT.as<double>(), K, N, md->d_AdT_planes.ptr);
-    std::vector<double> tables32, tables64;
+    std::vector<double> tables32, tables64, tables16;
     if (nufft_supported(md->channels)) {
         md->nufft32 = make_nufft_plan(md->channels, true);
         md->nufft64 = make_nufft_plan(md->channels, false);
+        md->nufft16 = make_nufft_plan(md->channels, false, true);
         tables32 = nufft_tables(md->nufft32);
         tables64 = nufft_tables(md->nufft64);
+        tables16 = nufft_tables(md->nufft16);
         md->d_nufft32.reserve(tables32.size() * sizeof(double));
         md->d_nufft64.reserve(tables64.size() * sizeof(double));
+        md->d_nufft16.reserve(tables16.size() * sizeof(double));
         LEAP_HIP(hipMemcpyAsync(md->d_nufft32.ptr, tables32.data(), tables32.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
         LEAP_HIP(hipMemcpyAsync(md->d_nufft64.ptr, tables64.data(), tables64.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+        LEAP_HIP(hipMemcpyAsync(md->d_nufft16.ptr, tables16.data(), tables16.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
     }
     LEAP_HIP(hipStreamSynchronize(s));
@@ -687,7 +711,7 @@ void reserve_solution(leap_ctx* ctx, const leap_metadata& md, uint32_t D, uint32
         const uint32_t tc = pack_chunk(T, md.nbl, md.channels, opts);
         ctx->d_packed.reserve(size_t(tc) * md.nbl * md.channels * (T > tc ? 2 : 1) * 2 * sizeof(double));
     }
-    if (opts.strategy == LEAP_STRATEGY_BATCHED && opts.precision == LEAP_PRECISION_MIXED)
+    if (opts.strategy == LEAP_STRATEGY_BATCHED)
         ctx->d_nufft_tail.reserve(nufft_tail_bytes(D));  // the NUFFT route's chunked last round
     ctx->d_phase.reserve(size_t(K1) * D * sizeof(double));
     ctx->d_cal1.reserve(size_t(N) * D * sizeof(double));
@@ -933,11 +957,11 @@ struct NufftGroup {
 // the NUFFT is asked for.
 bool set_takes_nufft(const leap_options& opts, const leap_metadata& md, const leap_ms_view& ms, uint32_t D) {
     if (opts.precision != LEAP_PRECISION_MIXED || D == 0) return false;
-    const bool cube_ok = nufft_cube_supported(ms.vis_dtype, ms.num_pols) && md.nufft64.N != 0;
+    const bool cube_ok = nufft_cube_ok(ms, false) && md.nufft64.N != 0;
     if (opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
         require(cube_ok, LEAP_ERR_INVALID_ARGUMENT,
                 "the multi-set NUFFT route needs 1.." + std::to_string(kNufftMaxChannels) +
-                    " channels and a complex64 cube with 1, 2 or 4 pols");
+                    " channels and a complex64 cube with 1, 2 or 4 pols at an aligned address");
         return true;
     }
     return opts.channel_sum == LEAP_CHANNEL_SUM_AUTO && cube_ok && md.channels >= kNufftMinChannels &&
@@ -1151,8 +1175,9 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             "several measurement sets in one launch need the batched strategy");
     require(!opts.all_pols && !opts.fp32_blocks, LEAP_ERR_NOT_IMPLEMENTED,
             "the multi-set launch accumulates polarisation 0 with the FP64 Horner or NUFFT channel sums only");
-    require(opts.channel_sum != LEAP_CHANNEL_SUM_NUFFT_FP32, LEAP_ERR_NOT_IMPLEMENTED,
-            "the multi-set launch takes the NUFFT with its FP64 transform (channel_sum 2)");
+    require(opts.channel_sum != LEAP_CHANNEL_SUM_NUFFT_FP32 && opts.channel_sum != LEAP_CHANNEL_SUM_NUFFT_W16,
+            LEAP_ERR_NOT_IMPLEMENTED, "the multi-set launch takes the width-10 NUFFT with its FP64 transform "
+                                      "(channel_sum 2)");
     std::vector<BatchState> st(n_sets);
     int32_t rounds = 0;
     for (uint32_t i = 0; i < n_sets; ++i) {
@@ -1746,29 +1771,38 @@ int leap_phasor64(const double* theta, uint32_t n, double* cos_out, double* sin_
 }
 
 namespace {
-int nufft_plan_out(uint32_t C, bool fp32, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
-                   double* scale) {
+// coef: (width / 2) x width doubles, coef[j * width + m] (width 12 for the
+// width-10/12 plans, 16 for the width-16 plan).
+int nufft_plan_out(uint32_t C, bool fp32, bool w16, uint32_t* N, uint32_t* W, uint32_t* h, double* beta,
+                   double* coef, double* scale) {
     return guarded(nullptr, [&] {
-        const NufftPlan p = make_nufft_plan(C, fp32);
+        const NufftPlan p = make_nufft_plan(C, fp32, w16);
         if (N) *N = p.N;
         if (W) *W = p.W;
         if (h) *h = p.h;
         if (beta) *beta = p.beta;
+        const int width = w16 ? 16 : 12;
+        static_assert(kNufftMaxWidth >= 16, "plan coefficients hold the width-16 kernel");
         if (coef)
-            for (int j = 0; j < kNufftMaxWidth / 2; ++j)
-                for (int m = 0; m < kNufftMaxWidth; ++m) coef[j * kNufftMaxWidth + m] = p.coef[j][m];
+            for (int j = 0; j < width / 2; ++j)
+                for (int m = 0; m < width; ++m) coef[j * width + m] = p.coef[j][m];
         if (scale) std::copy(p.scale.begin(), p.scale.end(), scale);
     });
 }
 }  // namespace
 
 int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef, double* scale) {
-    return nufft_plan_out(C, false, N, W, h, beta, coef, scale);
+    return nufft_plan_out(C, false, false, N, W, h, beta, coef, scale);
 }
 
 int leap_nufft_plan32(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
                       double* scale) {
-    return nufft_plan_out(C, true, N, W, h, beta, coef, scale);
+    return nufft_plan_out(C, true, false, N, W, h, beta, coef, scale);
+}
+
+int leap_nufft_plan_w16(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
+                        double* scale) {
+    return nufft_plan_out(C, false, true, N, W, h, beta, coef, scale);
 }
 
 double leap_multi_item_cost(uint32_t T, uint32_t C, int32_t tile) {

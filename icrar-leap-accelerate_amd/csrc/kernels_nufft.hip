@@ -34,7 +34,7 @@ uint64_t nufft_sets_key(const NufftPlan& plan, uint32_t C, uint32_t P) {
 void launch_rotate_nufft_sets(hipStream_t s, int32_t tile, const NufftPlan& plan, uint32_t C, uint32_t P,
                               const NufftSetDesc* sets, const uint32_t* items, uint32_t n_items) {
     if (n_items == 0) return;
-    require(!plan.fp32, LEAP_ERR_NOT_IMPLEMENTED, "the multi-set NUFFT launch uses the FP64 transform");
+    require(!plan.fp32 && !plan.w16, LEAP_ERR_NOT_IMPLEMENTED, "the multi-set NUFFT launch uses the width-10 or 12 FP64 plan");
     require(P == 1 || P == 2 || P == 4, LEAP_ERR_INTERNAL, "the multi-set NUFFT launch reads complex64 cubes");
 #define LEAP_NUFFT_SETS(N_, W_)                                                                  \
     if (plan.N == N_ && plan.W == W_) {                                                          \
@@ -61,23 +61,24 @@ size_t nufft_tail_bytes(uint32_t D) {
     return 2 * size_t(16) * size_t(std::max(1, cus)) * D * 2 * sizeof(double);
 }
 
-bool nufft_cube_supported(int32_t vis_dtype, uint32_t P) {
-    return vis_dtype == LEAP_VIS_C64 && (P == 1 || P == 2 || P == 4);
+bool nufft_cube_supported(int32_t vis_dtype, uint32_t P, bool w16) {
+    return (vis_dtype == LEAP_VIS_C64 || (w16 && vis_dtype == LEAP_VIS_C128)) && (P == 1 || P == 2 || P == 4);
 }
 
 void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
-                         bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
-                         uint32_t nbl_grid, uint32_t C, uint32_t P, uint32_t P_out, double f0, double df,
-                         double* avg, int32_t accumulate, double* tail, size_t tail_bytes) {
+                         bool cube, int32_t vis_dtype, const double* uvw, const double* dd, uint32_t D, uint32_t T,
+                         uint32_t nbl, uint32_t nbl_grid, uint32_t C, uint32_t P, uint32_t P_out, double f0,
+                         double df, double* avg, int32_t accumulate, double* tail, size_t tail_bytes) {
     if (nbl_grid == 0 || D == 0) return;
     require(P_out >= 1, LEAP_ERR_INTERNAL, "NUFFT output needs at least pol 0");
     require(plan.N != 0 && plan.scale.size() == C, LEAP_ERR_INTERNAL, "NUFFT plan does not match the channel count");
-    require(!cube || (P == 1 || P == 2 || P == 4), LEAP_ERR_INTERNAL, "NUFFT cube source needs 1, 2 or 4 pols");
+    require(!cube || nufft_cube_supported(vis_dtype, P, plan.w16), LEAP_ERR_INTERNAL,
+            "NUFFT cube source needs 1, 2 or 4 pols of complex64 (complex128 for the width-16 kernel)");
     nufft::NufftLaunch l{};
     l.tile = tile;
     l.plan = &plan;
     l.src = src;
-    l.src_pols = cube ? int32_t(P) : 0;
+    l.src_pols = cube ? int32_t(P) + (vis_dtype == LEAP_VIS_C128 ? 16 : 0) : 0;
     l.scale = tables + kNufftCoefDoubles;
     l.uvw = uvw;
     l.dd = dd;
@@ -96,7 +97,9 @@ void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, con
     l.accumulate = accumulate;
     l.tail = tail;
     l.tail_doubles = tail_bytes / sizeof(double);
-    if (plan.fp32)
+    if (plan.w16)
+        launch_rotate_nufft_w16(s, l);
+    else if (plan.fp32)
         launch_rotate_nufft_f32(s, l);
     else
         launch_rotate_nufft_f64(s, l);

@@ -111,15 +111,20 @@ MultiPlan plan_multi(const std::vector<MultiShape>& sets, int32_t tile, uint32_t
 
 // NUFFT channel sums (nufft_plan.cpp).
 constexpr uint32_t kNufftMaxChannels = 512;  // grid N <= 1024
-constexpr int kNufftMaxWidth = 12;
+constexpr int kNufftMaxWidth = 16;
 constexpr int kNufftQuadratureNodes = 100;
 constexpr int kNufftCoefDoubles = (kNufftMaxWidth / 2) * kNufftMaxWidth;
 // Automatic choice: fewer directions or channels stay on Horner (per row the FFT
 // costs about what 4 C FMAs per lane cost at C = 64..96).
 constexpr uint32_t kNufftMinDirections = 16;
 constexpr uint32_t kNufftMinChannels = 96;
+// FP64 precision: the width-16 NUFFT from this many directions (per row its
+// FFT and 16-point interpolations cost what the FP64 Horner chains of about
+// 20 directions cost at 384 channels).
+constexpr uint32_t kNufftW16MinDirections = 32;
 struct NufftPlan {
     bool fp32 = false;  // FP32 transform, grid and interpolation (W 8 or 10) instead of FP64 (W 10 or 12)
+    bool w16 = false;   // FP64 transform with kernel width 16 (LEAP_CHANNEL_SUM_NUFFT_W16)
     uint32_t N = 0;  // grid size (power of two, >= 2C, >= 256)
     uint32_t W = 0;  // kernel width in grid points
     uint32_t h = 0;  // 64 * round(C / 128): frequency k = c - h
@@ -130,9 +135,10 @@ struct NufftPlan {
 };
 bool nufft_supported(uint32_t C);
 inline bool is_nufft(int32_t channel_sum) {
-    return channel_sum == LEAP_CHANNEL_SUM_NUFFT || channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32;
+    return channel_sum == LEAP_CHANNEL_SUM_NUFFT || channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ||
+           channel_sum == LEAP_CHANNEL_SUM_NUFFT_W16;
 }
-NufftPlan make_nufft_plan(uint32_t C, bool fp32);
+NufftPlan make_nufft_plan(uint32_t C, bool fp32, bool w16 = false);
 // The plan's device tables: kNufftCoefDoubles coefficients (coef[j][m]), the C
 // precorrection factors, then the coefficients rounded to FP32 (from double
 // nufft_coef32_offset(C) on).
@@ -163,15 +169,21 @@ struct leap_metadata {
     leap::DeviceBuffer d_Gp;       // G+ = pinv(A^T A) then G1+ = pinv(A1^T A1), N x N each
     leap::DeviceBuffer d_AdT_planes;  // split-3 x BF16 planes of AdT in MFMA operand layout
     // NUFFT plans (N == 0 when the channel count is not supported) and their
-    // device tables (leap::nufft_tables): FP64 transform (LEAP_CHANNEL_SUM_NUFFT)
-    // and FP32 transform (LEAP_CHANNEL_SUM_NUFFT_FP32).
-    leap::NufftPlan nufft32, nufft64;
-    leap::DeviceBuffer d_nufft32, d_nufft64;
+    // device tables (leap::nufft_tables): FP64 transform (LEAP_CHANNEL_SUM_NUFFT),
+    // FP32 transform (LEAP_CHANNEL_SUM_NUFFT_FP32) and FP64 transform with the
+    // width-16 kernel (LEAP_CHANNEL_SUM_NUFFT_W16).
+    leap::NufftPlan nufft32, nufft64, nufft16;
+    leap::DeviceBuffer d_nufft32, d_nufft64, d_nufft16;
     const leap::NufftPlan& nufft_plan(int32_t channel_sum) const {
-        return channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? nufft32 : nufft64;
+        return channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? nufft32
+               : channel_sum == LEAP_CHANNEL_SUM_NUFFT_W16 ? nufft16
+                                                           : nufft64;
     }
     const double* nufft_tables(int32_t channel_sum) const {
-        return (channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? d_nufft32 : d_nufft64).as<double>();
+        return (channel_sum == LEAP_CHANNEL_SUM_NUFFT_FP32 ? d_nufft32
+                : channel_sum == LEAP_CHANNEL_SUM_NUFFT_W16 ? d_nufft16
+                                                            : d_nufft64)
+            .as<double>();
     }
     const double* nufft_scale(int32_t channel_sum) const {
         return nufft_tables(channel_sum) + leap::kNufftCoefDoubles;
@@ -183,6 +195,7 @@ struct leap_metadata {
 constexpr int32_t kTunePackedBit = 1024;
 constexpr int32_t kTuneNufftBit = 2048;
 constexpr int32_t kTuneNufft32Bit = 4096;
+constexpr int32_t kTuneNufftW16Bit = 8192;
 
 struct TuneKey {
     int32_t precision, vis_dtype, all_pols;
@@ -320,16 +333,18 @@ void launch_pack_pol0(hipStream_t s, const void* vis, int32_t vis_dtype, size_t 
 // x[i] *= scale[i % C] for n packed FP64 complex values (whole rows).
 void launch_scale_channels(hipStream_t s, double* x, size_t n, const double* scale, uint32_t C);
 // NUFFT channel sums (kernels_nufft.hip) over precorrected packed pol-0 rows
-// (cube false) or straight from a complex64 cube with 1, 2 or 4 pols (cube
-// true; scale = the plan's precorrection factors on the device); tiles
-// 641 / 1282 / 2564 as for launch_rotate_packed.
+// (cube false) or straight from a cube of vis_dtype with 1, 2 or 4 pols (cube
+// true; complex64 for every plan, complex128 for the width-16 plan; scale =
+// the plan's precorrection factors on the device); tiles 641 / 1282 / 2564 as
+// for launch_rotate_packed.
 // tables: the plan on the device, kNufftCoefDoubles coefficients (coef[j][m])
 // then the C precorrection factors.
-bool nufft_cube_supported(int32_t vis_dtype, uint32_t P);
+bool nufft_cube_supported(int32_t vis_dtype, uint32_t P, bool w16);
 // avg holds P_out pols per (direction, baseline) (pol 0 summed, the others
 // set to zero unless accumulating); P is the cube's pol count.
 void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, const double* tables, const void* src,
-                         bool cube, const double* uvw, const double* dd, uint32_t D, uint32_t T, uint32_t nbl,
+                         bool cube, int32_t vis_dtype, const double* uvw, const double* dd, uint32_t D, uint32_t T,
+                         uint32_t nbl,
                          uint32_t nbl_grid, uint32_t C, uint32_t P, uint32_t P_out, double f0, double df,
                          double* avg, int32_t accumulate, double* tail = nullptr, size_t tail_bytes = 0);
 // Scratch for launch_rotate_nufft's chunked last round (D directions, a single

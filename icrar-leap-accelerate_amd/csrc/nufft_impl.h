@@ -62,6 +62,7 @@ using namespace dev;
 using Z = C2<double>;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kWave = 64;
 
@@ -576,7 +577,10 @@ __device__ inline void turn_phasor_n(const Z* table, const double* dl, double sc
 //   P >= 1: the complex64 cube [row][channel][pol]: a channel whose 2P floats
 //           are not all finite is zero (CpuLeapCalibrator.cc:201-214), pol 0 is
 //           widened and multiplied by the channel's precorrection factor.
-template <int N, int P, int CPL>
+//   P = 16 + Q (Q = 1, 2, 4): the complex128 cube with Q pols, the same rule.
+// FULL (complex128 only): every pol loaded whole (16-byte loads) instead of
+// the upper words of pols 1 ..
+template <int N, int P, int CPL, bool WIDE = (P > 16), bool FULL = false>
 struct Source {
     float raw[CPL][2 * P];
     double sc[CPL];    // precorrection, 0 past C
@@ -627,8 +631,79 @@ struct Source {
     }
 };
 
+// The complex128 cube.  A lane's channels lie 16 Q bytes apart, so each load
+// instruction of the wave touches a cache line per lane or two, and the
+// vector-memory pipeline's work grows with the number of load instructions.
+// FULL: every pol is loaded whole with one 16-byte load (Q loads and 4 Q
+// registers per channel slot); otherwise pol 0 is loaded whole and the other
+// pols only as the upper words of their real and imaginary parts (sign,
+// exponent and the top of the mantissa: enough to tell whether the value is
+// finite), 2 Q - 1 loads and 4 + 2 (Q - 1) registers per slot.  Each row's
+// bytes cross HBM once either way (the words share the channel's lines).
+template <int N, int SP, int CPL, bool FULL>
+struct Source<N, SP, CPL, true, FULL> {
+    static constexpr int Q = SP - 16;
+    static_assert(Q == 1 || Q == 2 || Q == 4, "complex128 cube with 1, 2 or 4 pols");
+    static constexpr int NH = Q > 1 && !FULL ? 2 * (Q - 1) : 1;
+    static constexpr int NP = FULL ? Q : 1;
+    double re[CPL][NP], im[CPL][NP];  // pol 0, then (FULL) pols 1 .. Q - 1
+    uint32_t hi[CPL][NH];             // upper words of the real and imaginary parts of pols 1 .. Q - 1
+    double sc[CPL];        // precorrection, 0 past C
+    uint32_t off[CPL];     // double offset of the channel (clamped) in a row
+    __device__ void init(const LEAP_GLOBAL double* scale, uint32_t lane, uint32_t C) {
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) {
+            const uint32_t c = lane + kWave * i;
+            sc[i] = c < C ? scale[c] : 0.0;
+            off[i] = min(c, C - 1) * (2 * Q);
+        }
+    }
+    __device__ static const void* advance(const void* base, size_t rows, uint32_t C) {
+        return (const double*)base + rows * C * (2 * Q);
+    }
+    __device__ void load(const void* base, size_t row, uint32_t, uint32_t C) {
+        const LEAP_GLOBAL double* r = (const LEAP_GLOBAL double*)base + row * C * (2 * Q);
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) {
+            const LEAP_GLOBAL double* q = r + off[i];
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const f64x2 pk = ((const LEAP_GLOBAL f64x2*)q)[k];
+                re[i][k] = pk.x;
+                im[i][k] = pk.y;
+            }
+            if constexpr (Q > 1 && !FULL) {
+                const LEAP_GLOBAL uint32_t* w = (const LEAP_GLOBAL uint32_t*)(q + 2);
+#pragma unroll
+                for (int k = 0; k < Q - 1; ++k) {
+                    hi[i][2 * k] = w[4 * k + 1];
+                    hi[i][2 * k + 1] = w[4 * k + 3];
+                }
+            }
+        }
+    }
+    template <typename F>
+    __device__ void freq(C2<F>* x, uint32_t, uint32_t) const {
+        constexpr uint32_t kExp = 0x7ff00000u;  // an all-ones exponent: Inf or NaN
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) {
+            bool finite = true;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) finite = finite && __builtin_isfinite(re[i][k]) && __builtin_isfinite(im[i][k]);
+            if constexpr (Q > 1 && !FULL) {
+                uint32_t e = 0;
+#pragma unroll
+                for (int k = 0; k < NH; ++k) e = max(e, hi[i][k] & kExp);
+                finite = finite && e != kExp;
+            }
+            const double a = finite ? re[i][0] : 0.0, b = finite ? im[i][0] : 0.0;
+            x[i] = to_transform<F>(a * sc[i], b * sc[i]);
+        }
+    }
+};
+
 template <int N, int CPL>
-struct Source<N, 0, CPL> {
+struct Source<N, 0, CPL, false, false> {
     Z raw[CPL];
     __device__ void init(const LEAP_GLOBAL double*, uint32_t, uint32_t) {}
     __device__ static const void* advance(const void* base, size_t rows, uint32_t C) {
@@ -695,7 +770,10 @@ __device__ __forceinline__ void nufft_rows(const void* wr, const LEAP_GLOBAL dou
     // and stay in flight while it is transformed and interpolated.  The load is
     // unconditional (the last row reloads itself), so the row buffer keeps its
     // registers from one iteration to the next.
-    Source<N, SP, CPL> src;
+    // complex128 cubes: whole-pol loads where the registers allow it at two
+    // waves per SIMD (one direction per lane, the 1024-point grid, at most 6
+    // channel slots: 249 VGPRs with 4 pols)
+    Source<N, SP, CPL, (SP > 16), (SP > 16 && NDL == 1 && N == 1024 && CPL <= 6)> src;
     src.init(scale, lane, C);
     if (t_count > 0) src.load(wr, bl, lane, C);
     for (uint32_t t = 0; t < t_count; ++t) {
@@ -951,12 +1029,14 @@ using IC = std::integral_constant<int, V>;
 // of the two slots that are padding.
 template <int N, int W, typename F>
 constexpr int nufft_slots(uint32_t C) {
-    if constexpr (N == 1024 && W == 10 && std::is_same_v<F, double>) return C <= 6 * kWave ? 6 : 8;
+    if constexpr (N == 1024 && (W == 10 || W == 16) && std::is_same_v<F, double>) return C <= 6 * kWave ? 6 : 8;
     return N / (2 * kWave);
 }
 
 // Calls fn(IC<NDL>, IC<SP>, IC<CPL>) for a runtime tile (64 NDL directions per
-// workgroup: 641, 1282, 2564), row source (src_pols) and channel count.
+// workgroup: 641, 1282, 2564), row source (src_pols: 0 packed rows, 1 / 2 / 4
+// complex64 pols, 17 / 18 / 20 complex128 pols, the last for the width-16
+// kernel only) and channel count.
 template <int N, int W, typename F, class Fn>
 void dispatch_nufft(int32_t tile, int32_t src_pols, uint32_t C, Fn&& fn) {
     auto with_ndl = [&](auto sp, auto cpl) {
@@ -968,7 +1048,7 @@ void dispatch_nufft(int32_t tile, int32_t src_pols, uint32_t C, Fn&& fn) {
         }
     };
     auto with_cpl = [&](auto sp) {
-        if constexpr (N == 1024 && W == 10 && std::is_same_v<F, double>) {
+        if constexpr (N == 1024 && (W == 10 || W == 16) && std::is_same_v<F, double>) {
             if (nufft_slots<N, W, F>(C) == 6) {
                 with_ndl(sp, IC<6>{});
                 return;
@@ -981,6 +1061,19 @@ void dispatch_nufft(int32_t tile, int32_t src_pols, uint32_t C, Fn&& fn) {
         case 1: with_cpl(IC<1>{}); break;
         case 2: with_cpl(IC<2>{}); break;
         case 4: with_cpl(IC<4>{}); break;
+        case 17:
+        case 18:
+        case 20:
+            if constexpr (W == 16) {
+                if (src_pols == 17)
+                    with_cpl(IC<17>{});
+                else if (src_pols == 18)
+                    with_cpl(IC<18>{});
+                else
+                    with_cpl(IC<20>{});
+                break;
+            }
+            [[fallthrough]];
         default: throw Error(LEAP_ERR_INTERNAL, "unsupported NUFFT row source");
     }
 }
@@ -1041,7 +1134,7 @@ struct NufftLaunch {
     int32_t tile;
     const NufftPlan* plan;
     const void* src;
-    int32_t src_pols;  // 0: packed rows; 1 / 2 / 4: complex64 cube
+    int32_t src_pols;  // 0: packed rows; 1 / 2 / 4: complex64 cube; 17 / 18 / 20: complex128 cube
     const double* scale;
     const double* uvw;
     const double* dd;
@@ -1059,5 +1152,7 @@ struct NufftLaunch {
 // kernels_nufft32.hip: float).
 void launch_rotate_nufft_f64(hipStream_t s, const nufft::NufftLaunch& l);
 void launch_rotate_nufft_f32(hipStream_t s, const nufft::NufftLaunch& l);
+// The width-16 kernel (kernels_nufft_w16.hip).
+void launch_rotate_nufft_w16(hipStream_t s, const nufft::NufftLaunch& l);
 
 }  // namespace leap

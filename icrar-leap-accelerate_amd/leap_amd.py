@@ -23,6 +23,7 @@ LEAP_CHANNEL_SUM_AUTO = 0
 LEAP_CHANNEL_SUM_HORNER = 1
 LEAP_CHANNEL_SUM_NUFFT = 2
 LEAP_CHANNEL_SUM_NUFFT_FP32 = 3
+LEAP_CHANNEL_SUM_NUFFT_W16 = 4
 LEAP_STRATEGY_PER_DIRECTION = 0
 LEAP_STRATEGY_BATCHED = 1
 LEAP_ADJOINT_AUTO = 0
@@ -183,7 +184,7 @@ EXPORTS = [
     "leap_hip_set_adjoint_mode",
     "leap_hip_tuning_save", "leap_hip_tuning_load", "leap_phase_matrix", "leap_phase_matrix_pinv",
     "leap_hip_residual_loss_grad", "leap_synth_enu_to_uvw", "leap_hip_calibrate_batch", "leap_multi_schedule",
-    "leap_multi_item_cost", "leap_phasor64", "leap_nufft_plan", "leap_nufft_plan32",
+    "leap_multi_item_cost", "leap_phasor64", "leap_nufft_plan", "leap_nufft_plan32", "leap_nufft_plan_w16",
 ]
 
 
@@ -228,6 +229,7 @@ def lib():
     _u32p = ctypes.POINTER(ctypes.c_uint32)
     L.leap_nufft_plan.argtypes = [ctypes.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
     L.leap_nufft_plan32.argtypes = [ctypes.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
+    L.leap_nufft_plan_w16.argtypes = [ctypes.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
     L.leap_multi_item_cost.restype = ctypes.c_double
     L.leap_hip_metadata_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(MsView), ctypes.c_int32,
                                            ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
@@ -486,19 +488,23 @@ def multi_item_cost(T, C, tile):
     return lib().leap_multi_item_cost(T, C, tile)
 
 
-def nufft_plan(C, fp32=False):
-    """The NUFFT plan the mixed path uses for C channels (fp32: the FP32 transform of
-    LEAP_CHANNEL_SUM_NUFFT_FP32, else the FP64 one of LEAP_CHANNEL_SUM_NUFFT): dict of N, W,
-    h, beta, coef (W/2 x W weight-polynomial coefficients in s) and scale (C precorrection factors)."""
+def nufft_plan(C, fp32=False, w16=False):
+    """The NUFFT plan of C channels (fp32: the FP32 transform of LEAP_CHANNEL_SUM_NUFFT_FP32;
+    w16: the width-16 kernel of LEAP_CHANNEL_SUM_NUFFT_W16; else the FP64 transform of
+    LEAP_CHANNEL_SUM_NUFFT): dict of N, W, h, beta, coef (W/2 x W weight-polynomial
+    coefficients in s) and scale (C precorrection factors)."""
+    if fp32 and w16:
+        raise ValueError("the width-16 plan has an FP64 transform")
     N, W, h = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
     beta = ctypes.c_double()
-    coef = np.zeros(6 * 12)
+    width = 16 if w16 else 12
+    coef = np.zeros((width // 2) * width)
     scale = np.zeros(max(int(C), 1))
-    fn = lib().leap_nufft_plan32 if fp32 else lib().leap_nufft_plan
+    fn = lib().leap_nufft_plan_w16 if w16 else lib().leap_nufft_plan32 if fp32 else lib().leap_nufft_plan
     _check(fn(C, ctypes.byref(N), ctypes.byref(W), ctypes.byref(h), ctypes.byref(beta), _dptr(coef), _dptr(scale)))
     w = W.value
     return {"N": N.value, "W": w, "h": h.value, "beta": beta.value,
-            "coef": coef.reshape(6, 12)[: w // 2, :w].copy(), "scale": scale[: int(C)].copy()}
+            "coef": coef.reshape(width // 2, width)[: w // 2, :w].copy(), "scale": scale[: int(C)].copy()}
 
 
 # ---------------------------------------------------------------------------

@@ -91,7 +91,10 @@ typedef struct leap_ms_view {
 } leap_ms_view;
 
 typedef enum leap_precision {
-    /* FP64 phase, FP64 sincos, FP64 sums: the reference's arithmetic. */
+    /* FP64 phase, FP64 sincos, FP64 sums: the reference's arithmetic.  Batched pol-0
+     * calls with 96..512 channels and at least 32 directions sum the channels with the
+     * width-16 NUFFT by default (LEAP_CHANNEL_SUM_AUTO; see LEAP_CHANNEL_SUM_NUFFT_W16),
+     * other calls with the FP64 Horner chains. */
     LEAP_PRECISION_FP64 = 0,
     /* Visibilities stored as complex64 (or converted on load), the phase
      * argument and its reduction in FP64, Ad * dphi as split-3 x BF16 MFMA
@@ -113,8 +116,8 @@ typedef enum leap_strategy {
 } leap_strategy;
 
 typedef enum leap_channel_sum {
-    /* Mixed precision, pol 0, 96..512 channels and at least 16 directions: the NUFFT;
-     * otherwise Horner. */
+    /* Pol 0 and 96..512 channels: mixed precision from 16 directions the NUFFT, FP64
+     * precision from 32 directions the width-16 NUFFT; otherwise Horner. */
     LEAP_CHANNEL_SUM_AUTO = 0,
     /* FP64 Horner: four FMAs per channel and direction, exact to FP64 rounding. */
     LEAP_CHANNEL_SUM_HORNER = 1,
@@ -132,7 +135,15 @@ typedef enum leap_channel_sum {
      * precision: a baseline whose sum nearly cancels takes a phase error of that order,
      * which the inversion spreads over the calibration (measured 4e-5 rad on 48 stations,
      * 2.5e-2 rad on the 256-station, 128-timestep set). */
-    LEAP_CHANNEL_SUM_NUFFT_FP32 = 3
+    LEAP_CHANNEL_SUM_NUFFT_FP32 = 3,
+    /* The NUFFT with an FP64 transform and a kernel of width 16 (16 grid points per
+     * direction), for either precision; pol 0, 1..512 channels.  The kernel's truncation
+     * (exp(-beta), 2e-18 to 1.3e-16) lies below FP64 rounding, so a row's channel sum is
+     * within about 1e-15 of sum |v| of the exact sum at the same FP64 phase step, closer than
+     * the FP64 Horner chains (n eps, about 2e-14 at 384 channels).  A device-resident
+     * complex64 or complex128 cube with 1, 2 or 4 pols is read by the kernel itself; other
+     * sources are packed to pol-0 rows first. */
+    LEAP_CHANNEL_SUM_NUFFT_W16 = 4
 } leap_channel_sum;
 
 typedef struct leap_options {
@@ -390,6 +401,9 @@ int leap_nufft_plan(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* b
 /* The same for the FP32 transform of LEAP_CHANNEL_SUM_NUFFT_FP32 (narrower kernel). */
 int leap_nufft_plan32(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
                       double* scale);
+/* The same for LEAP_CHANNEL_SUM_NUFFT_W16 (W = 16): coef[j * 16 + m], 8 x 16 doubles. */
+int leap_nufft_plan_w16(uint32_t C, uint32_t* N, uint32_t* W, uint32_t* h, double* beta, double* coef,
+                        double* scale);
 
 /* ---- synthetic measurement sets ------------------------------------------- */
 typedef struct leap_synth_config {

@@ -270,7 +270,7 @@ def test_nufft_option_errors(leap, ctx):
                 md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=route, **bad)))
         with pytest.raises(leap.LeapError, match="1..512 channels"):
             md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=route)))
-    for bad in (-1, 4):
+    for bad in (-1, 5):
         with pytest.raises(leap.LeapError, match="channel_sum"):
             md.rotate(view, dirs, opts=leap.options(**mixed(channel_sum=bad)))
     # 520 channels: the automatic choice stays on Horner

@@ -151,6 +151,119 @@ def test_plan_rejects_unsupported(leap):
         for fp32 in (False, True):
             with pytest.raises(leap.LeapError, match="NUFFT"):
                 leap.nufft_plan(C, fp32)
+        with pytest.raises(leap.LeapError, match="NUFFT"):
+            leap.nufft_plan(C, w16=True)
+
+
+# ---- the width-16 plan (LEAP_CHANNEL_SUM_NUFFT_W16) ---------------------------
+
+PI_LD = np.longdouble("3.14159265358979323846264338327950288")
+
+
+def exact_sum_turns(v, x):
+    """sum_c v_c exp(2 pi i c x) in extended precision for evaluation points x
+    given in turns (c x reduced modulo 1 before the trigonometric functions)."""
+    ld = np.longdouble
+    k = np.arange(len(v)).astype(ld)
+    vr, vi = v.real.astype(ld), v.imag.astype(ld)
+    out = np.empty(len(x), complex)
+    for i, xx in enumerate(x):
+        ph = 2 * PI_LD * ((ld(xx) * k) % 1)
+        c, s = np.cos(ph), np.sin(ph)
+        out[i] = complex(float((vr * c - vi * s).sum()), float((vr * s + vi * c).sum()))
+    return out
+
+
+def nufft_numpy_turns(plan, v, x):
+    """nufft_numpy with the evaluation points in turns, as the kernels form
+    them (grid coordinate N frac(x), phase exp(2 pi i h x) with h x reduced
+    modulo 1 exactly)."""
+    N, W, h = plan["N"], plan["W"], plan["h"]
+    C = len(v)
+    ft = np.zeros(N, complex)
+    ft[(np.arange(C) - h) % N] = v * plan["scale"]
+    g = np.fft.ifft(ft) * N
+    u = (x - np.floor(x)) * N
+    m0 = np.ceil(u - W / 2)
+    s = (u - m0) - (W - 1) / 2
+    w = weights(plan, s)
+    acc = np.zeros(len(x), complex)
+    for j in range(W):
+        acc += g[(m0 + j).astype(int) % N] * w[j]
+    hx = (np.longdouble(h) * x.astype(np.longdouble)) % 1
+    return acc * np.exp(2j * np.pi * hx.astype(float))
+
+
+def horner_turns(v, x):
+    """The FP64 Horner evaluation of the same sums (the batched FP64 kernels'
+    method): z = exp(2 pi i x), P(z) by Horner's rule."""
+    out = np.empty(len(x), complex)
+    for i, xx in enumerate(x):
+        z = np.exp(2j * np.pi * (xx % 1.0))
+        acc = 0j
+        for c in range(len(v) - 1, -1, -1):
+            acc = acc * z + v[c]
+        out[i] = acc
+    return out
+
+
+@pytest.mark.parametrize("C", CHANNELS)
+def test_w16_plan_shape(leap, C):
+    p = leap.nufft_plan(C, w16=True)
+    q = leap.nufft_plan(C)
+    assert p["W"] == 16 and p["N"] == q["N"] and p["h"] == q["h"]
+    assert np.exp(-p["beta"]) < 2e-16
+    assert p["coef"].shape == (8, 16) and p["scale"].shape == (C,) and np.all(p["scale"] > 0)
+
+
+@pytest.mark.parametrize("C", [7, 96, 200, 300, 384, 447, 512])
+def test_w16_plan_is_as_accurate_as_fp64_horner(leap, C):
+    """With the evaluation point given in turns (as the kernels form it), the
+    width-16 plan's channel sum is within 2e-15 of sum |v| of the extended-
+    precision sum: the kernel's truncation lies below FP64 rounding, and the
+    FP64 Horner chains (n eps) are not closer."""
+    p = leap.nufft_plan(C, w16=True)
+    rng = np.random.default_rng(C + 7)
+    v = rng.normal(size=C) + 1j * rng.normal(size=C)
+    x = np.concatenate([rng.uniform(-10, 10, 150), [0.0, 0.5, -0.5, 1e-300, 0.25 + 1e-12]])
+    exact = exact_sum_turns(v, x)
+    scale = np.abs(v).sum()
+    err = np.abs(nufft_numpy_turns(p, v, x) - exact).max() / scale
+    assert err < 2e-15, err
+    if C >= 96:
+        herr = np.abs(horner_turns(v, x) - exact).max() / scale
+        assert err < herr, (err, herr)
+
+
+@pytest.mark.parametrize("C", [48, 200, 384, 512])
+def test_w16_weight_polynomials_and_precorrection(leap, C):
+    p = leap.nufft_plan(C, w16=True)
+    W, N, h, beta = p["W"], p["N"], p["h"], p["beta"]
+    s = np.linspace(-0.5, 0.5, 2001)
+    w = weights(p, s)
+    for j in range(W):
+        exact = es((s + (W - 1) / 2 - j) / (W / 2), beta)
+        # at FP64 rounding: exp(beta (sqrt(1 - z^2) - 1)) with beta near 40
+        # carries about 40 eps of relative rounding itself, in the plan's fit
+        # and in this reference alike
+        assert np.abs(w[j] - exact).max() < 1e-14, j
+    z, wq = legendre.leggauss(300)
+    half = W * np.pi / N
+    k = np.arange(C) - h
+    psihat = half * (wq[None, :] * es(z[None, :], beta) * np.cos(np.outer(k, z) * half)).sum(1)
+    assert np.abs(p["scale"] / ((2 * np.pi / N) / psihat) - 1).max() < 1e-13
+
+
+def test_w16_planted_narrow_kernel_fails(leap):
+    """The 2e-15 bound separates the width-16 plan from the width-10 one on
+    the same points, so a route that took the narrower plan fails it."""
+    C = 384
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=C) + 1j * rng.normal(size=C)
+    x = rng.uniform(-10, 10, 100)
+    exact = exact_sum_turns(v, x)
+    err10 = np.abs(nufft_numpy_turns(leap.nufft_plan(C), v, x) - exact).max() / np.abs(v).sum()
+    assert err10 > 1e-13
 
 
 def test_planted_mistakes_fail(leap):
