@@ -167,9 +167,12 @@ def run_c2(args):
         ctx.calibrate(view, dirs, md=md, opts=opts)
         step_s.append(time.perf_counter() - t)
     replayed = ctx.last_graph() == 2
+    route = ctx.last_route()[0]
     out["single_set"] = {"workload": "256 stations / 32896 baselines / 128 t / 384 ch / 4 pol, complex128 cube "
                                      "103.5 GB in HBM, 64 directions per step, FP64, one HIP graph replayed",
                          "graph_replayed": replayed, "tile": ctx.last_direction_block(),
+                         "channel_sums": {la.LEAP_CHANNEL_SUM_NUFFT_W16: "width-16 NUFFT (FP64 transform)",
+                                          la.LEAP_CHANNEL_SUM_HORNER: "FP64 Horner"}.get(route, str(route)),
                          "ms_per_step_median": round(float(np.median(step_s)) * 1e3, 3),
                          "directions_per_s": round(64 * steps / sum(step_s), 1)}
     log(f"[c2] 64 FP64 directions, graph: {out['single_set']['ms_per_step_median']} ms per step")
@@ -449,7 +452,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
                     "traffic": traffic_from_profile(prof + "_traffic.json"),
                     "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": alg_bytes,
                     # measured with rocprofv3 counters, not live
-                    "limiter": f"vector ALU and LDS (profiles/{prof}.md)",
+                    "limiter": f"{'HBM' if nufft16 else 'vector ALU and LDS'} (profiles/{prof}.md)",
                     "valu": {"achieved_tflops": round(tflops, 2), "peak_tflops": VALU_FP64_PEAK_TFLOPS,
                              "frac": round(tflops / VALU_FP64_PEAK_TFLOPS, 4),
                              "algorithmic_flops_per_launch": alg_flops},

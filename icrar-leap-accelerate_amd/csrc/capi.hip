@@ -213,8 +213,7 @@ leap_options routed(const leap_options& o, const leap_ms_view& ms, uint32_t T, u
     if (r.channel_sum == LEAP_CHANNEL_SUM_AUTO) {
         const bool fp64 = r.precision == LEAP_PRECISION_FP64;
         const bool nufft = use_packed(r) && nufft_supported(ms.num_channels) &&
-                           ms.num_channels >= kNufftMinChannels &&
-                           D >= (fp64 ? kNufftW16MinDirections : kNufftMinDirections);
+                           ms.num_channels >= kNufftMinChannels && D >= kNufftMinDirections;
         r.channel_sum = !nufft ? LEAP_CHANNEL_SUM_HORNER : fp64 ? LEAP_CHANNEL_SUM_NUFFT_W16 : LEAP_CHANNEL_SUM_NUFFT;
     }
     if (is_nufft(r.channel_sum)) {

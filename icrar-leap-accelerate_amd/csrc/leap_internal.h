@@ -115,13 +115,13 @@ constexpr int kNufftMaxWidth = 16;
 constexpr int kNufftQuadratureNodes = 100;
 constexpr int kNufftCoefDoubles = (kNufftMaxWidth / 2) * kNufftMaxWidth;
 // Automatic choice: fewer directions or channels stay on Horner (per row the FFT
-// costs about what 4 C FMAs per lane cost at C = 64..96).
+// costs about what 4 C FMAs per lane cost at C = 64..96).  The same bounds
+// hold for the width-16 NUFFT of FP64 precision: on the 256-station set (384
+// channels) it takes 16.5 / 16.6 / 16.9 / 21.8 / 24.5 ms at 16 / 32 / 64 /
+// 128 / 256 directions on the complex128 cube where Horner takes 19.9 / 21.4 /
+// 27.8 / 43.6 / 76.9 ms (complex64: 13.2 .. 18.6 against 15.1 .. 69.1 ms).
 constexpr uint32_t kNufftMinDirections = 16;
 constexpr uint32_t kNufftMinChannels = 96;
-// FP64 precision: the width-16 NUFFT from this many directions (per row its
-// FFT and 16-point interpolations cost what the FP64 Horner chains of about
-// 20 directions cost at 384 channels).
-constexpr uint32_t kNufftW16MinDirections = 32;
 struct NufftPlan {
     bool fp32 = false;  // FP32 transform, grid and interpolation (W 8 or 10) instead of FP64 (W 10 or 12)
     bool w16 = false;   // FP64 transform with kernel width 16 (LEAP_CHANNEL_SUM_NUFFT_W16)

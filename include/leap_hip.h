@@ -92,7 +92,7 @@ typedef struct leap_ms_view {
 
 typedef enum leap_precision {
     /* FP64 phase, FP64 sincos, FP64 sums: the reference's arithmetic.  Batched pol-0
-     * calls with 96..512 channels and at least 32 directions sum the channels with the
+     * calls with 96..512 channels and at least 16 directions sum the channels with the
      * width-16 NUFFT by default (LEAP_CHANNEL_SUM_AUTO; see LEAP_CHANNEL_SUM_NUFFT_W16),
      * other calls with the FP64 Horner chains. */
     LEAP_PRECISION_FP64 = 0,
@@ -116,8 +116,8 @@ typedef enum leap_strategy {
 } leap_strategy;
 
 typedef enum leap_channel_sum {
-    /* Pol 0 and 96..512 channels: mixed precision from 16 directions the NUFFT, FP64
-     * precision from 32 directions the width-16 NUFFT; otherwise Horner. */
+    /* Pol 0, 96..512 channels and at least 16 directions: the NUFFT in mixed precision,
+     * the width-16 NUFFT in FP64 precision; otherwise Horner. */
     LEAP_CHANNEL_SUM_AUTO = 0,
     /* FP64 Horner: four FMAs per channel and direction, exact to FP64 rounding. */
     LEAP_CHANNEL_SUM_HORNER = 1,

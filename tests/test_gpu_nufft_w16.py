@@ -3,7 +3,7 @@
 The width-16 plan keeps the kernel's truncation (exp(-beta) <= 1.3e-16) below FP64
 rounding, so its channel sums are as close to the exact sums as the FP64 Horner
 chains (tests/test_nufft_plan.py pins the plan against extended-precision sums).
-It is the automatic choice of FP64 batched pol-0 calls from 32 directions and
+It is the automatic choice of FP64 batched pol-0 calls from 16 directions and
 96..512 channels, read from a complex64 or complex128 cube on the device or from
 packed pol-0 rows.
 
@@ -231,7 +231,7 @@ def test_w16_is_the_automatic_fp64_choice(leap, ctx):
     ms = s.host_dict()
     with DeviceCube(leap, ctx, ms) as view:
         md = ctx.metadata(view)
-        for n_dirs, want in [(31, HORNER), (32, W16), (200, W16)]:
+        for n_dirs, want in [(15, HORNER), (16, W16), (200, W16)]:
             md.rotate(view, dirs_near(ms["phase_centre"], n_dirs, 13), opts=leap.options(**fp64()))
             assert ctx.last_route()[0] == want, n_dirs
     few = synth(leap, 90, n_t=2)
