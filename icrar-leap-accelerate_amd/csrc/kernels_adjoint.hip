@@ -71,7 +71,9 @@ __global__ __launch_bounds__(kThreads) void rotate_adjoint_kernel(
     constexpr int kCC = 256 / RP > 16 ? 256 / RP : 16;  // channels per chunk
     constexpr int DW = DL * ND;                          // directions of the workgroup
     using Ring = dev::CubeRing<VT, RP, kCC, sizeof(VT) == 4 ? 2 : 1>;
-    __shared__ Cx s_v[RP][kCC];
+    // one padding element per row for fewer than 16 lanes per row (rows kCC * 16 bytes apart
+    // would share LDS banks within a 16-lane group)
+    __shared__ Cx s_v[RP][kCC + (DL < 16 ? 1 : 0)];
     __shared__ dev::f32x4 s_raw[Ring::kBytes / 16];
     __shared__ double s_g[RP][DW][3];
     const uint32_t bl = blockIdx.x;

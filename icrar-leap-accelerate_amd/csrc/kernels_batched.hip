@@ -286,7 +286,10 @@ struct Horner64Smem {
     static constexpr int L = DB / ND;
     static constexpr int RP = kThreads / L;
     typedef double Red[(RP > 1) ? RP : 1][L][ND][NP][2];  // per-row sums of each direction
-    C2<double> v[RP][NP][CC];
+    // With fewer than 16 lanes per row a wave's 16-lane group of a 16-byte read spans two
+    // rows, whose entries CC * 16 bytes apart share LDS banks: one padding element staggers them.
+    static constexpr int kRowPad = L < 16 ? 1 : 0;
+    C2<double> v[RP][NP][CC + kRowPad];
     // pol-0 tiles combine their row sums in the raw chunk buffer, idle once the last chunk is in
     double red[NP == 1 ? 1 : sizeof(Red) / sizeof(double)];
 };
