@@ -117,7 +117,7 @@ constexpr int kNufftCoefDoubles = (kNufftMaxWidth / 2) * kNufftMaxWidth;
 // Automatic choice: fewer directions or channels stay on Horner (per row the FFT
 // costs about what 4 C FMAs per lane cost at C = 64..96).  The same bounds
 // hold for the width-16 NUFFT of FP64 precision: on the 256-station set (384
-// channels) it takes 16.5 / 16.6 / 16.9 / 21.8 / 24.5 ms at 16 / 32 / 64 /
+// channels) it takes 16.5 / 16.6 / 16.9 / 17.5 / 24.5 ms at 16 / 32 / 64 /
 // 128 / 256 directions on the complex128 cube where Horner takes 19.9 / 21.4 /
 // 27.8 / 43.6 / 76.9 ms (complex64: 13.2 .. 18.6 against 15.1 .. 69.1 ms).
 constexpr uint32_t kNufftMinDirections = 16;

@@ -771,9 +771,9 @@ __device__ __forceinline__ void nufft_rows(const void* wr, const LEAP_GLOBAL dou
     // unconditional (the last row reloads itself), so the row buffer keeps its
     // registers from one iteration to the next.
     // complex128 cubes: whole-pol loads where the registers allow it at two
-    // waves per SIMD (one direction per lane, the 1024-point grid, at most 6
-    // channel slots: 249 VGPRs with 4 pols)
-    Source<N, SP, CPL, (SP > 16), (SP > 16 && NDL == 1 && N == 1024 && CPL <= 6)> src;
+    // waves per SIMD (one or two directions per lane, the 1024-point grid, at
+    // most 6 channel slots: 249 / 251 VGPRs with 4 pols)
+    Source<N, SP, CPL, (SP > 16), (SP > 16 && NDL <= 2 && N == 1024 && CPL <= 6)> src;
     src.init(scale, lane, C);
     if (t_count > 0) src.load(wr, bl, lane, C);
     for (uint32_t t = 0; t < t_count; ++t) {
