@@ -10,8 +10,9 @@ baselines, 128 timesteps, 384 channels, 4 pols (the complex128 cube is
   same values, on the NUFFT and both FP64 Horner routes (<= 1e-5 rad
   wrapped; per-direction max and RMS printed).
 - Batched FP64 with 64 directions on the complex128 cube under HIP graph
-  capture: replays bit-equal to the eager call, within 1e-9 rad of the plain
-  path.
+  capture (the automatic route: the width-16 NUFFT): replays bit-equal to the
+  eager call, within 5e-9 rad of the plain path and 1e-9 rad of the FP64
+  Horner route.
 """
 import os
 import sys
@@ -121,8 +122,9 @@ def test_full_size_batched_fp64_graph(leap, ctx):
     the rounded wavelength, as the reference does (CpuLeapCalibrator.cc:195): for
     phases of 1e3 to 1e4 rad the two FP64 roundings differ by about 1e-12 rad per
     term, growing with the offset, and a baseline whose sum nearly cancels turns
-    that into a phase error the inversion spreads over the calibration (2.8e-9 rad
-    measured here, below 1e-9 with offsets up to 0.02 rad in test_full_size_fp64)."""
+    that into a phase error the inversion spreads over the calibration (2.5e-9 rad
+    measured here on the width-16 NUFFT and 2.8e-9 on the Horner route, which lie
+    2.7e-10 apart; below 1e-9 with offsets up to 0.02 rad in test_full_size_fp64)."""
     s = leap.SynthMS(**bench.c1_config())
     vis_p, uvw_p, view = device_cube(leap, ctx, s, leap.LEAP_VIS_C128, 16)
     try:
@@ -130,14 +132,19 @@ def test_full_size_batched_fp64_graph(leap, ctx):
         c = (view.phase_centre_ra_rad, view.phase_centre_dec_rad)
         dirs = bench.directions(c, 64, seed=13, spread=0.1)
         eager, ee = ctx.calibrate(view, dirs, opts=leap.options(strategy=1), md=md)
+        assert ctx.last_route()[0] == leap.LEAP_CHANNEL_SUM_NUFFT_W16
         for k in range(3):
             got, ge = ctx.calibrate(view, dirs, opts=leap.options(strategy=1, graph=1), md=md)
             assert ctx.last_graph() == (1 if k == 0 else 2)
             assert np.array_equal(got, eager) and np.array_equal(ge, ee), k
         plain, _ = ctx.calibrate(view, dirs, opts=leap.options(), md=md)
+        horner, _ = ctx.calibrate(view, dirs, opts=leap.options(strategy=1, channel_sum=1), md=md)
         err = wrapped(eager, plain).max()
-        print(f"batched FP64 (graph) vs plain, 64 directions on the complex128 cube: {err:.2e} rad")
+        err_h = wrapped(eager, horner).max()
+        print(f"batched FP64 (width-16 NUFFT, graph) vs plain, 64 directions on the complex128 cube: {err:.2e} rad; "
+              f"vs the FP64 Horner route {err_h:.2e} rad (Horner vs plain {wrapped(horner, plain).max():.2e} rad)")
         assert err < 5e-9
+        assert err_h < 1e-9
     finally:
         ctx.free(vis_p)
         ctx.free(uvw_p)
