@@ -145,8 +145,9 @@ UNEVEN = [(36, 0, 32, 300), (128, 1, 14, 48), (256, 1, 64, 96)]
 def run_c2(args):
     """Batching in FP64 and the uneven batch: 64 directions (offsets up to 0.1 rad) on the 256-station
     complex128 cube captured as one HIP graph and replayed each step; then the three uneven sets in one
-    call per step, in FP64 (Horner tile, captured) and in mixed precision on their complex64 cubes
-    (the automatic route: the NUFFT for the 300- and 96-channel sets, Horner for the 48-channel one)."""
+    call per step, in FP64 (captured) on their complex128 cubes and in mixed precision on their complex64
+    cubes (the automatic route: the NUFFT, width 16 in FP64, for the 300- and 96-channel sets, Horner for
+    the 48-channel one)."""
     steps = args.steps
     out = {}
     synth = la.SynthMS(**c1_config())
@@ -209,8 +210,9 @@ def run_c2(args):
             "ms_per_step_median": round(float(np.median(step_s)) * 1e3, 3),
             "rotate_kernel_ms": None if prec == 0 else round(float(np.median(k_ms)), 4),
             "directions_per_s": round(3 * 64 * steps / sum(step_s), 1),
-            "channel_sums": "FP64 Horner" if prec == 0 else
-                            ("NUFFT (300- and 96-channel sets) + Horner (48 channels)" if route == 2 else "Horner")}
+            "channel_sums": {la.LEAP_CHANNEL_SUM_NUFFT: "NUFFT (300- and 96-channel sets) + Horner (48 channels)",
+                             la.LEAP_CHANNEL_SUM_NUFFT_W16: "width-16 NUFFT (300- and 96-channel sets) + FP64 Horner "
+                                                            "(48 channels)"}.get(route, "FP64 Horner")}
         log(f"[c2] uneven batch {name}: {out[f'uneven_batch_{name}']['ms_per_step_median']} ms per step")
         for p_ in ptrs:
             ctx.free(p_)

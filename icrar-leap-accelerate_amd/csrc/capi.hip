@@ -950,21 +950,28 @@ struct NufftGroup {
     std::vector<uint32_t> items;
 };
 
-// Whether a set of a multi-set call takes the NUFFT channel sums (FP64
-// transform): mixed precision, a complex64 cube with 1, 2 or 4 pols, and the
-// single-set automatic rule (96..512 channels, 16 or more directions) unless
-// the NUFFT is asked for.
-bool set_takes_nufft(const leap_options& opts, const leap_metadata& md, const leap_ms_view& ms, uint32_t D) {
-    if (opts.precision != LEAP_PRECISION_MIXED || D == 0) return false;
-    const bool cube_ok = nufft_cube_ok(ms, false) && md.nufft64.N != 0;
-    if (opts.channel_sum == LEAP_CHANNEL_SUM_NUFFT) {
+// The channel sums a set of a multi-set call takes: 0 (Horner) or the NUFFT
+// route (LEAP_CHANNEL_SUM_NUFFT in mixed precision, LEAP_CHANNEL_SUM_NUFFT_W16
+// in FP64 or when asked for), on a cube the kernel reads (complex64, or
+// complex128 for the width-16 kernel, with 1, 2 or 4 pols), by the single-set
+// automatic rule (96..512 channels, 16 or more directions) unless the NUFFT is
+// asked for.
+int32_t set_nufft_route(const leap_options& opts, const leap_metadata& md, const leap_ms_view& ms, uint32_t D) {
+    if (D == 0) return 0;
+    int32_t cs = opts.channel_sum;
+    if (cs == LEAP_CHANNEL_SUM_AUTO)
+        cs = opts.precision == LEAP_PRECISION_FP64 ? LEAP_CHANNEL_SUM_NUFFT_W16 : LEAP_CHANNEL_SUM_NUFFT;
+    else if (cs != LEAP_CHANNEL_SUM_NUFFT && cs != LEAP_CHANNEL_SUM_NUFFT_W16)
+        return 0;
+    const bool cube_ok = nufft_cube_ok(ms, cs == LEAP_CHANNEL_SUM_NUFFT_W16) && md.nufft_plan(cs).N != 0;
+    if (opts.channel_sum != LEAP_CHANNEL_SUM_AUTO) {
         require(cube_ok, LEAP_ERR_INVALID_ARGUMENT,
                 "the multi-set NUFFT route needs 1.." + std::to_string(kNufftMaxChannels) +
-                    " channels and a complex64 cube with 1, 2 or 4 pols at an aligned address");
-        return true;
+                    " channels and a complex64 cube (complex128 too for the width-16 NUFFT) with 1, 2 or 4 "
+                    "pols at an aligned address");
+        return cs;
     }
-    return opts.channel_sum == LEAP_CHANNEL_SUM_AUTO && cube_ok && md.channels >= kNufftMinChannels &&
-           D >= kNufftMinDirections;
+    return cube_ok && md.channels >= kNufftMinChannels && D >= kNufftMinDirections ? cs : 0;
 }
 
 // NUFFT tile of a multi-set launch: the caller's when it is a NUFFT tile,
@@ -1164,7 +1171,7 @@ struct BatchState {
     int32_t solutions;
     uint32_t interval;
     std::vector<double> h_cal;
-    bool nufft = false;  // NUFFT channel sums (avgData holds pol 0 alone)
+    int32_t nufft = 0;  // NUFFT route of the channel sums, 0 for Horner (avgData holds pol 0 alone)
 };
 
 void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets, const leap_options& opts,
@@ -1174,9 +1181,8 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
             "several measurement sets in one launch need the batched strategy");
     require(!opts.all_pols && !opts.fp32_blocks, LEAP_ERR_NOT_IMPLEMENTED,
             "the multi-set launch accumulates polarisation 0 with the FP64 Horner or NUFFT channel sums only");
-    require(opts.channel_sum != LEAP_CHANNEL_SUM_NUFFT_FP32 && opts.channel_sum != LEAP_CHANNEL_SUM_NUFFT_W16,
-            LEAP_ERR_NOT_IMPLEMENTED, "the multi-set launch takes the width-10 NUFFT with its FP64 transform "
-                                      "(channel_sum 2)");
+    require(opts.channel_sum != LEAP_CHANNEL_SUM_NUFFT_FP32, LEAP_ERR_NOT_IMPLEMENTED,
+            "the multi-set launch takes the NUFFT with its FP64 transform (channel_sum 2 or 4)");
     std::vector<BatchState> st(n_sets);
     int32_t rounds = 0;
     for (uint32_t i = 0; i < n_sets; ++i) {
@@ -1193,13 +1199,13 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
         evaluate_slice(in.slice_start, in.slice_interval, in.slice_end, T, range);
         st[i] = BatchState{in.metadata, in.ms, in.directions, in.n_directions,
                            (range[2] - range[0]) / range[1], uint32_t(range[1]), {},
-                           set_takes_nufft(opts, *in.metadata, *in.ms, in.n_directions)};
+                           set_nufft_route(opts, *in.metadata, *in.ms, in.n_directions)};
         rounds = std::max(rounds, st[i].solutions);
     }
     while (ctx->batch_sets.size() < n_sets) ctx->batch_sets.push_back(std::make_unique<SetBuffers>());
-    ctx->last_channel_sum = std::any_of(st.begin(), st.end(), [](const BatchState& b) { return b.nufft; })
-                                ? LEAP_CHANNEL_SUM_NUFFT
-                                : LEAP_CHANNEL_SUM_HORNER;
+    ctx->last_channel_sum = LEAP_CHANNEL_SUM_HORNER;
+    for (const BatchState& b : st)
+        if (b.nufft) ctx->last_channel_sum = b.nufft;  // one NUFFT route per call (set by the precision)
     ctx->last_direct_cube = 1;
     begin_timing(ctx);
     for (uint32_t i = 0; i < n_sets; ++i) {
@@ -1231,9 +1237,11 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
                                                     uint32_t(size_t(r + 1) * b.interval * nbl), buf.d_vis, buf.d_uvw);
             active.push_back(i);
             if (b.nufft) {
-                const NufftPlan& plan = b.md->nufft64;
+                const NufftPlan& plan = b.md->nufft_plan(b.nufft);
                 const uint32_t C = b.md->channels;
-                const uint64_t key = nufft_sets_key(plan, C, b.md->pols);
+                // row source: the cube's pols, plus 16 for complex128
+                const uint32_t src = b.md->pols + (b.ms->vis_dtype == LEAP_VIS_C128 ? 16u : 0u);
+                const uint64_t key = nufft_sets_key(plan, C, src);
                 auto g = std::find_if(groups.begin(), groups.end(), [&](const NufftGroup& x) { return x.key == key; });
                 if (g == groups.end()) {
                     groups.emplace_back();
@@ -1241,9 +1249,9 @@ void calibrate_batch(leap_ctx* ctx, const leap_batch_set* sets, uint32_t n_sets,
                     g->plan = &plan;
                     g->key = key;
                     g->C = C;
-                    g->P = b.md->pols;
+                    g->P = src;
                 }
-                const double* tables = b.md->nufft_tables(LEAP_CHANNEL_SUM_NUFFT);
+                const double* tables = b.md->nufft_tables(b.nufft);
                 const double df = b.md->freq_inc_hz;
                 g->sets.push_back(NufftSetDesc{rows.vis, rows.uvw, buf.d_dd.as<double>(), buf.d_avg.as<double>(),
                                                tables + kNufftCoefDoubles, tables,

@@ -27,6 +27,7 @@ void launch_rotate_nufft_f64(hipStream_t s, const nufft::NufftLaunch& l) {
 uint64_t nufft_sets_key(const NufftPlan& plan, uint32_t C, uint32_t P) {
     uint32_t slots = plan.N / 128;
     if (!plan.fp32 && plan.N == 1024 && plan.W == 10) slots = uint32_t(nufft::nufft_slots<1024, 10, double>(C));
+    if (!plan.fp32 && plan.N == 1024 && plan.W == 16) slots = uint32_t(nufft::nufft_slots<1024, 16, double>(C));
     return (uint64_t(plan.N) << 32) | (uint64_t(plan.W) << 24) | (uint64_t(plan.fp32 ? 1 : 0) << 16) |
            (uint64_t(slots) << 8) | uint64_t(P);
 }
@@ -34,8 +35,12 @@ uint64_t nufft_sets_key(const NufftPlan& plan, uint32_t C, uint32_t P) {
 void launch_rotate_nufft_sets(hipStream_t s, int32_t tile, const NufftPlan& plan, uint32_t C, uint32_t P,
                               const NufftSetDesc* sets, const uint32_t* items, uint32_t n_items) {
     if (n_items == 0) return;
-    require(!plan.fp32 && !plan.w16, LEAP_ERR_NOT_IMPLEMENTED, "the multi-set NUFFT launch uses the width-10 or 12 FP64 plan");
-    require(P == 1 || P == 2 || P == 4, LEAP_ERR_INTERNAL, "the multi-set NUFFT launch reads complex64 cubes");
+    if (plan.w16) {
+        launch_rotate_nufft_sets_w16(s, tile, plan, C, P, sets, items, n_items);
+        return;
+    }
+    require(!plan.fp32, LEAP_ERR_NOT_IMPLEMENTED, "the multi-set NUFFT launch uses an FP64 transform");
+    require(P == 1 || P == 2 || P == 4, LEAP_ERR_INTERNAL, "the width-10 multi-set NUFFT launch reads complex64 cubes");
 #define LEAP_NUFFT_SETS(N_, W_)                                                                  \
     if (plan.N == N_ && plan.W == W_) {                                                          \
         nufft::launch_nw_sets<N_, W_, double>(s, tile, int32_t(P), C, sets, items, n_items);     \

@@ -352,7 +352,7 @@ void launch_rotate_nufft(hipStream_t s, int32_t tile, const NufftPlan& plan, con
 size_t nufft_tail_bytes(uint32_t D);
 // One set of a multi-set NUFFT launch (FP64 transform, complex64 cube).
 struct NufftSetDesc {
-    const void* vis;      // [T nbl][C][P] complex64
+    const void* vis;      // [T nbl][C][P] complex64 (or complex128 for the width-16 plan)
     const double* uvw;    // [T nbl][3]
     const double* dd;     // [D][3][3]
     double* avg;          // [D][nbl] complex128: pol 0 only
@@ -364,11 +364,13 @@ struct NufftSetDesc {
     uint32_t nbl, T, C, D;
 };
 // Sets whose keys are equal share one multi-set NUFFT launch (same grid, kernel
-// width, channel slots per lane and cube pols).
+// width, channel slots per lane and row source: P the cube's pols, plus 16 for
+// complex128).
 uint64_t nufft_sets_key(const NufftPlan& plan, uint32_t C, uint32_t P);
 // NUFFT channel sums of several sets in one launch: workgroup x takes
 // items[x] = (set | direction block << 16, baseline); tile 641 / 1282 / 2564.
-// plan, C and P: those of any set of the launch (equal keys).
+// plan, C and P (the row source: the pols, plus 16 for complex128): those of
+// any set of the launch (equal keys).
 void launch_rotate_nufft_sets(hipStream_t s, int32_t tile, const NufftPlan& plan, uint32_t C, uint32_t P,
                               const NufftSetDesc* sets, const uint32_t* items, uint32_t n_items);
 bool packed_tile_supported(int32_t tile);

@@ -1152,7 +1152,10 @@ struct NufftLaunch {
 // kernels_nufft32.hip: float).
 void launch_rotate_nufft_f64(hipStream_t s, const nufft::NufftLaunch& l);
 void launch_rotate_nufft_f32(hipStream_t s, const nufft::NufftLaunch& l);
-// The width-16 kernel (kernels_nufft_w16.hip).
+// The width-16 kernel (kernels_nufft_w16.hip) and its multi-set form
+// (kernels_nufft_w16_sets.hip; src: the cube's pols, plus 16 for complex128).
 void launch_rotate_nufft_w16(hipStream_t s, const nufft::NufftLaunch& l);
+void launch_rotate_nufft_sets_w16(hipStream_t s, int32_t tile, const NufftPlan& plan, uint32_t C, uint32_t src,
+                                  const NufftSetDesc* sets, const uint32_t* items, uint32_t n_items);
 
 }  // namespace leap

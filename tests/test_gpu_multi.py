@@ -145,6 +145,41 @@ def test_multi_nufft_equals_single_set(leap, ctx, nufft_sets, db, nd):
         assert np.array_equal(got, one), "multi-set and single-set NUFFT results differ"
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_multi_w16_matches_oracle_and_single_set(leap, ctx, oracle, dtype):
+    """FP64 multi-set calls: qualifying sets take the width-16 NUFFT (complex64 and
+    complex128 cubes), within 1e-9 rad of the oracle, and bit-equal to the
+    single-set launch of the same tile for sets with more directions than one block
+    (the single-set launch then runs every (baseline, direction block) whole)."""
+    sets = []
+    for kw, nd, slice_ in NUFFT_SETS:
+        s = leap.SynthMS(**kw)
+        ms = s.host_dict(s.fill_host(dtype))
+        sets.append((ms, dirs_near(ms["phase_centre"], nd, nd + 9, spread=0.05), slice_))
+    views = [leap.view_from_arrays(ms) for ms, _, _ in sets]
+    mds = [ctx.metadata(v) for v in views]
+    opts = leap.options(strategy=leap.LEAP_STRATEGY_BATCHED)
+    res = ctx.calibrate_batch([(md, v, d, sl) for md, v, (_, d, sl) in zip(mds, views, sets)], opts)
+    assert ctx.last_route()[0] == 4
+    worst = 0.0
+    for (ms, dirs, slice_), (got, ge) in zip(sets, res):
+        want, we = oracle.calibrate(ms, dirs, slice_)
+        assert np.array_equal(ge, we)
+        worst = max(worst, wrapped(got, want).max())
+    print(f"multi-set width-16 NUFFT, {np.dtype(dtype).name}: max wrapped error {worst:.2e} rad")
+    assert worst < 1e-9
+    pick = [i for i, (_, d, _) in enumerate(sets[:4]) if len(d) > 64]
+    assert pick
+    tiled = leap.options(strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=64, lane_directions=1)
+    res = ctx.calibrate_batch([(mds[i], views[i], sets[i][1], sets[i][2]) for i in pick], tiled)
+    assert ctx.last_route()[0] == 4
+    for i, (got, ge) in zip(pick, res):
+        one, oe = ctx.calibrate(views[i], sets[i][1], sets[i][2], opts=tiled, md=mds[i])
+        assert ctx.last_route()[0] == 4
+        assert np.array_equal(ge, oe)
+        assert np.array_equal(got, one), "multi-set and single-set width-16 NUFFT results differ"
+
+
 def test_multi_nufft_rejects_unsupported(leap, ctx, small_sets, nufft_sets):
     ms, dirs, slice_ = small_sets[0][1], small_sets[0][2], small_sets[0][3]
     v = leap.view_from_arrays(ms)  # complex128
@@ -202,16 +237,16 @@ def test_uneven_batch_full_size(leap, ctx, oracle, precision):
             sets.append((s, ms, host, view, ctx.metadata(view), dirs))
         batch = [(md, v, d, (0, -1, -1)) for (_, _, _, v, md, d) in sets]
         for db, nd in ((128, 2), (64, 1)):
-            # the Horner tile for every set (mixed: asked for), bit-equal to the single-set kernel
-            # that shares the multi-set tile code and reads the cube directly
+            # the Horner tile for every set (asked for), bit-equal to the single-set kernel that
+            # shares the multi-set tile code and reads the cube directly
             hopts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
-                                 lane_directions=nd, channel_sum=1 if precision else 0)
+                                 lane_directions=nd, channel_sum=1)
             hres = ctx.calibrate_batch(batch, hopts)
             assert ctx.last_route()[0] == 1
             horner_ms = ctx.last_kernel_times()[0]
             single_ms = 0.0
             one_opts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
-                                    lane_directions=nd, direct_cube=1, channel_sum=1 if precision else 0)
+                                    lane_directions=nd, direct_cube=1, channel_sum=1)
             for (s, ms, host, view, md, dirs), (got, _) in zip(sets, hres):
                 one, _ = ctx.calibrate(view, dirs, opts=one_opts, md=md)
                 single_ms += ctx.last_kernel_times()[0]
@@ -219,20 +254,23 @@ def test_uneven_batch_full_size(leap, ctx, oracle, precision):
             res = hres
             line = (f"uneven batch {'mixed' if precision else 'fp64'} tile {db}x{nd}: Horner, one launch "
                     f"{horner_ms:.3f} ms, three single-set launches {single_ms:.3f} ms")
-            if precision:
-                # the automatic route: the 300- and 96-channel sets on the NUFFT, the 48-channel set on Horner
-                opts = leap.options(precision=1, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
-                                    lane_directions=nd)
-                res = ctx.calibrate_batch(batch, opts)
-                assert ctx.last_route()[0] == 2
-                nufft_ms = ctx.last_kernel_times()[0]
-                assert np.array_equal(res[1][0], hres[1][0])  # the Horner set: the same tile code
-                for (s, ms, host, view, md, dirs), (got, _) in zip(sets, res):
-                    one, _ = ctx.calibrate(view, dirs, opts=opts, md=md)
-                    # the single-set NUFFT launch splits its last round's rows into chunks
-                    err = wrapped(got, one).max()
-                    assert err < 1e-10, err
-                line += f"; automatic route (NUFFT for 2 of 3 sets), one round {nufft_ms:.3f} ms"
+            # the automatic route: the 300- and 96-channel sets on the NUFFT (mixed precision) or the
+            # width-16 NUFFT (FP64), the 48-channel set on Horner
+            opts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=db,
+                                lane_directions=nd)
+            res = ctx.calibrate_batch(batch, opts)
+            assert ctx.last_route()[0] == (2 if precision else 4)
+            nufft_ms = ctx.last_kernel_times()[0]
+            assert np.array_equal(res[1][0], hres[1][0])  # the Horner set: the same tile code
+            for (s, ms, host, view, md, dirs), (got, _), (hgot, _) in zip(sets, res, hres):
+                one, _ = ctx.calibrate(view, dirs, opts=opts, md=md)
+                # the single-set NUFFT launch splits its last round's rows into chunks
+                err = wrapped(got, one).max()
+                assert err < 1e-10, err
+                if not precision:
+                    assert wrapped(got, hgot).max() < 1e-9  # FP64: the width-16 NUFFT and Horner agree
+            line += (f"; automatic route ({'NUFFT' if precision else 'width-16 NUFFT'} for 2 of 3 sets), one round "
+                     f"{nufft_ms:.3f} ms")
             print(line)
         # the same batch (tile 64 x 1) captured as one HIP graph and replayed: bit-equal to the eager launch
         gopts = leap.options(precision=precision, strategy=leap.LEAP_STRATEGY_BATCHED, direction_block=64,
