@@ -178,8 +178,12 @@ def test_w16_misaligned_cube_takes_the_packed_rows(leap, ctx):
     assert np.array_equal(aligned, off)
 
 
-@pytest.mark.parametrize("n_ch,n_dirs,n_t,n_pols", [(1, 1, 3, 4), (7, 5, 1, 2), (64, 65, 2, 1), (512, 3, 2, 4)])
+@pytest.mark.parametrize("n_ch,n_dirs,n_t,n_pols", [(1, 1, 3, 4), (7, 5, 1, 2), (64, 65, 2, 1), (512, 3, 2, 4),
+                                                   (300, 20, 2, 3)])
 def test_w16_edge_shapes_match_oracle(leap, ctx, oracle, n_ch, n_dirs, n_t, n_pols):
+    """Smallest and largest channel counts, one timestep, one direction, a direction
+    count one past a block, and 3 pols (which the kernel does not read itself: the
+    device cube is packed first), on the device cube and on packed rows."""
     print(f"w16 edge: {n_ch} ch, {n_dirs} dirs, {n_t} t, {n_pols} pols", flush=True)
     s = leap.SynthMS(n_antennas=10, n_timesteps=n_t, n_channels=n_ch, n_pols=n_pols, nan_fraction=0.01, seed=317)
     s.fill_host(np.complex128)
@@ -195,11 +199,13 @@ def test_w16_edge_shapes_match_oracle(leap, ctx, oracle, n_ch, n_dirs, n_t, n_po
     check_against_oracle(cube, oracle.metadata(ms), dirs)
 
 
-def test_w16_skips_non_finite_pols_like_horner(leap, ctx):
+@pytest.mark.parametrize("n_ch,nd", [(160, 1), (384, 1), (384, 2), (384, 4)])
+def test_w16_skips_non_finite_pols_like_horner(leap, ctx, n_ch, nd):
     """NaN or infinity in any pol of a (row, channel), pol 0 finite, drops that
-    channel from the sum on the complex128 cube (only the upper words of pols
-    1.. are loaded), as the Horner route does."""
-    s = synth(leap, 160, n_t=4, nan_fraction=0.0)
+    channel from the sum on the complex128 cube, as the Horner route does: with
+    the upper words of pols 1.. loaded (512-point grid; 4 directions per lane) and
+    with every pol loaded whole (1024-point grid, 1 or 2 directions per lane)."""
+    s = synth(leap, n_ch, n_t=4, nan_fraction=0.0)
     ms = s.host_dict()
     nbl = ms["num_baselines"]
     vis = np.array(ms["vis"]).reshape(ms["num_rows"], ms["num_channels"], ms["num_pols"])
@@ -208,10 +214,11 @@ def test_w16_skips_non_finite_pols_like_horner(leap, ctx):
     for t, bl, ch, pol, val in planted:
         vis[t * nbl + bl, ch, pol] = val
     ms = s.host_dict(vis.reshape(ms["vis"].shape))
-    dirs = dirs_near(ms["phase_centre"], 64, 12)
+    dirs = dirs_near(ms["phase_centre"], 64 * nd, 12)
+    tile = dict(direction_block=64 * nd, lane_directions=nd)
     with DeviceCube(leap, ctx, ms) as view:
         md = ctx.metadata(view)
-        w16 = md.rotate(view, dirs, opts=leap.options(**fp64(channel_sum=W16)))
+        w16 = md.rotate(view, dirs, opts=leap.options(**fp64(channel_sum=W16, **tile)))
         assert ctx.last_route() == (W16, 1)
         horner = md.rotate(view, dirs, opts=leap.options(**fp64(channel_sum=HORNER)))
     assert np.all(np.isfinite(w16))
@@ -221,7 +228,7 @@ def test_w16_skips_non_finite_pols_like_horner(leap, ctx):
     # a planted baseline differs from the cube without the plant by that channel's term
     clean = s.host_dict()
     with DeviceCube(leap, ctx, clean) as view:
-        ref = md.rotate(view, dirs, opts=leap.options(**fp64(channel_sum=W16)))
+        ref = md.rotate(view, dirs, opts=leap.options(**fp64(channel_sum=W16, **tile)))
     for t, bl, ch, pol, _ in planted:
         assert np.abs(w16[:, bl, 0] - ref[:, bl, 0]).max() > 1e-6 * np.abs(ref[:, bl, 0]).max()
 
