@@ -119,12 +119,13 @@ typedef enum leap_channel_sum {
     /* Pol 0, 96..512 channels and at least 16 directions: the NUFFT in mixed precision,
      * the width-16 NUFFT in FP64 precision; otherwise Horner. */
     LEAP_CHANNEL_SUM_AUTO = 0,
-    /* FP64 Horner: four FMAs per channel and direction, exact to FP64 rounding. */
+    /* FP64 Horner: four FMAs per channel and direction (rounding error n eps of sum |v|). */
     LEAP_CHANNEL_SUM_HORNER = 1,
     /* Type-2 non-uniform FFT per row (one inverse FFT of 256..1024 points, then W = 10 or 12
      * grid points per direction); mixed precision, pol 0, 1..512 channels.  A device-resident
-     * complex64 cube with 1, 2 or 4 pols is read by the NUFFT kernel itself; other sources
-     * (host-resident or complex128 cubes, 3 pols) are first packed to pol-0 rows.  Error of a
+     * complex64 cube with 1, 2 or 4 pols (16-byte aligned for 2 or 4) is read by the NUFFT kernel
+     * itself; other sources (host-resident or complex128 cubes, 3 pols) are first packed to
+     * pol-0 rows.  Error of a
      * row's channel sum: about 1e-11 of sum |v| (the tests bound avgData by 5e-10 of the sum
      * of |v| over the interval's rows and channels). */
     LEAP_CHANNEL_SUM_NUFFT = 2,
@@ -141,8 +142,8 @@ typedef enum leap_channel_sum {
      * (exp(-beta), 2e-18 to 1.3e-16) lies below FP64 rounding, so a row's channel sum is
      * within about 1e-15 of sum |v| of the exact sum at the same FP64 phase step, closer than
      * the FP64 Horner chains (n eps, about 2e-14 at 384 channels).  A device-resident
-     * complex64 or complex128 cube with 1, 2 or 4 pols is read by the kernel itself; other
-     * sources are packed to pol-0 rows first. */
+     * complex64 or complex128 cube with 1, 2 or 4 pols (16-byte aligned, except one complex64
+     * pol) is read by the kernel itself; other sources are packed to pol-0 rows first. */
     LEAP_CHANNEL_SUM_NUFFT_W16 = 4
 } leap_channel_sum;
 
