@@ -502,6 +502,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
                             "frac": round(pack_bytes / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if p_ms > 0 else None}
 
     error = None
+    fp64_batched = None
     if mode == "batched-mixed" and args.error_check:
         # Outside the timed region: the first timed step's directions again, against the plain FP64 path
         # (the reference's arithmetic, held to 1e-9 rad of the oracle by the parity tests) on the same cube.
@@ -520,6 +521,26 @@ def run_mode(args, mode, D, with_cpu, steps=None):
                  "per_direction_max_rad": [float(f"{x:.3g}") for x in per_max],
                  "per_direction_rms_rad": [float(f"{x:.3g}") for x in per_rms]}
         log(f"[{mode}] wrapped phase error vs FP64: max {error['max_rad']:.3e} rad, rms {error['rms_rad']:.3e} rad")
+        # FP64 precision on the same cube and directions, batched (its automatic route), also outside the
+        # timed region: its rate beside the headline's, and its distance from the plain FP64 path.
+        f64_opts = la.options(strategy=la.LEAP_STRATEGY_BATCHED)
+        ctx.calibrate(view, sel, md=md, opts=f64_opts)  # tunes its tile
+        f64_s = []
+        for _ in range(5):
+            t = time.perf_counter()
+            f64, _ = ctx.calibrate(view, sel, md=md, opts=f64_opts)
+            f64_s.append(time.perf_counter() - t)
+        f64_err = np.abs((f64[0] - want[0] + np.pi) % (2 * np.pi) - np.pi)
+        fp64_batched = {
+            "workload": f"batched FP64 calibration of the same {D} directions on the same complex64 cube "
+                        "(5 steps after a tuning call, not part of value)",
+            "channel_sums": {la.LEAP_CHANNEL_SUM_NUFFT_W16: "width-16 NUFFT (FP64 transform)",
+                             la.LEAP_CHANNEL_SUM_HORNER: "FP64 Horner"}.get(ctx.last_route()[0], "other"),
+            "ms_per_step_median": round(float(np.median(f64_s)) * 1e3, 3),
+            "directions_per_s": round(D / float(np.median(f64_s)), 1),
+            "max_rad_vs_plain_fp64": float(f64_err.max())}
+        log(f"[{mode}] batched FP64 on the same cube: {fp64_batched['ms_per_step_median']} ms per step, "
+            f"max {fp64_batched['max_rad_vs_plain_fp64']:.2e} rad from the plain path")
 
     cpu = None
     if with_cpu:
@@ -570,6 +591,7 @@ def run_mode(args, mode, D, with_cpu, steps=None):
             "vis_dtype": "complex64" if elem == 8 else "complex128", "directions_per_step": D, "mode": mode,
             "tile": tile, "metadata_s": round(t_md, 3),
             **({"error_vs_fp64": error} if error else {}),
+            **({"fp64_batched_same_cube": fp64_batched} if fp64_batched else {}),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
